@@ -1,0 +1,163 @@
+"""ctypes mirror of include/cj_api.h and the loader of libcalamarijoins_amd.so.
+
+The structs below follow the C declarations field by field.  Python code in
+this package (the benchmark harness and the tests) reaches the HIP engine
+only through these declarations.  Loading fails loudly when the library has
+not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PAGE_SIZE = 8192
+
+CJ_OK = 0
+CJ_INT32, CJ_INT64, CJ_FP64, CJ_VARCHAR = 0, 1, 2, 3
+CJ_NODE_JOIN, CJ_NODE_SCAN = 0, 1
+CJ_EXEC_OUTPUT_DEVICE = 1
+CJ_EXEC_NO_FAST_PATH = 2
+
+TYPE_NAMES = {"INT32": CJ_INT32, "INT64": CJ_INT64, "FP64": CJ_FP64, "VARCHAR": CJ_VARCHAR}
+TYPE_IDS = {v: k for k, v in TYPE_NAMES.items()}
+
+
+class ColumnDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("num_pages", C.c_uint32), ("pages", C.POINTER(C.c_void_p))]
+
+
+class TableDesc(C.Structure):
+    _fields_ = [("num_rows", C.c_uint64), ("num_columns", C.c_uint32), ("columns", C.POINTER(ColumnDesc))]
+
+
+class Attr(C.Structure):
+    _fields_ = [("index", C.c_uint64), ("type", C.c_int32)]
+
+
+class NodeDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("build_left", C.c_int32),
+        ("base_table_id", C.c_uint64),
+        ("left", C.c_uint64),
+        ("right", C.c_uint64),
+        ("left_attr", C.c_uint64),
+        ("right_attr", C.c_uint64),
+        ("num_output_attrs", C.c_uint32),
+        ("output_attrs", C.POINTER(Attr)),
+    ]
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [
+        ("num_nodes", C.c_uint32),
+        ("nodes", C.POINTER(NodeDesc)),
+        ("num_inputs", C.c_uint32),
+        ("inputs", C.POINTER(TableDesc)),
+        ("root", C.c_uint64),
+    ]
+
+
+class Timing(C.Structure):
+    _fields_ = [
+        ("hashtable_build_ms", C.c_int64),
+        ("hash_join_probe_ms", C.c_int64),
+        ("nested_loop_join_ms", C.c_int64),
+        ("materialize_ms", C.c_int64),
+        ("setup_ms", C.c_int64),
+        ("total_execution_ms", C.c_int64),
+        ("intermediate_ms", C.c_int64),
+        ("device_ms", C.c_double),
+        ("h2d_ms", C.c_double),
+        ("d2h_ms", C.c_double),
+        ("bytes_alg", C.c_uint64),
+    ]
+
+
+class ResultColumn(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("num_pages", C.c_uint32),
+        ("pages", C.POINTER(C.c_void_p)),
+        ("device_pages", C.c_void_p),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("num_rows", C.c_uint64),
+        ("num_columns", C.c_uint32),
+        ("columns", C.POINTER(ResultColumn)),
+        ("opaque", C.c_void_p),
+    ]
+
+
+class CtxOptions(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("device_budget", C.c_uint64),
+        ("num_streams", C.c_int32),
+        ("cache_decoded", C.c_int32),
+        ("kernel_timing", C.c_int32),
+        ("host_threads", C.c_int32),
+    ]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("total_ms", C.c_double), ("bytes", C.c_uint64)]
+
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libcalamarijoins_amd.so")
+
+# Every function include/cj_api.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "cj_ctx_create", "cj_ctx_destroy", "cj_last_error", "cj_ctx_device_info",
+    "cj_table_register", "cj_table_unregister",
+    "cj_execute", "cj_result_release", "cj_result_to_host",
+    "cj_kernel_stats", "cj_kernel_stats_reset",
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load the HIP engine.  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build the HIP engine first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    sig = {
+        "cj_ctx_create": (C.c_int, [P(CtxOptions), P(C.c_void_p)]),
+        "cj_ctx_destroy": (C.c_int, [C.c_void_p]),
+        "cj_last_error": (C.c_char_p, []),
+        "cj_ctx_device_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, P(C.c_int32)]),
+        "cj_table_register": (C.c_int, [C.c_void_p, P(TableDesc), P(C.c_uint64)]),
+        "cj_table_unregister": (C.c_int, [C.c_void_p, C.c_uint64]),
+        "cj_execute": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, P(Result), P(Timing)]),
+        "cj_result_release": (C.c_int, [C.c_void_p, P(Result)]),
+        "cj_result_to_host": (C.c_int, [C.c_void_p, P(Result)]),
+        "cj_kernel_stats": (C.c_int, [C.c_void_p, P(KernelStat), C.c_uint32, P(C.c_uint32)]),
+        "cj_kernel_stats_reset": (C.c_int, [C.c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class CJError(RuntimeError):
+    pass
+
+
+def check(rc: int) -> None:
+    if rc != CJ_OK:
+        msg = load_library().cj_last_error()
+        raise CJError(f"cj status {rc}: {msg.decode() if msg else ''}")

@@ -1,0 +1,130 @@
+// api.cpp -- extern "C" entry points of libcalamarijoins_amd.so (declared in
+// include/cj_api.h).  Exceptions never cross the boundary: they become a
+// negative status plus a per-thread message for cj_last_error().
+#include "engine.hpp"
+
+#include <cstring>
+#include <new>
+#include <string>
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F &&f) {
+    try {
+        f();
+        return CJ_OK;
+    } catch (const cj::UnsupportedError &e) {
+        g_err = e.what();
+        return CJ_ERR_UNSUPPORTED;
+    } catch (const cj::InvalidError &e) {
+        g_err = e.what();
+        return CJ_ERR_INVALID;
+    } catch (const cj::OomError &e) {
+        g_err = e.what();
+        return CJ_ERR_OOM;
+    } catch (const std::bad_alloc &e) {
+        g_err = "host allocation failed";
+        return CJ_ERR_OOM;
+    } catch (const cj::DeviceError &e) {
+        g_err = e.what();
+        return CJ_ERR_DEVICE;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return CJ_ERR_INVALID;
+    } catch (...) {
+        g_err = "unknown error";
+        return CJ_ERR_INVALID;
+    }
+}
+
+cj::Context *C(cj_ctx *c) { return reinterpret_cast<cj::Context *>(c); }
+}  // namespace
+
+extern "C" {
+
+const char *cj_last_error(void) { return g_err.c_str(); }
+
+int cj_ctx_create(const cj_ctx_options *opts, cj_ctx **out) {
+    if (!out) {
+        g_err = "out is null";
+        return CJ_ERR_INVALID;
+    }
+    *out = nullptr;
+    return guard([&] {
+        cj_ctx_options o{};
+        if (opts) o = *opts;
+        *out = reinterpret_cast<cj_ctx *>(new cj::Context(o));
+    });
+}
+
+int cj_ctx_destroy(cj_ctx *ctx) {
+    return guard([&] { delete C(ctx); });
+}
+
+int cj_ctx_device_info(cj_ctx *ctx, char *name, size_t name_cap, int32_t *num_cus) {
+    return guard([&] {
+        if (!ctx) throw cj::InvalidError("null context");
+        if (name && name_cap) std::snprintf(name, name_cap, "%s", C(ctx)->dev_name.c_str());
+        if (num_cus) *num_cus = C(ctx)->num_cus;
+    });
+}
+
+int cj_table_register(cj_ctx *ctx, const cj_table_desc *table, uint64_t *out_id) {
+    return guard([&] {
+        if (!ctx || !table || !out_id) throw cj::InvalidError("null argument");
+        CJ_HIP(hipSetDevice(C(ctx)->device));
+        *out_id = C(ctx)->register_table(*table);
+    });
+}
+
+int cj_table_unregister(cj_ctx *ctx, uint64_t id) {
+    return guard([&] {
+        if (!ctx) throw cj::InvalidError("null context");
+        CJ_HIP(hipSetDevice(C(ctx)->device));
+        C(ctx)->unregister_table(id);
+    });
+}
+
+int cj_execute(cj_ctx *ctx, const cj_plan_desc *plan, uint32_t flags, cj_result *out, cj_timing *timing) {
+    return guard([&] {
+        if (!ctx || !plan || !out) throw cj::InvalidError("null argument");
+        CJ_HIP(hipSetDevice(C(ctx)->device));
+        C(ctx)->execute(*plan, flags, out, timing);
+    });
+}
+
+int cj_result_release(cj_ctx *ctx, cj_result *res) {
+    return guard([&] {
+        if (!ctx) throw cj::InvalidError("null context");
+        C(ctx)->release(res);
+    });
+}
+
+int cj_result_to_host(cj_ctx *ctx, cj_result *res) {
+    return guard([&] {
+        if (!ctx) throw cj::InvalidError("null context");
+        CJ_HIP(hipSetDevice(C(ctx)->device));
+        C(ctx)->to_host(res);
+    });
+}
+
+int cj_kernel_stats(cj_ctx *ctx, cj_kernel_stat *out, uint32_t cap, uint32_t *count) {
+    return guard([&] {
+        if (!ctx || !count) throw cj::InvalidError("null argument");
+        std::vector<cj_kernel_stat> v;
+        C(ctx)->kernel_stats(v);
+        *count = (uint32_t)v.size();
+        for (uint32_t i = 0; i < cap && i < v.size(); ++i) out[i] = v[i];
+    });
+}
+
+int cj_kernel_stats_reset(cj_ctx *ctx) {
+    return guard([&] {
+        if (!ctx) throw cj::InvalidError("null context");
+        C(ctx)->kernel_stats_reset();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,229 @@
+// decode.hip -- page decode and device-wide scans.
+//
+// Page decode restates the page format the reference reads in
+// data_access/columnar_reader.h:120-201 (PageIndex::build: cumulative rows,
+// 0xffff counts one row, 0xfffe none) and :452-474 (bitmap + popcount to
+// find a value's slot), turned into one workgroup per page that writes a
+// dense, row-indexed column plus a validity bitmap.
+#include "cj_internal.hpp"
+#include "device_util.hpp"
+
+namespace cj {
+
+// ------------------------------------------------------------------ fills
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+static unsigned grid_for(uint64_t n) {
+    uint64_t g = (n + kWG - 1) / kWG;
+    return (unsigned)(g < 8192 ? (g ? g : 1) : 8192);
+}
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
+    if (!n) return;
+    k_fill_u32<<<grid_for(n), kWG, 0, s>>>(p, v, n);
+    CJ_HIP(hipGetLastError());
+}
+void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s) {
+    if (!n) return;
+    k_fill_u64<<<grid_for(n), kWG, 0, s>>>(p, v, n);
+    CJ_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ scans
+// Exclusive scan of u64, three phases (block sums, scan of sums, block
+// scan).  out has n + 1 entries; out[n] is the total.  In-place allowed.
+constexpr int kScanItems = 8;
+constexpr uint64_t kScanBlock = (uint64_t)kWG * kScanItems;
+
+__global__ void k_scan_block_sums(const uint64_t *in, uint64_t n, uint64_t *sums) {
+    __shared__ uint64_t lds[4];
+    uint64_t base = blockIdx.x * kScanBlock + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+        if (base + i < n) s += in[base + i];
+    uint64_t tot;
+    block_excl_scan<uint64_t>(s, &tot, lds);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_block(const uint64_t *in, uint64_t *out, uint64_t n, const uint64_t *block_off) {
+    __shared__ uint64_t lds[4];
+    uint64_t base = blockIdx.x * kScanBlock + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        s += v[i];
+    }
+    uint64_t tot;
+    uint64_t pre = block_excl_scan<uint64_t>(s, &tot, lds);
+    uint64_t off = (block_off ? block_off[blockIdx.x] : 0) + pre;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) out[base + i] = off;
+        off += v[i];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kWG - 1) out[n] = off;
+}
+
+size_t scan_tmp_elems(uint64_t n) {
+    size_t t = 0;
+    while (n > kScanBlock) {
+        uint64_t nb = (n + kScanBlock - 1) / kScanBlock;
+        t += 2 * (nb + 1);
+        n = nb;
+    }
+    return t + 2;
+}
+
+void scan_exclusive_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s, uint64_t *tmp,
+                        size_t tmp_elems) {
+    if (n == 0) {
+        fill_u64(out, 0, 1, s);
+        return;
+    }
+    uint64_t nb = (n + kScanBlock - 1) / kScanBlock;
+    if (nb == 1) {
+        k_scan_block<<<1, kWG, 0, s>>>(in, out, n, nullptr);
+        CJ_HIP(hipGetLastError());
+        return;
+    }
+    if (tmp_elems < 2 * (nb + 1)) throw DeviceError("scan scratch too small");
+    uint64_t *sums = tmp, *sums_off = tmp + nb + 1;
+    k_scan_block_sums<<<(unsigned)nb, kWG, 0, s>>>(in, n, sums);
+    CJ_HIP(hipGetLastError());
+    scan_exclusive_u64(sums, sums_off, nb, s, tmp + 2 * (nb + 1), tmp_elems - 2 * (nb + 1));
+    k_scan_block<<<(unsigned)nb, kWG, 0, s>>>(in, out, n, sums_off);
+    CJ_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------- page index
+// Rows contributed by each page (PageIndex::build, columnar_reader.h:120-160).
+__global__ void k_page_row_counts(const uint8_t *pages, uint32_t num_pages, uint64_t *counts) {
+    uint32_t pg = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pg >= num_pages) return;
+    uint32_t n = ld16(pages + (size_t)pg * kPage);
+    counts[pg] = n == 0xffffu ? 1u : (n == 0xfffeu ? 0u : n);
+}
+void page_row_counts(const uint8_t *pages, uint32_t num_pages, uint64_t *counts, hipStream_t s) {
+    if (!num_pages) return;
+    k_page_row_counts<<<(num_pages + kWG - 1) / kWG, kWG, 0, s>>>(pages, num_pages, counts);
+    CJ_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ dense decode
+// One workgroup per page.  Dense pages (n_rows == n_vals) map row r to value
+// r; sparse pages rank the bitmap with a wave ballot + popcount.
+template <int W>
+__global__ __launch_bounds__(kWG) void k_decode_fixed(const uint8_t *pages, const uint64_t *row_start,
+                                                      void *values, uint32_t *valid_bits) {
+    __shared__ uint32_t lds[4];
+    const uint32_t pg = blockIdx.x;
+    const uint8_t *p = pages + (size_t)pg * kPage;
+    const uint32_t n = ld16(p), nv = ld16(p + 2);
+    const uint64_t base = row_start[pg];
+    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+    const uint8_t *vals = p + (W == 4 ? 4 : 8);
+    const bool dense = n == nv;
+    uint32_t running = 0;
+    for (uint32_t t0 = 0; t0 < n; t0 += kWG) {
+        uint32_t r = t0 + threadIdx.x;
+        bool inr = r < n;
+        bool ok = inr && (dense || page_bit(bm, r));
+        uint64_t m;
+        uint32_t tot;
+        uint32_t vi = running + tile_rank(ok, &m, &tot, lds);
+        // a bitmap with more set bits than n_vals is malformed: never read
+        // past the page's value area
+        bool rd = ok && vi < nv;
+        if (inr) {
+            if (W == 4) static_cast<int32_t *>(values)[base + r] = rd ? ld32(vals + 4 * (size_t)vi) : 0;
+            else static_cast<uint64_t *>(values)[base + r] = rd ? ld64(vals + 8 * (size_t)vi) : 0;
+        }
+        if (lane_id() == 0) or_mask_bits(valid_bits, base + t0 + 64 * wave_id(), m);
+        running += tot;
+    }
+}
+
+void decode_fixed(const uint8_t *pages, uint32_t num_pages, const uint64_t *row_start, int width,
+                  void *values, uint32_t *valid_bits, hipStream_t s) {
+    if (!num_pages) return;
+    if (width == 4) k_decode_fixed<4><<<num_pages, kWG, 0, s>>>(pages, row_start, values, valid_bits);
+    else k_decode_fixed<8><<<num_pages, kWG, 0, s>>>(pages, row_start, values, valid_bits);
+    CJ_HIP(hipGetLastError());
+}
+
+// VARCHAR: per row a packed ref (page, byte offset of the chars, length);
+// a 0xffff page is one row whose ref records the length of its page run.
+__global__ __launch_bounds__(kWG) void k_decode_varchar(const uint8_t *pages, uint32_t num_pages,
+                                                        const uint64_t *row_start, uint64_t *refs,
+                                                        uint32_t *valid_bits) {
+    __shared__ uint32_t lds[4];
+    const uint32_t pg = blockIdx.x;
+    const uint8_t *p = pages + (size_t)pg * kPage;
+    const uint32_t n = ld16(p);
+    const uint64_t base = row_start[pg];
+    if (n == 0xfffeu) return;
+    if (n == 0xffffu) {
+        if (threadIdx.x == 0) {
+            uint32_t q = pg + 1;
+            while (q < num_pages && ld16(pages + (size_t)q * kPage) == 0xfffeu) ++q;
+            refs[base] = ((uint64_t)pg << 32) | ((uint64_t)kLongMarker << 16) | (uint64_t)(q - pg);
+            atomicOr(valid_bits + (base >> 5), 1u << (base & 31));
+        }
+        return;
+    }
+    const uint32_t nv = ld16(p + 2);
+    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+    const uint8_t *offs = p + 4;
+    const uint32_t chars = 4 + 2 * nv;
+    const uint32_t max_chars = kPage - chars - ((n + 7) >> 3);  // chars end before the bitmap
+    const bool dense = n == nv;
+    uint32_t running = 0;
+    for (uint32_t t0 = 0; t0 < n; t0 += kWG) {
+        uint32_t r = t0 + threadIdx.x;
+        bool inr = r < n;
+        bool ok = inr && (dense || page_bit(bm, r));
+        uint64_t m;
+        uint32_t tot;
+        uint32_t vi = running + tile_rank(ok, &m, &tot, lds);
+        if (inr) {
+            uint64_t ref = 0;
+            if (ok && vi < nv) {
+                uint32_t end = min((uint32_t)ld16(offs + 2 * vi), max_chars);
+                uint32_t start = min(vi ? (uint32_t)ld16(offs + 2 * (vi - 1)) : 0u, end);
+                ref = ((uint64_t)pg << 32) | ((uint64_t)(chars + start) << 16) | (uint64_t)(end - start);
+            }
+            refs[base + r] = ref;
+        }
+        if (lane_id() == 0) or_mask_bits(valid_bits, base + t0 + 64 * wave_id(), m);
+        running += tot;
+    }
+}
+
+void decode_varchar(const uint8_t *pages, uint32_t num_pages, const uint64_t *row_start, uint64_t *refs,
+                    uint32_t *valid_bits, hipStream_t s) {
+    if (!num_pages) return;
+    k_decode_varchar<<<num_pages, kWG, 0, s>>>(pages, num_pages, row_start, refs, valid_bits);
+    CJ_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- gathers
+__global__ void k_gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t *dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[idx[i]];
+}
+void gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t *dst, uint64_t n, hipStream_t s) {
+    if (!n) return;
+    k_gather_u32<<<grid_for(n), kWG, 0, s>>>(src, idx, dst, n);
+    CJ_HIP(hipGetLastError());
+}
+
+}  // namespace cj

@@ -1,0 +1,976 @@
+// engine.cpp -- context, memory pools, resident-table registry and the plan
+// executor of the MI355X join engine.
+//
+// The executor restates the reference's depth-first recursion
+// (src/execute.cpp:84-300): every JoinNode resolves its children, builds on
+// the side with fewer rows (ties go left, join_setup.h:108-136; the plan's
+// build_left hint is not consulted), uses the nested loop when the build
+// side has fewer than 8 rows (execute.cpp:236-238) and the radix-partitioned
+// unchained hash table otherwise, and materializes pages at the root only.
+// Intermediates are late-materialized: instead of the reference's 4-byte
+// value_t columns (intermediate.h:40-172) a child result is a set of row-id
+// lists, one per base scan that contributes columns, and values are read
+// from the base pages only at the root.  The value_t semantics that leak
+// into results are kept: keys read through an intermediate treat NULL and
+// INT32_MIN alike (hash_join.h:120-127, nested_loop.h:85), a hash build over
+// an intermediate inserts them as INT32_MIN (hashtable.h:357), and INT32
+// output values equal to INT32_MIN come out NULL (page_builders.h:123).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <set>
+#include <thread>
+
+namespace cj {
+
+// ========================================================== device pool
+static size_t pool_round(size_t n) {
+    if (n <= (1u << 20)) return (n + 4095) & ~size_t(4095);
+    const size_t g = size_t(2) << 20;
+    return (n + g - 1) & ~(g - 1);
+}
+
+DevicePool::~DevicePool() {
+    for (auto &kv : sizes_) hipFree(kv.first);
+}
+
+void DevicePool::trim() {
+    for (auto &kv : free_) {
+        sizes_.erase(kv.second);
+        reserved_ -= kv.first;
+        hipFree(kv.second);
+    }
+    free_.clear();
+}
+
+void *DevicePool::alloc(size_t bytes) {
+    size_t sz = pool_round(bytes ? bytes : 1);
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = free_.lower_bound(sz);
+    if (it != free_.end() && it->first <= sz + sz / 2 + (size_t(4) << 20)) {
+        void *p = it->second;
+        in_use_ += it->first;
+        free_.erase(it);
+        return p;
+    }
+    if (budget_ && reserved_ + sz > budget_) trim();
+    if (budget_ && reserved_ + sz > budget_)
+        throw OomError("device budget exhausted: need " + std::to_string(sz) + " B, reserved " +
+                       std::to_string(reserved_) + " of " + std::to_string(budget_));
+    void *p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        trim();
+        if (hipMalloc(&p, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            throw OomError("hipMalloc of " + std::to_string(sz) + " bytes failed");
+        }
+    }
+    sizes_[p] = sz;
+    reserved_ += sz;
+    in_use_ += sz;
+    return p;
+}
+
+void DevicePool::free(void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = sizes_.find(p);
+    if (it == sizes_.end()) return;
+    in_use_ -= it->second;
+    free_.emplace(it->second, p);
+}
+
+PinnedPool::~PinnedPool() {
+    for (auto &kv : sizes_) hipHostFree(kv.first);
+}
+void *PinnedPool::alloc(size_t bytes) {
+    size_t sz = pool_round(bytes ? bytes : 1);
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = free_.lower_bound(sz);
+    if (it != free_.end() && it->first <= 2 * sz + (size_t(4) << 20)) {
+        void *p = it->second;
+        free_.erase(it);
+        return p;
+    }
+    void *p = nullptr;
+    if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        throw OomError("hipHostMalloc of " + std::to_string(sz) + " bytes failed");
+    }
+    sizes_[p] = sz;
+    return p;
+}
+void PinnedPool::free(void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = sizes_.find(p);
+    if (it != sizes_.end()) free_.emplace(it->second, p);
+}
+
+// ========================================================== kernel timer
+void KernelTimer::begin(const char *name, uint64_t bytes) {
+    if (!on) return;
+    Rec r;
+    r.name = name;
+    r.bytes = bytes;
+    CJ_HIP(hipEventCreate(&r.a));
+    CJ_HIP(hipEventCreate(&r.b));
+    CJ_HIP(hipEventRecord(r.a, s));
+    recs.push_back(r);
+}
+void KernelTimer::end() {
+    if (!on || recs.empty()) return;
+    CJ_HIP(hipEventRecord(recs.back().b, s));
+}
+void KernelTimer::collect(std::map<std::string, KernelAgg> &acc) {
+    for (auto &r : recs) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto &e = acc[r.name];
+            e.launches += 1;
+            e.ms += ms;
+            e.bytes += r.bytes;
+        }
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    recs.clear();
+}
+
+struct TimedScope {
+    KernelTimer &t;
+    TimedScope(KernelTimer &tt, const char *n, uint64_t bytes) : t(tt) { t.begin(n, bytes); }
+    ~TimedScope() { t.end(); }
+};
+
+// Algorithmic (compulsory) bytes of each launch, for the per-kernel
+// roofline: what the kernel must read and write at minimum.
+static uint64_t decode_bytes(const DevColumn &c) {
+    uint64_t w = c.type == CJ_INT32 ? 4 : 8;
+    return (uint64_t)c.num_pages * kPage + c.num_rows * w + c.num_rows / 8;
+}
+static uint64_t key_bytes(const KeySource &k) {
+    if (k.mode == KEY_COLUMNAR) return (uint64_t)k.num_pages * kPage;
+    return k.rows * 8 + k.rows / 8;  // row id + key value (+ validity)
+}
+static uint64_t fixed_mat_bytes(uint64_t rows, int32_t type) {
+    uint64_t w = type == CJ_INT32 ? 4 : 8;
+    return rows * (4 + w) + rows / 8 + rows * w + rows / 8;  // row ids + values read, page values + bitmap written
+}
+
+// ============================================================== context
+Context::Context(const cj_ctx_options &o) : opts(o), pool(0) {
+    device = o.device;
+    CJ_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    CJ_HIP(hipGetDeviceProperties(&prop, device));
+    dev_name = prop.name;
+    num_cus = prop.multiProcessorCount;
+    max_lds = prop.sharedMemPerBlock;
+    size_t free_b = 0, total_b = 0;
+    CJ_HIP(hipMemGetInfo(&free_b, &total_b));
+    pool.set_budget(o.device_budget ? o.device_budget : (uint64_t)(free_b * 0.92));
+    int ns = o.num_streams > 0 ? o.num_streams : 4;
+    streams.resize(ns);
+    for (auto &st : streams) CJ_HIP(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+}
+
+Context::~Context() {
+    hipSetDevice(device);
+    hipDeviceSynchronize();
+    for (auto &kv : tables)
+        for (auto &c : kv.second->cols) free_column(*c);
+    tables.clear();
+    for (auto &st : streams) hipStreamDestroy(st.s);
+}
+
+hipStream_t Context::acquire_stream() {
+    std::unique_lock<std::mutex> g(smu);
+    for (;;) {
+        for (auto &st : streams)
+            if (!st.busy) {
+                st.busy = true;
+                return st.s;
+            }
+        scv.wait(g);
+    }
+}
+void Context::release_stream(hipStream_t s) {
+    {
+        std::lock_guard<std::mutex> g(smu);
+        for (auto &st : streams)
+            if (st.s == s) st.busy = false;
+    }
+    scv.notify_one();
+}
+
+void Context::free_column(DevColumn &c) {
+    pool.free(c.pages);
+    pool.free(c.row_start);
+    pool.free(c.values);
+    pool.free(c.valid);
+    c.pages = nullptr;
+    c.row_start = nullptr;
+    c.values = nullptr;
+    c.valid = nullptr;
+    c.loaded = false;
+}
+
+static uint32_t host_rows_of(const void *page) {
+    uint16_t n;
+    std::memcpy(&n, page, 2);
+    return n == 0xffff ? 1u : (n == 0xfffe ? 0u : n);
+}
+
+// Structural check of one page header against the format (reference
+// README.md:65-104) so that no kernel indexes outside its page: value /
+// offset area plus bitmap must fit 8 KiB; long-string pages only in VARCHAR
+// columns and 0xfffe only after 0xffff / 0xfffe.
+static void validate_page(const uint8_t *p, int32_t type, uint32_t idx, uint16_t prev_rows) {
+    uint16_t n, nv;
+    std::memcpy(&n, p, 2);
+    std::memcpy(&nv, p + 2, 2);
+    if (n == 0xffff || n == 0xfffe) {
+        if (type != CJ_VARCHAR) throw InvalidError("long-string page in a fixed-width column");
+        if (nv > kPage - 4) throw InvalidError("long-string page holds more than 8188 chars");
+        if (n == 0xfffe && (idx == 0 || (prev_rows != 0xffff && prev_rows != 0xfffe)))
+            throw InvalidError("0xfffe page does not continue a long string");
+        return;
+    }
+    if (nv > n) throw InvalidError("page has more values than rows");
+    size_t bm = (n + 7u) / 8u;
+    size_t need = type == CJ_VARCHAR ? 4 + 2 * (size_t)nv : (type == CJ_INT32 ? 4 + 4 * (size_t)nv : 8 + 8 * (size_t)nv);
+    if (need + bm > kPage) throw InvalidError("page values and bitmap overflow 8 KiB");
+}
+
+// Upload a column's pages (gathered through pinned staging, 32 MiB chunks,
+// double-buffered) and its row index.
+void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s) {
+    c.type = d.type;
+    c.num_pages = d.num_pages;
+    std::vector<uint64_t> rs(d.num_pages + 1);
+    uint64_t run = 0;
+    uint16_t prev = 0;
+    for (uint32_t i = 0; i < d.num_pages; ++i) {
+        if (!d.pages[i]) throw InvalidError("null page pointer");
+        const uint8_t *p = static_cast<const uint8_t *>(d.pages[i]);
+        validate_page(p, d.type, i, prev);
+        std::memcpy(&prev, p, 2);
+        rs[i] = run;
+        run += host_rows_of(p);
+    }
+    if (run > 0xffffffffull) throw UnsupportedError("column has more than 2^32 rows");
+    rs[d.num_pages] = run;
+    c.num_rows = run;
+    c.pages = static_cast<uint8_t *>(pool.alloc((size_t)d.num_pages * kPage + 16));
+    c.row_start = dalloc<uint64_t>(d.num_pages + 1);
+    const uint32_t chunk = 4096;
+    uint8_t *stage[2] = {nullptr, nullptr};
+    hipEvent_t ev[2];
+    size_t stage_bytes = (size_t)std::min<uint32_t>(chunk, std::max<uint32_t>(d.num_pages, 1)) * kPage;
+    stage[0] = static_cast<uint8_t *>(pinned.alloc(stage_bytes + (d.num_pages + 1) * 8));
+    stage[1] = static_cast<uint8_t *>(pinned.alloc(stage_bytes));
+    CJ_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    CJ_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    bool pending[2] = {false, false};
+    int nthreads = opts.host_threads > 0 ? opts.host_threads : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    for (uint32_t p0 = 0, k = 0; p0 < d.num_pages; p0 += chunk, k ^= 1) {
+        uint32_t np = std::min(chunk, d.num_pages - p0);
+        if (pending[k]) CJ_HIP(hipEventSynchronize(ev[k]));
+        uint8_t *dst = stage[k];
+        auto copy_range = [&](uint32_t a, uint32_t b) {
+            for (uint32_t i = a; i < b; ++i) std::memcpy(dst + (size_t)i * kPage, d.pages[p0 + i], kPage);
+        };
+        if (np >= 512 && nthreads > 1) {
+            std::vector<std::thread> th;
+            uint32_t per = (np + nthreads - 1) / nthreads;
+            for (int t = 0; t < nthreads; ++t) {
+                uint32_t a = t * per, b = std::min(np, a + per);
+                if (a < b) th.emplace_back(copy_range, a, b);
+            }
+            for (auto &x : th) x.join();
+        } else {
+            copy_range(0, np);
+        }
+        CJ_HIP(hipMemcpyAsync(c.pages + (size_t)p0 * kPage, dst, (size_t)np * kPage, hipMemcpyHostToDevice, s));
+        CJ_HIP(hipEventRecord(ev[k], s));
+        pending[k] = true;
+    }
+    CJ_HIP(hipStreamSynchronize(s));
+    std::memcpy(stage[0], rs.data(), rs.size() * 8);
+    CJ_HIP(hipMemcpyAsync(c.row_start, stage[0], rs.size() * 8, hipMemcpyHostToDevice, s));
+    CJ_HIP(hipStreamSynchronize(s));
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    pinned.free(stage[0]);
+    pinned.free(stage[1]);
+    c.loaded = true;
+}
+
+void Context::ensure_decoded(DevColumn &c, hipStream_t s) {
+    // caller holds c.mu when c is shared
+    if (c.values) return;
+    size_t w = c.type == CJ_INT32 ? 4 : 8;
+    c.values = pool.alloc(c.num_rows * w + 16);
+    size_t words = c.num_rows / 32 + 2;
+    c.valid = static_cast<uint32_t *>(pool.alloc(words * 4));
+    fill_u32(c.valid, 0, words, s);
+    if (c.type == CJ_VARCHAR) decode_varchar(c.pages, c.num_pages, c.row_start, (uint64_t *)c.values, c.valid, s);
+    else decode_fixed(c.pages, c.num_pages, c.row_start, (int)w, c.values, c.valid, s);
+}
+
+uint64_t Context::register_table(const cj_table_desc &t) {
+    hipStream_t s = acquire_stream();
+    auto tab = std::make_unique<DevTable>();
+    try {
+        tab->resident = true;
+        tab->num_rows = t.num_rows;
+        for (uint32_t i = 0; i < t.num_columns; ++i) {
+            auto c = std::make_unique<DevColumn>();
+            upload_column(*c, t.columns[i], s);
+            tab->host_pages.emplace_back(t.columns[i].pages, t.columns[i].pages + t.columns[i].num_pages);
+            tab->cols.push_back(std::move(c));
+        }
+    } catch (...) {
+        for (auto &c : tab->cols) free_column(*c);
+        release_stream(s);
+        throw;
+    }
+    release_stream(s);
+    std::lock_guard<std::mutex> g(mu);
+    uint64_t id = next_id++;
+    tab->id = id;
+    for (uint32_t i = 0; i < t.num_columns; ++i)
+        if (t.columns[i].num_pages)
+            by_first_page[{t.columns[i].pages[0], t.columns[i].num_pages}] = {tab.get(), i};
+    tables[id] = std::move(tab);
+    return id;
+}
+
+void Context::unregister_table(uint64_t id) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = tables.find(id);
+    if (it == tables.end()) throw InvalidError("unknown table id");
+    for (auto jt = by_first_page.begin(); jt != by_first_page.end();)
+        jt = jt->second.first == it->second.get() ? by_first_page.erase(jt) : std::next(jt);
+    CJ_HIP(hipDeviceSynchronize());
+    for (auto &c : it->second->cols) free_column(*c);
+    tables.erase(it);
+}
+
+DevColumn *Context::find_resident(const cj_column_desc &c) {
+    if (!c.num_pages) return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = by_first_page.find({c.pages[0], c.num_pages});
+    if (it == by_first_page.end()) return nullptr;
+    DevTable *t = it->second.first;
+    const auto &hp = t->host_pages[it->second.second];
+    if (hp.back() != c.pages[c.num_pages - 1]) return nullptr;
+    DevColumn *col = t->cols[it->second.second].get();
+    return col->type == c.type ? col : nullptr;
+}
+
+void Context::kernel_stats(std::vector<cj_kernel_stat> &out) {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto &kv : kstats) {
+        cj_kernel_stat s{};
+        std::snprintf(s.name, sizeof s.name, "%s", kv.first.c_str());
+        s.launches = kv.second.launches;
+        s.total_ms = kv.second.ms;
+        s.bytes = kv.second.bytes;
+        out.push_back(s);
+    }
+}
+void Context::kernel_stats_reset() {
+    std::lock_guard<std::mutex> g(mu);
+    kstats.clear();
+}
+
+// ============================================================== executor
+namespace {
+
+struct Slot {
+    uint32_t table;          // plan input index
+    const uint32_t *rids;    // row ids into the base table (nullptr = identity)
+};
+struct ColRef {
+    uint32_t slot;
+    uint32_t col;            // base column index
+    int32_t type;
+};
+struct Rel {
+    bool columnar = false;   // a ScanNode (single identity slot)
+    uint64_t rows = 0;
+    std::vector<Slot> slots;
+    std::vector<ColRef> cols;
+};
+
+struct ColView {
+    DevColumn *col = nullptr;
+    const void *values = nullptr;
+    const uint32_t *valid = nullptr;
+};
+
+class Exec {
+  public:
+    Exec(Context &c, const cj_plan_desc &p, hipStream_t s) : ctx(c), plan(p), st(s) {
+        timer.on = c.opts.kernel_timing != 0;
+        timer.s = s;
+        cols.resize(p.num_inputs);
+        for (uint32_t i = 0; i < p.num_inputs; ++i) cols[i].resize(p.inputs[i].num_columns, nullptr);
+    }
+    ~Exec() {
+        for (void *p : temps) ctx.pool.free(p);
+        for (auto &c : own_cols) ctx.free_column(*c);
+    }
+
+    Context &ctx;
+    const cj_plan_desc &plan;
+    hipStream_t st;
+    KernelTimer timer;
+    std::vector<void *> temps;
+    std::vector<std::unique_ptr<DevColumn>> own_cols;
+    std::vector<std::vector<DevColumn *>> cols;
+    std::map<DevColumn *, ColView> views;
+    std::set<DevColumn *> referenced;
+    uint64_t scratch_elems = 0;
+    uint64_t *scratch = nullptr;
+    double h2d_ms = 0;
+
+    template <class T>
+    T *tmp(size_t n) {
+        T *p = ctx.dalloc<T>(n);
+        temps.push_back(p);
+        return p;
+    }
+
+    DevColumn *column(uint32_t table, uint32_t c) {
+        if (cols[table][c]) return cols[table][c];
+        const cj_column_desc &d = plan.inputs[table].columns[c];
+        DevColumn *dc = ctx.find_resident(d);
+        if (!dc) {
+            auto t0 = std::chrono::steady_clock::now();
+            auto own = std::make_unique<DevColumn>();
+            ctx.upload_column(*own, d, st);
+            dc = own.get();
+            own_cols.push_back(std::move(own));
+            h2d_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        cols[table][c] = dc;
+        referenced.insert(dc);
+        return dc;
+    }
+
+    // Dense decode of a base column, cached in the context for resident
+    // tables when cache_decoded is set, otherwise private to this call.
+    ColView view(DevColumn *c) {
+        auto it = views.find(c);
+        if (it != views.end()) return it->second;
+        ColView v;
+        v.col = c;
+        bool shared = false;
+        for (auto &o : own_cols) shared |= o.get() == c;
+        shared = !shared;
+        if (shared && ctx.opts.cache_decoded) {
+            std::lock_guard<std::mutex> g(c->mu);
+            if (!c->values) {
+                TimedScope ts(timer, c->type == CJ_VARCHAR ? "decode_varchar" : "decode_fixed", decode_bytes(*c));
+                ctx.ensure_decoded(*c, st);
+                CJ_HIP(hipStreamSynchronize(st));
+            }
+            v.values = c->values;
+            v.valid = c->valid;
+        } else if (!shared) {
+            TimedScope ts(timer, c->type == CJ_VARCHAR ? "decode_varchar" : "decode_fixed", decode_bytes(*c));
+            ctx.ensure_decoded(*c, st);
+            v.values = c->values;
+            v.valid = c->valid;
+        } else {
+            size_t w = c->type == CJ_INT32 ? 4 : 8;
+            void *vals = tmp<uint8_t>(c->num_rows * w + 16);
+            size_t words = c->num_rows / 32 + 2;
+            uint32_t *valid = tmp<uint32_t>(words);
+            fill_u32(valid, 0, words, st);
+            TimedScope ts(timer, c->type == CJ_VARCHAR ? "decode_varchar" : "decode_fixed", decode_bytes(*c));
+            if (c->type == CJ_VARCHAR) decode_varchar(c->pages, c->num_pages, c->row_start, (uint64_t *)vals, valid, st);
+            else decode_fixed(c->pages, c->num_pages, c->row_start, (int)w, vals, valid, st);
+            v.values = vals;
+            v.valid = valid;
+        }
+        views[c] = v;
+        return v;
+    }
+
+    uint64_t *scan_scratch(uint64_t n) {
+        uint64_t need = scan_tmp_elems(n);
+        if (need > scratch_elems) {
+            scratch = tmp<uint64_t>(need);
+            scratch_elems = need;
+        }
+        return scratch;
+    }
+    void scan(const uint64_t *in, uint64_t *out, uint64_t n) {
+        TimedScope ts(timer, "scan", 16 * n);
+        uint64_t *sc = scan_scratch(n);
+        scan_exclusive_u64(in, out, n, st, sc, scratch_elems);
+    }
+    uint64_t read_u64(const uint64_t *dev) {
+        uint64_t v = 0;
+        CJ_HIP(hipMemcpyAsync(&v, dev, 8, hipMemcpyDeviceToHost, st));
+        CJ_HIP(hipStreamSynchronize(st));
+        return v;
+    }
+
+    Rel scan_rel(const cj_node_desc &n) {
+        if (n.base_table_id >= plan.num_inputs) throw InvalidError("scan base_table_id out of range");
+        const cj_table_desc &t = plan.inputs[n.base_table_id];
+        Rel r;
+        r.columnar = true;
+        r.rows = t.num_rows;
+        r.slots.push_back(Slot{(uint32_t)n.base_table_id, nullptr});
+        for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
+            const cj_attr &at = n.output_attrs[a];
+            if (at.index >= t.num_columns) throw InvalidError("scan output attr out of range");
+            if (t.columns[at.index].type != at.type) throw UnsupportedError("scan output attr type mismatch");
+            r.cols.push_back(ColRef{0, (uint32_t)at.index, at.type});
+        }
+        return r;
+    }
+
+    Rel resolve(uint64_t idx, int depth) {
+        if (idx >= plan.num_nodes) throw InvalidError("node index out of range");
+        if (depth > 256) throw InvalidError("plan too deep (cycle?)");
+        const cj_node_desc &n = plan.nodes[idx];
+        if (n.kind == CJ_NODE_SCAN) return scan_rel(n);
+        if (n.kind != CJ_NODE_JOIN) throw InvalidError("unknown node kind");
+        return join(n, false, depth, nullptr);
+    }
+
+    KeySource key_source(const Rel &r, uint32_t attr) {
+        const ColRef &cr = r.cols[attr];
+        DevColumn *c = column(r.slots[cr.slot].table, cr.col);
+        KeySource k{};
+        if (r.columnar) {
+            k.mode = KEY_COLUMNAR;
+            k.pages = c->pages;
+            k.num_pages = c->num_pages;
+            k.row_start = c->row_start;
+        } else {
+            ColView v = view(c);
+            k.mode = KEY_INTER;
+            k.values = static_cast<const int32_t *>(v.values);
+            k.valid_bits = v.valid;
+            k.rids = r.slots[cr.slot].rids;
+            k.rows = r.rows;
+        }
+        return k;
+    }
+
+    // Hash build: partition pass (histogram, scan, scatter) then one
+    // workgroup per partition.
+    HashTableDev build_hash(const KeySource &k, uint64_t rows, uint64_t cap_rows) {
+        int log_slots = 11;  // directory >= 2048 slots (hashtable.h:255-264)
+        while ((1ull << log_slots) < rows) ++log_slots;
+        // <= 4096 partitions (LDS histogram of the partition pass) and
+        // 2^log_part slots per build workgroup (8 B of LDS per slot)
+        int log_part = std::max(std::min(log_slots, 12), log_slots - 12);
+        if ((8ull << log_part) + 64 > ctx.max_lds)
+            throw UnsupportedError("build side too large for the LDS build");
+        HashGeom g{log_slots, log_part, 1u << (log_slots - log_part)};
+        uint32_t P = g.num_partitions;
+        uint64_t *pc = tmp<uint64_t>(P + 1);
+        uint64_t *poff = tmp<uint64_t>(P + 1);
+        uint64_t *cur = tmp<uint64_t>(P + 1);
+        fill_u64(pc, 0, P + 1, st);
+        {
+            TimedScope ts(timer, "part_count", key_bytes(k));
+            build_partition_hist(k, g, pc, st);
+        }
+        scan(pc, poff, P);
+        CJ_HIP(hipMemcpyAsync(cur, poff, (P + 1) * 8, hipMemcpyDeviceToDevice, st));
+        uint64_t cap = std::max<uint64_t>(std::max(rows, cap_rows), 1);
+        int32_t *tk = tmp<int32_t>(cap);
+        uint32_t *tr = tmp<uint32_t>(cap);
+        {
+            TimedScope ts(timer, "part_scatter", key_bytes(k) + 8 * cap);
+            build_partition_scatter(k, g, cur, tk, tr, st);
+        }
+        uint64_t *dir = tmp<uint64_t>(1ull << log_slots);
+        int32_t *keys = tmp<int32_t>(cap);
+        uint32_t *rids = tmp<uint32_t>(cap);
+        {
+            TimedScope ts(timer, "build_partitions", 16 * cap + (8ull << log_slots));
+            build_partitions(g, poff, tk, tr, dir, keys, rids, st);
+        }
+        return HashTableDev{dir, keys, rids, log_slots, rows};
+    }
+
+    Rel join(const cj_node_desc &n, bool is_root, int depth, cj_result *out);
+    void materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
+                          const std::vector<uint32_t> &remap, uint64_t total, const uint32_t *ob, const uint32_t *op,
+                          cj_result *out);
+
+    // output placement
+    bool device_output = false;
+    uint64_t bytes_out = 0;
+    std::vector<std::pair<uint8_t *, std::pair<uint32_t, int32_t>>> out_cols;  // pages, (num_pages, type)
+    uint64_t out_rows = 0;
+    std::vector<int32_t> out_types;
+};
+
+Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
+    Rel L = resolve(n.left, depth + 1);
+    Rel R = resolve(n.right, depth + 1);
+    if (n.left_attr >= L.cols.size() || n.right_attr >= R.cols.size()) throw InvalidError("join attr out of range");
+    if (L.cols[n.left_attr].type != CJ_INT32 || R.cols[n.right_attr].type != CJ_INT32)
+        throw UnsupportedError("join keys must be INT32");
+    const bool build_left = L.rows <= R.rows;
+    Rel &B = build_left ? L : R;
+    Rel &P = build_left ? R : L;
+    const uint32_t battr = (uint32_t)(build_left ? n.left_attr : n.right_attr);
+    const uint32_t pattr = (uint32_t)(build_left ? n.right_attr : n.left_attr);
+    const uint32_t lw = (uint32_t)L.cols.size(), bw = (uint32_t)B.cols.size();
+    std::vector<uint32_t> remap(n.num_output_attrs);
+    bool need_b = false, need_p = false;
+    for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
+        uint64_t ci = n.output_attrs[a].index;
+        if (ci >= (uint64_t)lw + R.cols.size()) throw InvalidError("join output attr out of range");
+        remap[a] = build_left ? (uint32_t)ci : (ci < lw ? bw + (uint32_t)ci : (uint32_t)(ci - lw));
+        (remap[a] < bw ? need_b : need_p) = true;
+    }
+
+    KeySource bk = key_source(B, battr);
+    KeySource pk = key_source(P, pattr);
+    const bool use_nl = B.rows < 8;
+    HashTableDev ht{};
+    NLKeys *nl = nullptr;
+    uint64_t total = 0;
+    uint32_t *ob = nullptr, *op = nullptr;
+    const bool empty = B.rows == 0 || P.rows == 0;
+    if (!empty) {
+        if (use_nl) {
+            nl = tmp<NLKeys>(1);
+            TimedScope ts(timer, "nl_collect", 0);
+            nl_collect(bk, nl, st);
+        } else {
+            // tuple capacity: the rows the key pages actually hold (the
+            // declared num_rows only sizes the directory)
+            uint64_t cap = B.rows;
+            if (B.columnar) cap = column(B.slots[0].table, B.cols[battr].col)->num_rows;
+            ht = build_hash(bk, B.rows, cap);
+        }
+        uint32_t units = key_source_units(pk);
+        uint64_t *cnt = tmp<uint64_t>(units + 1);
+        uint64_t *off = tmp<uint64_t>(units + 1);
+        {
+            TimedScope ts(timer, use_nl ? "nl_probe_count" : "probe_count", key_bytes(pk));
+            probe_count(pk, use_nl ? nullptr : &ht, nl, cnt, st);
+        }
+        scan(cnt, off, units);
+        total = read_u64(off + units);
+        if (total > 0xffffffffull) throw UnsupportedError("join result exceeds 2^32 rows");
+        if (total) {
+            if (need_b) ob = tmp<uint32_t>(total);
+            if (need_p) op = tmp<uint32_t>(total);
+            TimedScope ts(timer, use_nl ? "nl_probe_write" : "probe_write", key_bytes(pk) + total * (4ull * (need_b + need_p)));
+            probe_write(pk, use_nl ? nullptr : &ht, nl, off, ob, op, st);
+        }
+    }
+
+    if (is_root) {
+        materialize_root(n, B, P, bw, remap, total, ob, op, out);
+        return Rel{};
+    }
+    // Non-root: late-materialized intermediate (row-id list per slot).
+    Rel res;
+    res.columnar = false;
+    res.rows = total;
+    std::map<std::pair<int, uint32_t>, uint32_t> slot_map;
+    for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
+        bool fb = remap[a] < bw;
+        const Rel &S = fb ? B : P;
+        uint32_t local = fb ? remap[a] : remap[a] - bw;
+        const ColRef &cr = S.cols[local];
+        auto key = std::make_pair(fb ? 0 : 1, cr.slot);
+        auto it = slot_map.find(key);
+        uint32_t ns;
+        if (it == slot_map.end()) {
+            const uint32_t *idx = fb ? ob : op;
+            const uint32_t *old = S.slots[cr.slot].rids;
+            const uint32_t *nr = idx;
+            if (old && total) {
+                uint32_t *g = tmp<uint32_t>(total);
+                TimedScope ts(timer, "gather_rids", 12 * total);
+                gather_u32(old, idx, g, total, st);
+                nr = g;
+            }
+            ns = (uint32_t)res.slots.size();
+            res.slots.push_back(Slot{S.slots[cr.slot].table, total ? nr : nullptr});
+            if (!total) {  // an empty list still needs a non-null marker
+                res.slots.back().rids = tmp<uint32_t>(1);
+            }
+            slot_map[key] = ns;
+        } else {
+            ns = it->second;
+        }
+        res.cols.push_back(ColRef{ns, cr.col, cr.type});
+    }
+    return res;
+}
+
+void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
+                            const std::vector<uint32_t> &remap, uint64_t total, const uint32_t *ob,
+                            const uint32_t *op, cj_result *) {
+    out_rows = total;
+    std::map<std::pair<int, uint32_t>, const uint32_t *> final_rids;
+    for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
+        const int32_t type = n.output_attrs[a].type;
+        out_types.push_back(type);
+        bool fb = remap[a] < bw;
+        const Rel &S = fb ? B : P;
+        uint32_t local = fb ? remap[a] : remap[a] - bw;
+        const ColRef &cr = S.cols[local];
+        if (cr.type != type) throw UnsupportedError("output attr type does not match its source column");
+        if (total == 0) {  // create_empty_result (materialize.h:61-69)
+            out_cols.push_back({nullptr, {0u, type}});
+            continue;
+        }
+        auto key = std::make_pair(fb ? 0 : 1, cr.slot);
+        const uint32_t *rids;
+        auto it = final_rids.find(key);
+        if (it != final_rids.end()) {
+            rids = it->second;
+        } else {
+            const uint32_t *idx = fb ? ob : op;
+            const uint32_t *old = S.slots[cr.slot].rids;
+            if (old) {
+                uint32_t *g = tmp<uint32_t>(total);
+                TimedScope ts(timer, "gather_rids", 12 * total);
+                gather_u32(old, idx, g, total, st);
+                rids = g;
+            } else {
+                rids = idx;
+            }
+            final_rids[key] = rids;
+        }
+        DevColumn *c = column(S.slots[cr.slot].table, cr.col);
+        ColView v = view(c);
+        uint8_t *pages;
+        uint32_t np;
+        if (type == CJ_VARCHAR) {
+            uint64_t chunks = (total + kVarcharChunk - 1) / kVarcharChunk;
+            uint64_t *cp = tmp<uint64_t>(chunks + 1);
+            uint64_t *co = tmp<uint64_t>(chunks + 1);
+            VarcharSource vs{c->pages, static_cast<const uint64_t *>(v.values), v.valid, rids};
+            {
+                TimedScope ts(timer, "varchar_plan", 12 * total);
+                varchar_plan(vs, total, cp, st);
+            }
+            scan(cp, co, chunks);
+            uint64_t npg = read_u64(co + chunks);
+            np = (uint32_t)npg;
+            pages = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
+            TimedScope ts(timer, "varchar_write", (uint64_t)np * kPage);
+            varchar_write(vs, total, co, pages, st);
+        } else {
+            uint32_t R = type == CJ_INT32 ? kRowsI32 : kRowsI64;
+            np = (uint32_t)((total + R - 1) / R);
+            pages = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
+            FixedSource fs{type == CJ_INT32 ? 4 : 8, type == CJ_INT32, v.values, v.valid, rids};
+            TimedScope ts(timer, type == CJ_INT32 ? "materialize_i32" : "materialize_i64", fixed_mat_bytes(total, type));
+            materialize_fixed(fs, total, R, pages, st);
+        }
+        out_cols.push_back({pages, {np, type}});
+    }
+}
+
+}  // namespace
+
+// Algorithmic output bytes of a result column, from its page headers:
+// non-NULL value bytes (VARCHAR: chars + 2 per value) plus bitmap bytes.
+static uint64_t host_payload_bytes(const uint8_t *pages, uint32_t np, int32_t type) {
+    uint64_t b = 0;
+    for (uint32_t i = 0; i < np; ++i) {
+        const uint8_t *p = pages + (size_t)i * kPage;
+        uint16_t n, nv;
+        std::memcpy(&n, p, 2);
+        std::memcpy(&nv, p + 2, 2);
+        if (type == CJ_VARCHAR && (n == 0xffff || n == 0xfffe)) { b += nv; continue; }
+        b += (n + 7) / 8;
+        if (type == CJ_VARCHAR) {
+            uint16_t last = 0;
+            if (nv) std::memcpy(&last, p + 4 + 2 * (nv - 1), 2);
+            b += last + 2ull * nv;
+        } else {
+            b += (uint64_t)nv * (type == CJ_INT32 ? 4 : 8);
+        }
+    }
+    return b;
+}
+
+void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing) {
+    auto t_start = std::chrono::steady_clock::now();
+    std::memset(out, 0, sizeof *out);
+    if (plan.root >= plan.num_nodes) throw InvalidError("plan root out of range");
+    if (plan.nodes[plan.root].kind != CJ_NODE_JOIN) throw UnsupportedError("plan root must be a join node");
+    for (uint32_t i = 0; i < plan.num_inputs; ++i)
+        for (uint32_t c = 0; c < plan.inputs[i].num_columns; ++c)
+            if (plan.inputs[i].columns[c].type < CJ_INT32 || plan.inputs[i].columns[c].type > CJ_VARCHAR)
+                throw InvalidError("bad column type");
+    hipStream_t s = acquire_stream();
+    struct Rel_ {
+        Context &c;
+        hipStream_t s;
+        ~Rel_() { c.release_stream(s); }
+    } rel_guard{*this, s};
+    CJ_HIP(hipSetDevice(device));
+    Exec ex(*this, plan, s);
+    ex.device_output = (flags & CJ_EXEC_OUTPUT_DEVICE) != 0;
+    hipEvent_t e0, e1, e2;
+    CJ_HIP(hipEventCreate(&e0));
+    CJ_HIP(hipEventCreate(&e1));
+    CJ_HIP(hipEventCreate(&e2));
+    auto *impl = new ResultImpl();
+    try {
+        // inputs first (uploads of non-resident columns are not device time)
+        for (uint32_t i = 0; i < plan.num_nodes; ++i) {
+            const cj_node_desc &nd = plan.nodes[i];
+            if (nd.kind != CJ_NODE_SCAN || nd.base_table_id >= plan.num_inputs) continue;
+            for (uint32_t a = 0; a < nd.num_output_attrs; ++a)
+                if (nd.output_attrs[a].index < plan.inputs[nd.base_table_id].num_columns)
+                    ex.column((uint32_t)nd.base_table_id, (uint32_t)nd.output_attrs[a].index);
+        }
+        CJ_HIP(hipEventRecord(e0, s));
+        ex.join(plan.nodes[plan.root], true, 0, out);
+        CJ_HIP(hipEventRecord(e1, s));
+        out->num_rows = ex.out_rows;
+        out->num_columns = (uint32_t)ex.out_cols.size();
+        impl->cols.resize(ex.out_cols.size());
+        impl->host_page_ptrs.resize(ex.out_cols.size());
+        uint64_t total_pages = 0;
+        for (size_t i = 0; i < ex.out_cols.size(); ++i) {
+            impl->cols[i].type = ex.out_cols[i].second.second;
+            impl->cols[i].num_pages = ex.out_cols[i].second.first;
+            impl->cols[i].device_pages = ex.out_cols[i].first;
+            impl->dev_pages.push_back(ex.out_cols[i].first);
+            total_pages += ex.out_cols[i].second.first;
+        }
+        double d2h = 0;
+        if (!ex.device_output && total_pages) {
+            impl->host_block = pinned.alloc(total_pages * kPage);
+            uint64_t at = 0;
+            CJ_HIP(hipEventRecord(e1, s));
+            for (size_t i = 0; i < impl->cols.size(); ++i) {
+                auto &c = impl->cols[i];
+                uint8_t *h = static_cast<uint8_t *>(impl->host_block) + at * kPage;
+                if (c.num_pages)
+                    CJ_HIP(hipMemcpyAsync(h, c.device_pages, (size_t)c.num_pages * kPage, hipMemcpyDeviceToHost, s));
+                impl->host_page_ptrs[i].resize(c.num_pages);
+                for (uint32_t q = 0; q < c.num_pages; ++q) impl->host_page_ptrs[i][q] = h + (size_t)q * kPage;
+                at += c.num_pages;
+            }
+            CJ_HIP(hipEventRecord(e2, s));
+        }
+        CJ_HIP(hipStreamSynchronize(s));
+        if (!ex.device_output) {
+            for (size_t i = 0; i < impl->cols.size(); ++i) {
+                impl->cols[i].pages = impl->host_page_ptrs[i].data();
+                pool.free(impl->dev_pages[i]);
+                impl->dev_pages[i] = nullptr;
+                impl->cols[i].device_pages = nullptr;
+            }
+            if (total_pages) {
+                float ms = 0;
+                hipEventElapsedTime(&ms, e1, e2);
+                d2h = ms;
+            }
+        }
+        out->columns = impl->cols.data();
+        out->opaque = impl;
+        if (timing) {
+            float dev_ms = 0;
+            hipEventElapsedTime(&dev_ms, e0, e1);
+            std::memset(timing, 0, sizeof *timing);
+            timing->device_ms = dev_ms;
+            timing->h2d_ms = ex.h2d_ms;
+            timing->d2h_ms = d2h;
+            uint64_t in_bytes = 0;
+            for (DevColumn *c : ex.referenced) in_bytes += (uint64_t)c->num_pages * kPage;
+            uint64_t out_bytes = 0;
+            if (ex.device_output) {
+                uint64_t *acc = ex.tmp<uint64_t>(1);
+                fill_u64(acc, 0, 1, s);
+                for (auto &c : impl->cols)
+                    if (c.num_pages)
+                        page_payload_bytes(static_cast<const uint8_t *>(c.device_pages), c.num_pages, c.type, acc, s);
+                out_bytes = ex.read_u64(acc);
+            } else {
+                for (size_t i = 0; i < impl->cols.size(); ++i) {
+                    auto &c = impl->cols[i];
+                    if (c.num_pages)
+                        out_bytes += host_payload_bytes(static_cast<const uint8_t *>(impl->host_page_ptrs[i][0]),
+                                                        c.num_pages, c.type);
+                }
+            }
+            timing->bytes_alg = in_bytes + out_bytes;
+        }
+    } catch (...) {
+        hipStreamSynchronize(s);
+        for (void *p : impl->dev_pages) pool.free(p);
+        for (auto &c : ex.out_cols)
+            if (std::find(impl->dev_pages.begin(), impl->dev_pages.end(), c.first) == impl->dev_pages.end())
+                pool.free(c.first);
+        pinned.free(impl->host_block);
+        delete impl;
+        std::memset(out, 0, sizeof *out);
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        hipEventDestroy(e2);
+        throw;
+    }
+    if (ex.timer.on) {
+        std::lock_guard<std::mutex> g(mu);
+        ex.timer.collect(kstats);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(e2);
+    if (timing) {
+        auto ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        timing->total_execution_ms = (int64_t)ms;
+    }
+}
+
+void Context::release(cj_result *res) {
+    if (!res || !res->opaque) return;
+    auto *impl = static_cast<ResultImpl *>(res->opaque);
+    for (void *p : impl->dev_pages) pool.free(p);
+    pinned.free(impl->host_block);
+    delete impl;
+    std::memset(res, 0, sizeof *res);
+}
+
+void Context::to_host(cj_result *res) {
+    if (!res || !res->opaque) throw InvalidError("empty result");
+    auto *impl = static_cast<ResultImpl *>(res->opaque);
+    if (impl->host_block) return;
+    uint64_t total_pages = 0;
+    for (auto &c : impl->cols) total_pages += c.num_pages;
+    if (!total_pages) return;
+    impl->host_block = pinned.alloc(total_pages * kPage);
+    uint64_t at = 0;
+    for (size_t i = 0; i < impl->cols.size(); ++i) {
+        auto &c = impl->cols[i];
+        uint8_t *h = static_cast<uint8_t *>(impl->host_block) + at * kPage;
+        if (c.num_pages) CJ_HIP(hipMemcpy(h, c.device_pages, (size_t)c.num_pages * kPage, hipMemcpyDeviceToHost));
+        impl->host_page_ptrs[i].resize(c.num_pages);
+        for (uint32_t q = 0; q < c.num_pages; ++q) impl->host_page_ptrs[i][q] = h + (size_t)q * kPage;
+        c.pages = impl->host_page_ptrs[i].data();
+        at += c.num_pages;
+    }
+}
+
+}  // namespace cj

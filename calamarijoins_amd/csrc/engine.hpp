@@ -1,0 +1,158 @@
+// engine.hpp -- host side of the MI355X join engine: context, device memory
+// pool, HBM-resident table registry and the plan executor.
+#pragma once
+
+#include "../../include/cj_api.h"
+#include "cj_internal.hpp"
+
+#include <atomic>
+#include <condition_variable>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace cj {
+
+// Caching device allocator: blocks are rounded to 2 MiB (small ones to
+// 4 KiB) and recycled best-fit, so steady-state execution calls no hipMalloc.
+class DevicePool {
+  public:
+    explicit DevicePool(uint64_t budget) : budget_(budget) {}
+    void set_budget(uint64_t b) { budget_ = b; }
+    ~DevicePool();
+    void *alloc(size_t bytes);
+    void free(void *p);
+    void trim();  // return cached blocks to the driver
+    uint64_t in_use() const { return in_use_; }
+    uint64_t budget() const { return budget_; }
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    std::unordered_map<void *, size_t> sizes_;
+    uint64_t budget_;
+    uint64_t reserved_ = 0;  // bytes obtained from hipMalloc
+    uint64_t in_use_ = 0;
+};
+
+// Cached pinned host buffers (result pages, upload staging).
+class PinnedPool {
+  public:
+    ~PinnedPool();
+    void *alloc(size_t bytes);
+    void free(void *p);
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    std::unordered_map<void *, size_t> sizes_;
+};
+
+// A base-table column in HBM.
+struct DevColumn {
+    int32_t type = 0;
+    uint32_t num_pages = 0;
+    uint64_t num_rows = 0;      // rows described by the pages
+    uint8_t *pages = nullptr;   // num_pages * 8 KiB, contiguous
+    uint64_t *row_start = nullptr;  // [num_pages + 1] first row of each page
+    void *values = nullptr;     // decoded: i32 / u64 per row, VARCHAR refs
+    uint32_t *valid = nullptr;  // decoded validity bitmap
+    bool loaded = false;        // pages + row_start present
+    std::mutex mu;              // guards lazy decode
+};
+
+struct DevTable {
+    uint64_t id = 0;
+    bool resident = false;
+    uint64_t num_rows = 0;
+    std::vector<std::unique_ptr<DevColumn>> cols;
+    std::vector<std::vector<const void *>> host_pages;  // identity of each column
+};
+
+// Per-kernel hipEvent timing (enabled by cj_ctx_options::kernel_timing).
+struct KernelAgg {
+    uint64_t launches = 0;
+    double ms = 0;
+    uint64_t bytes = 0;
+};
+struct KernelTimer {
+    struct Rec {
+        std::string name;
+        uint64_t bytes;
+        hipEvent_t a, b;
+    };
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<Rec> recs;
+    void begin(const char *name, uint64_t bytes);
+    void end();
+    void collect(std::map<std::string, KernelAgg> &acc);
+};
+
+struct ExecStream {
+    hipStream_t s = nullptr;
+    bool busy = false;
+};
+
+class Context {
+  public:
+    explicit Context(const cj_ctx_options &o);
+    ~Context();
+
+    int device = 0;
+    cj_ctx_options opts{};
+    DevicePool pool;
+    PinnedPool pinned;
+    std::string dev_name;
+    int num_cus = 0;
+    size_t max_lds = 65536;  // LDS bytes one workgroup may use
+
+    uint64_t register_table(const cj_table_desc &t);
+    void unregister_table(uint64_t id);
+
+    void execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing);
+    void release(cj_result *res);
+    void to_host(cj_result *res);
+
+    void kernel_stats(std::vector<cj_kernel_stat> &out);
+    void kernel_stats_reset();
+
+    // registry lookup: a host column matching a resident column
+    DevColumn *find_resident(const cj_column_desc &c);
+
+    hipStream_t acquire_stream();
+    void release_stream(hipStream_t s);
+
+    // device memory helpers used by the executor
+    template <class T>
+    T *dalloc(size_t n) {
+        return static_cast<T *>(pool.alloc(n * sizeof(T) + 16));
+    }
+    void ensure_index(DevColumn &c, hipStream_t s);
+    void ensure_decoded(DevColumn &c, hipStream_t s);
+    void upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s);
+    void free_column(DevColumn &c);
+
+    std::mutex mu;  // registry + stats
+    std::map<uint64_t, std::unique_ptr<DevTable>> tables;
+    std::map<std::pair<const void *, uint32_t>, std::pair<DevTable *, uint32_t>> by_first_page;
+    uint64_t next_id = 1;
+    std::map<std::string, KernelAgg> kstats;
+
+    std::mutex smu;
+    std::condition_variable scv;
+    std::vector<ExecStream> streams;
+};
+
+// Result bookkeeping hidden behind cj_result::opaque.
+struct ResultImpl {
+    std::vector<cj_result_column> cols;
+    std::vector<std::vector<void *>> host_page_ptrs;
+    std::vector<uint8_t *> dev_pages;  // per column (pool memory)
+    void *host_block = nullptr;        // pinned
+};
+
+}  // namespace cj

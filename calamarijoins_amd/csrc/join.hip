@@ -1,0 +1,342 @@
+// join.hip -- radix partitioning, unchained hash-table build, probe and the
+// tiny-build nested loop.
+//
+// Algorithm follows the reference's UnchainedHashtable
+// (join_execution/hashtable.h:63-505): a power-of-two directory of u64
+// entries (end_offset << 16 | 16-bit Bloom tag), keys and row ids stored
+// contiguously in slot order, built partition by partition after a radix
+// partition pass; probe = Bloom test, then a scan of [prev_end, end)
+// (hashtable.h:298-313, hash_join.h:156-244).  Tiny build sides (< 8 rows)
+// use the nested loop of join_execution/nested_loop.h:107-244 with the
+// build keys broadcast to every lane.
+//
+// gfx950 mapping: one workgroup per key page (or 2048-row chunk of a row-id
+// list); partition histograms and slot counts live in LDS; the partition
+// pass reserves output ranges with one global atomic per (workgroup,
+// partition); a build workgroup owns one partition (<= 4096 slots) so its
+// counts, prefix and Bloom tags stay in LDS and the directory is written
+// coalesced.
+#include "cj_internal.hpp"
+#include "device_util.hpp"
+
+namespace cj {
+
+// Bloom tags: the 2048 smallest 16-bit integers with exactly five bits set,
+// ascending -- the closed form of BLOOM_TAGS (bloom_tags.h:35-263).
+struct BloomTable {
+    uint16_t t[2048];
+};
+constexpr BloomTable make_bloom() {
+    BloomTable b{};
+    uint32_t n = 0;
+    for (uint32_t v = 0; v < 65536u && n < 2048u; ++v) {
+        uint32_t c = 0;
+        for (uint32_t x = v; x; x &= x - 1) ++c;
+        if (c == 5) b.t[n++] = (uint16_t)v;
+    }
+    return b;
+}
+__constant__ BloomTable c_bloom = make_bloom();
+
+constexpr uint32_t kInterChunk = 2048;  // rows of a row-id list per workgroup
+
+uint32_t key_source_units(const KeySource &k) {
+    if (k.mode == KEY_COLUMNAR) return k.num_pages;
+    return (uint32_t)((k.rows + kInterChunk - 1) / kInterChunk);
+}
+
+// Iterate the keys of one work unit in row order, 256 rows per tile.  For
+// each tile, `fn(in_range, ok, key, rid)` is called by every thread (so it
+// may use workgroup barriers).  `want_nulls` (hash build over an
+// intermediate) passes NULL keys through as INT32_MIN, the reference's
+// value_t encoding (hashtable.h:357 inserts them unfiltered).
+template <class F>
+__device__ __forceinline__ void for_each_key(const KeySource &k, uint32_t unit, bool want_nulls, uint32_t *lds4,
+                                             F &&fn) {
+    if (k.mode == KEY_COLUMNAR) {
+        const uint8_t *p = k.pages + (size_t)unit * kPage;
+        const uint32_t n = ld16(p), nv = ld16(p + 2);
+        const uint64_t base = k.row_start[unit];
+        const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+        const bool dense = n == nv;
+        uint32_t running = 0;
+        for (uint32_t t0 = 0; t0 < n; t0 += kWG) {
+            uint32_t r = t0 + threadIdx.x;
+            bool inr = r < n;
+            bool ok = inr && (dense || page_bit(bm, r));
+            uint64_t m;
+            uint32_t tot;
+            uint32_t vi = dense ? r : running + tile_rank(ok, &m, &tot, lds4);
+            ok = ok && vi < nv;  // malformed bitmap: stay inside the value area
+            int32_t key = ok ? ld32(p + 4 + 4 * (size_t)vi) : 0;
+            fn(inr, ok, key, (uint32_t)(base + r));
+            if (!dense) running += tot;
+        }
+    } else {
+        const uint64_t a = (uint64_t)unit * kInterChunk;
+        const uint64_t e = a + kInterChunk < k.rows ? a + kInterChunk : k.rows;
+        for (uint64_t t0 = a; t0 < e; t0 += kWG) {
+            uint64_t j = t0 + threadIdx.x;
+            bool inr = j < e;
+            int32_t key = kI32Null;
+            if (inr) {
+                uint32_t rid = k.rids ? k.rids[j] : (uint32_t)j;
+                if (valid_bit(k.valid_bits, rid)) key = k.values[rid];
+            }
+            bool ok = inr && (want_nulls || key != kI32Null);
+            fn(inr, ok, key, (uint32_t)j);
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t slot_of(uint64_t h, int log_slots) { return h >> (64 - log_slots); }
+
+// ---------------------------------------------------- radix partition pass
+// Partition of a key = the top bits of its directory slot, so each build
+// partition owns a contiguous slot range.
+__global__ __launch_bounds__(kWG) void k_part_count(KeySource k, HashGeom g, uint64_t *part_count) {
+    extern __shared__ uint32_t hist[];  // num_partitions
+    __shared__ uint32_t lds4[4];
+    for (uint32_t i = threadIdx.x; i < g.num_partitions; i += kWG) hist[i] = 0;
+    __syncthreads();
+    const bool want_nulls = k.mode == KEY_INTER;
+    for_each_key(k, blockIdx.x, want_nulls, lds4, [&](bool, bool ok, int32_t key, uint32_t) {
+        if (ok) atomicAdd(&hist[slot_of(hash_key(key), g.log_slots) >> g.log_part], 1u);
+    });
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < g.num_partitions; i += kWG)
+        if (hist[i]) atomicAdd((unsigned long long *)&part_count[i], (unsigned long long)hist[i]);
+}
+
+__global__ __launch_bounds__(kWG) void k_part_scatter(KeySource k, HashGeom g, uint64_t *part_cursor,
+                                                      int32_t *tup_keys, uint32_t *tup_rids) {
+    extern __shared__ uint32_t sm[];  // hist[P] then cursor[P] (u32 pairs as u64)
+    uint32_t *hist = sm;
+    uint64_t *cur = reinterpret_cast<uint64_t *>(sm + ((g.num_partitions + 1) & ~1u));
+    __shared__ uint32_t lds4[4];
+    for (uint32_t i = threadIdx.x; i < g.num_partitions; i += kWG) hist[i] = 0;
+    __syncthreads();
+    const bool want_nulls = k.mode == KEY_INTER;
+    for_each_key(k, blockIdx.x, want_nulls, lds4, [&](bool, bool ok, int32_t key, uint32_t) {
+        if (ok) atomicAdd(&hist[slot_of(hash_key(key), g.log_slots) >> g.log_part], 1u);
+    });
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < g.num_partitions; i += kWG)
+        cur[i] = hist[i] ? atomicAdd((unsigned long long *)&part_cursor[i], (unsigned long long)hist[i]) : 0;
+    __syncthreads();
+    for_each_key(k, blockIdx.x, want_nulls, lds4, [&](bool, bool ok, int32_t key, uint32_t rid) {
+        if (ok) {
+            uint32_t p = (uint32_t)(slot_of(hash_key(key), g.log_slots) >> g.log_part);
+            uint64_t pos = atomicAdd((unsigned long long *)&cur[p], 1ull);
+            tup_keys[pos] = key;
+            tup_rids[pos] = rid;
+        }
+    });
+}
+
+void build_partition_hist(const KeySource &k, const HashGeom &g, uint64_t *part_count, hipStream_t s) {
+    uint32_t units = key_source_units(k);
+    if (!units) return;
+    size_t sh = g.num_partitions * sizeof(uint32_t);
+    k_part_count<<<units, kWG, sh, s>>>(k, g, part_count);
+    CJ_HIP(hipGetLastError());
+}
+
+void build_partition_scatter(const KeySource &k, const HashGeom &g, uint64_t *part_cursor, int32_t *tup_keys,
+                             uint32_t *tup_rids, hipStream_t s) {
+    uint32_t units = key_source_units(k);
+    if (!units) return;
+    // part_cursor starts as a copy of the partition offsets and is advanced
+    // by one atomic per (workgroup, partition).
+    size_t sh = ((g.num_partitions + 1) & ~1u) * sizeof(uint32_t) + g.num_partitions * sizeof(uint64_t);
+    k_part_scatter<<<units, kWG, sh, s>>>(k, g, part_cursor, tup_keys, tup_rids);
+    CJ_HIP(hipGetLastError());
+}
+
+// -------------------------------------------- per-partition build in LDS
+// build_partition (hashtable.h:197-246): count per slot, exclusive prefix,
+// scatter keys/row ids, OR Bloom tags, write (end << 16) | tag.
+__global__ __launch_bounds__(kWG) void k_build_partitions(HashGeom g, const uint64_t *part_start,
+                                                          const uint64_t *part_end, const int32_t *tup_keys,
+                                                          const uint32_t *tup_rids, uint64_t *dir, int32_t *keys,
+                                                          uint32_t *rids) {
+    extern __shared__ uint32_t sm[];
+    const uint32_t S = 1u << g.log_part;
+    uint32_t *cnt = sm, *tag = sm + S;
+    __shared__ uint32_t lds4[4];
+    const uint32_t p = blockIdx.x;
+    const uint64_t a = part_start[p], e = part_end[p];
+    for (uint32_t i = threadIdx.x; i < S; i += kWG) { cnt[i] = 0; tag[i] = 0; }
+    __syncthreads();
+    const uint32_t smask = S - 1;
+    for (uint64_t i = a + threadIdx.x; i < e; i += kWG) {
+        uint64_t h = hash_key(tup_keys[i]);
+        uint32_t ls = (uint32_t)slot_of(h, g.log_slots) & smask;
+        atomicAdd(&cnt[ls], 1u);
+        atomicOr(&tag[ls], (uint32_t)c_bloom.t[h & 0x7FF]);
+    }
+    __syncthreads();
+    // exclusive prefix over the S slot counts: each thread owns S/256 slots
+    const uint32_t per = (S + kWG - 1) / kWG;
+    const uint32_t s0 = threadIdx.x * per;
+    uint32_t local = 0;
+    for (uint32_t i = 0; i < per; ++i)
+        if (s0 + i < S) local += cnt[s0 + i];
+    uint32_t tot;
+    uint32_t pre = block_excl_scan<uint32_t>(local, &tot, lds4);
+    for (uint32_t i = 0; i < per; ++i)
+        if (s0 + i < S) {
+            uint32_t c = cnt[s0 + i];
+            cnt[s0 + i] = pre;
+            pre += c;
+        }
+    __syncthreads();
+    for (uint64_t i = a + threadIdx.x; i < e; i += kWG) {
+        int32_t key = tup_keys[i];
+        uint32_t ls = (uint32_t)slot_of(hash_key(key), g.log_slots) & smask;
+        uint64_t pos = a + atomicAdd(&cnt[ls], 1u);
+        keys[pos] = key;
+        rids[pos] = tup_rids[i];
+    }
+    __syncthreads();
+    uint64_t *d = dir + (uint64_t)p * S;
+    for (uint32_t i = threadIdx.x; i < S; i += kWG) d[i] = ((a + cnt[i]) << 16) | (uint64_t)(tag[i] & 0xFFFF);
+}
+
+void build_partitions(const HashGeom &g, const uint64_t *part_off, const int32_t *tup_keys, const uint32_t *tup_rids,
+                      uint64_t *dir, int32_t *keys, uint32_t *rids, hipStream_t s) {
+    size_t sh = 2 * (size_t)(1u << g.log_part) * sizeof(uint32_t);
+    k_build_partitions<<<g.num_partitions, kWG, sh, s>>>(g, part_off, part_off + 1, tup_keys, tup_rids, dir, keys,
+                                                         rids);
+    CJ_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ nested loop
+// visit_rows (nested_loop.h:50-95): the first (up to) 8 non-NULL build keys
+// in row order.  The build side has fewer than 8 rows.
+__global__ void k_nl_collect(KeySource k, NLKeys *out) {
+    if (threadIdx.x != 0) return;
+    NLKeys r{};
+    r.count = 0;
+    uint32_t units = k.mode == KEY_COLUMNAR ? k.num_pages : (uint32_t)((k.rows + kInterChunk - 1) / kInterChunk);
+    for (uint32_t u = 0; u < units && r.count < 8; ++u) {
+        if (k.mode == KEY_COLUMNAR) {
+            const uint8_t *p = k.pages + (size_t)u * kPage;
+            uint32_t n = ld16(p), nv = ld16(p + 2), vi = 0;
+            const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+            for (uint32_t rr = 0; rr < n && r.count < 8; ++rr) {
+                bool ok = n == nv || page_bit(bm, rr);
+                if (!ok || vi >= nv) continue;
+                r.keys[r.count] = ld32(p + 4 + 4 * (size_t)(n == nv ? rr : vi));
+                r.rids[r.count] = (uint32_t)(k.row_start[u] + rr);
+                r.count++;
+                vi++;
+            }
+        } else {
+            uint64_t a = (uint64_t)u * kInterChunk, e = a + kInterChunk < k.rows ? a + kInterChunk : k.rows;
+            for (uint64_t j = a; j < e && r.count < 8; ++j) {
+                uint32_t rid = k.rids ? k.rids[j] : (uint32_t)j;
+                int32_t key = valid_bit(k.valid_bits, rid) ? k.values[rid] : kI32Null;
+                if (key == kI32Null) continue;
+                r.keys[r.count] = key;
+                r.rids[r.count] = (uint32_t)j;
+                r.count++;
+            }
+        }
+    }
+    *out = r;
+}
+void nl_collect(const KeySource &k, NLKeys *out, hipStream_t s) {
+    k_nl_collect<<<1, 64, 0, s>>>(k, out);
+    CJ_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ probe
+// Matches of one probe key.  With WRITE, the (build_rid, probe_rid) pairs
+// go to out[pos..]; either output array may be null (the reference's
+// LEFT_ONLY / RIGHT_ONLY collection modes, match_collector.h:31-35).
+template <bool NL, bool WRITE>
+__device__ __forceinline__ uint32_t probe_key(int32_t key, uint32_t prid, const HashTableDev &ht, const NLKeys &nl,
+                                              uint64_t pos, uint32_t *ob, uint32_t *op) {
+    uint32_t m = 0;
+    if (NL) {
+        for (int i = 0; i < nl.count; ++i)
+            if (nl.keys[i] == key) {
+                if (WRITE) {
+                    if (ob) ob[pos + m] = nl.rids[i];
+                    if (op) op[pos + m] = prid;
+                }
+                ++m;
+            }
+        return m;
+    }
+    uint64_t h = hash_key(key);
+    uint64_t slot = slot_of(h, ht.log_slots);
+    uint64_t e = ht.dir[slot];
+    uint32_t tag = c_bloom.t[h & 0x7FF];
+    if ((e & tag) != tag) return 0;
+    uint64_t end = e >> 16, start = slot ? (ht.dir[slot - 1] >> 16) : 0;
+    for (uint64_t i = start; i < end; ++i)
+        if (ht.keys[i] == key) {
+            if (WRITE) {
+                if (ob) ob[pos + m] = ht.rids[i];
+                if (op) op[pos + m] = prid;
+            }
+            ++m;
+        }
+    return m;
+}
+
+template <bool NL>
+__global__ __launch_bounds__(kWG) void k_probe_count(KeySource k, HashTableDev ht, const NLKeys *nlp,
+                                                     uint64_t *counts) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint64_t red[4];
+    NLKeys nl;
+    if (NL) nl = *nlp;
+    uint64_t mine = 0;
+    for_each_key(k, blockIdx.x, false, lds4, [&](bool, bool ok, int32_t key, uint32_t rid) {
+        if (ok) mine += probe_key<NL, false>(key, rid, ht, nl, 0, nullptr, nullptr);
+    });
+    uint64_t tot;
+    block_excl_scan<uint64_t>(mine, &tot, red);
+    if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+template <bool NL>
+__global__ __launch_bounds__(kWG) void k_probe_write(KeySource k, HashTableDev ht, const NLKeys *nlp,
+                                                     const uint64_t *offsets, uint32_t *ob, uint32_t *op) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint64_t red[4];
+    NLKeys nl;
+    if (NL) nl = *nlp;
+    uint64_t base = offsets[blockIdx.x];
+    for_each_key(k, blockIdx.x, false, lds4, [&](bool, bool ok, int32_t key, uint32_t rid) {
+        uint32_t m = ok ? probe_key<NL, false>(key, rid, ht, nl, 0, nullptr, nullptr) : 0;
+        uint64_t tot;
+        uint64_t pre = block_excl_scan<uint64_t>((uint64_t)m, &tot, red);
+        if (m) probe_key<NL, true>(key, rid, ht, nl, base + pre, ob, op);
+        base += tot;
+    });
+}
+
+void probe_count(const KeySource &k, const HashTableDev *ht, const NLKeys *nl, uint64_t *counts, hipStream_t s) {
+    uint32_t units = key_source_units(k);
+    if (!units) return;
+    HashTableDev h = ht ? *ht : HashTableDev{};
+    if (nl) k_probe_count<true><<<units, kWG, 0, s>>>(k, h, nl, counts);
+    else k_probe_count<false><<<units, kWG, 0, s>>>(k, h, nl, counts);
+    CJ_HIP(hipGetLastError());
+}
+
+void probe_write(const KeySource &k, const HashTableDev *ht, const NLKeys *nl, const uint64_t *offsets,
+                 uint32_t *out_build, uint32_t *out_probe, hipStream_t s) {
+    uint32_t units = key_source_units(k);
+    if (!units) return;
+    HashTableDev h = ht ? *ht : HashTableDev{};
+    if (nl) k_probe_write<true><<<units, kWG, 0, s>>>(k, h, nl, offsets, out_build, out_probe);
+    else k_probe_write<false><<<units, kWG, 0, s>>>(k, h, nl, offsets, out_build, out_probe);
+    CJ_HIP(hipGetLastError());
+}
+
+}  // namespace cj

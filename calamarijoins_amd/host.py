@@ -1,0 +1,185 @@
+"""Python mirror of the contest API objects over the C ABI.
+
+Column / Table / Plan follow the reference's Plan builder
+(include/data_model/plan.h:149-185) and produce the cj_api.h descriptors;
+Engine wraps a cj_ctx.  Used by the benchmark harness and the tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+PAGE = abi.PAGE_SIZE
+T = abi.TYPE_NAMES
+
+
+class Column:
+    """A column: its type id and its pages (uint8 array, n_pages x 8192)."""
+
+    def __init__(self, type_id: int, pages: np.ndarray, owner=None):
+        self.type = type_id
+        self.pages = pages.reshape(-1, PAGE) if pages.size else pages.reshape(0, PAGE)
+        self.owner = owner
+        self._ptrs = None
+
+    @property
+    def num_pages(self) -> int:
+        return self.pages.shape[0]
+
+    def page_ptrs(self):
+        if self._ptrs is None:
+            base = self.pages.ctypes.data
+            n = self.num_pages
+            self._ptrs = (C.c_void_p * max(n, 1))(*[base + i * PAGE for i in range(n)])
+        return self._ptrs
+
+
+class Table:
+    def __init__(self, num_rows: int, columns: list, owner=None):
+        self.num_rows = num_rows
+        self.columns = columns
+        self.owner = owner
+        self._desc = None
+
+    def desc(self) -> abi.TableDesc:
+        if self._desc is None:
+            cols = (abi.ColumnDesc * max(len(self.columns), 1))()
+            for i, c in enumerate(self.columns):
+                cols[i] = abi.ColumnDesc(c.type, c.num_pages, C.cast(c.page_ptrs(), C.POINTER(C.c_void_p)))
+            self._cols = cols
+            self._desc = abi.TableDesc(self.num_rows, len(self.columns), cols)
+        return self._desc
+
+    def input_bytes(self, col_indices) -> int:
+        return sum(self.columns[i].num_pages * PAGE for i in col_indices)
+
+
+class Plan:
+    def __init__(self):
+        self.nodes = []
+        self.inputs: list = []
+        self.root = 0
+
+    def new_scan_node(self, table_id, attrs):
+        self.nodes.append(dict(kind=abi.CJ_NODE_SCAN, table=table_id, attrs=list(attrs)))
+        return len(self.nodes) - 1
+
+    def new_join_node(self, build_left, left, right, left_attr, right_attr, attrs):
+        self.nodes.append(dict(kind=abi.CJ_NODE_JOIN, build_left=build_left, left=left, right=right,
+                               left_attr=left_attr, right_attr=right_attr, attrs=list(attrs)))
+        return len(self.nodes) - 1
+
+    def new_input(self, table):
+        self.inputs.append(table)
+        return len(self.inputs) - 1
+
+    def desc(self) -> abi.PlanDesc:
+        keep = []
+        nodes = (abi.NodeDesc * max(len(self.nodes), 1))()
+        for i, n in enumerate(self.nodes):
+            attrs = (abi.Attr * max(len(n["attrs"]), 1))()
+            for j, (idx, t) in enumerate(n["attrs"]):
+                attrs[j] = abi.Attr(idx, T[t] if isinstance(t, str) else t)
+            keep.append(attrs)
+            d = abi.NodeDesc()
+            d.kind = n["kind"]
+            if n["kind"] == abi.CJ_NODE_SCAN:
+                d.base_table_id = n["table"]
+            else:
+                d.build_left = int(n["build_left"])
+                d.left, d.right = n["left"], n["right"]
+                d.left_attr, d.right_attr = n["left_attr"], n["right_attr"]
+            d.num_output_attrs = len(n["attrs"])
+            d.output_attrs = attrs
+            nodes[i] = d
+        tabs = (abi.TableDesc * max(len(self.inputs), 1))()
+        for i, t in enumerate(self.inputs):
+            tabs[i] = t.desc()
+        pd = abi.PlanDesc(len(self.nodes), nodes, len(self.inputs), tabs, self.root)
+        self._keep = (keep, nodes, tabs)
+        return pd
+
+
+class Engine:
+    """A cj_ctx (one GPU)."""
+
+    def __init__(self, **opts):
+        self.lib = abi.load_library()
+        o = abi.CtxOptions()
+        for k, v in opts.items():
+            setattr(o, k, v)
+        self.ctx = C.c_void_p()
+        abi.check(self.lib.cj_ctx_create(C.byref(o), C.byref(self.ctx)))
+
+    def close(self):
+        if self.ctx:
+            self.lib.cj_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        cus = C.c_int32()
+        abi.check(self.lib.cj_ctx_device_info(self.ctx, name, 256, C.byref(cus)))
+        return name.value.decode(), cus.value
+
+    def register(self, table: Table) -> int:
+        tid = C.c_uint64()
+        abi.check(self.lib.cj_table_register(self.ctx, C.byref(table.desc()), C.byref(tid)))
+        return tid.value
+
+    def unregister(self, tid: int):
+        abi.check(self.lib.cj_table_unregister(self.ctx, tid))
+
+    def execute(self, plan: Plan, flags: int = 0, timing: bool = False):
+        res = abi.Result()
+        tm = abi.Timing()
+        abi.check(self.lib.cj_execute(self.ctx, C.byref(plan.desc()), flags, C.byref(res),
+                                      C.byref(tm) if timing else None))
+        return EngineResult(self, res, tm if timing else None)
+
+    def kernel_stats(self):
+        arr = (abi.KernelStat * 128)()
+        n = C.c_uint32()
+        abi.check(self.lib.cj_kernel_stats(self.ctx, arr, 128, C.byref(n)))
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms, arr[i].bytes)
+                for i in range(min(n.value, 128))}
+
+    def kernel_stats_reset(self):
+        abi.check(self.lib.cj_kernel_stats_reset(self.ctx))
+
+
+class EngineResult:
+    def __init__(self, eng: Engine, res: abi.Result, timing):
+        self.eng = eng
+        self.res = res
+        self.timing = timing
+        self.num_rows = res.num_rows
+        self.types = [res.columns[i].type for i in range(res.num_columns)]
+        self._refresh()
+
+    def _refresh(self):
+        res = self.res
+        self.page_lists = []
+        for i in range(res.num_columns):
+            c = res.columns[i]
+            self.page_lists.append([c.pages[q] for q in range(c.num_pages)] if c.pages else [])
+
+    def to_host(self):
+        abi.check(self.eng.lib.cj_result_to_host(self.eng.ctx, C.byref(self.res)))
+        self._refresh()
+
+    def num_pages(self):
+        return [self.res.columns[i].num_pages for i in range(self.res.num_columns)]
+
+    def release(self):
+        if self.res.opaque:
+            self.eng.lib.cj_result_release(self.eng.ctx, C.byref(self.res))
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

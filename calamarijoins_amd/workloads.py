@@ -1,0 +1,163 @@
+"""Synthetic workloads (include/cj_workload.h) as host Tables and Plans.
+
+Configurations (decimal M; seed = 1000 + config index):
+  C1  100k build x 1M probe, INT32 keys U[0, 2^17) with 5 % NULL, one full-range
+      INT32 payload per side; output [build key, build payload, probe payload].
+  C2  4M build keys = seeded permutation of [0, 4M); 64M probe keys =
+      perm(rank), rank ~ Zipf(1.0), 30 % misses in [4M, 2^31); payloads INT32 +
+      INT64 per side and FP64 (U[0, 1e6)) on the probe side, 2 % NULL;
+      output [b.i32, b.i64, p.i32, p.i64, p.f64].
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .host import PAGE, Column, Plan, Table
+
+GEN_PERM, GEN_UNIFORM, GEN_ZIPF_FK, GEN_I32_FULL, GEN_I64_FULL, GEN_F64_SCALED, GEN_VARCHAR, GEN_UNIFORM_FK = range(8)
+
+
+class ColGen(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("type", C.c_int32), ("stream", C.c_uint64), ("null_frac", C.c_double),
+        ("domain", C.c_uint64), ("perm_stream", C.c_uint64), ("zipf_s", C.c_double), ("miss_frac", C.c_double),
+        ("lo", C.c_int64), ("hi", C.c_int64), ("miss_lo", C.c_int64), ("miss_hi", C.c_int64),
+        ("scale", C.c_double), ("long_frac", C.c_double), ("long_lo", C.c_int64), ("long_hi", C.c_int64),
+    ]
+
+
+class GenTable(C.Structure):
+    _fields_ = [
+        ("num_rows", C.c_uint64), ("num_columns", C.c_uint32), ("types", C.POINTER(C.c_int32)),
+        ("num_pages", C.POINTER(C.c_uint32)), ("pages", C.POINTER(C.POINTER(C.c_uint8))), ("opaque", C.c_void_p),
+    ]
+
+
+def _lib():
+    lib = abi.load_library()
+    if not getattr(lib, "_gen_sigs", False):
+        lib.cj_gen_table_create.restype = C.c_int
+        lib.cj_gen_table_create.argtypes = [C.c_uint64, C.c_double, C.c_uint64, C.c_uint32, C.POINTER(ColGen),
+                                            C.c_int32, C.POINTER(GenTable)]
+        lib.cj_gen_table_free.restype = None
+        lib.cj_gen_table_free.argtypes = [C.POINTER(GenTable)]
+        lib.cj_pages_encode_fixed.restype = C.c_int
+        lib.cj_pages_encode_fixed.argtypes = [C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                              C.POINTER(C.c_uint64)]
+        lib.cj_pages_encode_varchar.restype = C.c_int
+        lib.cj_pages_encode_varchar.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_uint64, C.POINTER(C.c_uint64)]
+        lib._gen_sigs = True
+    return lib
+
+
+class _Owned:
+    """Keeps a generated table's library-owned pages alive."""
+
+    def __init__(self, gt: GenTable):
+        self.gt = gt
+
+    def __del__(self):
+        try:
+            _lib().cj_gen_table_free(C.byref(self.gt))
+        except Exception:
+            pass
+
+
+def col(kind, type_name, stream, null_frac=0.0, **kw) -> ColGen:
+    g = ColGen()
+    g.kind = kind
+    g.type = abi.TYPE_NAMES[type_name]
+    g.stream = stream
+    g.null_frac = null_frac
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+def generate(candidate_rows: int, seed: int, cols: list, keep_frac: float = 1.0, threads: int = 0) -> Table:
+    arr = (ColGen * len(cols))(*cols)
+    gt = GenTable()
+    rc = _lib().cj_gen_table_create(candidate_rows, keep_frac, seed, len(cols), arr, threads, C.byref(gt))
+    if rc != 0:
+        raise RuntimeError("table generation failed")
+    owner = _Owned(gt)
+    columns = []
+    for i in range(gt.num_columns):
+        np_ = gt.num_pages[i]
+        buf = np.ctypeslib.as_array(gt.pages[i], shape=(max(np_, 1) * PAGE,))[: np_ * PAGE]
+        columns.append(Column(gt.types[i], buf.reshape(np_, PAGE), owner=owner))
+    return Table(gt.num_rows, columns, owner=owner)
+
+
+def encode_fixed(type_id: int, values: np.ndarray, valid: np.ndarray) -> np.ndarray:
+    """Product page writer (cj_pages_encode_fixed)."""
+    n = len(valid)
+    per = 1984 if type_id == abi.CJ_INT32 else 1007
+    maxp = n // per + n // 60000 + 4
+    out = np.zeros((maxp, PAGE), dtype=np.uint8)
+    got = C.c_uint64()
+    v = np.ascontiguousarray(values)
+    ok = np.ascontiguousarray(valid, dtype=np.uint8)
+    rc = _lib().cj_pages_encode_fixed(type_id, n, v.ctypes.data, ok.ctypes.data, out.ctypes.data, maxp, C.byref(got))
+    if rc != 0:
+        raise RuntimeError("encode failed")
+    return out[: got.value].copy()
+
+
+# ------------------------------------------------------------------ configs
+def c1_tables(seed: int = 1001, build_rows: int = 100_000, probe_rows: int = 1_000_000):
+    b = generate(build_rows, seed, [
+        col(GEN_UNIFORM, "INT32", 1, 0.05, lo=0, hi=1 << 17),
+        col(GEN_I32_FULL, "INT32", 2),
+    ])
+    p = generate(probe_rows, seed, [
+        col(GEN_UNIFORM, "INT32", 11, 0.05, lo=0, hi=1 << 17),
+        col(GEN_I32_FULL, "INT32", 12),
+    ])
+    return b, p
+
+
+def c1_plan(build: Table, probe: Table) -> Plan:
+    p = Plan()
+    p.new_input(build)
+    p.new_input(probe)
+    p.new_scan_node(0, [(0, "INT32"), (1, "INT32")])
+    p.new_scan_node(1, [(0, "INT32"), (1, "INT32")])
+    p.root = p.new_join_node(True, 0, 1, 0, 0, [(0, "INT32"), (1, "INT32"), (3, "INT32")])
+    return p
+
+
+C2_BUILD, C2_PROBE = 4_000_000, 64_000_000
+
+
+def c2_tables(seed: int = 1002, build_rows: int = C2_BUILD, probe_rows: int = C2_PROBE, threads: int = 0):
+    perm_stream = 7
+    b = generate(build_rows, seed, [
+        col(GEN_PERM, "INT32", 1, 0.0, domain=build_rows, perm_stream=perm_stream),
+        col(GEN_I32_FULL, "INT32", 2, 0.02),
+        col(GEN_I64_FULL, "INT64", 3, 0.02),
+    ], threads=threads)
+    p = generate(probe_rows, seed, [
+        col(GEN_ZIPF_FK, "INT32", 11, 0.0, domain=build_rows, perm_stream=perm_stream, zipf_s=1.0, miss_frac=0.30,
+            miss_lo=build_rows, miss_hi=1 << 31),
+        col(GEN_I32_FULL, "INT32", 12, 0.02),
+        col(GEN_I64_FULL, "INT64", 13, 0.02),
+        col(GEN_F64_SCALED, "FP64", 14, 0.02, scale=1e6),
+    ], threads=threads)
+    return b, p
+
+
+def c2_plan(build: Table, probe: Table) -> Plan:
+    p = Plan()
+    p.new_input(build)
+    p.new_input(probe)
+    p.new_scan_node(0, [(0, "INT32"), (1, "INT32"), (2, "INT64")])
+    p.new_scan_node(1, [(0, "INT32"), (1, "INT32"), (2, "INT64"), (3, "FP64")])
+    # output: b.i32, b.i64, p.i32, p.i64, p.f64
+    p.root = p.new_join_node(True, 0, 1, 0, 0,
+                             [(1, "INT32"), (2, "INT64"), (4, "INT32"), (5, "INT64"), (6, "FP64")])
+    return p

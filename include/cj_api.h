@@ -1,0 +1,182 @@
+/*
+ * cj_api.h -- C ABI of the MI355X join engine (libcalamarijoins_amd.so).
+ *
+ * This is the drop-in boundary for the join pipeline behind the contest API
+ *
+ *     void*         Contest::build_context();
+ *     void          Contest::destroy_context(void*);
+ *     ColumnarTable Contest::execute(const Plan&, void*, TimingStats*);
+ *
+ * (reference: include/data_model/plan.h:380-385, src/execute.cpp:311-329).
+ * The C++ header include/cj/plan.h re-declares that API on top of these
+ * functions; any other language binds the functions below directly.
+ *
+ * All signatures use plain pointers and sizes.  Pages are the contest's
+ * 8192-byte columnar pages (reference README.md:65-104, plan.h:78-81):
+ *   u16 n_rows | u16 n_vals | values (INT32 at +4, INT64/FP64 at +8) or
+ *   u16 VARCHAR end offsets at +4 with chars at +4+2*n_vals | NULL bitmap
+ *   in the last ceil(n_rows/8) bytes (LSB first).  n_rows 0xffff / 0xfffe
+ *   mark the first / continuation page of a long string (u16 char count at
+ *   +2, chars at +4).
+ *
+ * Error convention: every function returns CJ_OK (0) or a negative status;
+ * cj_last_error() gives the message of the calling thread's last failure.
+ */
+#ifndef CJ_API_H
+#define CJ_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CJ_PAGE_SIZE 8192
+
+/* Status codes. */
+enum {
+    CJ_OK = 0,
+    CJ_ERR_INVALID = -1,     /* malformed plan / table / argument          */
+    CJ_ERR_UNSUPPORTED = -2, /* shape the engine rejects (e.g. non-INT32 key) */
+    CJ_ERR_DEVICE = -3,      /* HIP runtime failure                         */
+    CJ_ERR_OOM = -4,         /* device memory budget exhausted              */
+    CJ_ERR_BUSY = -5         /* serving queue full / not admitted           */
+};
+
+/* DataType, same numbering as the reference enum (foundation/attribute.h:7-12). */
+enum { CJ_INT32 = 0, CJ_INT64 = 1, CJ_FP64 = 2, CJ_VARCHAR = 3 };
+
+/* Plan node kinds (reference NodeType, plan.h:46-49: HashJoin, Scan). */
+enum { CJ_NODE_JOIN = 0, CJ_NODE_SCAN = 1 };
+
+typedef struct cj_ctx cj_ctx;
+
+/* One column of a ColumnarTable: reference Column{type, pages} (plan.h:84-136). */
+typedef struct cj_column_desc {
+    int32_t type;             /* CJ_INT32 .. CJ_VARCHAR                 */
+    uint32_t num_pages;
+    const void *const *pages; /* host pointers to 8 KiB pages, 8-aligned */
+} cj_column_desc;
+
+/* Reference ColumnarTable{num_rows, columns} (plan.h:139-142). */
+typedef struct cj_table_desc {
+    uint64_t num_rows;
+    uint32_t num_columns;
+    const cj_column_desc *columns;
+} cj_table_desc;
+
+/* One entry of PlanNode::output_attrs: tuple<size_t, DataType>. */
+typedef struct cj_attr {
+    uint64_t index;
+    int32_t type;
+} cj_attr;
+
+/* Reference PlanNode{variant<ScanNode, JoinNode>, output_attrs} (plan.h:51-70). */
+typedef struct cj_node_desc {
+    int32_t kind;           /* CJ_NODE_SCAN or CJ_NODE_JOIN           */
+    int32_t build_left;     /* JoinNode::build_left (a hint; see below) */
+    uint64_t base_table_id; /* ScanNode::base_table_id                */
+    uint64_t left, right;   /* JoinNode children (node indices)       */
+    uint64_t left_attr, right_attr;
+    uint32_t num_output_attrs;
+    const cj_attr *output_attrs;
+} cj_node_desc;
+
+/* Reference Plan{nodes, inputs, root} (plan.h:149-185). */
+typedef struct cj_plan_desc {
+    uint32_t num_nodes;
+    const cj_node_desc *nodes;
+    uint32_t num_inputs;
+    const cj_table_desc *inputs;
+    uint64_t root;
+} cj_plan_desc;
+
+/* Reference Contest::TimingStats (plan.h:370-378), plus device detail. */
+typedef struct cj_timing {
+    int64_t hashtable_build_ms;
+    int64_t hash_join_probe_ms;
+    int64_t nested_loop_join_ms;
+    int64_t materialize_ms;
+    int64_t setup_ms;
+    int64_t total_execution_ms;
+    int64_t intermediate_ms;
+    double device_ms;   /* first kernel .. last output page in HBM (hipEvents) */
+    double h2d_ms;      /* page upload of non-resident inputs              */
+    double d2h_ms;      /* result page download (host output only)         */
+    uint64_t bytes_alg; /* algorithmic bytes: input pages read + output written */
+} cj_timing;
+
+/* A result column. Host output: pages[i] are host pointers valid until
+ * cj_result_release.  Device output: device_pages is the HBM address of
+ * num_pages contiguous 8 KiB pages (pages == NULL). */
+typedef struct cj_result_column {
+    int32_t type;
+    uint32_t num_pages;
+    void **pages;
+    void *device_pages;
+} cj_result_column;
+
+typedef struct cj_result {
+    uint64_t num_rows;
+    uint32_t num_columns;
+    cj_result_column *columns;
+    void *opaque; /* owned by the library */
+} cj_result;
+
+/* Context options (zero-initialise, then set what you need). */
+typedef struct cj_ctx_options {
+    int32_t device;             /* HIP device ordinal                       */
+    uint64_t device_budget;     /* bytes of HBM the context may use; 0=auto */
+    int32_t num_streams;        /* concurrent execution slots; 0 = 4        */
+    int32_t cache_decoded;      /* keep decoded columns of resident tables  */
+    int32_t kernel_timing;      /* record per-kernel hipEvents (profiling)  */
+    int32_t host_threads;       /* threads for host page staging; 0 = auto  */
+} cj_ctx_options;
+
+/* Execution flags. */
+enum {
+    CJ_EXEC_OUTPUT_DEVICE = 1, /* keep result pages in HBM (no D2H)          */
+    CJ_EXEC_NO_FAST_PATH = 2   /* force the general (row-id list) path        */
+};
+
+/* ---- context: replaces Contest::build_context / destroy_context
+ *      (reference src/execute.cpp:327-329). ------------------------------- */
+int cj_ctx_create(const cj_ctx_options *opts, cj_ctx **out);
+int cj_ctx_destroy(cj_ctx *ctx);
+const char *cj_last_error(void);
+/* Device name and CU count of the context's GPU (diagnostics). */
+int cj_ctx_device_info(cj_ctx *ctx, char *name, size_t name_cap, int32_t *num_cus);
+
+/* ---- HBM-resident base tables.  A registered table's pages are uploaded
+ *      once; later plans whose inputs use the same page pointers read the
+ *      HBM copy.  The caller must not modify or free registered pages until
+ *      cj_table_unregister.  (No reference counterpart: the reference reads
+ *      host pages in place, plan.h:84-142.) ------------------------------- */
+int cj_table_register(cj_ctx *ctx, const cj_table_desc *table, uint64_t *out_id);
+int cj_table_unregister(cj_ctx *ctx, uint64_t id);
+
+/* ---- execute: replaces Contest::execute (reference src/execute.cpp:311-325).
+ *      Synchronous.  The result is owned by the caller until
+ *      cj_result_release.  timing may be NULL. ----------------------------- */
+int cj_execute(cj_ctx *ctx, const cj_plan_desc *plan, uint32_t flags,
+               cj_result *out, cj_timing *timing);
+int cj_result_release(cj_ctx *ctx, cj_result *res);
+/* Copy a device-output result into host pages (fills columns[i].pages). */
+int cj_result_to_host(cj_ctx *ctx, cj_result *res);
+
+/* Per-kernel device time accumulated while kernel_timing is on. */
+typedef struct cj_kernel_stat {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+    uint64_t bytes; /* algorithmic bytes moved by those launches */
+} cj_kernel_stat;
+int cj_kernel_stats(cj_ctx *ctx, cj_kernel_stat *out, uint32_t cap, uint32_t *count);
+int cj_kernel_stats_reset(cj_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CJ_API_H */

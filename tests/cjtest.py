@@ -1,0 +1,293 @@
+"""Test helpers: tables, plans, the CPU oracle and the HIP engine via ctypes.
+
+The oracle (oracle/libcj_oracle.so) is the checker; the engine
+(calamarijoins_amd/libcalamarijoins_amd.so) is what is tested.  Both take
+the same cj_api.h descriptors, built here from Python rows or from the
+engine's synthetic-table generator.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import struct
+import subprocess
+
+import numpy as np
+
+from calamarijoins_amd import abi
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(REPO, "oracle", "libcj_oracle.so")
+PAGE = abi.PAGE_SIZE
+T = abi.TYPE_NAMES  # name -> id
+
+_oracle = None
+
+
+class OracleResult(C.Structure):
+    _fields_ = [
+        ("num_rows", C.c_uint64),
+        ("num_columns", C.c_uint32),
+        ("types", C.POINTER(C.c_int32)),
+        ("num_pages", C.POINTER(C.c_uint32)),
+        ("pages", C.POINTER(C.POINTER(C.c_void_p))),
+    ]
+
+
+def oracle() -> C.CDLL:
+    global _oracle
+    if _oracle is not None:
+        return _oracle
+    if not os.path.exists(ORACLE_SO):
+        subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True, capture_output=True)
+    lib = C.CDLL(ORACLE_SO)
+    P = C.POINTER
+    lib.cjo_encode_fixed.restype = C.c_size_t
+    lib.cjo_encode_fixed.argtypes = [C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    lib.cjo_encode_varchar.restype = C.c_size_t
+    lib.cjo_encode_varchar.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+    lib.cjo_decode_fixed.restype = C.c_int64
+    lib.cjo_decode_fixed.argtypes = [C.c_int32, P(C.c_void_p), C.c_uint32, C.c_void_p, C.c_void_p]
+    lib.cjo_decode_varchar.restype = C.c_int64
+    lib.cjo_decode_varchar.argtypes = [P(C.c_void_p), C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                       P(C.c_uint64)]
+    lib.cjo_row_hashes.restype = C.c_int
+    lib.cjo_row_hashes.argtypes = [P(abi.TableDesc), C.c_void_p]
+    lib.cjo_execute.restype = C.c_int
+    lib.cjo_execute.argtypes = [P(abi.PlanDesc), C.c_int, P(OracleResult)]
+    lib.cjo_result_free.restype = None
+    lib.cjo_result_free.argtypes = [P(OracleResult)]
+    lib.cjo_last_error.restype = C.c_char_p
+    lib.cjo_crc32c_u32.restype = C.c_uint32
+    lib.cjo_crc32c_u32.argtypes = [C.c_uint32, C.c_uint32]
+    lib.cjo_crc32c_bytes.restype = C.c_uint32
+    lib.cjo_crc32c_bytes.argtypes = [C.c_uint32, C.c_char_p, C.c_size_t]
+    lib.cjo_bloom_tag.restype = C.c_uint16
+    lib.cjo_bloom_tag.argtypes = [C.c_uint32]
+    lib.cjo_crc32c_hw_mismatches.restype = C.c_int64
+    lib.cjo_crc32c_hw_mismatches.argtypes = [C.c_uint64]
+    _oracle = lib
+    return lib
+
+
+# --------------------------------------------------------------- columns
+from calamarijoins_amd.host import Column, Table, Plan, Engine, EngineResult  # noqa: E402,F401
+
+
+def encode_fixed(type_id: int, values, valid) -> np.ndarray:
+    n = len(valid)
+    w = 4 if type_id == abi.CJ_INT32 else 8
+    dt = {abi.CJ_INT32: np.int32, abi.CJ_INT64: np.int64, abi.CJ_FP64: np.float64}[type_id]
+    vals = np.ascontiguousarray(values, dtype=dt)
+    val = np.ascontiguousarray(valid, dtype=np.uint8)
+    per = 1984 if w == 4 else 1007
+    maxp = n // per + n // 60000 + 4
+    out = np.zeros((maxp, PAGE), dtype=np.uint8)
+    got = oracle().cjo_encode_fixed(type_id, n, vals.ctypes.data, val.ctypes.data, out.ctypes.data, maxp)
+    assert got != C.c_size_t(-1).value
+    return out[:got].copy()
+
+
+def encode_varchar(strings) -> np.ndarray:
+    n = len(strings)
+    valid = np.array([s is not None for s in strings], dtype=np.uint8)
+    blobs = [(s.encode() if isinstance(s, str) else s) if s is not None else b"" for s in strings]
+    offs = np.zeros(n + 1, dtype=np.uint64)
+    if n:
+        offs[1:] = np.cumsum([len(b) for b in blobs])
+    data = b"".join(blobs) + b"\0"
+    buf = C.create_string_buffer(data, len(data))
+    maxp = n + int(offs[-1]) // (PAGE - 8) + 4
+    out = np.zeros((maxp, PAGE), dtype=np.uint8)
+    got = oracle().cjo_encode_varchar(n, offs.ctypes.data, C.addressof(buf), valid.ctypes.data, out.ctypes.data, maxp)
+    assert got != C.c_size_t(-1).value
+    return out[:got].copy()
+
+
+def column_from_values(type_name: str, values: list) -> Column:
+    t = T[type_name]
+    if t == abi.CJ_VARCHAR:
+        return Column(t, encode_varchar(values))
+    valid = [v is not None for v in values]
+    vals = [v if v is not None else 0 for v in values]
+    return Column(t, encode_fixed(t, vals, valid))
+
+
+def table_from_rows(types: list[str], rows: list[list]) -> Table:
+    cols = [column_from_values(tn, [r[i] for r in rows]) for i, tn in enumerate(types)]
+    return Table(len(rows), cols)
+
+
+# ------------------------------------------------------------ decoding
+def decode_pages(type_id: int, page_ptrs, num_pages: int) -> list:
+    """Decode a column (list of page pointers) into Python values (None = NULL)."""
+    lib = oracle()
+    arr = (C.c_void_p * max(num_pages, 1))(*[page_ptrs[i] for i in range(num_pages)])
+    if type_id == abi.CJ_VARCHAR:
+        total = C.c_uint64(0)
+        n = lib.cjo_decode_varchar(arr, num_pages, None, None, 0, None, C.byref(total))
+        assert n >= 0, "malformed VARCHAR column"
+        offs = np.zeros(n + 1, dtype=np.uint64)
+        data = np.zeros(int(total.value) + 1, dtype=np.uint8)
+        valid = np.zeros(max(n, 1), dtype=np.uint8)
+        lib.cjo_decode_varchar(arr, num_pages, offs.ctypes.data, data.ctypes.data, total.value, valid.ctypes.data, None)
+        raw = data.tobytes()
+        return [raw[int(offs[i]):int(offs[i + 1])] if valid[i] else None for i in range(n)]
+    n = lib.cjo_decode_fixed(type_id, arr, num_pages, None, None)
+    assert n >= 0
+    dt = {abi.CJ_INT32: np.int32, abi.CJ_INT64: np.int64, abi.CJ_FP64: np.float64}[type_id]
+    vals = np.zeros(max(n, 1), dtype=dt)
+    valid = np.zeros(max(n, 1), dtype=np.uint8)
+    lib.cjo_decode_fixed(type_id, arr, num_pages, vals.ctypes.data, valid.ctypes.data)
+    out = vals[:n].tolist()
+    return [out[i] if valid[i] else None for i in range(n)]
+
+
+def decode_column_arrays(type_id: int, page_ptrs, num_pages: int):
+    """Fixed-width column -> (values ndarray, valid ndarray)."""
+    lib = oracle()
+    arr = (C.c_void_p * max(num_pages, 1))(*[page_ptrs[i] for i in range(num_pages)])
+    n = lib.cjo_decode_fixed(type_id, arr, num_pages, None, None)
+    dt = {abi.CJ_INT32: np.int32, abi.CJ_INT64: np.int64, abi.CJ_FP64: np.float64}[type_id]
+    vals = np.zeros(max(n, 1), dtype=dt)
+    valid = np.zeros(max(n, 1), dtype=np.uint8)
+    lib.cjo_decode_fixed(type_id, arr, num_pages, vals.ctypes.data, valid.ctypes.data)
+    return vals[:n], valid[:n].astype(bool)
+
+
+def canon_key(row):
+    out = []
+    for v in row:
+        if v is None:
+            out.append((1, 0))
+        elif isinstance(v, float):
+            out.append((0, struct.unpack("<q", struct.pack("<d", v))[0]))
+        elif isinstance(v, str):
+            out.append((0, v.encode()))
+        else:
+            out.append((0, v))
+    return tuple(out)
+
+
+def canon_rows(rows):
+    return sorted([tuple(r) for r in rows], key=canon_key)
+
+
+# -------------------------------------------------------------- results
+# A "result" is anything with num_rows, types and page_lists (per-column
+# lists of host page addresses): an oracle run or an EngineResult.
+def result_columns(r):
+    return [decode_pages(t, pl, len(pl)) for t, pl in zip(r.types, r.page_lists)]
+
+
+def result_rows(r):
+    cols = result_columns(r)
+    for c in cols:
+        assert len(c) == r.num_rows, (len(c), r.num_rows)
+    return [tuple(c[i] for c in cols) for i in range(r.num_rows)]
+
+
+def result_table_desc(r, keep: list):
+    cols = (abi.ColumnDesc * max(len(r.types), 1))()
+    for i, (t, pl) in enumerate(zip(r.types, r.page_lists)):
+        arr = (C.c_void_p * max(len(pl), 1))(*pl)
+        keep.append(arr)
+        cols[i] = abi.ColumnDesc(t, len(pl), C.cast(arr, C.POINTER(C.c_void_p)))
+    keep.append(cols)
+    return abi.TableDesc(r.num_rows, len(r.types), cols)
+
+
+def result_row_hashes(r) -> np.ndarray:
+    """Sorted per-row hashes: equal arrays <=> equal row multisets (w.h.p.)."""
+    keep = []
+    desc = result_table_desc(r, keep)
+    out = np.zeros(max(r.num_rows, 1), dtype=np.uint64)
+    rc = oracle().cjo_row_hashes(C.byref(desc), out.ctypes.data)
+    assert rc == 0, oracle().cjo_last_error()
+    return np.sort(out[: r.num_rows])
+
+
+class OracleRun:
+    def __init__(self, plan: Plan, threads: int = 0):
+        lib = oracle()
+        self.res = OracleResult()
+        rc = lib.cjo_execute(C.byref(plan.desc()), threads, C.byref(self.res))
+        if rc != 0:
+            raise RuntimeError(f"oracle status {rc}: {lib.cjo_last_error().decode()}")
+        self.num_rows = self.res.num_rows
+        self.types = [self.res.types[i] for i in range(self.res.num_columns)]
+        self.page_lists = [[self.res.pages[i][q] for q in range(self.res.num_pages[i])]
+                           for i in range(self.res.num_columns)]
+
+    def rows(self):
+        return result_rows(self)
+
+    def __del__(self):
+        try:
+            oracle().cjo_result_free(C.byref(self.res))
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------- KAT loading
+def kat_tables(case) -> list[Table]:
+    out = []
+    for t in case["tables"]:
+        types = t["types"]
+        if "rows" in t:
+            rows = t["rows"]
+        elif "seq" in t:
+            a, b, s = t["seq"]
+            rows = [[i] for i in range(a, b, s)]
+        else:
+            m = t["mod"]
+            rows = []
+            for i in range(m["n"]):
+                if m["null_every"] and i % m["null_every"] == 0:
+                    rows.append([None])
+                else:
+                    rows.append([(i % m["mod"]) + m["add"]])
+        out.append(table_from_rows(types, rows))
+    return out
+
+
+def kat_plan(case) -> Plan:
+    p = Plan()
+    for t in kat_tables(case):
+        p.new_input(t)
+    for n in case["nodes"]:
+        attrs = [(a, t) for a, t in n["out"]]
+        if "scan" in n:
+            p.new_scan_node(n["scan"], attrs)
+        else:
+            bl, l, r, la, ra = n["join"]
+            p.new_join_node(bl, l, r, la, ra, attrs)
+    p.root = case["root"]
+    return p
+
+
+def check_kat(case, result):
+    """Assert a result against a KAT's expectations."""
+    exp = case["expect"]
+    assert [abi.TYPE_IDS[t] for t in result.types] == exp["types"], result.types
+    if "num_rows" in exp:
+        assert result.num_rows == exp["num_rows"]
+    if exp.get("num_rows_positive"):
+        assert result.num_rows > 0
+    if result.num_rows == 0:
+        return
+    rows = result_rows(result)
+    norm = [tuple(v.decode() if isinstance(v, bytes) else v for v in r) for r in rows]
+    if "rows" in exp:
+        assert canon_rows(norm) == canon_rows([tuple(r) for r in exp["rows"]])
+    if "seq_rows" in exp:
+        a, b, k = exp["seq_rows"]
+        assert canon_rows(norm) == [tuple([v] * k) for v in range(a, b)]
+    if exp.get("no_nulls"):
+        assert all(v is not None for r in norm for v in r)
+    if exp.get("cols_equal"):
+        assert all(len(set(r)) == 1 for r in norm)
+    if "first_col_mod" in exp:
+        m, rr = exp["first_col_mod"]
+        assert all(r[0] % m == rr for r in norm)

@@ -1,0 +1,162 @@
+"""GPU parity: the HIP engine through the C ABI against the oracle, bit-exact
+(the same multiset of output rows, NULLs and raw bytes included)."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import bruteforce
+import cjtest
+import test_oracle as to
+from calamarijoins_amd import abi, workloads
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KATS = json.load(open(os.path.join(GOLDEN, "unit_kats.json")))["cases"]
+
+
+def same_multiset(got, exp):
+    assert got.num_rows == exp.num_rows, (got.num_rows, exp.num_rows)
+    assert got.types == exp.types
+    assert np.array_equal(cjtest.result_row_hashes(got), cjtest.result_row_hashes(exp))
+
+
+@pytest.mark.parametrize("case", KATS, ids=[c["name"] for c in KATS])
+def test_gpu_unit_kat(engine, case):
+    res = engine.execute(cjtest.kat_plan(case))
+    cjtest.check_kat(case, res)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_gpu_random_plans_vs_bruteforce(engine, seed):
+    rng = random.Random(1000 + seed)
+    njoins = 1 + seed % 3
+    tables = []
+    for _ in range(njoins + 1):
+        nrows = 0 if rng.random() < 0.04 else rng.choice([1, 3, 5, 7, 8, 9, 20, 60, 150])
+        tables.append(to.rand_table(rng, nrows, key_domain=rng.choice([2, 4, 8, 16])))
+    plan = to.rand_plan(rng, njoins, tables)
+    exp = bruteforce.evaluate(plan.nodes, tables, plan.root)
+    got = engine.execute(plan)
+    assert got.num_rows == len(exp)
+    assert to.norm_rows(cjtest.result_rows(got)) == to.norm_rows(exp)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_gpu_random_plans_vs_oracle_larger(engine, seed):
+    rng = random.Random(5000 + seed)
+    njoins = 1 + seed % 3
+    tables = []
+    for _ in range(njoins + 1):
+        nrows = rng.choice([500, 3000, 20000])
+        tables.append(to.rand_table(rng, nrows, key_domain=rng.choice([100, 2000, 50000])))
+    plan = to.rand_plan(rng, njoins, tables)
+    same_multiset(engine.execute(plan), cjtest.OracleRun(plan, threads=8))
+
+
+def test_gpu_c1_config_bit_exact(engine):
+    b, p = workloads.c1_tables()
+    plan = workloads.c1_plan(b, p)
+    same_multiset(engine.execute(plan), cjtest.OracleRun(plan, threads=8))
+
+
+def test_gpu_c2_scaled_bit_exact(engine):
+    b, p = workloads.c2_tables(build_rows=200_000, probe_rows=3_200_000)
+    plan = workloads.c2_plan(b, p)
+    got = engine.execute(plan)
+    exp = cjtest.OracleRun(plan, threads=8)
+    same_multiset(got, exp)
+    # each output row's payloads come from rows whose keys matched: all 5
+    # columns present, and the row count equals the non-miss probe rows
+    pk = cjtest.decode_column_arrays(abi.CJ_INT32, p.columns[0].page_ptrs(), p.columns[0].num_pages)[0]
+    assert got.num_rows == int(np.sum(pk < 200_000))
+
+
+def test_gpu_resident_tables_and_device_output(engine):
+    b, p = workloads.c2_tables(seed=77, build_rows=50_000, probe_rows=400_000)
+    plan = workloads.c2_plan(b, p)
+    exp = cjtest.OracleRun(plan, threads=8)
+    ib, ip = engine.register(b), engine.register(p)
+    try:
+        for _ in range(2):  # second run hits the decoded-column cache
+            got = engine.execute(plan, flags=abi.CJ_EXEC_OUTPUT_DEVICE, timing=True)
+            assert got.page_lists == [[]] * 5
+            got.to_host()
+            same_multiset(got, exp)
+            assert got.timing.bytes_alg > 0 and got.timing.device_ms > 0
+    finally:
+        engine.unregister(ib)
+        engine.unregister(ip)
+
+
+def test_gpu_varchar_long_strings(engine):
+    rng = random.Random(3)
+    n = 4000
+    keys = [rng.randrange(500) if rng.random() > 0.05 else None for _ in range(n)]
+    strs = []
+    for _ in range(n):
+        r = rng.random()
+        strs.append(None if r < 0.1 else ("Q" * rng.choice([8185, 8186, 9000, 20000]) if r < 0.11 else
+                                          "".join(chr(rng.randrange(33, 127)) for _ in range(rng.randrange(0, 70)))))
+    t1 = cjtest.table_from_rows(["INT32", "VARCHAR"], [[k, s] for k, s in zip(keys, strs)])
+    t2 = cjtest.table_from_rows(["INT32", "VARCHAR"], [[k, s] for k, s in zip(keys[:700], strs[:700])])
+    plan = cjtest.Plan()
+    plan.new_input(t1)
+    plan.new_input(t2)
+    plan.new_scan_node(0, [(0, "INT32"), (1, "VARCHAR")])
+    plan.new_scan_node(1, [(1, "VARCHAR"), (0, "INT32")])
+    plan.root = plan.new_join_node(False, 0, 1, 0, 1, [(1, "VARCHAR"), (2, "VARCHAR"), (0, "INT32")])
+    same_multiset(engine.execute(plan), cjtest.OracleRun(plan, threads=8))
+
+
+def test_gpu_int32_min_quirks(engine):
+    """INT32_MIN payloads come out NULL; through an intermediate, NULL and
+    INT32_MIN keys behave as the reference's value_t sentinel."""
+    I = bruteforce.I32_MIN
+    rows_a = [[1, I], [I, 5], [None, 6], [2, 7]] + [[k, k] for k in range(10, 40)]
+    rows_b = [[1, 1], [I, 2], [2, I], [None, 3]] + [[k, -k] for k in range(10, 60)]
+    rows_c = [[I, 100], [1, 101], [5, 102]] + [[k, k] for k in range(10, 70)]
+    tabs = [(["INT32", "INT32"], r) for r in (rows_a, rows_b, rows_c)]
+    plan = cjtest.Plan()
+    for t in tabs:
+        plan.new_input(cjtest.table_from_rows(*t))
+    for i in range(3):
+        plan.new_scan_node(i, [(0, "INT32"), (1, "INT32")])
+    j = plan.new_join_node(True, 0, 1, 0, 0, [(1, "INT32"), (2, "INT32"), (3, "INT32")])
+    plan.root = plan.new_join_node(True, j, 2, 1, 0, [(0, "INT32"), (1, "INT32"), (2, "INT32"), (4, "INT32")])
+    exp = bruteforce.evaluate(plan.nodes, tabs, plan.root)
+    got = engine.execute(plan)
+    assert to.norm_rows(cjtest.result_rows(got)) == to.norm_rows(exp)
+
+
+def test_gpu_empty_inputs(engine):
+    t0 = cjtest.table_from_rows(["INT32"], [])
+    t1 = cjtest.table_from_rows(["INT32"], [[1], [2]])
+    for a, bt in ((t0, t1), (t1, t0), (t0, t0)):
+        plan = cjtest.Plan()
+        plan.new_input(a)
+        plan.new_input(bt)
+        plan.new_scan_node(0, [(0, "INT32")])
+        plan.new_scan_node(1, [(0, "INT32")])
+        plan.root = plan.new_join_node(True, 0, 1, 0, 0, [(0, "INT32"), (1, "INT32")])
+        got = engine.execute(plan)
+        assert got.num_rows == 0 and got.types == [0, 0] and got.num_pages() == [0, 0]
+
+
+def test_gpu_rejects_bad_plans(engine):
+    t = cjtest.table_from_rows(["INT32", "INT64"], [[1, 2]])
+    plan = cjtest.Plan()
+    plan.new_input(t)
+    plan.new_scan_node(0, [(0, "INT32"), (1, "INT64")])
+    plan.new_scan_node(0, [(0, "INT32"), (1, "INT64")])
+    plan.root = plan.new_join_node(True, 0, 1, 1, 1, [(0, "INT32")])  # INT64 key
+    with pytest.raises(abi.CJError):
+        engine.execute(plan)
+    plan.nodes[-1]["left_attr"] = 0
+    plan.nodes[-1]["right_attr"] = 0
+    plan.nodes[-1]["attrs"] = [(1, "INT32")]  # declared type != source type
+    with pytest.raises(abi.CJError):
+        engine.execute(plan)
