@@ -40,17 +40,17 @@ def cpu_baseline(build, probe_rows, seed, threads, repeats):
     """The oracle's CPU restatement of execute() on a bounded sample: the full
     4M-row build side and the first `probe_rows` probe rows of the same
     generated workload (rows are a pure function of seed and row index)."""
-    sys.path.insert(0, os.path.join(REPO, "tests"))
-    import cjtest
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import cj_oracle
     from calamarijoins_amd import workloads
 
     _, p = workloads.c2_tables(seed=seed, build_rows=workloads.C2_BUILD, probe_rows=probe_rows)
     plan = workloads.c2_plan(build, p)
-    cjtest.OracleRun(plan, threads=threads)  # cold run
+    cj_oracle.OracleRun(plan, threads=threads)  # cold run
     times, rows = [], 0
     for _ in range(repeats):
         t0 = time.perf_counter()
-        r = cjtest.OracleRun(plan, threads=threads)
+        r = cj_oracle.OracleRun(plan, threads=threads)
         times.append(time.perf_counter() - t0)
         rows = r.num_rows
         del r

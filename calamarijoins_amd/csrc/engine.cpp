@@ -16,6 +16,7 @@
 // an intermediate inserts them as INT32_MIN (hashtable.h:357), and INT32
 // output values equal to INT32_MIN come out NULL (page_builders.h:123).
 #include "engine.hpp"
+#include "fused.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -212,10 +213,13 @@ void Context::free_column(DevColumn &c) {
     pool.free(c.row_start);
     pool.free(c.values);
     pool.free(c.valid);
+    pool.free(c.byte_prefix);  // byte_start lives in the row_start block
     c.pages = nullptr;
     c.row_start = nullptr;
     c.values = nullptr;
     c.valid = nullptr;
+    c.byte_start = nullptr;
+    c.byte_prefix = nullptr;
     c.loaded = false;
 }
 
@@ -251,8 +255,11 @@ static void validate_page(const uint8_t *p, int32_t type, uint32_t idx, uint16_t
 void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s) {
     c.type = d.type;
     c.num_pages = d.num_pages;
-    std::vector<uint64_t> rs(d.num_pages + 1);
-    uint64_t run = 0;
+    // host index: row_start (first row of each page) and byte_start (first
+    // bitmap byte of each page in the page index), both length num_pages + 1
+    std::vector<uint64_t> rs(2 * (size_t)d.num_pages + 2);
+    uint64_t *bs = rs.data() + d.num_pages + 1;
+    uint64_t run = 0, bytes = 0;
     uint16_t prev = 0;
     for (uint32_t i = 0; i < d.num_pages; ++i) {
         if (!d.pages[i]) throw InvalidError("null page pointer");
@@ -260,18 +267,23 @@ void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s
         validate_page(p, d.type, i, prev);
         std::memcpy(&prev, p, 2);
         rs[i] = run;
+        bs[i] = bytes;
         run += host_rows_of(p);
+        if (prev < 0xfffe) bytes += (prev + 7u) / 8u;
     }
     if (run > 0xffffffffull) throw UnsupportedError("column has more than 2^32 rows");
     rs[d.num_pages] = run;
+    bs[d.num_pages] = bytes;
     c.num_rows = run;
+    c.bitmap_bytes = bytes;
     c.pages = static_cast<uint8_t *>(pool.alloc((size_t)d.num_pages * kPage + 16));
-    c.row_start = dalloc<uint64_t>(d.num_pages + 1);
+    c.row_start = dalloc<uint64_t>(2 * (size_t)d.num_pages + 2);
+    c.byte_start = nullptr;  // lives inside the row_start block
     const uint32_t chunk = 4096;
     uint8_t *stage[2] = {nullptr, nullptr};
     hipEvent_t ev[2];
     size_t stage_bytes = (size_t)std::min<uint32_t>(chunk, std::max<uint32_t>(d.num_pages, 1)) * kPage;
-    stage[0] = static_cast<uint8_t *>(pinned.alloc(stage_bytes + (d.num_pages + 1) * 8));
+    stage[0] = static_cast<uint8_t *>(pinned.alloc(std::max(stage_bytes, rs.size() * 8)));
     stage[1] = static_cast<uint8_t *>(pinned.alloc(stage_bytes));
     CJ_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     CJ_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -307,6 +319,7 @@ void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s
     hipEventDestroy(ev[1]);
     pinned.free(stage[0]);
     pinned.free(stage[1]);
+    c.byte_start = c.row_start + d.num_pages + 1;
     c.loaded = true;
 }
 
@@ -435,6 +448,7 @@ class Exec {
     std::vector<std::unique_ptr<DevColumn>> own_cols;
     std::vector<std::vector<DevColumn *>> cols;
     std::map<DevColumn *, ColView> views;
+    std::map<DevColumn *, const uint16_t *> page_indexes;
     std::set<DevColumn *> referenced;
     uint64_t scratch_elems = 0;
     uint64_t *scratch = nullptr;
@@ -459,6 +473,11 @@ class Exec {
             own_cols.push_back(std::move(own));
             h2d_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
+        // every column of a table must describe the same rows: row ids of one
+        // column index all the others
+        for (DevColumn *o : cols[table])
+            if (o && o->num_rows != dc->num_rows)
+                throw InvalidError("columns of input table " + std::to_string(table) + " hold different row counts");
         cols[table][c] = dc;
         referenced.insert(dc);
         return dc;
@@ -609,6 +628,12 @@ class Exec {
     }
 
     Rel join(const cj_node_desc &n, bool is_root, int depth, cj_result *out);
+    bool fused_ok(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw, const std::vector<uint32_t> &remap);
+    void fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw, const std::vector<uint32_t> &remap,
+                    uint32_t units, const KeySource &pk, const HashTableDev &ht, const NLKeys *nl,
+                    const uint64_t *cnt, const uint64_t *off, uint64_t total);
+    const uint16_t *page_index(DevColumn *c);
+    uint32_t flags = 0;
     void materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
                           const std::vector<uint32_t> &remap, uint64_t total, const uint32_t *ob, const uint32_t *op,
                           cj_result *out);
@@ -672,6 +697,10 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         scan(cnt, off, units);
         total = read_u64(off + units);
         if (total > 0xffffffffull) throw UnsupportedError("join result exceeds 2^32 rows");
+        if (total && is_root && fused_ok(n, B, P, bw, remap)) {
+            fused_root(n, B, P, bw, remap, units, pk, ht, nl, cnt, off, total);
+            return Rel{};
+        }
         if (total) {
             if (need_b) ob = tmp<uint32_t>(total);
             if (need_p) op = tmp<uint32_t>(total);
@@ -719,6 +748,101 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         res.cols.push_back(ColRef{ns, cr.col, cr.type});
     }
     return res;
+}
+
+// The fused probe + materialization applies to a root join whose probe side
+// is a base-table scan (payload pages read in place) and whose output columns
+// are all fixed-width.
+bool Exec::fused_ok(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
+                    const std::vector<uint32_t> &remap) {
+    if (flags & CJ_EXEC_NO_FAST_PATH) return false;
+    if (!P.columnar || n.num_output_attrs == 0 || n.num_output_attrs > kFusedMaxCols) return false;
+    for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
+        int32_t t = n.output_attrs[a].type;
+        if (t == CJ_VARCHAR) return false;
+        const Rel &S = remap[a] < bw ? B : P;
+        const ColRef &cr = S.cols[remap[a] < bw ? remap[a] : remap[a] - bw];
+        if (cr.type != t) return false;  // the general path reports the mismatch
+    }
+    return true;
+}
+
+// Page index of a base column (see fused.hip): cached with resident tables
+// when decoded columns are cached, otherwise private to this call.
+const uint16_t *Exec::page_index(DevColumn *c) {
+    if (c->byte_prefix) return c->byte_prefix;
+    bool own = false;
+    for (auto &o : own_cols) own |= o.get() == c;
+    TimedScope ts(timer, "page_index", c->bitmap_bytes * 3);
+    if (own || ctx.opts.cache_decoded) {
+        std::lock_guard<std::mutex> g(c->mu);
+        if (!c->byte_prefix) {
+            uint16_t *bp = static_cast<uint16_t *>(ctx.pool.alloc(c->bitmap_bytes * 2 + 16));
+            page_byte_prefix(c->pages, c->num_pages, c->byte_start, bp, st);
+            if (!own) CJ_HIP(hipStreamSynchronize(st));
+            c->byte_prefix = bp;
+        }
+        return c->byte_prefix;
+    }
+    auto it = page_indexes.find(c);
+    if (it != page_indexes.end()) return it->second;
+    uint16_t *bp = tmp<uint16_t>(c->bitmap_bytes + 8);
+    page_byte_prefix(c->pages, c->num_pages, c->byte_start, bp, st);
+    page_indexes[c] = bp;
+    return bp;
+}
+
+void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
+                      const std::vector<uint32_t> &remap, uint32_t units, const KeySource &pk, const HashTableDev &ht,
+                      const NLKeys *nl, const uint64_t *cnt, const uint64_t *off, uint64_t total) {
+    uint64_t *p32 = tmp<uint64_t>(units + 1), *p64 = tmp<uint64_t>(units + 1);
+    uint64_t *o32 = tmp<uint64_t>(units + 1), *o64 = tmp<uint64_t>(units + 1);
+    fused_batch_pages(cnt, units, p32, p64, st);
+    scan(p32, o32, units);
+    scan(p64, o64, units);
+    const uint64_t n32 = read_u64(o32 + units), n64 = read_u64(o64 + units);
+    FusedArgs a{};
+    a.key = pk;
+    a.ht = ht;
+    a.nl = nl;
+    a.off_rows = off;
+    a.off_p32 = o32;
+    a.off_p64 = o64;
+    a.ncols = n.num_output_attrs;
+    out_rows = total;
+    uint64_t bytes = key_bytes(pk);
+    for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
+        const int32_t type = n.output_attrs[i].type;
+        const bool fb = remap[i] < bw;
+        const Rel &S = fb ? B : P;
+        const ColRef &cr = S.cols[fb ? remap[i] : remap[i] - bw];
+        DevColumn *c = column(S.slots[cr.slot].table, cr.col);
+        FusedCol &fc = a.cols[i];
+        fc.from_build = fb;
+        fc.width = type == CJ_INT32 ? 4 : 8;
+        fc.is_int32 = type == CJ_INT32;
+        if (fb) {
+            ColView v = view(c);
+            fc.values = v.values;
+            fc.valid_bits = v.valid;
+            fc.rids = S.slots[cr.slot].rids;
+            bytes += total * (uint64_t)fc.width;
+        } else {
+            fc.pages = c->pages;
+            fc.num_pages = c->num_pages;
+            fc.row_start = c->row_start;
+            fc.byte_start = c->byte_start;
+            fc.byte_prefix = page_index(c);
+            bytes += (uint64_t)c->num_pages * kPage;
+        }
+        uint64_t np = fc.width == 4 ? n32 : n64;
+        fc.out = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
+        bytes += total * (uint64_t)fc.width + total / 8;
+        out_types.push_back(type);
+        out_cols.push_back({fc.out, {(uint32_t)np, type}});
+    }
+    TimedScope ts(timer, "fused_probe_materialize", bytes);
+    fused_write(a, units, nl != nullptr, st);
 }
 
 void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
@@ -829,6 +953,7 @@ void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, 
     CJ_HIP(hipSetDevice(device));
     Exec ex(*this, plan, s);
     ex.device_output = (flags & CJ_EXEC_OUTPUT_DEVICE) != 0;
+    ex.flags = flags;
     hipEvent_t e0, e1, e2;
     CJ_HIP(hipEventCreate(&e0));
     CJ_HIP(hipEventCreate(&e1));

@@ -60,6 +60,9 @@ struct DevColumn {
     uint64_t *row_start = nullptr;  // [num_pages + 1] first row of each page
     void *values = nullptr;     // decoded: i32 / u64 per row, VARCHAR refs
     uint32_t *valid = nullptr;  // decoded validity bitmap
+    uint64_t *byte_start = nullptr;   // [num_pages + 1] first bitmap byte of each page
+    uint16_t *byte_prefix = nullptr;  // page index: non-NULL rows before each bitmap byte
+    uint64_t bitmap_bytes = 0;        // sum over pages of ceil(rows / 8)
     bool loaded = false;        // pages + row_start present
     std::mutex mu;              // guards lazy decode
 };

@@ -17,26 +17,10 @@
 // counts, prefix and Bloom tags stay in LDS and the directory is written
 // coalesced.
 #include "cj_internal.hpp"
+#include "bloom.hpp"
 #include "device_util.hpp"
 
 namespace cj {
-
-// Bloom tags: the 2048 smallest 16-bit integers with exactly five bits set,
-// ascending -- the closed form of BLOOM_TAGS (bloom_tags.h:35-263).
-struct BloomTable {
-    uint16_t t[2048];
-};
-constexpr BloomTable make_bloom() {
-    BloomTable b{};
-    uint32_t n = 0;
-    for (uint32_t v = 0; v < 65536u && n < 2048u; ++v) {
-        uint32_t c = 0;
-        for (uint32_t x = v; x; x &= x - 1) ++c;
-        if (c == 5) b.t[n++] = (uint16_t)v;
-    }
-    return b;
-}
-__constant__ BloomTable c_bloom = make_bloom();
 
 constexpr uint32_t kInterChunk = 2048;  // rows of a row-id list per workgroup
 

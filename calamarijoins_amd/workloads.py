@@ -151,6 +151,61 @@ def c2_tables(seed: int = 1002, build_rows: int = C2_BUILD, probe_rows: int = C2
     return b, p
 
 
+C3_FACT = 36_000_000
+C3_DIMS = (2_500_000, 4_000_000, 130_000, 1_200_000)
+
+
+def _unit(seed: int, stream: int) -> float:
+    """A seeded U[0, 1) draw (SplitMix64 of seed and stream)."""
+    def mix(z):
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9 & 0xFFFFFFFFFFFFFFFF
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EB & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+    phi = 0x9E3779B97F4A7C15
+    return (mix((seed ^ (stream * phi & 0xFFFFFFFFFFFFFFFF)) + phi & 0xFFFFFFFFFFFFFFFF) >> 11) / float(1 << 53)
+
+
+def c3_tables(seed: int = 1003, fact_rows: int = C3_FACT, dims=C3_DIMS, threads: int = 0):
+    """Star schema: a fact table with one INT32 foreign key per dimension
+    (uniform over the dimension's unfiltered key domain) and four dimensions
+    with unique shuffled INT32 keys, pre-filtered to a seeded 1-40 % of their
+    rows, each with a VARCHAR payload (length U[5, 60] printable ASCII; one
+    string in 10^5 is 9-20 KiB and goes to long-string pages)."""
+    fact = generate(fact_rows, seed, [
+        col(GEN_UNIFORM, "INT32", 20 + j, 0.0, lo=0, hi=d) for j, d in enumerate(dims)
+    ], threads=threads)
+    out = []
+    keep = []
+    for j, d in enumerate(dims):
+        p = 0.01 + 0.39 * _unit(seed, 900 + j)
+        keep.append(p)
+        out.append(generate(d, seed + 100 * (j + 1), [
+            col(GEN_PERM, "INT32", 1, 0.0, domain=d, perm_stream=50 + j),
+            col(GEN_VARCHAR, "VARCHAR", 2, 0.0, lo=5, hi=60, long_frac=1e-5, long_lo=9216, long_hi=20480),
+        ], keep_frac=p, threads=threads))
+    return fact, out, keep
+
+
+def c3_plan(fact: Table, dims) -> Plan:
+    """(((F x D1) x D2) x D3) x D4, build_left = false, output
+    [D1.s, D2.s, D3.s, F.fk1, F.fk4]."""
+    p = Plan()
+    p.new_input(fact)
+    for d in dims:
+        p.new_input(d)
+    f = p.new_scan_node(0, [(0, "INT32"), (1, "INT32"), (2, "INT32"), (3, "INT32")])
+    ds = [p.new_scan_node(1 + j, [(0, "INT32"), (1, "VARCHAR")]) for j in range(4)]
+    j1 = p.new_join_node(False, f, ds[0], 0, 0,
+                         [(0, "INT32"), (1, "INT32"), (2, "INT32"), (3, "INT32"), (5, "VARCHAR")])
+    j2 = p.new_join_node(False, j1, ds[1], 1, 0,
+                         [(0, "INT32"), (2, "INT32"), (3, "INT32"), (4, "VARCHAR"), (6, "VARCHAR")])
+    j3 = p.new_join_node(False, j2, ds[2], 1, 0,
+                         [(0, "INT32"), (2, "INT32"), (3, "VARCHAR"), (4, "VARCHAR"), (6, "VARCHAR")])
+    p.root = p.new_join_node(False, j3, ds[3], 1, 0,
+                             [(2, "VARCHAR"), (3, "VARCHAR"), (4, "VARCHAR"), (0, "INT32"), (1, "INT32")])
+    return p
+
+
 def c2_plan(build: Table, probe: Table) -> Plan:
     p = Plan()
     p.new_input(build)

@@ -12,6 +12,7 @@ import math
 import os
 import struct
 import subprocess
+import sys
 
 import numpy as np
 
@@ -22,53 +23,9 @@ ORACLE_SO = os.path.join(REPO, "oracle", "libcj_oracle.so")
 PAGE = abi.PAGE_SIZE
 T = abi.TYPE_NAMES  # name -> id
 
-_oracle = None
-
-
-class OracleResult(C.Structure):
-    _fields_ = [
-        ("num_rows", C.c_uint64),
-        ("num_columns", C.c_uint32),
-        ("types", C.POINTER(C.c_int32)),
-        ("num_pages", C.POINTER(C.c_uint32)),
-        ("pages", C.POINTER(C.POINTER(C.c_void_p))),
-    ]
-
-
-def oracle() -> C.CDLL:
-    global _oracle
-    if _oracle is not None:
-        return _oracle
-    if not os.path.exists(ORACLE_SO):
-        subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True, capture_output=True)
-    lib = C.CDLL(ORACLE_SO)
-    P = C.POINTER
-    lib.cjo_encode_fixed.restype = C.c_size_t
-    lib.cjo_encode_fixed.argtypes = [C.c_int32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
-    lib.cjo_encode_varchar.restype = C.c_size_t
-    lib.cjo_encode_varchar.argtypes = [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
-    lib.cjo_decode_fixed.restype = C.c_int64
-    lib.cjo_decode_fixed.argtypes = [C.c_int32, P(C.c_void_p), C.c_uint32, C.c_void_p, C.c_void_p]
-    lib.cjo_decode_varchar.restype = C.c_int64
-    lib.cjo_decode_varchar.argtypes = [P(C.c_void_p), C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
-                                       P(C.c_uint64)]
-    lib.cjo_row_hashes.restype = C.c_int
-    lib.cjo_row_hashes.argtypes = [P(abi.TableDesc), C.c_void_p]
-    lib.cjo_execute.restype = C.c_int
-    lib.cjo_execute.argtypes = [P(abi.PlanDesc), C.c_int, P(OracleResult)]
-    lib.cjo_result_free.restype = None
-    lib.cjo_result_free.argtypes = [P(OracleResult)]
-    lib.cjo_last_error.restype = C.c_char_p
-    lib.cjo_crc32c_u32.restype = C.c_uint32
-    lib.cjo_crc32c_u32.argtypes = [C.c_uint32, C.c_uint32]
-    lib.cjo_crc32c_bytes.restype = C.c_uint32
-    lib.cjo_crc32c_bytes.argtypes = [C.c_uint32, C.c_char_p, C.c_size_t]
-    lib.cjo_bloom_tag.restype = C.c_uint16
-    lib.cjo_bloom_tag.argtypes = [C.c_uint32]
-    lib.cjo_crc32c_hw_mismatches.restype = C.c_int64
-    lib.cjo_crc32c_hw_mismatches.argtypes = [C.c_uint64]
-    _oracle = lib
-    return lib
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+from cj_oracle import OracleResult, OracleRun  # noqa: E402,F401
+from cj_oracle import lib as oracle  # noqa: E402,F401
 
 
 # --------------------------------------------------------------- columns
@@ -206,28 +163,6 @@ def result_row_hashes(r) -> np.ndarray:
     rc = oracle().cjo_row_hashes(C.byref(desc), out.ctypes.data)
     assert rc == 0, oracle().cjo_last_error()
     return np.sort(out[: r.num_rows])
-
-
-class OracleRun:
-    def __init__(self, plan: Plan, threads: int = 0):
-        lib = oracle()
-        self.res = OracleResult()
-        rc = lib.cjo_execute(C.byref(plan.desc()), threads, C.byref(self.res))
-        if rc != 0:
-            raise RuntimeError(f"oracle status {rc}: {lib.cjo_last_error().decode()}")
-        self.num_rows = self.res.num_rows
-        self.types = [self.res.types[i] for i in range(self.res.num_columns)]
-        self.page_lists = [[self.res.pages[i][q] for q in range(self.res.num_pages[i])]
-                           for i in range(self.res.num_columns)]
-
-    def rows(self):
-        return result_rows(self)
-
-    def __del__(self):
-        try:
-            oracle().cjo_result_free(C.byref(self.res))
-        except Exception:
-            pass
 
 
 # ---------------------------------------------------------- KAT loading

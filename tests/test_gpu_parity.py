@@ -30,8 +30,13 @@ def test_gpu_unit_kat(engine, case):
     cjtest.check_kat(case, res)
 
 
+PATHS = [0, abi.CJ_EXEC_NO_FAST_PATH]
+PATH_IDS = ["fused", "general"]
+
+
+@pytest.mark.parametrize("flags", PATHS, ids=PATH_IDS)
 @pytest.mark.parametrize("seed", range(40))
-def test_gpu_random_plans_vs_bruteforce(engine, seed):
+def test_gpu_random_plans_vs_bruteforce(engine, seed, flags):
     rng = random.Random(1000 + seed)
     njoins = 1 + seed % 3
     tables = []
@@ -40,13 +45,14 @@ def test_gpu_random_plans_vs_bruteforce(engine, seed):
         tables.append(to.rand_table(rng, nrows, key_domain=rng.choice([2, 4, 8, 16])))
     plan = to.rand_plan(rng, njoins, tables)
     exp = bruteforce.evaluate(plan.nodes, tables, plan.root)
-    got = engine.execute(plan)
+    got = engine.execute(plan, flags=flags)
     assert got.num_rows == len(exp)
     assert to.norm_rows(cjtest.result_rows(got)) == to.norm_rows(exp)
 
 
+@pytest.mark.parametrize("flags", PATHS, ids=PATH_IDS)
 @pytest.mark.parametrize("seed", range(8))
-def test_gpu_random_plans_vs_oracle_larger(engine, seed):
+def test_gpu_random_plans_vs_oracle_larger(engine, seed, flags):
     rng = random.Random(5000 + seed)
     njoins = 1 + seed % 3
     tables = []
@@ -54,25 +60,37 @@ def test_gpu_random_plans_vs_oracle_larger(engine, seed):
         nrows = rng.choice([500, 3000, 20000])
         tables.append(to.rand_table(rng, nrows, key_domain=rng.choice([100, 2000, 50000])))
     plan = to.rand_plan(rng, njoins, tables)
-    same_multiset(engine.execute(plan), cjtest.OracleRun(plan, threads=8))
+    same_multiset(engine.execute(plan, flags=flags), cjtest.OracleRun(plan, threads=8))
 
 
-def test_gpu_c1_config_bit_exact(engine):
+@pytest.mark.parametrize("flags", PATHS, ids=PATH_IDS)
+def test_gpu_c1_config_bit_exact(engine, flags):
     b, p = workloads.c1_tables()
     plan = workloads.c1_plan(b, p)
-    same_multiset(engine.execute(plan), cjtest.OracleRun(plan, threads=8))
+    same_multiset(engine.execute(plan, flags=flags), cjtest.OracleRun(plan, threads=8))
 
 
-def test_gpu_c2_scaled_bit_exact(engine):
+@pytest.mark.parametrize("flags", PATHS, ids=PATH_IDS)
+def test_gpu_c2_scaled_bit_exact(engine, flags):
     b, p = workloads.c2_tables(build_rows=200_000, probe_rows=3_200_000)
     plan = workloads.c2_plan(b, p)
-    got = engine.execute(plan)
+    got = engine.execute(plan, flags=flags)
     exp = cjtest.OracleRun(plan, threads=8)
     same_multiset(got, exp)
     # each output row's payloads come from rows whose keys matched: all 5
     # columns present, and the row count equals the non-miss probe rows
     pk = cjtest.decode_column_arrays(abi.CJ_INT32, p.columns[0].page_ptrs(), p.columns[0].num_pages)[0]
     assert got.num_rows == int(np.sum(pk < 200_000))
+
+
+@pytest.mark.parametrize("scale", [100, 10])
+def test_gpu_c3_star_pipeline_bit_exact(engine, scale):
+    """Left-deep 4-join star with VARCHAR (incl. long strings) through
+    late-materialized intermediates."""
+    f, ds, _ = workloads.c3_tables(fact_rows=workloads.C3_FACT // scale,
+                                   dims=tuple(d // scale for d in workloads.C3_DIMS))
+    plan = workloads.c3_plan(f, ds)
+    same_multiset(engine.execute(plan), cjtest.OracleRun(plan, threads=8))
 
 
 def test_gpu_resident_tables_and_device_output(engine):
