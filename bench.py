@@ -3,9 +3,11 @@
 MI355X through the C ABI, base-table pages resident in HBM, result pages left
 in HBM.
 
-One step = one cj_execute() of the plan (page decode, radix partition, hash
-build, probe count + write, page materialization of 5 payload columns).
-Prints one JSON line (see the round contract in DESIGN.md).
+One step = one cj_execute() of the plan: key-column decode and the build
+structure (direct map for the unique dense build keys, else radix-partitioned
+hash table), probe count + pair write, and the page writers of the 5 payload
+columns.  Prints one JSON line with the step throughput, the roofline of the
+dominant kernel and a CPU baseline.
 """
 import argparse
 import json
@@ -79,7 +81,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md but not built)", file=sys.stderr)
+              "(one process drives one GPU; sharding across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
 
     from calamarijoins_amd import abi, workloads

@@ -630,10 +630,12 @@ class Exec {
     Rel join(const cj_node_desc &n, bool is_root, int depth, cj_result *out);
     bool fused_ok(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw, const std::vector<uint32_t> &remap);
     void fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw, const std::vector<uint32_t> &remap,
-                    uint32_t units, const KeySource &pk, const HashTableDev &ht, const NLKeys *nl,
-                    const uint64_t *cnt, const uint64_t *off, uint64_t total);
+                    const ProbeArgs &a, const uint64_t *off, uint64_t total);
+    bool try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint32_t bw,
+                    const std::vector<uint32_t> &remap, DirectTable *dt);
     const uint16_t *page_index(DevColumn *c);
     uint32_t flags = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> direct_field;  // per output attr: record offset, flag bit
     void materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
                           const std::vector<uint32_t> &remap, uint64_t total, const uint32_t *ob, const uint32_t *op,
                           cj_result *out);
@@ -676,29 +678,39 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
     uint32_t *ob = nullptr, *op = nullptr;
     const bool empty = B.rows == 0 || P.rows == 0;
     if (!empty) {
+        const bool fused = is_root && fused_ok(n, B, P, bw, remap);
+        ProbeArgs fa{};
+        fa.mode = kModeHash;
         if (use_nl) {
             nl = tmp<NLKeys>(1);
             TimedScope ts(timer, "nl_collect", 0);
             nl_collect(bk, nl, st);
-        } else {
+            fa.mode = kModeNL;
+        } else if (!(fused && B.columnar && try_direct(n, B, battr, bw, remap, &fa.dt))) {
             // tuple capacity: the rows the key pages actually hold (the
             // declared num_rows only sizes the directory)
             uint64_t cap = B.rows;
             if (B.columnar) cap = column(B.slots[0].table, B.cols[battr].col)->num_rows;
             ht = build_hash(bk, B.rows, cap);
+        } else {
+            fa.mode = kModeDirect;
         }
+        fa.key = pk;
+        fa.ht = ht;
+        fa.nl = nl;
         uint32_t units = key_source_units(pk);
         uint64_t *cnt = tmp<uint64_t>(units + 1);
         uint64_t *off = tmp<uint64_t>(units + 1);
         {
             TimedScope ts(timer, use_nl ? "nl_probe_count" : "probe_count", key_bytes(pk));
-            probe_count(pk, use_nl ? nullptr : &ht, nl, cnt, st);
+            if (pk.mode == KEY_COLUMNAR) probe_count_columnar(fa, cnt, st);
+            else probe_count(pk, use_nl ? nullptr : &ht, nl, cnt, st);
         }
         scan(cnt, off, units);
         total = read_u64(off + units);
         if (total > 0xffffffffull) throw UnsupportedError("join result exceeds 2^32 rows");
-        if (total && is_root && fused_ok(n, B, P, bw, remap)) {
-            fused_root(n, B, P, bw, remap, units, pk, ht, nl, cnt, off, total);
+        if (total && fused) {
+            fused_root(n, B, P, bw, remap, fa, off, total);
             return Rel{};
         }
         if (total) {
@@ -756,7 +768,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
 bool Exec::fused_ok(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
                     const std::vector<uint32_t> &remap) {
     if (flags & CJ_EXEC_NO_FAST_PATH) return false;
-    if (!P.columnar || n.num_output_attrs == 0 || n.num_output_attrs > kFusedMaxCols) return false;
+    if (!P.columnar || n.num_output_attrs == 0 || n.num_output_attrs > kMatMaxCols) return false;
     for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
         int32_t t = n.output_attrs[a].type;
         if (t == CJ_VARCHAR) return false;
@@ -792,57 +804,143 @@ const uint16_t *Exec::page_index(DevColumn *c) {
     return bp;
 }
 
-void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
-                      const std::vector<uint32_t> &remap, uint32_t units, const KeySource &pk, const HashTableDev &ht,
-                      const NLKeys *nl, const uint64_t *cnt, const uint64_t *off, uint64_t total) {
-    uint64_t *p32 = tmp<uint64_t>(units + 1), *p64 = tmp<uint64_t>(units + 1);
-    uint64_t *o32 = tmp<uint64_t>(units + 1), *o64 = tmp<uint64_t>(units + 1);
-    fused_batch_pages(cnt, units, p32, p64, st);
-    scan(p32, o32, units);
-    scan(p64, o64, units);
-    const uint64_t n32 = read_u64(o32 + units), n64 = read_u64(o64 + units);
-    FusedArgs a{};
-    a.key = pk;
-    a.ht = ht;
-    a.nl = nl;
-    a.off_rows = off;
-    a.off_p32 = o32;
-    a.off_p64 = o64;
-    a.ncols = n.num_output_attrs;
-    out_rows = total;
-    uint64_t bytes = key_bytes(pk);
-    for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
-        const int32_t type = n.output_attrs[i].type;
-        const bool fb = remap[i] < bw;
-        const Rel &S = fb ? B : P;
-        const ColRef &cr = S.cols[fb ? remap[i] : remap[i] - bw];
-        DevColumn *c = column(S.slots[cr.slot].table, cr.col);
-        FusedCol &fc = a.cols[i];
-        fc.from_build = fb;
-        fc.width = type == CJ_INT32 ? 4 : 8;
-        fc.is_int32 = type == CJ_INT32;
-        if (fb) {
-            ColView v = view(c);
-            fc.values = v.values;
-            fc.valid_bits = v.valid;
-            fc.rids = S.slots[cr.slot].rids;
-            bytes += total * (uint64_t)fc.width;
-        } else {
-            fc.pages = c->pages;
-            fc.num_pages = c->num_pages;
-            fc.row_start = c->row_start;
-            fc.byte_start = c->byte_start;
-            fc.byte_prefix = page_index(c);
-            bytes += (uint64_t)c->num_pages * kPage;
-        }
-        uint64_t np = fc.width == 4 ? n32 : n64;
-        fc.out = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
-        bytes += total * (uint64_t)fc.width + total / 8;
-        out_types.push_back(type);
-        out_cols.push_back({fc.out, {(uint32_t)np, type}});
+// Direct-mapped build side (direct.hip) for a columnar build whose non-NULL
+// keys are unique and span at most ~2x its rows.  Returns false (and the
+// caller builds the hash table) when the keys are too sparse or repeat.
+bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint32_t bw,
+                      const std::vector<uint32_t> &remap, DirectTable *dt) {
+    if (flags & CJ_EXEC_NO_DIRECT) return false;
+    DevColumn *kc = column(B.slots[0].table, B.cols[battr].col);
+    if (kc->num_rows == 0) return false;
+    ColView kv = view(kc);
+    int32_t *mm = tmp<int32_t>(2);
+    fill_u32(reinterpret_cast<uint32_t *>(mm), 0x7fffffffu, 1, st);
+    fill_u32(reinterpret_cast<uint32_t *>(mm) + 1, 0x80000000u, 1, st);
+    {
+        TimedScope ts(timer, "key_minmax", kc->num_rows * 4);
+        key_minmax(static_cast<const int32_t *>(kv.values), kv.valid, kc->num_rows, mm, st);
     }
-    TimedScope ts(timer, "fused_probe_materialize", bytes);
-    fused_write(a, units, nl != nullptr, st);
+    int32_t h[2];
+    CJ_HIP(hipMemcpyAsync(h, mm, 8, hipMemcpyDeviceToHost, st));
+    CJ_HIP(hipStreamSynchronize(st));
+    if (h[0] > h[1]) return false;  // no non-NULL key
+    const uint64_t range = (uint64_t)((int64_t)h[1] - (int64_t)h[0]) + 1;
+    if (range > 2 * kc->num_rows + 1024) return false;
+    // record: 8-byte fields, then 4-byte fields, then the u32 of NULL flags
+    DirectBuildArgs a{};
+    std::vector<uint32_t> build_attrs;
+    for (uint32_t i = 0; i < n.num_output_attrs; ++i)
+        if (remap[i] < bw) build_attrs.push_back(i);
+    if (build_attrs.size() > kDirectMaxFields) return false;
+    direct_field.assign(n.num_output_attrs, {0u, 0u});
+    uint32_t off = 0;
+    for (int pass = 0; pass < 2; ++pass)
+        for (uint32_t f = 0; f < build_attrs.size(); ++f) {
+            uint32_t i = build_attrs[f];
+            int32_t type = n.output_attrs[i].type;
+            uint32_t w = type == CJ_INT32 ? 4 : 8;
+            if ((pass == 0) != (w == 8)) continue;
+            const ColRef &cr = B.cols[remap[i]];
+            ColView v = view(column(B.slots[0].table, cr.col));
+            a.fields[f] = DirectField{(int32_t)w, off, v.values, v.valid};
+            direct_field[i] = {off, f};
+            off += w;
+        }
+    a.nfields = (uint32_t)build_attrs.size();
+    a.flags_offset = off;
+    a.stride = (off + 4 + 7) & ~7u;
+    a.keys = static_cast<const int32_t *>(kv.values);
+    a.key_valid = kv.valid;
+    a.rows = kc->num_rows;
+    a.min_key = h[0];
+    const uint64_t words = range / 32 + 2;
+    a.presence = tmp<uint32_t>(words);
+    a.records = tmp<uint8_t>(range * a.stride);
+    a.dup_flag = tmp<uint32_t>(1);
+    fill_u32(a.presence, 0, words, st);
+    fill_u32(a.dup_flag, 0, 1, st);
+    {
+        TimedScope ts(timer, "direct_build", kc->num_rows * (8 + a.stride));
+        direct_build(a, st);
+    }
+    uint32_t dup = 0;
+    CJ_HIP(hipMemcpyAsync(&dup, a.dup_flag, 4, hipMemcpyDeviceToHost, st));
+    CJ_HIP(hipStreamSynchronize(st));
+    if (dup) return false;
+    *dt = DirectTable{h[0], range, a.presence, a.records, a.stride, a.flags_offset};
+    return true;
+}
+
+void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
+                      const std::vector<uint32_t> &remap, const ProbeArgs &a, const uint64_t *off, uint64_t total) {
+    const bool direct = a.mode == kModeDirect;
+    bool need_b = false, need_p = false;
+    for (uint32_t i = 0; i < n.num_output_attrs; ++i) (remap[i] < bw ? need_b : need_p) = true;
+    uint32_t *ob = need_b ? tmp<uint32_t>(total) : nullptr;
+    uint32_t *op = need_p ? tmp<uint32_t>(total) : nullptr;
+    {
+        TimedScope ts(timer, "probe_write", key_bytes(a.key) + 4 * total * ((uint64_t)need_b + need_p));
+        probe_write_columnar(a, off, ob, op, st);
+    }
+    out_rows = total;
+    out_types.resize(n.num_output_attrs);
+    out_cols.resize(n.num_output_attrs);
+    // one page writer per (side, width class) covering all its columns
+    for (int side = 0; side < 2; ++side)
+        for (int width = 4; width <= 8; width += 4) {
+            MatArgs m{};
+            m.from_build = side == 0;
+            m.direct = m.from_build && direct;
+            m.dt = a.dt;
+            m.ids = m.from_build ? ob : op;
+            m.rows = total;
+            m.rows_per_page = width == 4 ? kRowsPage4 : kRowsPage8;
+            const uint64_t np = (total + m.rows_per_page - 1) / m.rows_per_page;
+            uint64_t bytes = 4 * total;
+            for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
+                const int32_t type = n.output_attrs[i].type;
+                const bool fb = remap[i] < bw;
+                if (fb != m.from_build || (type == CJ_INT32 ? 4 : 8) != width) continue;
+                const Rel &S = fb ? B : P;
+                const ColRef &cr = S.cols[fb ? remap[i] : remap[i] - bw];
+                DevColumn *c = column(S.slots[cr.slot].table, cr.col);
+                MatCol &mc = m.cols[m.ncols++];
+                mc.width = width;
+                mc.is_int32 = type == CJ_INT32;
+                if (m.direct) {
+                    mc.rec_offset = direct_field[i].first;
+                    mc.rec_bit = direct_field[i].second;
+                    bytes += total * (uint64_t)width;
+                } else if (fb) {
+                    ColView v = view(c);
+                    mc.values = v.values;
+                    mc.valid_bits = v.valid;
+                    mc.rids = S.slots[cr.slot].rids;
+                    bytes += total * (uint64_t)width + total / 8;
+                } else {
+                    mc.pages = c->pages;
+                    mc.num_pages = c->num_pages;
+                    mc.row_start = c->row_start;
+                    mc.byte_start = c->byte_start;
+                    mc.byte_prefix = page_index(c);
+                    uint32_t *wf = tmp<uint32_t>(np + 1);
+                    {
+                        TimedScope ts(timer, "window_first", np * 12);
+                        window_first(op, total, m.rows_per_page, c->row_start, c->num_pages, wf, st);
+                    }
+                    mc.win_first = wf;
+                    bytes += (uint64_t)c->num_pages * kPage;
+                }
+                mc.out = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
+                bytes += total * (uint64_t)width + total / 8;
+                out_types[i] = type;
+                out_cols[i] = {mc.out, {(uint32_t)np, type}};
+            }
+            if (!m.ncols) continue;
+            static const char *names[2][2] = {{"mat_build_w4", "mat_build_w8"}, {"mat_probe_w4", "mat_probe_w8"}};
+            TimedScope ts(timer, names[side][width / 8], bytes);
+            materialize_side(m, st);
+        }
 }
 
 void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,

@@ -1,0 +1,190 @@
+// probe8.hpp -- device helpers of the probe and root-materialization kernels:
+// 2048-row tiles with 8 rows per thread, ranks from wave ballots, probe-key
+// lookups against the three build structures, and payload page windows.
+#pragma once
+#include "bloom.hpp"
+#include "cj_internal.hpp"
+#include "device_util.hpp"
+#include "fused.hpp"
+
+namespace cj {
+
+constexpr uint32_t kRPT = 8;           // rows per thread
+constexpr uint32_t kTile = kWG * kRPT;  // rows per workgroup tile
+
+// A tile gives thread t the rows t, t + 256, ..., t + 256 (S - 1), so every
+// load or store instruction of a wave covers 64 consecutive rows.  rank_slots
+// ranks S predicates per thread in row order (S wave ballots, one LDS
+// exchange of the 4 S wave counts); `lds32` holds 32 u32; all threads must
+// call it.  mask[j] is the wave's ballot of slot j.  Returns the total.
+template <uint32_t S>
+__device__ __forceinline__ uint32_t rank_slots(uint32_t pred, uint32_t rank[S], uint64_t mask[S], uint32_t *lds32) {
+    const uint32_t w = wave_id(), lane = lane_id();
+#pragma unroll
+    for (uint32_t j = 0; j < S; ++j) mask[j] = __ballot((pred >> j) & 1u);
+    if (lane == 0) {
+#pragma unroll
+        for (uint32_t j = 0; j < S; ++j) lds32[4 * j + w] = (uint32_t)__popcll(mask[j]);
+    }
+    __syncthreads();
+    uint32_t run = 0, base[S];
+#pragma unroll
+    for (uint32_t j = 0; j < S; ++j) {
+#pragma unroll
+        for (uint32_t ww = 0; ww < 4; ++ww) {
+            if (ww == w) base[j] = run;
+            run += lds32[4 * j + ww];
+        }
+    }
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (uint32_t j = 0; j < S; ++j) rank[j] = base[j] + (uint32_t)__popcll(mask[j] & lt);
+    return run;
+}
+
+// Keys of this thread's 8 rows (t0 + 256 j + t) of a key page.  Bit j of
+// the returned mask: the row exists and its key is not NULL.
+__device__ __forceinline__ uint32_t load_keys8(const uint8_t *p, uint32_t n, uint32_t nv, bool dense, uint32_t t0,
+                                               uint32_t *running, int32_t key[kRPT], uint32_t *lds32) {
+    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+    uint32_t valid = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) {
+        const uint32_t r = t0 + kWG * j + threadIdx.x;
+        if (r < n && (dense || page_bit(bm, r))) valid |= 1u << j;
+    }
+    uint32_t vi[kRPT];
+    if (dense) {
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j) vi[j] = t0 + kWG * j + threadIdx.x;
+    } else {
+        uint64_t mask[kRPT];
+        const uint32_t tot = rank_slots<kRPT>(valid, vi, mask, lds32);
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j) vi[j] += *running;
+        *running += tot;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) {
+        key[j] = 0;
+        if ((valid >> j) & 1u) {
+            if (vi[j] < nv) key[j] = ld32(p + 4 + 4 * (size_t)vi[j]);
+            else valid &= ~(1u << j);  // malformed bitmap: stay inside the value area
+        }
+    }
+    return valid;
+}
+
+// Lookups of 8 keys, all loads issued before any is used.  m[j] = matches of
+// key j; hash mode: [s[j], e[j]) is its directory range; direct mode: s[j]
+// is its record slot.
+template <int MODE>
+__device__ __forceinline__ void lookup8(uint32_t valid, const int32_t key[kRPT], const HashTableDev &ht,
+                                        const NLKeys &nl, const DirectTable &dt, uint32_t m[kRPT], uint32_t s[kRPT],
+                                        uint32_t e[kRPT]) {
+    if (MODE == kModeDirect) {
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j) {
+            int64_t off = (int64_t)key[j] - (int64_t)dt.min_key;
+            bool in = ((valid >> j) & 1u) && off >= 0 && (uint64_t)off < dt.range;
+            s[j] = in ? (uint32_t)off : 0u;
+            e[j] = 0;
+            m[j] = in ? (dt.presence[(uint64_t)off >> 5] >> (off & 31)) & 1u : 0u;
+        }
+        return;
+    }
+    if (MODE == kModeNL) {
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j) {
+            m[j] = 0;
+            s[j] = e[j] = 0;
+            if ((valid >> j) & 1u)
+                for (int i = 0; i < nl.count; ++i) m[j] += nl.keys[i] == key[j];
+        }
+        return;
+    }
+    uint64_t slot[kRPT], d[kRPT];
+    uint32_t tag[kRPT];
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) {
+        uint64_t h = hash_key(key[j]);
+        slot[j] = h >> (64 - ht.log_slots);
+        tag[j] = c_bloom.t[h & 0x7FF];
+        d[j] = ((valid >> j) & 1u) ? ht.dir[slot[j]] : 0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) {
+        bool pass = ((valid >> j) & 1u) && (d[j] & tag[j]) == tag[j];
+        e[j] = pass ? (uint32_t)(d[j] >> 16) : 0;
+        s[j] = pass && slot[j] ? (uint32_t)(ht.dir[slot[j] - 1] >> 16) : 0;
+        if (!pass) s[j] = e[j] = 0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) {
+        uint32_t c = 0;
+        for (uint32_t i = s[j]; i < e[j]; ++i) c += ht.keys[i] == key[j];
+        m[j] = c;
+    }
+}
+
+// ---- payload page windows -------------------------------------------------
+constexpr uint32_t kWin = 32;  // payload pages a window may hold
+
+struct WinState {
+    uint64_t rs[kWin + 1];  // row starts of the window pages (+ end)
+    uint16_t nrows[kWin];   // page headers of the window pages
+    uint16_t nvals[kWin];
+    uint32_t first;         // first page of the window
+    uint32_t count;         // pages in the window (0: window overflowed)
+};
+
+// Wave-cooperative set-up of the window of pages holding rows [r0, r1]
+// starting at page `first` (the page holding r0).  Called by one wave.
+__device__ __forceinline__ void window_setup(WinState &win, const uint8_t *pages, uint32_t num_pages,
+                                             const uint64_t *row_start, uint32_t first, uint64_t r_last) {
+    const uint32_t lane = lane_id(), p = first + lane;
+    const uint64_t rs = (lane <= kWin && p <= num_pages) ? row_start[p] : ~0ull;
+    const bool in = lane <= kWin && p < num_pages && rs <= r_last;  // pages are in row order
+    const uint64_t bal = __ballot(in);
+    const uint32_t cw = (uint32_t)__popcll(bal & 0xffffffffull);
+    const bool overflow = (bal >> kWin) & 1ull;
+    if (lane <= cw) win.rs[lane] = rs;
+    if (lane < cw) {
+        const uint8_t *wp = pages + (size_t)p * kPage;
+        win.nrows[lane] = ld16(wp);
+        win.nvals[lane] = ld16(wp + 2);
+    }
+    if (lane == 0) {
+        win.first = first;
+        win.count = overflow ? 0 : cw;
+    }
+}
+
+// Page of `row` (window first, else a binary search), its first row and header.
+__device__ __forceinline__ uint32_t find_page(const uint8_t *pages, uint32_t num_pages, const uint64_t *row_start,
+                                              const WinState &w, uint64_t row, uint64_t *rs, uint32_t *n,
+                                              uint32_t *nv) {
+    if (w.count) {
+        uint32_t k = 0;
+        while (k + 1 < w.count && w.rs[k + 1] <= row) ++k;
+        *rs = w.rs[k];
+        *n = w.nrows[k];
+        *nv = w.nvals[k];
+        return w.first + k;
+    }
+    uint32_t lo = 0, hi = num_pages;  // upper_bound(row_start, row) - 1
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (row_start[mid + 1] <= row) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t pg = lo < num_pages ? lo : num_pages - 1;
+    const uint8_t *p = pages + (size_t)pg * kPage;
+    *rs = row_start[pg];
+    *n = ld16(p);
+    *nv = ld16(p + 2);
+    return pg;
+}
+
+}  // namespace cj
