@@ -67,6 +67,23 @@ def cpu_baseline(build, probe_rows, seed, threads, repeats):
     }
 
 
+COPY_BYTES = 1 << 30
+
+
+def copy_calibration():
+    """Three device-to-device copies of COPY_BYTES: rocprofv3's FETCH_SIZE and
+    WRITE_SIZE of this copy kernel should read COPY_BYTES / 1024 KiB each, and
+    the ratio scales the same counters of the join kernels to bytes."""
+    import torch
+    src = torch.empty(COPY_BYTES, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "kib": COPY_BYTES // 1024}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,11 +95,16 @@ def main():
     ap.add_argument("--cpu-sample-rows", type=int, default=16_000_000)
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(one process drives one GPU; sharding across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     from calamarijoins_amd import abi, workloads
     from calamarijoins_amd.host import Engine

@@ -18,6 +18,7 @@ CJ_NODE_JOIN, CJ_NODE_SCAN = 0, 1
 CJ_EXEC_OUTPUT_DEVICE = 1
 CJ_EXEC_NO_FAST_PATH = 2
 CJ_EXEC_NO_DIRECT = 4
+CJ_EXEC_NO_ALIGNED = 8
 
 TYPE_NAMES = {"INT32": CJ_INT32, "INT64": CJ_INT64, "FP64": CJ_FP64, "VARCHAR": CJ_VARCHAR}
 TYPE_IDS = {v: k for k, v in TYPE_NAMES.items()}

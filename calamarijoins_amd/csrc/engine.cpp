@@ -17,6 +17,7 @@
 // output values equal to INT32_MIN come out NULL (page_builders.h:123).
 #include "engine.hpp"
 #include "fused.hpp"
+#include "pagemat.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -269,7 +270,10 @@ void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s
         rs[i] = run;
         bs[i] = bytes;
         run += host_rows_of(p);
-        if (prev < 0xfffe) bytes += (prev + 7u) / 8u;
+        if (prev < 0xfffe) {
+            bytes += (prev + 7u) / 8u;
+            c.max_page_rows = std::max<uint32_t>(c.max_page_rows, prev);
+        }
     }
     if (run > 0xffffffffull) throw UnsupportedError("column has more than 2^32 rows");
     rs[d.num_pages] = run;
@@ -542,6 +546,20 @@ class Exec {
         CJ_HIP(hipStreamSynchronize(st));
         return v;
     }
+    // Several device counters with one read-back.
+    std::vector<uint64_t> read_u64s(const std::vector<const uint64_t *> &ptrs) {
+        std::vector<uint64_t> v(ptrs.size());
+        for (size_t i = 0; i < ptrs.size(); i += 16) {
+            U64List l{};
+            l.n = (uint32_t)std::min<size_t>(16, ptrs.size() - i);
+            for (uint32_t k = 0; k < l.n; ++k) l.p[k] = ptrs[i + k];
+            uint64_t *d = tmp<uint64_t>(16);
+            gather_u64s(l, d, st);
+            CJ_HIP(hipMemcpyAsync(v.data() + i, d, 8 * l.n, hipMemcpyDeviceToHost, st));
+        }
+        CJ_HIP(hipStreamSynchronize(st));
+        return v;
+    }
 
     Rel scan_rel(const cj_node_desc &n) {
         if (n.base_table_id >= plan.num_inputs) throw InvalidError("scan base_table_id out of range");
@@ -631,6 +649,9 @@ class Exec {
     bool fused_ok(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw, const std::vector<uint32_t> &remap);
     void fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw, const std::vector<uint32_t> &remap,
                     const ProbeArgs &a, const uint64_t *off, uint64_t total);
+    void aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
+                      const std::vector<uint32_t> &remap, const ProbeArgs &a, const uint64_t *cnt,
+                      const uint32_t *match_bits, uint64_t total);
     bool try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint32_t bw,
                     const std::vector<uint32_t> &remap, DirectTable *dt);
     const uint16_t *page_index(DevColumn *c);
@@ -701,14 +722,33 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         uint32_t units = key_source_units(pk);
         uint64_t *cnt = tmp<uint64_t>(units + 1);
         uint64_t *off = tmp<uint64_t>(units + 1);
+        // page-aligned root output (pagemat.hpp): every probe row matches at
+        // most once, so the count pass also records which rows matched
+        const bool aligned = fused && fa.mode == kModeDirect && pk.mode == KEY_COLUMNAR &&
+                             !(flags & CJ_EXEC_NO_ALIGNED);
+        uint32_t *match_bits = nullptr;
+        const uint64_t probe_rows = pk.mode == KEY_COLUMNAR ? column(P.slots[0].table, P.cols[pattr].col)->num_rows : 0;
+        if (aligned) {
+            const uint64_t words = probe_rows / 32 + 2;
+            match_bits = tmp<uint32_t>(words);
+            fill_u32(match_bits, 0, words, st);
+        }
         {
             TimedScope ts(timer, use_nl ? "nl_probe_count" : "probe_count", key_bytes(pk));
-            if (pk.mode == KEY_COLUMNAR) probe_count_columnar(fa, cnt, st);
+            if (aligned) probe_count_match(fa, cnt, match_bits, st);
+            else if (pk.mode == KEY_COLUMNAR) probe_count_columnar(fa, cnt, st);
             else probe_count(pk, use_nl ? nullptr : &ht, nl, cnt, st);
         }
         scan(cnt, off, units);
         total = read_u64(off + units);
         if (total > 0xffffffffull) throw UnsupportedError("join result exceeds 2^32 rows");
+        // dense matching (at least half the probe rows) keeps the filtered
+        // pages at least half full on average; sparser results take the
+        // block writers, which fill every page
+        if (total && aligned && 2 * total >= probe_rows) {
+            aligned_root(n, B, P, bw, remap, fa, cnt, match_bits, total);
+            return Rel{};
+        }
         if (total && fused) {
             fused_root(n, B, P, bw, remap, fa, off, total);
             return Rel{};
@@ -848,7 +888,7 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
         }
     a.nfields = (uint32_t)build_attrs.size();
     a.flags_offset = off;
-    a.stride = (off + 4 + 7) & ~7u;
+    a.stride = (off + 4 + 15) & ~15u;  // whole 16-byte chunks: one vector load each
     a.keys = static_cast<const int32_t *>(kv.values);
     a.key_valid = kv.valid;
     a.rows = kc->num_rows;
@@ -871,6 +911,16 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     return true;
 }
 
+// Columns one page-writer workgroup handles: one column per workgroup keeps
+// registers low and puts more independent workgroups in flight; the ids of a
+// block are then read once per column, mostly from L2.
+static uint32_t mat_cols_per_wg(bool from_build, uint32_t width, uint32_t ncols) {
+    (void)from_build;
+    (void)width;
+    (void)ncols;
+    return 1;
+}
+
 void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
                       const std::vector<uint32_t> &remap, const ProbeArgs &a, const uint64_t *off, uint64_t total) {
     const bool direct = a.mode == kModeDirect;
@@ -885,32 +935,33 @@ void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_
     out_rows = total;
     out_types.resize(n.num_output_attrs);
     out_cols.resize(n.num_output_attrs);
-    // one page writer per (side, width class) covering all its columns
+    // page writers per (side, width); each workgroup writes cols_per_wg
+    // columns of one block of result rows
+    const uint64_t blocks = (total + kBlockRows - 1) / kBlockRows;
     for (int side = 0; side < 2; ++side)
-        for (int width = 4; width <= 8; width += 4) {
+        for (uint32_t width = 4; width <= 8; width += 4) {
             MatArgs m{};
             m.from_build = side == 0;
             m.direct = m.from_build && direct;
             m.dt = a.dt;
             m.ids = m.from_build ? ob : op;
             m.rows = total;
-            m.rows_per_page = width == 4 ? kRowsPage4 : kRowsPage8;
-            const uint64_t np = (total + m.rows_per_page - 1) / m.rows_per_page;
-            uint64_t bytes = 4 * total;
+            uint64_t bytes = 0;
             for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
                 const int32_t type = n.output_attrs[i].type;
                 const bool fb = remap[i] < bw;
-                if (fb != m.from_build || (type == CJ_INT32 ? 4 : 8) != width) continue;
+                if (fb != (bool)m.from_build || (type == CJ_INT32 ? 4u : 8u) != width) continue;
                 const Rel &S = fb ? B : P;
                 const ColRef &cr = S.cols[fb ? remap[i] : remap[i] - bw];
                 DevColumn *c = column(S.slots[cr.slot].table, cr.col);
                 MatCol &mc = m.cols[m.ncols++];
-                mc.width = width;
+                mc.width = (int32_t)width;
                 mc.is_int32 = type == CJ_INT32;
+                bytes += 4 * total;  // ids
                 if (m.direct) {
                     mc.rec_offset = direct_field[i].first;
                     mc.rec_bit = direct_field[i].second;
-                    bytes += total * (uint64_t)width;
+                    bytes += total * (uint64_t)(width + 4);  // field + NULL flags
                 } else if (fb) {
                     ColView v = view(c);
                     mc.values = v.values;
@@ -923,24 +974,109 @@ void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_
                     mc.row_start = c->row_start;
                     mc.byte_start = c->byte_start;
                     mc.byte_prefix = page_index(c);
-                    uint32_t *wf = tmp<uint32_t>(np + 1);
+                    uint32_t *wf = tmp<uint32_t>(blocks + 1);
                     {
-                        TimedScope ts(timer, "window_first", np * 12);
-                        window_first(op, total, m.rows_per_page, c->row_start, c->num_pages, wf, st);
+                        TimedScope ts(timer, "window_first", blocks * 12);
+                        window_first(op, total, kBlockRows, c->row_start, c->num_pages, wf, st);
                     }
                     mc.win_first = wf;
                     bytes += (uint64_t)c->num_pages * kPage;
                 }
-                mc.out = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
+                // INT32: page k per block; 8-byte: pages 2k, 2k + 1 (the last may be unused)
+                const uint64_t np = width == 4 ? blocks : (total + kRowsPage8 - 1) / kRowsPage8;
+                mc.out = static_cast<uint8_t *>(ctx.pool.alloc((size_t)(width == 4 ? blocks : 2 * blocks) * kPage + 16));
                 bytes += total * (uint64_t)width + total / 8;
                 out_types[i] = type;
                 out_cols[i] = {mc.out, {(uint32_t)np, type}};
             }
             if (!m.ncols) continue;
+            m.cols_per_wg = mat_cols_per_wg(m.from_build != 0, width, m.ncols);
             static const char *names[2][2] = {{"mat_build_w4", "mat_build_w8"}, {"mat_probe_w4", "mat_probe_w8"}};
             TimedScope ts(timer, names[side][width / 8], bytes);
             materialize_side(m, st);
         }
+}
+
+// Page-aligned root output (pagemat.hip): build-side pages per key page of
+// the probe side, probe-side pages per payload page.  Page counts are read
+// back once, together, after the writers have been queued.
+void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
+                        const std::vector<uint32_t> &remap, const ProbeArgs &a, const uint64_t *cnt,
+                        const uint32_t *match_bits, uint64_t total) {
+    out_rows = total;
+    out_types.assign(n.num_output_attrs, 0);
+    out_cols.assign(n.num_output_attrs, {nullptr, {0u, 0}});
+    const uint32_t kp = a.key.num_pages;
+    std::vector<const uint64_t *> count_at;  // device page count per output column
+    std::vector<uint32_t> count_col;
+    // ---- build side
+    BuildPagesArgs ba{};
+    ba.key = a.key;
+    ba.dt = a.dt;
+    uint64_t bbytes = key_bytes(a.key) + total * a.dt.stride;
+    for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
+        if (remap[i] >= bw) continue;
+        if (!ba.p4_off) {
+            uint64_t *p4 = tmp<uint64_t>(kp + 1), *p8 = tmp<uint64_t>(kp + 1);
+            uint64_t *o4 = tmp<uint64_t>(kp + 1), *o8 = tmp<uint64_t>(kp + 1);
+            key_page_out_pages(cnt, kp, p4, p8, st);
+            scan(p4, o4, kp);
+            scan(p8, o8, kp);
+            ba.p4_off = o4;
+            ba.p8_off = o8;
+        }
+        const int32_t type = n.output_attrs[i].type;
+        const uint32_t width = type == CJ_INT32 ? 4 : 8;
+        // at most one partial page per key page beyond the full ones
+        const uint64_t bound = total / (width == 4 ? kCap4 : kCap8) + kp + 1;
+        BuildPageCol &c = ba.cols[ba.ncols++];
+        c.width = (int32_t)width;
+        c.is_int32 = type == CJ_INT32;
+        c.rec_offset = direct_field[i].first;
+        c.rec_bit = direct_field[i].second;
+        c.out = static_cast<uint8_t *>(ctx.pool.alloc((size_t)bound * kPage + 16));
+        bbytes += total * (uint64_t)width + total / 8;
+        out_types[i] = type;
+        out_cols[i] = {c.out, {0u, type}};
+        count_at.push_back((width == 4 ? ba.p4_off : ba.p8_off) + kp);
+        count_col.push_back(i);
+    }
+    if (ba.ncols) {
+        TimedScope ts(timer, "build_pages", bbytes);
+        build_pages(ba, st);
+    }
+    // ---- probe side: one filtered output page per payload page with a match
+    for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
+        if (remap[i] < bw) continue;
+        const int32_t type = n.output_attrs[i].type;
+        const ColRef &cr = P.cols[remap[i] - bw];
+        DevColumn *c = column(P.slots[cr.slot].table, cr.col);
+        uint64_t *flag = tmp<uint64_t>(c->num_pages + 1), *pos = tmp<uint64_t>(c->num_pages + 1);
+        {
+            TimedScope ts(timer, "page_match_flags", (uint64_t)c->num_pages * 16 + c->num_rows / 8);
+            page_match_flags(c->pages, c->num_pages, c->row_start, match_bits, flag, st);
+        }
+        scan(flag, pos, c->num_pages);
+        FilterArgs f{};
+        f.pages = c->pages;
+        f.num_pages = c->num_pages;
+        f.row_start = c->row_start;
+        f.match_bits = match_bits;
+        f.pos = pos;
+        f.width = type == CJ_INT32 ? 4 : 8;
+        f.is_int32 = type == CJ_INT32;
+        f.max_page_rows = c->max_page_rows;
+        f.out =static_cast<uint8_t *>(ctx.pool.alloc((size_t)c->num_pages * kPage + 16));
+        out_types[i] = type;
+        out_cols[i] = {f.out, {0u, type}};
+        count_at.push_back(pos + c->num_pages);
+        count_col.push_back(i);
+        TimedScope ts(timer, f.width == 4 ? "probe_filter_w4" : "probe_filter_w8",
+                      (uint64_t)c->num_pages * kPage + c->num_rows / 8 + total * (uint64_t)f.width + total / 8);
+        probe_filter(f, st);
+    }
+    const std::vector<uint64_t> np = read_u64s(count_at);
+    for (size_t k = 0; k < np.size(); ++k) out_cols[count_col[k]].second.first = (uint32_t)np[k];
 }
 
 void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,

@@ -63,6 +63,7 @@ struct DevColumn {
     uint64_t *byte_start = nullptr;   // [num_pages + 1] first bitmap byte of each page
     uint16_t *byte_prefix = nullptr;  // page index: non-NULL rows before each bitmap byte
     uint64_t bitmap_bytes = 0;        // sum over pages of ceil(rows / 8)
+    uint32_t max_page_rows = 0;       // largest n_rows of a (non long-string) page
     bool loaded = false;        // pages + row_start present
     std::mutex mu;              // guards lazy decode
 };

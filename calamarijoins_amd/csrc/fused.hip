@@ -9,12 +9,14 @@
 //   1. k_probe_count8: matches per probe key page (one lookup per row);
 //   2. k_probe_write8: after an exclusive scan of those counts, the
 //      (build id, probe row) of every match, in probe-row order;
-//   3. k_mat_side: one workgroup per output page and per (side, width class)
-//      writes that page for every output column of the class: build-side
-//      values come from the dense decoded build columns (or the direct-map
-//      records), probe-side values straight from the probe table's pages.
-// Output page k holds result rows [k R, (k + 1) R): R = 1984 for INT32 and
-// 1007 for INT64/FP64, the largest counts whose all-non-NULL page fits.
+//   3. k_mat_block: one workgroup per block of 1984 result rows and per side
+//      writes the block's pages of every output column of that side:
+//      build-side values come from the dense decoded build columns (or the
+//      direct-map records), probe-side values straight from the probe
+//      table's pages.  Each id and each record is read once per side.
+// Block k holds result rows [1984 k, 1984 (k + 1)): INT32 page k (1984 rows,
+// the most an all-non-NULL INT32 page holds) and 8-byte pages 2k and 2k + 1
+// (992 rows each).
 //
 // gfx950 mapping: 2048-row tiles, thread t owning rows t + 256 j, so each
 // wave instruction loads or stores 64 consecutive rows; ranks come from wave
@@ -219,66 +221,110 @@ void probe_write_columnar(const ProbeArgs &a, const uint64_t *off, uint32_t *bui
 }
 
 // ----------------------------------------------------- output page writer
-// One workgroup per output page of every column in `a` (one side, one
-// width W); S = slots per thread (R / 256 rounded up).
-template <int W, uint32_t S, bool FROM_BUILD, bool DIRECT>
-__global__ __launch_bounds__(kWG) void k_mat_side(MatArgs a) {
-    __shared__ uint32_t lds32[32];
-    __shared__ WinState win[FROM_BUILD ? 1 : kMatMaxCols];
-    const uint64_t k = blockIdx.x;
-    const uint64_t r0 = k * a.rows_per_page;
-    const uint32_t cnt = (uint32_t)(a.rows - r0 < a.rows_per_page ? a.rows - r0 : a.rows_per_page);
-    uint32_t id[S];
+// Ranks of 8 predicates per thread in block-row order (row 256 j + t) and
+// the number of true rows before kRowsPage8 (the first 8-byte page).
+__device__ __forceinline__ uint32_t rank_block(uint32_t pred, uint32_t rank[kRPT], uint64_t mask[kRPT],
+                                               uint32_t *lds32, uint32_t *before_split) {
+    // row kRowsPage8 = 992 = slot 3, wave 3, lane 32
+    constexpr uint32_t js = kRowsPage8 / kWG, ws = (kRowsPage8 % kWG) / 64, ls = kRowsPage8 % 64;
+    const uint32_t w = wave_id(), lane = lane_id();
 #pragma unroll
-    for (uint32_t j = 0; j < S; ++j) {
+    for (uint32_t j = 0; j < kRPT; ++j) mask[j] = __ballot((pred >> j) & 1u);
+    if (lane == 0) {
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j) lds32[4 * j + w] = (uint32_t)__popcll(mask[j]);
+        if (w == ws) lds32[4 * kRPT] = (uint32_t)__popcll(mask[js] & ((1ull << ls) - 1ull));
+    }
+    __syncthreads();
+    uint32_t run = 0, base[kRPT], split = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) {
+#pragma unroll
+        for (uint32_t ww = 0; ww < 4; ++ww) {
+            if (ww == w) base[j] = run;
+            if (j == js && ww == ws) split = run + lds32[4 * kRPT];
+            run += lds32[4 * j + ww];
+        }
+    }
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) rank[j] = base[j] + (uint32_t)__popcll(mask[j] & lt);
+    *before_split = split;
+    return run;
+}
+
+// A workgroup writes, for `a.cols_per_wg` output columns of width W of one
+// side, the pages of one block of kBlockRows result rows: one INT32 page
+// (1984 rows) or two 8-byte pages (992 rows each).  Thread t owns block rows
+// t + 256 j.  Workgroup i runs on XCD i % 8; the column groups of one block
+// are given to consecutive workgroups of the same XCD so they share its L2
+// for the block's ids.
+template <int W, bool FROM_BUILD, bool DIRECT>
+__global__ __launch_bounds__(kWG) void k_mat_block(MatArgs a) {
+    __shared__ uint32_t lds32[4 * kRPT + 4];
+    __shared__ WinState win[FROM_BUILD ? 1 : kMatMaxCols];
+    const uint32_t groups = (a.ncols + a.cols_per_wg - 1) / a.cols_per_wg;
+    const uint32_t xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const uint64_t b = (uint64_t)(slot / groups) * 8 + xcd;
+    const uint32_t c0 = (slot % groups) * a.cols_per_wg;
+    const uint32_t c1 = c0 + a.cols_per_wg < a.ncols ? c0 + a.cols_per_wg : a.ncols;
+    const uint64_t r0 = b * kBlockRows;
+    if (r0 >= a.rows) return;
+    const uint32_t cnt = (uint32_t)(a.rows - r0 < kBlockRows ? a.rows - r0 : kBlockRows);
+    const uint32_t cnt0 = cnt < kRowsPage8 ? cnt : kRowsPage8, cnt1 = cnt - cnt0;
+    uint32_t id[kRPT];
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) {
         const uint32_t i = kWG * j + threadIdx.x;
         id[j] = i < cnt ? a.ids[r0 + i] : 0u;
     }
-    if (!FROM_BUILD) {  // probe rows ascend: page windows per column, one wave each
+    if (!FROM_BUILD) {  // probe rows ascend: a page window per column, one wave each
         const uint64_t r_last = a.ids[r0 + cnt - 1];
-        for (uint32_t c = wave_id(); c < a.ncols; c += kWG / 64) {
+        for (uint32_t c = c0 + wave_id(); c < c1; c += kWG / 64) {
             const MatCol &col = a.cols[c];
-            window_setup(win[c], col.pages, col.num_pages, col.row_start, col.win_first[k], r_last);
+            window_setup(win[c - c0], col.pages, col.num_pages, col.row_start, col.win_first[b], r_last);
         }
         __syncthreads();
     }
-    for (uint32_t c = 0; c < a.ncols; ++c) {
+    for (uint32_t c = c0; c < c1; ++c) {
         const MatCol &col = a.cols[c];
-        uint64_t v[S];
+        constexpr bool w4 = W == 4;
+        uint64_t v[kRPT];
         uint32_t ok = 0;
         if (FROM_BUILD && DIRECT) {  // ids are direct-map record slots
 #pragma unroll
-            for (uint32_t j = 0; j < S; ++j) {
+            for (uint32_t j = 0; j < kRPT; ++j) {
                 v[j] = 0;
                 if (kWG * j + threadIdx.x < cnt) {
                     const uint8_t *rec = a.dt.records + (uint64_t)id[j] * a.dt.stride;
                     const uint32_t fl = *reinterpret_cast<const uint32_t *>(rec + a.dt.flags_offset);
-                    v[j] = W == 4 ? (uint64_t) * reinterpret_cast<const uint32_t *>(rec + col.rec_offset)
-                                  : *reinterpret_cast<const uint64_t *>(rec + col.rec_offset);
+                    v[j] = w4 ? (uint64_t) * reinterpret_cast<const uint32_t *>(rec + col.rec_offset)
+                              : *reinterpret_cast<const uint64_t *>(rec + col.rec_offset);
                     ok |= ((fl >> col.rec_bit) & 1u) << j;
                 }
             }
         } else if (FROM_BUILD) {  // ids are build row ids (through a row-id chain)
 #pragma unroll
-            for (uint32_t j = 0; j < S; ++j) {
+            for (uint32_t j = 0; j < kRPT; ++j) {
                 v[j] = 0;
                 if (kWG * j + threadIdx.x < cnt) {
                     const uint32_t rid = col.rids ? col.rids[id[j]] : id[j];
-                    v[j] = W == 4 ? (uint64_t) static_cast<const uint32_t *>(col.values)[rid]
-                                  : static_cast<const uint64_t *>(col.values)[rid];
+                    v[j] = w4 ? (uint64_t) static_cast<const uint32_t *>(col.values)[rid]
+                              : static_cast<const uint64_t *>(col.values)[rid];
                     ok |= (uint32_t)valid_bit(col.valid_bits, rid) << j;
                 }
             }
         } else {  // ids are probe rows: read their pages
-            uint32_t pgi[S], lr[S];
+            uint32_t pgi[kRPT], lr[kRPT];
 #pragma unroll
-            for (uint32_t j = 0; j < S; ++j) {
+            for (uint32_t j = 0; j < kRPT; ++j) {
                 pgi[j] = 0;
                 lr[j] = 0;
                 if (kWG * j + threadIdx.x < cnt) {
                     uint64_t rs;
                     uint32_t nn, nvv;
-                    pgi[j] = find_page(col.pages, col.num_pages, col.row_start, win[c], id[j], &rs, &nn, &nvv);
+                    pgi[j] = find_page(col.pages, col.num_pages, col.row_start, win[c - c0], id[j], &rs, &nn, &nvv);
                     const uint8_t *pp = col.pages + (size_t)pgi[j] * kPage;
                     lr[j] = (uint32_t)(id[j] - rs);
                     if (lr[j] >= nn) {  // row outside its page: inconsistent row starts
@@ -296,60 +342,93 @@ __global__ __launch_bounds__(kWG) void k_mat_side(MatArgs a) {
                 }
             }
 #pragma unroll
-            for (uint32_t j = 0; j < S; ++j) {
+            for (uint32_t j = 0; j < kRPT; ++j) {
                 v[j] = 0;
                 if ((ok >> j) & 1u) {
-                    const uint8_t *vals = col.pages + (size_t)pgi[j] * kPage + (W == 4 ? 4 : 8);
-                    v[j] = W == 4 ? (uint64_t)(uint32_t)ld32(vals + 4 * (size_t)lr[j]) : ld64(vals + 8 * (size_t)lr[j]);
+                    const uint8_t *vals = col.pages + (size_t)pgi[j] * kPage + (w4 ? 4 : 8);
+                    v[j] = w4 ? (uint64_t)(uint32_t)ld32(vals + 4 * (size_t)lr[j]) : ld64(vals + 8 * (size_t)lr[j]);
                 }
             }
         }
-        if (W == 4 && col.is_int32) {  // value_t NULL sentinel (page_builders.h:123)
+        if (col.is_int32) {  // value_t NULL sentinel (page_builders.h:123)
 #pragma unroll
-            for (uint32_t j = 0; j < S; ++j)
+            for (uint32_t j = 0; j < kRPT; ++j)
                 if ((int32_t)(uint32_t)v[j] == kI32Null) ok &= ~(1u << j);
         }
-        uint32_t vi[S];
-        uint64_t mask[S];
-        const uint32_t tot = rank_slots<S>(ok, vi, mask, lds32);
-        uint8_t *page = col.out + k * kPage;
-        uint8_t *vals = page + (W == 4 ? 4 : 8);
-#pragma unroll
-        for (uint32_t j = 0; j < S; ++j)
-            if ((ok >> j) & 1u) {
-                if (W == 4) *reinterpret_cast<uint32_t *>(vals + 4 * (size_t)vi[j]) = (uint32_t)v[j];
-                else *reinterpret_cast<uint64_t *>(vals + 8 * (size_t)vi[j]) = v[j];
-            }
-        // bitmap: slot j of wave w covers rows 256 j + 64 w .. + 63 = 8 bytes
-        const uint32_t nbytes = (cnt + 7) >> 3;
-        uint8_t *bm = page + kPage - nbytes;
+        uint32_t vi[kRPT], split;
+        uint64_t mask[kRPT];
+        const uint32_t tot = rank_block(ok, vi, mask, lds32, &split);
         const uint32_t lane = lane_id();
-        if (lane < 8) {
+        if (w4) {  // W == 4
+            uint8_t *page = col.out + b * kPage;
 #pragma unroll
-            for (uint32_t j = 0; j < S; ++j) {
-                const uint32_t b = ((kWG * j + 64 * wave_id()) >> 3) + lane;
-                if (b < nbytes) bm[b] = (uint8_t)(mask[j] >> (8 * lane));
+            for (uint32_t j = 0; j < kRPT; ++j)
+                if ((ok >> j) & 1u) *reinterpret_cast<uint32_t *>(page + 4 + 4 * (size_t)vi[j]) = (uint32_t)v[j];
+            // bitmap: slot j of wave w covers rows 256 j + 64 w .. + 63 = 8 bytes
+            const uint32_t nbytes = (cnt + 7) >> 3;
+            uint8_t *bm = page + kPage - nbytes;
+            if (lane < 8) {
+#pragma unroll
+                for (uint32_t j = 0; j < kRPT; ++j) {
+                    const uint32_t by = ((kWG * j + 64 * wave_id()) >> 3) + lane;
+                    if (by < nbytes) bm[by] = (uint8_t)(mask[j] >> (8 * lane));
+                }
             }
-        }
-        if (threadIdx.x == 0) {
-            *reinterpret_cast<uint16_t *>(page) = (uint16_t)cnt;
-            *reinterpret_cast<uint16_t *>(page + 2) = (uint16_t)tot;
+            if (threadIdx.x == 0) {
+                *reinterpret_cast<uint16_t *>(page) = (uint16_t)cnt;
+                *reinterpret_cast<uint16_t *>(page + 2) = (uint16_t)tot;
+            }
+        } else {
+            uint8_t *page0 = col.out + 2 * b * kPage, *page1 = page0 + kPage;
+#pragma unroll
+            for (uint32_t j = 0; j < kRPT; ++j)
+                if ((ok >> j) & 1u) {
+                    const bool hi = kWG * j + threadIdx.x >= kRowsPage8;
+                    uint8_t *dst = hi ? page1 + 8 + 8 * (size_t)(vi[j] - split) : page0 + 8 + 8 * (size_t)vi[j];
+                    *reinterpret_cast<uint64_t *>(dst) = v[j];
+                }
+            const uint32_t nb0 = (cnt0 + 7) >> 3, nb1 = (cnt1 + 7) >> 3;
+            constexpr uint32_t kSplitByte = kRowsPage8 / 8;
+            if (lane < 8) {
+#pragma unroll
+                for (uint32_t j = 0; j < kRPT; ++j) {
+                    const uint32_t by = ((kWG * j + 64 * wave_id()) >> 3) + lane;
+                    const uint8_t bits = (uint8_t)(mask[j] >> (8 * lane));
+                    if (by < kSplitByte) {
+                        if (by < nb0) page0[kPage - nb0 + by] = bits;
+                    } else if (by - kSplitByte < nb1) {
+                        page1[kPage - nb1 + (by - kSplitByte)] = bits;
+                    }
+                }
+            }
+            if (threadIdx.x == 0) {
+                *reinterpret_cast<uint16_t *>(page0) = (uint16_t)cnt0;
+                *reinterpret_cast<uint16_t *>(page0 + 2) = (uint16_t)split;
+                if (cnt1) {
+                    *reinterpret_cast<uint16_t *>(page1) = (uint16_t)cnt1;
+                    *reinterpret_cast<uint16_t *>(page1 + 2) = (uint16_t)(tot - split);
+                }
+            }
         }
     }
 }
 
-template <int W, uint32_t S>
-static void launch_side(const MatArgs &a, unsigned pages, hipStream_t s) {
-    if (a.from_build && a.direct) k_mat_side<W, S, true, true><<<pages, kWG, 0, s>>>(a);
-    else if (a.from_build) k_mat_side<W, S, true, false><<<pages, kWG, 0, s>>>(a);
-    else k_mat_side<W, S, false, false><<<pages, kWG, 0, s>>>(a);
+template <int W>
+static void launch_block(const MatArgs &a, unsigned grid, hipStream_t s) {
+    if (a.from_build && a.direct) k_mat_block<W, true, true><<<grid, kWG, 0, s>>>(a);
+    else if (a.from_build) k_mat_block<W, true, false><<<grid, kWG, 0, s>>>(a);
+    else k_mat_block<W, false, false><<<grid, kWG, 0, s>>>(a);
 }
 
 void materialize_side(const MatArgs &a, hipStream_t s) {
     if (!a.rows || !a.ncols) return;
-    const uint64_t pages = (a.rows + a.rows_per_page - 1) / a.rows_per_page;
-    if (a.cols[0].width == 4) launch_side<4, (kRowsPage4 + kWG - 1) / kWG>(a, (unsigned)pages, s);
-    else launch_side<8, (kRowsPage8 + kWG - 1) / kWG>(a, (unsigned)pages, s);
+    if (a.cols_per_wg < 1 || a.cols_per_wg > kMatMaxCols) throw InvalidError("bad columns per workgroup");
+    const uint64_t blocks = (a.rows + kBlockRows - 1) / kBlockRows;
+    const uint64_t groups = (a.ncols + a.cols_per_wg - 1) / a.cols_per_wg;
+    const uint64_t grid = (blocks + 7) / 8 * 8 * groups;
+    if (grid > 0x7fffffffull) throw UnsupportedError("result too large for one launch");
+    if (a.cols[0].width == 4) launch_block<4>(a, (unsigned)grid, s);
+    else launch_block<8>(a, (unsigned)grid, s);
     CJ_HIP(hipGetLastError());
 }
 

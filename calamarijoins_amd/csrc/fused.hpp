@@ -48,21 +48,22 @@ struct MatCol {
     uint8_t *out;                // output page block of this column
 };
 
-// Output columns of one side and one width class: output page k holds rows
-// [k R, (k + 1) R) of the join result, R = 1984 (INT32) or 1007 (8-byte).
+// Output columns of one side and one width.  Result rows are written in
+// blocks of kBlockRows: block k fills INT32 page k and 8-byte pages 2k and
+// 2k + 1.
 struct MatArgs {
     int32_t from_build;
     int32_t direct;
     DirectTable dt;
     const uint32_t *ids;  // per output row: build id or probe row
     uint64_t rows;
-    uint32_t rows_per_page;
     uint32_t ncols;
+    uint32_t cols_per_wg;  // columns one workgroup writes (1..ncols)
     MatCol cols[kMatMaxCols];
 };
 
-constexpr uint32_t kRowsPage4 = kRowsI32;
-constexpr uint32_t kRowsPage8 = kRowsI64;
+constexpr uint32_t kBlockRows = kRowsI32;      // 1984: a full INT32 page
+constexpr uint32_t kRowsPage8 = kBlockRows / 2;  // 992 rows per 8-byte page
 
 void materialize_side(const MatArgs &a, hipStream_t s);
 
@@ -70,9 +71,9 @@ void materialize_side(const MatArgs &a, hipStream_t s);
 // its page before it.
 void page_byte_prefix(const uint8_t *pages, uint32_t num_pages, const uint64_t *byte_start, uint16_t *prefix,
                       hipStream_t s);
-// For each output page, the payload page holding its first probe row:
-// first[k] = upper_bound(col_row_start, probe_rows[k * R]) - 1.
-void window_first(const uint32_t *probe_rows, uint64_t rows, uint32_t rows_per_page, const uint64_t *col_row_start,
+// For each block of R result rows, the payload page holding its first probe
+// row: first[k] = upper_bound(col_row_start, probe_rows[k * R]) - 1.
+void window_first(const uint32_t *probe_rows, uint64_t rows, uint32_t rows_per_block, const uint64_t *col_row_start,
                   uint32_t col_pages, uint32_t *first, hipStream_t s);
 
 }  // namespace cj

@@ -44,21 +44,33 @@ __device__ __forceinline__ uint32_t rank_slots(uint32_t pred, uint32_t rank[S], 
 }
 
 // Keys of this thread's 8 rows (t0 + 256 j + t) of a key page.  Bit j of
-// the returned mask: the row exists and its key is not NULL.
+// the returned mask: the row exists and its key is not NULL.  Loads are
+// unconditional with clamped in-page addresses (no branch around a load), so
+// all 8 are in flight together.
 __device__ __forceinline__ uint32_t load_keys8(const uint8_t *p, uint32_t n, uint32_t nv, bool dense, uint32_t t0,
                                                uint32_t *running, int32_t key[kRPT], uint32_t *lds32) {
-    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+    const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
     uint32_t valid = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kRPT; ++j) {
-        const uint32_t r = t0 + kWG * j + threadIdx.x;
-        if (r < n && (dense || page_bit(bm, r))) valid |= 1u << j;
-    }
     uint32_t vi[kRPT];
     if (dense) {
 #pragma unroll
-        for (uint32_t j = 0; j < kRPT; ++j) vi[j] = t0 + kWG * j + threadIdx.x;
+        for (uint32_t j = 0; j < kRPT; ++j) {
+            vi[j] = t0 + kWG * j + threadIdx.x;
+            if (vi[j] < n) valid |= 1u << j;
+        }
     } else {
+        const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+        uint32_t byte[kRPT];
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j) {
+            const uint32_t r = t0 + kWG * j + threadIdx.x;
+            byte[j] = bm[(r < n ? r : nlast) >> 3];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j) {
+            const uint32_t r = t0 + kWG * j + threadIdx.x;
+            if (r < n && ((byte[j] >> (r & 7)) & 1u)) valid |= 1u << j;
+        }
         uint64_t mask[kRPT];
         const uint32_t tot = rank_slots<kRPT>(valid, vi, mask, lds32);
 #pragma unroll
@@ -67,30 +79,33 @@ __device__ __forceinline__ uint32_t load_keys8(const uint8_t *p, uint32_t n, uin
     }
 #pragma unroll
     for (uint32_t j = 0; j < kRPT; ++j) {
-        key[j] = 0;
-        if ((valid >> j) & 1u) {
-            if (vi[j] < nv) key[j] = ld32(p + 4 + 4 * (size_t)vi[j]);
-            else valid &= ~(1u << j);  // malformed bitmap: stay inside the value area
-        }
+        key[j] = ld32(p + 4 + 4 * (size_t)(vi[j] < nv ? vi[j] : vlast));
+        if (vi[j] >= nv) valid &= ~(1u << j);  // malformed bitmap: stay inside the value area
     }
     return valid;
 }
 
 // Lookups of 8 keys, all loads issued before any is used.  m[j] = matches of
 // key j; hash mode: [s[j], e[j]) is its directory range; direct mode: s[j]
-// is its record slot.
+// is its record slot (0 when it has none).
 template <int MODE>
 __device__ __forceinline__ void lookup8(uint32_t valid, const int32_t key[kRPT], const HashTableDev &ht,
                                         const NLKeys &nl, const DirectTable &dt, uint32_t m[kRPT], uint32_t s[kRPT],
                                         uint32_t e[kRPT]) {
     if (MODE == kModeDirect) {
+        uint32_t w[kRPT];
 #pragma unroll
         for (uint32_t j = 0; j < kRPT; ++j) {
-            int64_t off = (int64_t)key[j] - (int64_t)dt.min_key;
-            bool in = ((valid >> j) & 1u) && off >= 0 && (uint64_t)off < dt.range;
+            const int64_t off = (int64_t)key[j] - (int64_t)dt.min_key;
+            const bool in = ((valid >> j) & 1u) && off >= 0 && (uint64_t)off < dt.range;
             s[j] = in ? (uint32_t)off : 0u;
+            e[j] = in;
+            w[j] = dt.presence[s[j] >> 5];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j) {
+            m[j] = e[j] ? (w[j] >> (s[j] & 31)) & 1u : 0u;
             e[j] = 0;
-            m[j] = in ? (dt.presence[(uint64_t)off >> 5] >> (off & 31)) & 1u : 0u;
         }
         return;
     }
@@ -104,26 +119,29 @@ __device__ __forceinline__ void lookup8(uint32_t valid, const int32_t key[kRPT],
         }
         return;
     }
-    uint64_t slot[kRPT], d[kRPT];
+    uint64_t slot[kRPT], d[kRPT], dp[kRPT];
     uint32_t tag[kRPT];
 #pragma unroll
     for (uint32_t j = 0; j < kRPT; ++j) {
         uint64_t h = hash_key(key[j]);
         slot[j] = h >> (64 - ht.log_slots);
         tag[j] = c_bloom.t[h & 0x7FF];
-        d[j] = ((valid >> j) & 1u) ? ht.dir[slot[j]] : 0;
+        d[j] = ht.dir[slot[j]];
+        dp[j] = ht.dir[slot[j] ? slot[j] - 1 : 0];
     }
+    int32_t k0[kRPT];
 #pragma unroll
     for (uint32_t j = 0; j < kRPT; ++j) {
-        bool pass = ((valid >> j) & 1u) && (d[j] & tag[j]) == tag[j];
+        const bool pass = ((valid >> j) & 1u) && (d[j] & tag[j]) == tag[j];
         e[j] = pass ? (uint32_t)(d[j] >> 16) : 0;
-        s[j] = pass && slot[j] ? (uint32_t)(ht.dir[slot[j] - 1] >> 16) : 0;
-        if (!pass) s[j] = e[j] = 0;
+        s[j] = pass && slot[j] ? (uint32_t)(dp[j] >> 16) : 0;
+        if (s[j] >= e[j]) s[j] = e[j] = 0;
+        k0[j] = ht.keys[s[j]];  // first candidate (index 0 is always a tuple when the table is not empty)
     }
 #pragma unroll
     for (uint32_t j = 0; j < kRPT; ++j) {
-        uint32_t c = 0;
-        for (uint32_t i = s[j]; i < e[j]; ++i) c += ht.keys[i] == key[j];
+        uint32_t c = s[j] < e[j] && k0[j] == key[j];
+        for (uint32_t i = s[j] + 1; i < e[j]; ++i) c += ht.keys[i] == key[j];
         m[j] = c;
     }
 }

@@ -138,7 +138,8 @@ typedef struct cj_ctx_options {
 enum {
     CJ_EXEC_OUTPUT_DEVICE = 1, /* keep result pages in HBM (no D2H)          */
     CJ_EXEC_NO_FAST_PATH = 2,  /* force the general (row-id list) path        */
-    CJ_EXEC_NO_DIRECT = 4      /* never use the direct-mapped build side      */
+    CJ_EXEC_NO_DIRECT = 4,     /* never use the direct-mapped build side      */
+    CJ_EXEC_NO_ALIGNED = 8     /* never write page-aligned root output        */
 };
 
 /* ---- context: replaces Contest::build_context / destroy_context

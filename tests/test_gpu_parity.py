@@ -24,15 +24,15 @@ def same_multiset(got, exp):
     assert np.array_equal(cjtest.result_row_hashes(got), cjtest.result_row_hashes(exp))
 
 
-@pytest.mark.parametrize("flags", [0, 4, 2], ids=["fused", "fused-hash", "general"])
+@pytest.mark.parametrize("flags", [0, 8, 4, 2], ids=["aligned", "fused", "fused-hash", "general"])
 @pytest.mark.parametrize("case", KATS, ids=[c["name"] for c in KATS])
 def test_gpu_unit_kat(engine, case, flags):
     res = engine.execute(cjtest.kat_plan(case), flags=flags)
     cjtest.check_kat(case, res)
 
 
-PATHS = [0, abi.CJ_EXEC_NO_DIRECT, abi.CJ_EXEC_NO_FAST_PATH]
-PATH_IDS = ["fused", "fused-hash", "general"]
+PATHS = [0, abi.CJ_EXEC_NO_ALIGNED, abi.CJ_EXEC_NO_DIRECT, abi.CJ_EXEC_NO_FAST_PATH]
+PATH_IDS = ["aligned", "fused", "fused-hash", "general"]
 
 
 @pytest.mark.parametrize("flags", PATHS, ids=PATH_IDS)
