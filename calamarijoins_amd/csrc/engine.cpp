@@ -1013,6 +1013,7 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
     BuildPagesArgs ba{};
     ba.key = a.key;
     ba.dt = a.dt;
+    ba.match_bits = match_bits;
     uint64_t bbytes = key_bytes(a.key) + total * a.dt.stride;
     for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
         if (remap[i] >= bw) continue;

@@ -84,13 +84,119 @@ __global__ __launch_bounds__(kWG) void k_probe_count_match(ProbeArgs a, uint64_t
     }
 }
 
+// Direct mode: one wave per key page, no barriers.  Per segment of 2048 rows
+// the 32 chunk masks (64 rows each) go to the wave's LDS slice, then lane i
+// assembles global bitmap word i of the segment from them: interior words
+// are stored, the two edge words OR-ed with device atomics.
+constexpr uint32_t kCountBatch = 16;
+
+__global__ __launch_bounds__(kWG) void k_count_match_direct(ProbeArgs a, uint64_t *counts, uint32_t *match_bits) {
+    __shared__ uint32_t msk_all[kWG / 64][2 * (kTile / 64) + 2];
+    const uint32_t lane = lane_id();
+    const KeySource &k = a.key;
+    const DirectTable &dt = a.dt;
+    const uint32_t pg = blockIdx.x * (kWG / 64) + wave_id();
+    if (pg >= k.num_pages) return;
+    const uint8_t *p = k.pages + (size_t)pg * kPage;
+    const uint32_t n = ld16(p), nv = ld16(p + 2);
+    const bool dense = n == nv;
+    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+    const int32_t *keys = reinterpret_cast<const int32_t *>(p + 4);
+    const uint64_t r0 = k.row_start[pg];
+    uint32_t *msk = msk_all[wave_id()];
+    const uint64_t lt = lanemask_lt();
+    const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
+    uint32_t vbase = 0;
+    uint64_t total = 0;
+    for (uint32_t s0 = 0; s0 < n; s0 += kTile) {
+        const uint32_t nseg = n - s0 < kTile ? n - s0 : kTile;
+        for (uint32_t b0 = 0; b0 < kTile / 64; b0 += kCountBatch) {
+            uint32_t vbits = 0, vi[kCountBatch];
+            if (dense) {
+#pragma unroll
+                for (uint32_t c = 0; c < kCountBatch; ++c) {
+                    const uint32_t lr = s0 + 64 * (b0 + c) + lane;
+                    vi[c] = lr;
+                    if (lr < n) vbits |= 1u << c;
+                }
+            } else {
+                uint32_t byte[kCountBatch];
+#pragma unroll
+                for (uint32_t c = 0; c < kCountBatch; ++c) {
+                    const uint32_t lr = s0 + 64 * (b0 + c) + lane;
+                    byte[c] = bm[(lr < n ? lr : nlast) >> 3];
+                }
+#pragma unroll
+                for (uint32_t c = 0; c < kCountBatch; ++c) {
+                    const uint32_t lr = s0 + 64 * (b0 + c) + lane;
+                    const bool v = lr < n && ((byte[c] >> (lr & 7)) & 1u);
+                    const uint64_t vm = __ballot(v);
+                    vi[c] = vbase + (uint32_t)__popcll(vm & lt);
+                    vbase += (uint32_t)__popcll(vm);
+                    if (v) vbits |= 1u << c;
+                }
+            }
+            uint32_t slot[kCountBatch], word[kCountBatch], in = 0;
+#pragma unroll
+            for (uint32_t c = 0; c < kCountBatch; ++c) {
+                const int32_t key = keys[vi[c] < nv ? vi[c] : vlast];
+                const int64_t off = (int64_t)key - (int64_t)dt.min_key;
+                const bool ok = ((vbits >> c) & 1u) && vi[c] < nv && off >= 0 && (uint64_t)off < dt.range;
+                slot[c] = ok ? (uint32_t)off : 0u;
+                if (ok) in |= 1u << c;
+            }
+#pragma unroll
+            for (uint32_t c = 0; c < kCountBatch; ++c) word[c] = dt.presence[slot[c] >> 5];
+#pragma unroll
+            for (uint32_t c = 0; c < kCountBatch; ++c) {
+                const bool m = ((in >> c) & 1u) && ((word[c] >> (slot[c] & 31)) & 1u);
+                const uint64_t mm = __ballot(m);
+                total += (uint32_t)__popcll(mm);
+                if (lane == 0) {
+                    msk[2 * (b0 + c)] = (uint32_t)mm;
+                    msk[2 * (b0 + c) + 1] = (uint32_t)(mm >> 32);
+                }
+            }
+        }
+        if (lane == 0) msk[2 * (kTile / 64)] = msk[2 * (kTile / 64) + 1] = 0;
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t g0 = r0 + s0;
+        const uint32_t sh = (uint32_t)(g0 & 31);
+        const uint32_t nwords = (sh + nseg + 31) >> 5;
+        for (uint32_t i = lane; i < nwords; i += 64) {
+            // bits [32 i - sh, 32 i - sh + 32) of the segment's row mask
+            uint32_t w;
+            if (i == 0) {
+                w = msk[0] << sh;
+            } else {
+                const uint32_t st = 32 * i - sh, d = st >> 5, o = st & 31;
+                w = o ? (msk[d] >> o) | (msk[d + 1] << (32 - o)) : msk[d];
+            }
+            const bool edge = (i == 0 && sh) || (i == nwords - 1 && ((sh + nseg) & 31));
+            uint32_t *dst = match_bits + (g0 >> 5) + i;
+            if (edge) {
+                if (w) atomicOr(dst, w);
+            } else {
+                *dst = w;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) counts[pg] = total;
+}
+
 void probe_count_match(const ProbeArgs &a, uint64_t *counts, uint32_t *match_bits, hipStream_t s) {
     const uint32_t units = a.key.num_pages;
     if (!units) return;
     const uint32_t grid = units < kPagesGrid ? units : kPagesGrid;
-    if (a.mode == kModeNL) k_probe_count_match<kModeNL><<<grid, kWG, 0, s>>>(a, counts, match_bits);
-    else if (a.mode == kModeDirect) k_probe_count_match<kModeDirect><<<grid, kWG, 0, s>>>(a, counts, match_bits);
-    else k_probe_count_match<kModeHash><<<grid, kWG, 0, s>>>(a, counts, match_bits);
+    if (a.mode == kModeDirect) {
+        const uint32_t per = kWG / 64;
+        k_count_match_direct<<<(units + per - 1) / per, kWG, 0, s>>>(a, counts, match_bits);
+    } else if (a.mode == kModeNL) {
+        k_probe_count_match<kModeNL><<<grid, kWG, 0, s>>>(a, counts, match_bits);
+    } else {
+        k_probe_count_match<kModeHash><<<grid, kWG, 0, s>>>(a, counts, match_bits);
+    }
     CJ_HIP(hipGetLastError());
 }
 
@@ -125,111 +231,173 @@ __device__ __forceinline__ uint32_t rec_dword(const uint4 (&r)[NQ], uint32_t d) 
     return x;
 }
 
+constexpr uint32_t kBuildColsPerWave = 4;  // build columns one launch writes
+constexpr uint32_t kBuildBatch = 8;        // 64-row chunks whose loads are issued together
+constexpr uint32_t kObmWords = kCap4 / 32 + 1;
+
+// One wave per key page, no barriers.  For each batch of 8 chunks of 64 rows:
+// bitmap bytes (sparse pages), keys, presence words and records are loaded
+// with clamped addresses so every load of a phase is in flight together;
+// then, chunk by chunk, ballots give each matched row its result row within
+// the key page (ko) and each column's non-NULL values their slot.  Result
+// row ko goes to output page ko / cap of the key page; the values before
+// each page start are kept per column (ps) as the boundary passes.
 template <int NQ>
-__global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a) {
-    __shared__ uint32_t lds32[32];
-    __shared__ uint32_t obm[kMatMaxCols][kMaxPagesPerKeyPage][kCap4 / 32 + 1];  // output bitmaps
-    __shared__ uint32_t pstart[kMatMaxCols][kMaxPagesPerKeyPage + 1];  // values before each page
-    __shared__ uint32_t nn_run[kMatMaxCols];
-    NLKeys nl{};
-    const KeySource &k = a.key;
-    const uint32_t words = kMaxPagesPerKeyPage * (kCap4 / 32 + 1);
-    for (uint32_t pg = blockIdx.x; pg < k.num_pages; pg += gridDim.x) {
-        const uint8_t *p = k.pages + (size_t)pg * kPage;
-        const uint32_t n = ld16(p), nv = ld16(p + 2);
-        const bool dense = n == nv;
-        for (uint32_t i = threadIdx.x; i < a.ncols * words; i += kWG) (&obm[0][0][0])[i] = 0;
-        if (threadIdx.x < kMatMaxCols) nn_run[threadIdx.x] = 0;
-        __syncthreads();
-        uint32_t running = 0, run_m = 0;
-        for (uint32_t t0 = 0; t0 < n; t0 += kTile) {
-            int32_t key[kRPT];
-            const uint32_t valid = load_keys8(p, n, nv, dense, t0, &running, key, lds32);
-            uint32_t m[kRPT], slot[kRPT], e[kRPT];
-            HashTableDev ht{};
-            lookup8<kModeDirect>(valid, key, ht, nl, a.dt, m, slot, e);
-            uint4 rec[kRPT][NQ];
-            uint32_t mm = 0;
+__global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a, uint32_t c_lo) {
+    __shared__ uint32_t obm_all[kWG / 64][kBuildColsPerWave][kMaxPagesPerKeyPage][kObmWords];
+    const uint32_t lane = lane_id();
+    const uint32_t pg = blockIdx.x * (kWG / 64) + wave_id();
+    if (pg >= a.key.num_pages) return;
+    const uint32_t nc = a.ncols - c_lo < kBuildColsPerWave ? a.ncols - c_lo : kBuildColsPerWave;
+    const uint8_t *p = a.key.pages + (size_t)pg * kPage;
+    const uint32_t n = ld16(p), nv = ld16(p + 2);
+    const bool dense = n == nv;
+    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+    const int32_t *keys = reinterpret_cast<const int32_t *>(p + 4);
+    uint32_t(*obm)[kMaxPagesPerKeyPage][kObmWords] = obm_all[wave_id()];
+    for (uint32_t i = lane; i < kBuildColsPerWave * kMaxPagesPerKeyPage * kObmWords; i += 64)
+        (&obm[0][0][0])[i] = 0;
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t first4 = a.p4_off[pg], first8 = a.p8_off[pg];
+    const uint64_t r0 = a.key.row_start[pg];
+    const uint32_t *mw = a.match_bits + (r0 >> 5);
+    const uint32_t msh = (uint32_t)(r0 & 31);
+    const uint64_t lt = lanemask_lt();
+    const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
+    const uint32_t fl_dw = a.dt.flags_offset >> 2;
+    uint32_t vbase = 0, mbase = 0;
+    uint32_t nn[kBuildColsPerWave] = {0, 0, 0, 0};
+    uint32_t ps[kBuildColsPerWave][kMaxPagesPerKeyPage] = {};
+    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kBuildBatch) {
+        uint32_t vbits = 0;
+        uint32_t vi[kBuildBatch];
+        if (dense) {
 #pragma unroll
-            for (uint32_t j = 0; j < kRPT; ++j) {
-                mm |= (m[j] ? 1u : 0u) << j;
-                // unconditional (slot 0 when unmatched): all record loads in flight together
-                const uint4 *r = reinterpret_cast<const uint4 *>(a.dt.records + (uint64_t)slot[j] * a.dt.stride);
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) rec[j][q] = r[q];
+            for (uint32_t c = 0; c < kBuildBatch; ++c) {
+                const uint32_t lr = c0 + 64 * c + lane;
+                vi[c] = lr;
+                if (lr < n) vbits |= 1u << c;
             }
-            uint32_t ko[kRPT];
-            uint64_t mask[kRPT];
-            const uint32_t tot_m = rank_slots<kRPT>(mm, ko, mask, lds32);
-            const uint32_t fl_dw = a.dt.flags_offset >> 2;
-            for (uint32_t c = 0; c < a.ncols; ++c) {
-                const BuildPageCol &col = a.cols[c];
+        } else {
+            uint32_t byte[kBuildBatch];
+#pragma unroll
+            for (uint32_t c = 0; c < kBuildBatch; ++c) {
+                const uint32_t lr = c0 + 64 * c + lane;
+                byte[c] = bm[(lr < n ? lr : nlast) >> 3];
+            }
+#pragma unroll
+            for (uint32_t c = 0; c < kBuildBatch; ++c) {
+                const uint32_t lr = c0 + 64 * c + lane;
+                const bool v = lr < n && ((byte[c] >> (lr & 7)) & 1u);
+                const uint64_t vm = __ballot(v);
+                vi[c] = vbase + (uint32_t)__popcll(vm & lt);
+                vbase += (uint32_t)__popcll(vm);
+                if (v) vbits |= 1u << c;
+            }
+        }
+        int32_t key[kBuildBatch];
+#pragma unroll
+        for (uint32_t c = 0; c < kBuildBatch; ++c) {
+            key[c] = keys[vi[c] < nv ? vi[c] : vlast];
+            if (vi[c] >= nv) vbits &= ~(1u << c);
+        }
+        // matched rows come from the count pass's match bitmap (coalesced
+        // words) instead of a second random presence lookup per row
+        uint32_t mword[kBuildBatch];
+#pragma unroll
+        for (uint32_t c = 0; c < kBuildBatch; ++c) {
+            const uint32_t lr = c0 + 64 * c + lane;
+            mword[c] = mw[(msh + (lr < n ? lr : nlast)) >> 5];
+        }
+        uint32_t slot[kBuildBatch], mbits = 0;
+        uint4 rec[kBuildBatch][NQ];
+#pragma unroll
+        for (uint32_t c = 0; c < kBuildBatch; ++c) {
+            const uint32_t lr = c0 + 64 * c + lane;
+            const int64_t off = (int64_t)key[c] - (int64_t)a.dt.min_key;
+            const bool m = ((vbits >> c) & 1u) && ((mword[c] >> ((msh + lr) & 31)) & 1u) && off >= 0 &&
+                           (uint64_t)off < a.dt.range;
+            slot[c] = m ? (uint32_t)off : 0u;
+            if (m) mbits |= 1u << c;
+            const uint4 *r = reinterpret_cast<const uint4 *>(a.dt.records + (uint64_t)slot[c] * a.dt.stride);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) rec[c][q] = r[q];
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < kBuildBatch; ++c) {
+            const bool matched = (mbits >> c) & 1u;
+            const uint64_t mm = __ballot(matched);
+            if (!mm) continue;
+            const uint32_t ko = mbase + (uint32_t)__popcll(mm & lt);
+            const uint32_t tm = (uint32_t)__popcll(mm);
+#pragma unroll
+            for (uint32_t k = 0; k < kBuildColsPerWave; ++k) {
+                if (k >= nc) break;
+                const BuildPageCol &col = a.cols[c_lo + k];
                 const uint32_t cap = col.width == 4 ? kCap4 : kCap8;
-                const uint32_t dw = col.rec_offset >> 2;
-                uint64_t v[kRPT];
-                uint32_t ok = 0;
+                const uint32_t lo = rec_dword<NQ>(rec[c], col.rec_offset >> 2);
+                const uint32_t hi = col.width == 8 ? rec_dword<NQ>(rec[c], (col.rec_offset >> 2) + 1) : 0u;
+                const uint32_t fl = rec_dword<NQ>(rec[c], fl_dw);
+                bool ok = matched && ((fl >> col.rec_bit) & 1u);
+                if (col.is_int32 && (int32_t)lo == kI32Null) ok = false;  // page_builders.h:123
+                const uint64_t om = __ballot(ok);
+                // a page of this column starts inside the chunk: values before it
+                const uint32_t b = (mbase + cap - 1) / cap * cap;
+                if (b < mbase + tm && b / cap < kMaxPagesPerKeyPage) {
+                    const uint64_t before = __ballot(matched && ko < b);
+                    const uint32_t v = nn[k] + (uint32_t)__popcll(om & before);
 #pragma unroll
-                for (uint32_t j = 0; j < kRPT; ++j) {
-                    const uint32_t lo = rec_dword<NQ>(rec[j], dw);
-                    const uint32_t hi = col.width == 8 ? rec_dword<NQ>(rec[j], dw + 1) : 0u;
-                    const uint32_t fl = rec_dword<NQ>(rec[j], fl_dw);
-                    v[j] = ((uint64_t)hi << 32) | lo;
-                    bool o = ((mm >> j) & 1u) && ((fl >> col.rec_bit) & 1u);
-                    if (col.is_int32 && (int32_t)lo == kI32Null) o = false;  // page_builders.h:123
-                    ok |= (o ? 1u : 0u) << j;
+                    for (uint32_t q = 0; q < kMaxPagesPerKeyPage; ++q)
+                        if (q == b / cap) ps[k][q] = v;
                 }
-                uint32_t so[kRPT];
-                uint64_t omask[kRPT];
-                const uint32_t base_nn = nn_run[c];
-                const uint32_t tot_ok = rank_slots<kRPT>(ok, so, omask, lds32);
-                // the first row of each output page records the values before it
+                if (ok) {
+                    const uint32_t pp = ko / cap;
+                    uint32_t base = ps[k][0];
 #pragma unroll
-                for (uint32_t j = 0; j < kRPT; ++j)
-                    if ((mm >> j) & 1u) {
-                        const uint32_t r = run_m + ko[j];
-                        if (r % cap == 0 && r / cap < kMaxPagesPerKeyPage) pstart[c][r / cap] = base_nn + so[j];
-                    }
-                __syncthreads();
-                const uint64_t first = col.width == 4 ? a.p4_off[pg] : a.p8_off[pg];
-#pragma unroll
-                for (uint32_t j = 0; j < kRPT; ++j)
-                    if ((ok >> j) & 1u) {
-                        const uint32_t r = run_m + ko[j];
-                        const uint32_t pp = r / cap;
-                        if (pp >= kMaxPagesPerKeyPage) continue;
-                        const uint32_t vi = base_nn + so[j] - pstart[c][pp];
-                        uint8_t *page = col.out + (first + pp) * kPage;
-                        if (col.width == 4) *reinterpret_cast<uint32_t *>(page + 4 + 4 * (size_t)vi) = (uint32_t)v[j];
-                        else *reinterpret_cast<uint64_t *>(page + 8 + 8 * (size_t)vi) = v[j];
-                        const uint32_t rr = r - pp * cap;
-                        atomicOr(&obm[c][pp][rr >> 5], 1u << (rr & 31));
-                    }
-                __syncthreads();
-                if (threadIdx.x == 0) nn_run[c] = base_nn + tot_ok;
-            }
-            run_m += tot_m;
-        }
-        __syncthreads();
-        // headers and bitmaps
-        for (uint32_t c = 0; c < a.ncols; ++c) {
-            const BuildPageCol &col = a.cols[c];
-            const uint32_t cap = col.width == 4 ? kCap4 : kCap8;
-            const uint32_t np = (run_m + cap - 1) / cap;
-            const uint64_t first = col.width == 4 ? a.p4_off[pg] : a.p8_off[pg];
-            for (uint32_t pp = 0; pp < np && pp < kMaxPagesPerKeyPage; ++pp) {
-                uint8_t *page = col.out + (first + pp) * kPage;
-                const uint32_t rows = run_m - pp * cap < cap ? run_m - pp * cap : cap;
-                const uint32_t nb = (rows + 7) >> 3;
-                const uint8_t *src = reinterpret_cast<const uint8_t *>(obm[c][pp]);
-                for (uint32_t b = threadIdx.x; b < nb; b += kWG) page[kPage - nb + b] = src[b];
-                if (threadIdx.x == 0) {
-                    const uint32_t end = pp + 1 < np ? pstart[c][pp + 1] : nn_run[c];
-                    *reinterpret_cast<uint16_t *>(page) = (uint16_t)rows;
-                    *reinterpret_cast<uint16_t *>(page + 2) = (uint16_t)(end - pstart[c][pp]);
+                    for (uint32_t q = 1; q < kMaxPagesPerKeyPage; ++q) base = pp == q ? ps[k][q] : base;
+                    const uint32_t x = nn[k] + (uint32_t)__popcll(om & lt) - base;
+                    uint8_t *page = col.out + ((col.width == 4 ? first4 : first8) + pp) * kPage;
+                    if (col.width == 4) *reinterpret_cast<uint32_t *>(page + 4 + 4 * (size_t)x) = lo;
+                    else *reinterpret_cast<uint64_t *>(page + 8 + 8 * (size_t)x) = ((uint64_t)hi << 32) | lo;
                 }
+                // output bitmap: the chunk's result rows [mbase, mbase + tm) carry the
+                // packed ok bits; split at a page boundary
+                const uint64_t pk = pack_selected(matched, ok, mm);
+                if (lane == 0) {
+                    const uint32_t p0 = mbase / cap, r0 = mbase - p0 * cap;
+                    const uint32_t n0 = cap - r0 < tm ? cap - r0 : tm;
+                    if (p0 < kMaxPagesPerKeyPage) or_bits_at(obm[k][p0], r0, pk, n0);
+                    if (n0 < tm && p0 + 1 < kMaxPagesPerKeyPage) or_bits_at(obm[k][p0 + 1], 0, pk >> n0, tm - n0);
+                }
+                nn[k] += (uint32_t)__popcll(om);
+            }
+            mbase += tm;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // headers and bitmaps
+#pragma unroll
+    for (uint32_t k = 0; k < kBuildColsPerWave; ++k) {
+        if (k >= nc) break;
+        const BuildPageCol &col = a.cols[c_lo + k];
+        const uint32_t cap = col.width == 4 ? kCap4 : kCap8;
+        const uint32_t np = (mbase + cap - 1) / cap;
+#pragma unroll
+        for (uint32_t pp = 0; pp < kMaxPagesPerKeyPage; ++pp) {
+            if (pp >= np) break;
+            uint8_t *page = col.out + ((col.width == 4 ? first4 : first8) + pp) * kPage;
+            const uint32_t rows = mbase - pp * cap < cap ? mbase - pp * cap : cap;
+            const uint32_t nb = (rows + 7) >> 3;
+            const uint8_t *src = reinterpret_cast<const uint8_t *>(obm[k][pp]);
+            for (uint32_t b = lane; b < nb; b += 64) page[kPage - nb + b] = src[b];
+            if (lane == 0) {
+                uint32_t end = nn[k];
+#pragma unroll
+                for (uint32_t q = 1; q < kMaxPagesPerKeyPage; ++q) end = pp + 1 == q && q < np ? ps[k][q] : end;
+                *reinterpret_cast<uint16_t *>(page) = (uint16_t)rows;
+                *reinterpret_cast<uint16_t *>(page + 2) = (uint16_t)(end - ps[k][pp]);
             }
         }
-        __syncthreads();
     }
 }
 
@@ -237,13 +405,16 @@ void build_pages(const BuildPagesArgs &a, hipStream_t s) {
     const uint32_t units = a.key.num_pages;
     if (!units || !a.ncols) return;
     if (a.dt.stride % 16 || a.dt.stride == 0 || a.dt.stride > 80) throw InvalidError("bad direct-map record stride");
-    const uint32_t grid = units < kPagesGrid ? units : kPagesGrid;
-    switch (a.dt.stride / 16) {
-        case 1: k_build_pages<1><<<grid, kWG, 0, s>>>(a); break;
-        case 2: k_build_pages<2><<<grid, kWG, 0, s>>>(a); break;
-        case 3: k_build_pages<3><<<grid, kWG, 0, s>>>(a); break;
-        case 4: k_build_pages<4><<<grid, kWG, 0, s>>>(a); break;
-        default: k_build_pages<5><<<grid, kWG, 0, s>>>(a); break;
+    const uint32_t per = kWG / 64;
+    const uint32_t grid = (units + per - 1) / per;
+    for (uint32_t c_lo = 0; c_lo < a.ncols; c_lo += kBuildColsPerWave) {
+        switch (a.dt.stride / 16) {
+            case 1: k_build_pages<1><<<grid, kWG, 0, s>>>(a, c_lo); break;
+            case 2: k_build_pages<2><<<grid, kWG, 0, s>>>(a, c_lo); break;
+            case 3: k_build_pages<3><<<grid, kWG, 0, s>>>(a, c_lo); break;
+            case 4: k_build_pages<4><<<grid, kWG, 0, s>>>(a, c_lo); break;
+            default: k_build_pages<5><<<grid, kWG, 0, s>>>(a, c_lo); break;
+        }
     }
     CJ_HIP(hipGetLastError());
 }
@@ -380,7 +551,7 @@ __global__ __launch_bounds__(kWG) void k_probe_filter(FilterArgs a) {
     const uint32_t q = blockIdx.x;
     const uint64_t out_pg = a.pos[q];
     if (a.pos[q + 1] == out_pg) return;  // no matched row
-    if (ld16(a.pages + (size_t)q * kPage) <= kWaveMaxRows) return;  // k_probe_filter_wave's page
+    if (a.row_start[q + 1] - a.row_start[q] <= kWaveMaxRows) return;  // k_probe_filter_wave's page
     const uint4 *src = reinterpret_cast<const uint4 *>(a.pages + (size_t)q * kPage);
     for (uint32_t i = threadIdx.x; i < kPage / 16; i += kWG) stage[i] = src[i];
     const uint64_t rs = a.row_start[q];

@@ -35,6 +35,7 @@ struct BuildPageCol {
 struct BuildPagesArgs {
     KeySource key;  // columnar probe key column
     DirectTable dt;
+    const uint32_t *match_bits;  // from probe_count_match
     const uint64_t *p4_off;  // first output page per key page (exclusive scans)
     const uint64_t *p8_off;
     uint32_t ncols;

@@ -179,3 +179,34 @@ def test_gpu_rejects_bad_plans(engine):
     plan.nodes[-1]["attrs"] = [(1, "INT32")]  # declared type != source type
     with pytest.raises(abi.CJError):
         engine.execute(plan)
+
+
+@pytest.mark.parametrize("flags", PATHS, ids=PATH_IDS)
+@pytest.mark.parametrize("key_null,pay_null", [(0.3, 0.95), (0.6, 0.97), (0.0, 0.5)],
+                         ids=["sparse-payload", "sparse-keys", "half-null"])
+def test_gpu_sparse_pages(engine, key_null, pay_null, flags):
+    """Pages far beyond the dense row counts: 95-97 % NULL payloads give
+    pages of 20k-40k rows, 60 % NULL keys give key pages of several 2048-row
+    tiles; unique dense build keys take the direct-mapped build."""
+    rng = random.Random(int(key_null * 100) * 7 + int(pay_null * 100))
+    nb, npr = 3000, 60000
+    perm = list(range(nb))
+    rng.shuffle(perm)
+
+    def val(p, lo, hi):
+        return None if rng.random() < p else rng.randrange(lo, hi)
+
+    brows = [[perm[i], val(pay_null, -2**31, 2**31), val(pay_null, -2**63, 2**63)] for i in range(nb)]
+    prows = []
+    for _ in range(npr):
+        k = None if rng.random() < key_null else (rng.randrange(nb) if rng.random() < 0.85 else rng.randrange(nb, 10**6))
+        prows.append([k, val(pay_null, -2**31, 2**31), val(0.0, -2**63, 2**63),
+                      None if rng.random() < pay_null else rng.random() * 1e6])
+    plan = cjtest.Plan()
+    plan.new_input(cjtest.table_from_rows(["INT32", "INT32", "INT64"], brows))
+    plan.new_input(cjtest.table_from_rows(["INT32", "INT32", "INT64", "FP64"], prows))
+    plan.new_scan_node(0, [(0, "INT32"), (1, "INT32"), (2, "INT64")])
+    plan.new_scan_node(1, [(0, "INT32"), (1, "INT32"), (2, "INT64"), (3, "FP64")])
+    plan.root = plan.new_join_node(True, 0, 1, 0, 0, [(1, "INT32"), (2, "INT64"), (4, "INT32"), (5, "INT64"),
+                                                     (6, "FP64"), (3, "INT32")])
+    same_multiset(engine.execute(plan, flags=flags), cjtest.OracleRun(plan, threads=8))
