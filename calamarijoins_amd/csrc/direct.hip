@@ -6,77 +6,249 @@
 // key - min: a presence bitmap (one bit per key of the range, small enough
 // to stay in L2) answers the counting pass, and a record array holds, per
 // key, the build side's projected payload values with their NULL flags, so
-// the probe's write pass fetches one record per match instead of a
-// directory entry, a key, a row id and one value per build column.  The
-// join result is identical: a probe key matches exactly the build row
-// carrying that key.  Duplicate keys are detected while the table is built
-// and the executor then falls back to the hash table.
+// the probe fetches one 16-byte record per match instead of a directory
+// entry, a key, a row id and one value per build column.  The join result
+// is identical: a probe key matches exactly the build row carrying that key.
+//
+// Building takes no atomics on the table: records are zeroed, each build
+// row stores its record (one 16-byte store per chunk) with a present bit,
+// and a streaming pass turns the present bits into the bitmap, counting
+// them.  Two rows with the same key store to the same slot, so fewer present
+// slots than keyed rows means duplicates, and the executor falls back to the
+// hash table.
 #include "cj_internal.hpp"
 #include "device_util.hpp"
 #include "direct.hpp"
 
 namespace cj {
 
-__global__ void k_key_minmax(const int32_t *keys, const uint32_t *valid, uint64_t n, int32_t *mm) {
-    __shared__ int32_t smin[kWG], smax[kWG];
+// Each thread reduces 16 consecutive keys (four 16-byte loads, half a
+// validity word); a workgroup reduces through shuffles and LDS and writes
+// its partial to its own slot; one workgroup combines the partials.  (Many
+// workgroups updating one address with device atomics serialize.)
+__global__ __launch_bounds__(kWG) void k_key_minmax(const int32_t *keys, const uint32_t *valid, uint64_t n,
+                                                    int32_t *part) {
+    __shared__ int32_t red[2][kWG / 64];
     int32_t lo = INT32_MAX, hi = INT32_MIN;
-    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kWG)
-        if (valid_bit(valid, i)) {
-            int32_t k = keys[i];
-            lo = k < lo ? k : lo;
-            hi = k > hi ? k : hi;
+    const uint64_t groups = (n + 15) / 16;
+    for (uint64_t g = blockIdx.x * (uint64_t)kWG + threadIdx.x; g < groups; g += (uint64_t)gridDim.x * kWG) {
+        const uint64_t r0 = g * 16;
+        const uint32_t vw = valid[r0 >> 5] >> (r0 & 16);
+        if (r0 + 16 <= n) {
+            const int4 *k4 = reinterpret_cast<const int4 *>(keys + r0);
+            int4 q[4] = {k4[0], k4[1], k4[2], k4[3]};
+            const int32_t *k = reinterpret_cast<const int32_t *>(q);
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                if ((vw >> i) & 1u) {
+                    lo = min(lo, k[i]);
+                    hi = max(hi, k[i]);
+                }
+        } else {
+            for (uint64_t r = r0; r < n; ++r)
+                if ((vw >> (r - r0)) & 1u) {
+                    lo = min(lo, keys[r]);
+                    hi = max(hi, keys[r]);
+                }
         }
-    smin[threadIdx.x] = lo;
-    smax[threadIdx.x] = hi;
-    __syncthreads();
-    for (int s = kWG / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
-            smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
-        }
-        __syncthreads();
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    if (lane_id() == 0) {
+        red[0][wave_id()] = lo;
+        red[1][wave_id()] = hi;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        atomicMin(&mm[0], smin[0]);
-        atomicMax(&mm[1], smax[0]);
+        for (uint32_t w = 1; w < kWG / 64; ++w) {
+            lo = min(lo, red[0][w]);
+            hi = max(hi, red[1][w]);
+        }
+        part[2 * blockIdx.x] = lo;
+        part[2 * blockIdx.x + 1] = hi;
     }
 }
 
-void key_minmax(const int32_t *keys, const uint32_t *valid, uint64_t n, int32_t *mm, hipStream_t s) {
-    if (!n) return;
-    uint64_t g = (n + kWG - 1) / kWG;
-    k_key_minmax<<<(unsigned)(g < 2048 ? g : 2048), kWG, 0, s>>>(keys, valid, n, mm);
+__global__ __launch_bounds__(kWG) void k_minmax_final(const int32_t *part, uint32_t nparts, int32_t *mm) {
+    __shared__ int32_t red[2][kWG / 64];
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (uint32_t i = threadIdx.x; i < nparts; i += kWG) {
+        lo = min(lo, part[2 * i]);
+        hi = max(hi, part[2 * i + 1]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    if (lane_id() == 0) {
+        red[0][wave_id()] = lo;
+        red[1][wave_id()] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < kWG / 64; ++w) {
+            lo = min(lo, red[0][w]);
+            hi = max(hi, red[1][w]);
+        }
+        mm[0] = lo;
+        mm[1] = hi;
+    }
+}
+
+void key_minmax(const int32_t *keys, const uint32_t *valid, uint64_t n, int32_t *mm, int32_t *part,
+                hipStream_t s) {
+    const uint64_t groups = (n + 15) / 16;
+    uint64_t g = (groups + kWG - 1) / kWG;
+    g = g < kMinmaxParts ? g : kMinmaxParts;
+    if (g) k_key_minmax<<<(unsigned)g, kWG, 0, s>>>(keys, valid, n, part);
+    k_minmax_final<<<1, kWG, 0, s>>>(part, (uint32_t)g, mm);
     CJ_HIP(hipGetLastError());
 }
 
-// One thread per build row: set the key's presence bit (a bit already set
-// means a duplicate key) and write its record.
-__global__ void k_direct_build(DirectBuildArgs a) {
-    const uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x;
-    if (i >= a.rows) return;
-    if (!valid_bit(a.key_valid, i)) return;
-    const uint64_t slot = (uint64_t)((int64_t)a.keys[i] - (int64_t)a.min_key);
-    const uint32_t bit = 1u << (slot & 31);
-    const uint32_t old = atomicOr(&a.presence[slot >> 5], bit);
-    if (old & bit) {
-        atomicOr(a.dup_flag, 1u);
-        return;
+// Key range of the build side, or 0 when it has no key or exceeds max_range.
+__device__ __forceinline__ uint64_t direct_range(const DirectBuildArgs &a) {
+    const int32_t lo = a.minmax[0], hi = a.minmax[1];
+    if (lo > hi) return 0;
+    const uint64_t r = (uint64_t)((int64_t)hi - (int64_t)lo) + 1;
+    return r <= a.max_range ? r : 0;
+}
+
+__global__ __launch_bounds__(kWG) void k_direct_zero(DirectBuildArgs a) {
+    const uint64_t range = direct_range(a);
+    const uint64_t chunks = range * (a.stride / 16);
+    uint4 *r = reinterpret_cast<uint4 *>(a.records);
+    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < chunks; i += (uint64_t)gridDim.x * kWG)
+        r[i] = make_uint4(0, 0, 0, 0);
+}
+
+// Field f's value, placed at byte offset `off` of a record built in the
+// registers of 16-byte chunks q[] (offsets are uniform).
+template <int NQ>
+__device__ __forceinline__ void rec_put(uint4 (&q)[NQ], uint32_t off, uint32_t v) {
+    const uint32_t d = off >> 2;
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        if (d == 4u * c + 0) q[c].x = v;
+        if (d == 4u * c + 1) q[c].y = v;
+        if (d == 4u * c + 2) q[c].z = v;
+        if (d == 4u * c + 3) q[c].w = v;
     }
-    uint8_t *rec = a.records + slot * a.stride;
-    uint32_t flags = 0;
-    for (uint32_t f = 0; f < a.nfields; ++f) {
-        const DirectField &fd = a.fields[f];
-        const bool ok = valid_bit(fd.valid_bits, i);
-        flags |= (uint32_t)ok << f;
-        if (fd.width == 4) *reinterpret_cast<uint32_t *>(rec + fd.offset) = ok ? static_cast<const uint32_t *>(fd.values)[i] : 0u;
-        else *reinterpret_cast<uint64_t *>(rec + fd.offset) = ok ? static_cast<const uint64_t *>(fd.values)[i] : 0ull;
+}
+
+template <int NQ>
+__global__ __launch_bounds__(kWG) void k_direct_scatter(DirectBuildArgs a) {
+    __shared__ uint32_t red[kWG / 64];
+    const uint64_t range = direct_range(a);
+    uint32_t cnt = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; range && i < a.rows; i += (uint64_t)gridDim.x * kWG) {
+        if (!valid_bit(a.key_valid, i)) continue;
+        ++cnt;
+        const uint64_t slot = (uint64_t)((int64_t)a.keys[i] - (int64_t)a.minmax[0]);
+        uint4 q[NQ];
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) q[c] = make_uint4(0, 0, 0, 0);
+        uint32_t flags = 1u << kDirectPresentBit;
+        for (uint32_t f = 0; f < a.nfields; ++f) {
+            const DirectField &fd = a.fields[f];
+            const bool ok = valid_bit(fd.valid_bits, i);
+            flags |= (uint32_t)ok << f;
+            if (fd.width == 4) {
+                rec_put<NQ>(q, fd.offset, ok ? static_cast<const uint32_t *>(fd.values)[i] : 0u);
+            } else {
+                const uint64_t v = ok ? static_cast<const uint64_t *>(fd.values)[i] : 0ull;
+                rec_put<NQ>(q, fd.offset, (uint32_t)v);
+                rec_put<NQ>(q, fd.offset + 4, (uint32_t)(v >> 32));
+            }
+        }
+        rec_put<NQ>(q, a.flags_offset, flags);
+        uint4 *dst = reinterpret_cast<uint4 *>(a.records + slot * a.stride);
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) dst[c] = q[c];
     }
-    *reinterpret_cast<uint32_t *>(rec + a.flags_offset) = flags;
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if (lane_id() == 0) red[wave_id()] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kWG / 64; ++w) t += red[w];
+        a.partials[blockIdx.x] = t;
+    }
+}
+
+// 64 slots per wave step: ballot of their present bits -> two presence words.
+__global__ __launch_bounds__(kWG) void k_direct_presence(DirectBuildArgs a) {
+    __shared__ uint32_t red[kWG / 64];
+    const uint64_t range = direct_range(a);
+    const uint32_t lane = lane_id();
+    const uint64_t steps = (range + 63) / 64;
+    const uint64_t wave = blockIdx.x * (uint64_t)(kWG / 64) + wave_id();
+    const uint64_t waves = (uint64_t)gridDim.x * (kWG / 64);
+    uint32_t cnt = 0;
+    for (uint64_t st = wave; st < steps; st += waves) {
+        const uint64_t slot = st * 64 + lane;
+        const uint64_t sc = slot < range ? slot : range - 1;
+        const uint32_t fl = *reinterpret_cast<const uint32_t *>(a.records + sc * a.stride + a.flags_offset);
+        const uint64_t b = __ballot(slot < range && ((fl >> kDirectPresentBit) & 1u));
+        if (lane < 2) a.presence[2 * st + lane] = (uint32_t)(b >> (32 * lane));
+        cnt += (uint32_t)__popcll(b);
+    }
+    if (lane == 0) red[wave_id()] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kWG / 64; ++w) t += red[w];
+        a.partials[kDirectParts + blockIdx.x] = t;
+    }
+}
+
+// counters[0] = keyed rows, counters[1] = present slots.
+__global__ __launch_bounds__(kWG) void k_direct_finish(DirectBuildArgs a, uint32_t nscatter, uint32_t npresence) {
+    __shared__ uint32_t red[2][kWG / 64];
+    uint32_t k = 0, p = 0;
+    for (uint32_t i = threadIdx.x; i < nscatter; i += kWG) k += a.partials[i];
+    for (uint32_t i = threadIdx.x; i < npresence; i += kWG) p += a.partials[kDirectParts + i];
+    for (int o = 32; o > 0; o >>= 1) {
+        k += __shfl_xor(k, o, 64);
+        p += __shfl_xor(p, o, 64);
+    }
+    if (lane_id() == 0) {
+        red[0][wave_id()] = k;
+        red[1][wave_id()] = p;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        k = p = 0;
+        for (uint32_t w = 0; w < kWG / 64; ++w) {
+            k += red[0][w];
+            p += red[1][w];
+        }
+        a.counters[0] = k;
+        a.counters[1] = p;
+    }
 }
 
 void direct_build(const DirectBuildArgs &a, hipStream_t s) {
-    if (!a.rows) return;
-    k_direct_build<<<(unsigned)((a.rows + kWG - 1) / kWG), kWG, 0, s>>>(a);
+    if (!a.rows || !a.max_range) return;
+    if (a.stride % 16 || a.stride == 0 || a.stride > 80) throw InvalidError("bad direct-map record stride");
+    const uint64_t chunks = a.max_range * (a.stride / 16);
+    const uint64_t zg = (chunks + kWG - 1) / kWG;
+    k_direct_zero<<<(unsigned)(zg < 8192 ? zg : 8192), kWG, 0, s>>>(a);
+    uint64_t sg = (a.rows + kWG - 1) / kWG;
+    sg = sg < kDirectParts ? sg : kDirectParts;
+    switch (a.stride / 16) {
+        case 1: k_direct_scatter<1><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+        case 2: k_direct_scatter<2><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+        case 3: k_direct_scatter<3><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+        case 4: k_direct_scatter<4><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+        default: k_direct_scatter<5><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+    }
+    const uint64_t steps = (a.max_range + 63) / 64;
+    uint64_t pg = (steps + kWG / 64 - 1) / (kWG / 64);
+    pg = pg < kDirectParts ? pg : kDirectParts;
+    k_direct_presence<<<(unsigned)pg, kWG, 0, s>>>(a);
+    k_direct_finish<<<1, kWG, 0, s>>>(a, (uint32_t)sg, (uint32_t)pg);
     CJ_HIP(hipGetLastError());
 }
 

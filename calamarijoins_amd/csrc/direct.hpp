@@ -5,6 +5,9 @@
 namespace cj {
 
 constexpr uint32_t kDirectMaxFields = 8;
+constexpr uint32_t kDirectPresentBit = 31;  // flags bit: the slot holds a build row
+constexpr uint32_t kDirectParts = 1024;     // workgroup partials of each build pass
+constexpr uint32_t kMinmaxParts = 1024;     // workgroup partials of key_minmax
 
 struct DirectField {
     int32_t width;  // 4 or 8
@@ -17,12 +20,14 @@ struct DirectBuildArgs {
     const int32_t *keys;  // dense decoded build key column
     const uint32_t *key_valid;
     uint64_t rows;
-    int32_t min_key;
-    uint32_t *presence;   // ceil(range / 32) words, zeroed
-    uint8_t *records;     // range * stride bytes
-    uint32_t stride;
-    uint32_t flags_offset;  // u32 of per-field non-NULL bits
-    uint32_t *dup_flag;
+    const int32_t *minmax;  // device [min, max] of the non-NULL keys (key_minmax)
+    uint64_t max_range;     // capacity of presence / records, in slots
+    uint32_t *presence;     // max_range / 32 + 2 words
+    uint8_t *records;       // max_range * stride bytes
+    uint32_t stride;        // multiple of 16
+    uint32_t flags_offset;  // u32: bit f = field f non-NULL, bit 31 = present
+    uint32_t *counters;     // [0] build rows with a key, [1] slots present
+    uint32_t *partials;     // 2 * kDirectParts scratch words
     uint32_t nfields;
     DirectField fields[kDirectMaxFields];
 };
@@ -37,7 +42,13 @@ struct DirectTable {
     uint32_t flags_offset;
 };
 
-void key_minmax(const int32_t *keys, const uint32_t *valid, uint64_t n, int32_t *mm, hipStream_t s);
+// mm[0] = min, mm[1] = max of the non-NULL keys (INT32_MAX, INT32_MIN when
+// there is none); `part` holds 2 * kMinmaxParts scratch ints.
+void key_minmax(const int32_t *keys, const uint32_t *valid, uint64_t n, int32_t *mm, int32_t *part,
+                hipStream_t s);
+// Zero the records of the key range, scatter one record per build row, then
+// derive the presence bitmap from the records.  Nothing is written when the
+// key range exceeds max_range.  Keys are unique iff counters[0] == counters[1].
 void direct_build(const DirectBuildArgs &a, hipStream_t s);
 
 }  // namespace cj

@@ -852,26 +852,13 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     if (flags & CJ_EXEC_NO_DIRECT) return false;
     DevColumn *kc = column(B.slots[0].table, B.cols[battr].col);
     if (kc->num_rows == 0) return false;
-    ColView kv = view(kc);
-    int32_t *mm = tmp<int32_t>(2);
-    fill_u32(reinterpret_cast<uint32_t *>(mm), 0x7fffffffu, 1, st);
-    fill_u32(reinterpret_cast<uint32_t *>(mm) + 1, 0x80000000u, 1, st);
-    {
-        TimedScope ts(timer, "key_minmax", kc->num_rows * 4);
-        key_minmax(static_cast<const int32_t *>(kv.values), kv.valid, kc->num_rows, mm, st);
-    }
-    int32_t h[2];
-    CJ_HIP(hipMemcpyAsync(h, mm, 8, hipMemcpyDeviceToHost, st));
-    CJ_HIP(hipStreamSynchronize(st));
-    if (h[0] > h[1]) return false;  // no non-NULL key
-    const uint64_t range = (uint64_t)((int64_t)h[1] - (int64_t)h[0]) + 1;
-    if (range > 2 * kc->num_rows + 1024) return false;
     // record: 8-byte fields, then 4-byte fields, then the u32 of NULL flags
     DirectBuildArgs a{};
     std::vector<uint32_t> build_attrs;
     for (uint32_t i = 0; i < n.num_output_attrs; ++i)
         if (remap[i] < bw) build_attrs.push_back(i);
     if (build_attrs.size() > kDirectMaxFields) return false;
+    ColView kv = view(kc);
     direct_field.assign(n.num_output_attrs, {0u, 0u});
     uint32_t off = 0;
     for (int pass = 0; pass < 2; ++pass)
@@ -892,22 +879,35 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     a.keys = static_cast<const int32_t *>(kv.values);
     a.key_valid = kv.valid;
     a.rows = kc->num_rows;
-    a.min_key = h[0];
-    const uint64_t words = range / 32 + 2;
-    a.presence = tmp<uint32_t>(words);
-    a.records = tmp<uint8_t>(range * a.stride);
-    a.dup_flag = tmp<uint32_t>(1);
-    fill_u32(a.presence, 0, words, st);
-    fill_u32(a.dup_flag, 0, 1, st);
+    // the table is sized for the widest range the direct map accepts, so the
+    // key range, the build and the duplicate check need one read-back
+    a.max_range = 2 * kc->num_rows + 1024;
+    uint32_t *st4 = tmp<uint32_t>(4);  // min, max, keyed rows, present slots (written by kernels)
+    int32_t *mm = reinterpret_cast<int32_t *>(st4);
     {
-        TimedScope ts(timer, "direct_build", kc->num_rows * (8 + a.stride));
+        TimedScope ts(timer, "key_minmax", kc->num_rows * 4 + kc->num_rows / 8);
+        key_minmax(a.keys, a.key_valid, kc->num_rows, mm, tmp<int32_t>(2 * kMinmaxParts), st);
+    }
+    a.minmax = mm;
+    a.counters = st4 + 2;
+    a.partials = tmp<uint32_t>(2 * kDirectParts);
+    const uint64_t words = a.max_range / 32 + 4;
+    a.presence = tmp<uint32_t>(words);
+    a.records = tmp<uint8_t>(a.max_range * a.stride);
+    {
+        // zeroed records + one record per row + the bitmap pass over the records
+        TimedScope ts(timer, "direct_build", kc->num_rows * (4ull + a.stride + a.stride) + kc->num_rows / 8);
         direct_build(a, st);
     }
-    uint32_t dup = 0;
-    CJ_HIP(hipMemcpyAsync(&dup, a.dup_flag, 4, hipMemcpyDeviceToHost, st));
+    uint32_t h[4];
+    CJ_HIP(hipMemcpyAsync(h, st4, 16, hipMemcpyDeviceToHost, st));
     CJ_HIP(hipStreamSynchronize(st));
-    if (dup) return false;
-    *dt = DirectTable{h[0], range, a.presence, a.records, a.stride, a.flags_offset};
+    const int32_t lo = (int32_t)h[0], hi = (int32_t)h[1];
+    if (lo > hi) return false;  // no non-NULL key
+    const uint64_t range = (uint64_t)((int64_t)hi - (int64_t)lo) + 1;
+    if (range > a.max_range) return false;
+    if (h[2] != h[3]) return false;  // duplicate keys
+    *dt = DirectTable{lo, range, a.presence, a.records, a.stride, a.flags_offset};
     return true;
 }
 
