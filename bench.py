@@ -5,23 +5,41 @@ in HBM.
 
 One step = one cj_execute() of the plan: key-column decode and the build
 structure (direct map for the unique dense build keys, else radix-partitioned
-hash table), probe count + pair write, and the page writers of the 5 payload
+hash table), the counting pass, and the page writers of the 5 payload
 columns.  Prints one JSON line with the step throughput, the roofline of the
-dominant kernel and a CPU baseline.
+dominant kernel, a measured device-copy ceiling and a CPU baseline.
+
+Algorithmic bytes follow the per-unit definition used throughout: every
+input page a step references is read once (8192 B), every output value and
+bitmap byte is written once; build structures, match bitmaps and row-id lists
+are not counted.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import platform
+import re
 import statistics
 import sys
 import time
+
+import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 METRIC = "join output rows/s + HBM GB/s (1 GPU); p50/p99 plan latency at fixed QPS"
 HBM_PEAK_GBS = 8000.0
+PAGE = 8192
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "c2_join_4Mx64M_pmc_traffic.md")
+# timed scope of the engine -> HIP kernel it launches (for the PMC traffic table)
+SCOPE_KERNEL = {
+    "build_pages": "k_build_pages",
+    "probe_count": "k_count_match_direct",
+    "probe_filter_w4": "k_probe_filter_wave<4>",
+    "probe_filter_w8": "k_probe_filter_wave<8>",
+}
 
 
 def log(msg):
@@ -36,6 +54,91 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor()
+
+
+class Hip:
+    """The few HIP runtime calls the copy ceiling needs (libamdhip64)."""
+
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so")
+        for n in ("hipMalloc", "hipFree", "hipMemcpy", "hipMemset", "hipEventCreate", "hipEventRecord",
+                  "hipEventSynchronize", "hipEventElapsedTime", "hipEventDestroy", "hipDeviceSynchronize"):
+            getattr(self.lib, n).restype = C.c_int
+        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.lib.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        self.lib.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.lib.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+
+    def ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    def copy_ms(self, nbytes, reps):
+        """Best time of a device-to-device hipMemcpy of nbytes (a plain copy kernel)."""
+        a, b = C.c_void_p(), C.c_void_p()
+        self.ok(self.lib.hipMalloc(C.byref(a), C.c_size_t(nbytes)), "hipMalloc")
+        self.ok(self.lib.hipMalloc(C.byref(b), C.c_size_t(nbytes)), "hipMalloc")
+        e0, e1 = C.c_void_p(), C.c_void_p()
+        self.ok(self.lib.hipEventCreate(C.byref(e0)), "hipEventCreate")
+        self.ok(self.lib.hipEventCreate(C.byref(e1)), "hipEventCreate")
+        try:
+            self.ok(self.lib.hipMemset(a, 1, nbytes), "hipMemset")
+            self.ok(self.lib.hipMemcpy(b, a, nbytes, 3), "hipMemcpy")  # warm
+            best = float("inf")
+            for _ in range(reps):
+                self.ok(self.lib.hipEventRecord(e0, None), "hipEventRecord")
+                self.ok(self.lib.hipMemcpy(b, a, nbytes, 3), "hipMemcpy")
+                self.ok(self.lib.hipEventRecord(e1, None), "hipEventRecord")
+                self.ok(self.lib.hipEventSynchronize(e1), "hipEventSynchronize")
+                ms = C.c_float()
+                self.ok(self.lib.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+                best = min(best, ms.value)
+            return best
+        finally:
+            self.lib.hipEventDestroy(e0)
+            self.lib.hipEventDestroy(e1)
+            self.lib.hipFree(a)
+            self.lib.hipFree(b)
+
+
+COPY_BYTES = 1 << 30
+
+
+def copy_calibration():
+    """Three device-to-device copies of COPY_BYTES: rocprofv3's FETCH_SIZE and
+    WRITE_SIZE of this copy kernel should read COPY_BYTES / 1024 KiB each, and
+    the ratio scales the same counters of the join kernels to bytes."""
+    Hip().copy_ms(COPY_BYTES, 3)
+    print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "kib": COPY_BYTES // 1024}), flush=True)
+
+
+def page_payload_bytes(ptr, num_pages, type_id):
+    """Non-NULL value bytes + bitmap bytes of a fixed-width result column
+    whose pages are contiguous in host memory starting at ptr."""
+    if not num_pages:
+        return 0
+    buf = np.ctypeslib.as_array((C.c_uint8 * (num_pages * PAGE)).from_address(ptr)).reshape(num_pages, PAGE)
+    n = buf[:, 0].astype(np.int64) | (buf[:, 1].astype(np.int64) << 8)
+    nv = buf[:, 2].astype(np.int64) | (buf[:, 3].astype(np.int64) << 8)
+    w = 4 if type_id == 0 else 8
+    return int((nv * w).sum() + ((n + 7) // 8).sum())
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration)."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0].startswith(kernel):
+            try:
+                return float(cells[-1])
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(build, probe_rows, seed, threads, repeats):
@@ -62,26 +165,10 @@ def cpu_baseline(build, probe_rows, seed, threads, repeats):
         "unit": "rows/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"4M build x first {probe_rows // 1_000_000}M probe rows of the same workload; "
-                  f"1 cold + {repeats} timed runs, median {med * 1e3:.0f} ms; {cpu_model()}",
+        "sample": f"CPU restatement of execute() (reported baseline only): 4M build x first "
+                  f"{probe_rows // 1_000_000}M probe rows of the same workload; 1 cold + {repeats} timed runs, "
+                  f"median {med * 1e3:.0f} ms; {cpu_model()}",
     }
-
-
-COPY_BYTES = 1 << 30
-
-
-def copy_calibration():
-    """Three device-to-device copies of COPY_BYTES: rocprofv3's FETCH_SIZE and
-    WRITE_SIZE of this copy kernel should read COPY_BYTES / 1024 KiB each, and
-    the ratio scales the same counters of the join kernels to bytes."""
-    import torch
-    src = torch.empty(COPY_BYTES, dtype=torch.uint8, device="cuda").fill_(1)
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "kib": COPY_BYTES // 1024}), flush=True)
 
 
 def main():
@@ -121,11 +208,23 @@ def main():
     ids = [eng.register(build), eng.register(probe)]
     log(f"registered tables in HBM in {time.time() - t0:.1f}s")
 
+    # runs with the result copied to host: exact output bytes per column and
+    # the end-to-end time including the D2H copy (reported, never `value`;
+    # the second run reuses the pinned output buffers of the first)
+    eng.execute(plan, flags=0).release()
+    t0 = time.perf_counter()
+    host = eng.execute(plan, flags=0, timing=True)
+    e2e_ms = (time.perf_counter() - t0) * 1e3
+    out_rows = host.num_rows
+    out_bytes = [page_payload_bytes(host.page_lists[i][0] if host.page_lists[i] else 0, n, t)
+                 for i, (n, t) in enumerate(zip(host.num_pages(), host.types))]
+    d2h_ms = host.timing.d2h_ms
+    host.release()
     flags = abi.CJ_EXEC_OUTPUT_DEVICE
     first = eng.execute(plan, flags=flags, timing=True)
-    out_rows = first.num_rows
     bytes_alg = first.timing.bytes_alg
-    log(f"first run: {out_rows} rows, device {first.timing.device_ms:.3f} ms, bytes_alg {bytes_alg / 1e9:.3f} GB")
+    log(f"first run: {out_rows} rows, device {first.timing.device_ms:.3f} ms, bytes_alg {bytes_alg / 1e9:.3f} GB, "
+        f"output bytes per column {out_bytes}")
     first.release()
     for _ in range(args.warmup):
         eng.execute(plan, flags=flags).release()
@@ -141,9 +240,10 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     value = out_rows / (ms_per_step / 1e3)
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, device median {statistics.median(dev_ms):.3f} ms")
+    dev_med = statistics.median(dev_ms)
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, device median {dev_med:.3f} ms")
 
-    # per-kernel roofline from hipEvents on the execution stream
+    # per-kernel durations from hipEvents on the execution stream
     eng.close()
     eng = Engine(cache_decoded=0, kernel_timing=1)
     ids = [eng.register(build), eng.register(probe)]
@@ -154,15 +254,31 @@ def main():
         eng.execute(plan, flags=flags).release()
     stats = eng.kernel_stats()
     eng.close()
-    dominant = max(stats.items(), key=lambda kv: kv[1][1])
-    kname, (launches, tot_ms, kbytes) = dominant
-    avg_ms = tot_ms / launches
-    per_launch_bytes = kbytes / launches
-    achieved = per_launch_bytes / (avg_ms / 1e3) / 1e9
-    kernels = {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4),
-                   "GBps": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[1] > 0 else None}
+    kernels = {k: {"launches": v[0] // prof_steps, "avg_ms": round(v[1] / v[0], 4),
+                   "compulsory_GBps": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[1] > 0 else None}
                for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])}
     log("kernels: " + json.dumps(kernels))
+
+    # roofline of the dominant kernel: algorithmic bytes of one launch
+    kname, (launches, tot_ms, kbytes) = max(stats.items(), key=lambda kv: kv[1][1])
+    avg_ms = tot_ms / launches
+    in_pages = [c.num_pages * PAGE for c in probe.columns]  # key, i32, i64, f64
+    per_launch = {
+        "build_pages": in_pages[0] + out_bytes[0] + out_bytes[1],
+        "probe_count": in_pages[0],
+        "probe_filter_w4": in_pages[1] + out_bytes[2],
+        "probe_filter_w8": (in_pages[2] + out_bytes[3] + in_pages[3] + out_bytes[4]) / 2,
+    }
+    alg = per_launch.get(kname, kbytes / launches)
+    achieved = alg / (avg_ms / 1e3) / 1e9
+    traffic = pmc_traffic(SCOPE_KERNEL[kname]) if kname in SCOPE_KERNEL else None
+
+    copy_gbps = None
+    try:
+        cms = Hip().copy_ms(COPY_BYTES, 5)
+        copy_gbps = 2 * COPY_BYTES / (cms / 1e3) / 1e9
+    except Exception as e:  # reported only
+        log(f"copy ceiling unavailable: {e}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -172,7 +288,6 @@ def main():
         except Exception as e:  # the oracle is optional on the timing box
             log(f"cpu baseline unavailable: {e}")
 
-    dev_med = statistics.median(dev_ms)
     line = {
         "metric": METRIC,
         "value": value,
@@ -199,16 +314,18 @@ def main():
         "bytes_alg_per_step": bytes_alg,
         "hbm_GBps_alg": bytes_alg / (dev_med / 1e3) / 1e9,
         "hbm_frac_alg": bytes_alg / (dev_med / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "copy_ceiling_GBps": copy_gbps,
+        "end_to_end_host_output": {"ms": e2e_ms, "d2h_ms": d2h_ms, "rows_per_s": out_rows / (e2e_ms / 1e3)},
         "roofline": {
-            "kernel": kname,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "kernel": kname,
             "avg_launch_ms": avg_ms,
-            "alg_bytes_per_launch": per_launch_bytes,
+            "alg_bytes_per_launch": alg,
         },
         "kernels": kernels,
         "cpu_baseline": cpu,

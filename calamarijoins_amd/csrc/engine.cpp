@@ -1014,7 +1014,9 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
     ba.key = a.key;
     ba.dt = a.dt;
     ba.match_bits = match_bits;
-    uint64_t bbytes = key_bytes(a.key) + total * a.dt.stride;
+    // algorithmic bytes: key pages read once + output values and bitmaps (the
+    // direct-map records are a build structure and are not counted)
+    uint64_t bbytes = key_bytes(a.key);
     for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
         if (remap[i] >= bw) continue;
         if (!ba.p4_off) {
@@ -1073,7 +1075,7 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
         count_at.push_back(pos + c->num_pages);
         count_col.push_back(i);
         TimedScope ts(timer, f.width == 4 ? "probe_filter_w4" : "probe_filter_w8",
-                      (uint64_t)c->num_pages * kPage + c->num_rows / 8 + total * (uint64_t)f.width + total / 8);
+                      (uint64_t)c->num_pages * kPage + total * (uint64_t)f.width + total / 8);
         probe_filter(f, st);
     }
     const std::vector<uint64_t> np = read_u64s(count_at);

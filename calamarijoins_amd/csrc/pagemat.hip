@@ -235,6 +235,56 @@ constexpr uint32_t kBuildColsPerWave = 4;  // build columns one launch writes
 constexpr uint32_t kBuildBatch = 8;        // 64-row chunks whose loads are issued together
 constexpr uint32_t kObmWords = kCap4 / 32 + 1;
 
+// One build column's part of a 64-row chunk: the chunk's matched rows are
+// result rows [mbase, mbase + tm) of the key page (ko per lane).  The page
+// capacity is a compile-time constant, so the page arithmetic is a multiply
+// and a shift rather than a division.
+template <int NQ, int W>
+__device__ __forceinline__ void emit_col(const BuildPageCol &col, const uint4 (&rec)[NQ], uint32_t fl_dw,
+                                         uint64_t first, bool matched, uint64_t mm, uint32_t ko, uint32_t mbase,
+                                         uint32_t tm, uint32_t *nn_l, uint32_t *ps,
+                                         uint32_t (*obm)[kObmWords], uint32_t lane, uint64_t lt) {
+    // nn_l: values so far, ps[q]: values before page q (the wave's LDS slice;
+    // written by lane 0, read back by the wave in order)
+    const uint32_t nn = *nn_l;
+    constexpr uint32_t cap = W == 4 ? kCap4 : kCap8;
+    const uint32_t lo = rec_dword<NQ>(rec, col.rec_offset >> 2);
+    const uint32_t hi = W == 8 ? rec_dword<NQ>(rec, (col.rec_offset >> 2) + 1) : 0u;
+    const uint32_t fl = rec_dword<NQ>(rec, fl_dw);
+    bool ok = matched && ((fl >> col.rec_bit) & 1u);
+    if (W == 4 && col.is_int32 && (int32_t)lo == kI32Null) ok = false;  // page_builders.h:123
+    const uint64_t om = __ballot(ok);
+    // a page of this column starts inside the chunk: record the values before it
+    const uint32_t b = (mbase + cap - 1) / cap * cap;
+    if (b < mbase + tm) {
+        const uint64_t before = __ballot(matched && ko < b);
+        const uint32_t v = nn + (uint32_t)__popcll(om & before);
+        const uint32_t pi = b / cap;
+        if (lane == 0 && pi < kMaxPagesPerKeyPage) ps[pi] = v;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (ok) {
+        const uint32_t pp = ko / cap;
+        const uint32_t base = ps[pp < kMaxPagesPerKeyPage ? pp : 0];
+        const uint32_t x = nn + (uint32_t)__popcll(om & lt) - base;
+        uint8_t *page = col.out + (first + pp) * kPage;
+        if (W == 4) *reinterpret_cast<uint32_t *>(page + 4 + 4 * (size_t)x) = lo;
+        else *reinterpret_cast<uint64_t *>(page + 8 + 8 * (size_t)x) = ((uint64_t)hi << 32) | lo;
+    }
+    // output bitmap: result rows [mbase, mbase + tm) carry the packed ok bits;
+    // split at a page boundary
+    const uint64_t pk = pack_selected(matched, ok, mm);
+    if (lane == 0) {
+        const uint32_t p0 = mbase / cap, r0 = mbase - p0 * cap;
+        const uint32_t n0 = cap - r0 < tm ? cap - r0 : tm;
+        if (p0 < kMaxPagesPerKeyPage) or_bits_at(obm[p0], r0, pk, n0);
+        if (n0 < tm && p0 + 1 < kMaxPagesPerKeyPage) or_bits_at(obm[p0 + 1], 0, pk >> n0, tm - n0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) *nn_l = nn + (uint32_t)__popcll(om);
+    __builtin_amdgcn_wave_barrier();
+}
+
 // One wave per key page, no barriers.  For each batch of 8 chunks of 64 rows:
 // bitmap bytes (sparse pages), keys, presence words and records are loaded
 // with clamped addresses so every load of a phase is in flight together;
@@ -245,6 +295,8 @@ constexpr uint32_t kObmWords = kCap4 / 32 + 1;
 template <int NQ>
 __global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a, uint32_t c_lo) {
     __shared__ uint32_t obm_all[kWG / 64][kBuildColsPerWave][kMaxPagesPerKeyPage][kObmWords];
+    __shared__ uint32_t nn_all[kWG / 64][kBuildColsPerWave];
+    __shared__ uint32_t ps_all[kWG / 64][kBuildColsPerWave][kMaxPagesPerKeyPage];
     const uint32_t lane = lane_id();
     const uint32_t pg = blockIdx.x * (kWG / 64) + wave_id();
     if (pg >= a.key.num_pages) return;
@@ -257,6 +309,10 @@ __global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a, uint32_t 
     uint32_t(*obm)[kMaxPagesPerKeyPage][kObmWords] = obm_all[wave_id()];
     for (uint32_t i = lane; i < kBuildColsPerWave * kMaxPagesPerKeyPage * kObmWords; i += 64)
         (&obm[0][0][0])[i] = 0;
+    uint32_t *nn = nn_all[wave_id()];
+    uint32_t(*ps)[kMaxPagesPerKeyPage] = ps_all[wave_id()];
+    if (lane < kBuildColsPerWave) nn[lane] = 0;
+    if (lane < kBuildColsPerWave * kMaxPagesPerKeyPage) (&ps[0][0])[lane] = 0;
     __builtin_amdgcn_wave_barrier();
     const uint64_t first4 = a.p4_off[pg], first8 = a.p8_off[pg];
     const uint64_t r0 = a.key.row_start[pg];
@@ -266,8 +322,6 @@ __global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a, uint32_t 
     const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
     const uint32_t fl_dw = a.dt.flags_offset >> 2;
     uint32_t vbase = 0, mbase = 0;
-    uint32_t nn[kBuildColsPerWave] = {0, 0, 0, 0};
-    uint32_t ps[kBuildColsPerWave][kMaxPagesPerKeyPage] = {};
     for (uint32_t c0 = 0; c0 < n; c0 += 64 * kBuildBatch) {
         uint32_t vbits = 0;
         uint32_t vi[kBuildBatch];
@@ -330,46 +384,13 @@ __global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a, uint32_t 
             if (!mm) continue;
             const uint32_t ko = mbase + (uint32_t)__popcll(mm & lt);
             const uint32_t tm = (uint32_t)__popcll(mm);
-#pragma unroll
-            for (uint32_t k = 0; k < kBuildColsPerWave; ++k) {
-                if (k >= nc) break;
+#pragma unroll 1
+            for (uint32_t k = 0; k < nc; ++k) {
                 const BuildPageCol &col = a.cols[c_lo + k];
-                const uint32_t cap = col.width == 4 ? kCap4 : kCap8;
-                const uint32_t lo = rec_dword<NQ>(rec[c], col.rec_offset >> 2);
-                const uint32_t hi = col.width == 8 ? rec_dword<NQ>(rec[c], (col.rec_offset >> 2) + 1) : 0u;
-                const uint32_t fl = rec_dword<NQ>(rec[c], fl_dw);
-                bool ok = matched && ((fl >> col.rec_bit) & 1u);
-                if (col.is_int32 && (int32_t)lo == kI32Null) ok = false;  // page_builders.h:123
-                const uint64_t om = __ballot(ok);
-                // a page of this column starts inside the chunk: values before it
-                const uint32_t b = (mbase + cap - 1) / cap * cap;
-                if (b < mbase + tm && b / cap < kMaxPagesPerKeyPage) {
-                    const uint64_t before = __ballot(matched && ko < b);
-                    const uint32_t v = nn[k] + (uint32_t)__popcll(om & before);
-#pragma unroll
-                    for (uint32_t q = 0; q < kMaxPagesPerKeyPage; ++q)
-                        if (q == b / cap) ps[k][q] = v;
-                }
-                if (ok) {
-                    const uint32_t pp = ko / cap;
-                    uint32_t base = ps[k][0];
-#pragma unroll
-                    for (uint32_t q = 1; q < kMaxPagesPerKeyPage; ++q) base = pp == q ? ps[k][q] : base;
-                    const uint32_t x = nn[k] + (uint32_t)__popcll(om & lt) - base;
-                    uint8_t *page = col.out + ((col.width == 4 ? first4 : first8) + pp) * kPage;
-                    if (col.width == 4) *reinterpret_cast<uint32_t *>(page + 4 + 4 * (size_t)x) = lo;
-                    else *reinterpret_cast<uint64_t *>(page + 8 + 8 * (size_t)x) = ((uint64_t)hi << 32) | lo;
-                }
-                // output bitmap: the chunk's result rows [mbase, mbase + tm) carry the
-                // packed ok bits; split at a page boundary
-                const uint64_t pk = pack_selected(matched, ok, mm);
-                if (lane == 0) {
-                    const uint32_t p0 = mbase / cap, r0 = mbase - p0 * cap;
-                    const uint32_t n0 = cap - r0 < tm ? cap - r0 : tm;
-                    if (p0 < kMaxPagesPerKeyPage) or_bits_at(obm[k][p0], r0, pk, n0);
-                    if (n0 < tm && p0 + 1 < kMaxPagesPerKeyPage) or_bits_at(obm[k][p0 + 1], 0, pk >> n0, tm - n0);
-                }
-                nn[k] += (uint32_t)__popcll(om);
+                if (col.width == 4)
+                    emit_col<NQ, 4>(col, rec[c], fl_dw, first4, matched, mm, ko, mbase, tm, &nn[k], ps[k], obm[k], lane, lt);
+                else
+                    emit_col<NQ, 8>(col, rec[c], fl_dw, first8, matched, mm, ko, mbase, tm, &nn[k], ps[k], obm[k], lane, lt);
             }
             mbase += tm;
         }
