@@ -13,6 +13,7 @@ import os
 PAGE_SIZE = 8192
 
 CJ_OK = 0
+CJ_ERR_INVALID, CJ_ERR_UNSUPPORTED, CJ_ERR_DEVICE, CJ_ERR_OOM, CJ_ERR_BUSY = -1, -2, -3, -4, -5
 CJ_INT32, CJ_INT64, CJ_FP64, CJ_VARCHAR = 0, 1, 2, 3
 CJ_NODE_JOIN, CJ_NODE_SCAN = 0, 1
 CJ_EXEC_OUTPUT_DEVICE = 1
@@ -73,7 +74,13 @@ class Timing(C.Structure):
         ("h2d_ms", C.c_double),
         ("d2h_ms", C.c_double),
         ("bytes_alg", C.c_uint64),
+        ("queue_ms", C.c_double),
+        ("service_ms", C.c_double),
     ]
+
+
+class ServeOptions(C.Structure):
+    _fields_ = [("workers", C.c_int32), ("admit_bytes", C.c_uint64)]
 
 
 class ResultColumn(C.Structure):
@@ -118,6 +125,7 @@ EXPORTS = [
     "cj_table_register", "cj_table_unregister",
     "cj_execute", "cj_result_release", "cj_result_to_host",
     "cj_kernel_stats", "cj_kernel_stats_reset",
+    "cj_serve_start", "cj_submit", "cj_wait", "cj_serve_stop", "cj_plan_footprint",
 ]
 
 _lib = None
@@ -146,6 +154,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "cj_result_to_host": (C.c_int, [C.c_void_p, P(Result)]),
         "cj_kernel_stats": (C.c_int, [C.c_void_p, P(KernelStat), C.c_uint32, P(C.c_uint32)]),
         "cj_kernel_stats_reset": (C.c_int, [C.c_void_p]),
+        "cj_serve_start": (C.c_int, [C.c_void_p, P(ServeOptions)]),
+        "cj_submit": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, P(C.c_uint64)]),
+        "cj_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, P(Result), P(Timing)]),
+        "cj_serve_stop": (C.c_int, [C.c_void_p]),
+        "cj_plan_footprint": (C.c_int, [P(PlanDesc), P(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -2,8 +2,12 @@
 // include/cj_api.h).  Exceptions never cross the boundary: they become a
 // negative status plus a per-thread message for cj_last_error().
 #include "engine.hpp"
+#include "serve.hpp"
 
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -24,6 +28,9 @@ int guard(F &&f) {
     } catch (const cj::OomError &e) {
         g_err = e.what();
         return CJ_ERR_OOM;
+    } catch (const cj::BusyError &e) {
+        g_err = e.what();
+        return CJ_ERR_BUSY;
     } catch (const std::bad_alloc &e) {
         g_err = "host allocation failed";
         return CJ_ERR_OOM;
@@ -40,6 +47,29 @@ int guard(F &&f) {
 }
 
 cj::Context *C(cj_ctx *c) { return reinterpret_cast<cj::Context *>(c); }
+
+// serving loops by context
+std::mutex g_srv_mu;
+std::map<cj::Context *, std::shared_ptr<cj::Server>> g_servers;
+
+std::shared_ptr<cj::Server> server_of(cj_ctx *ctx) {
+    std::lock_guard<std::mutex> g(g_srv_mu);
+    auto it = g_servers.find(C(ctx));
+    if (it == g_servers.end()) throw cj::InvalidError("serving loop not started on this context");
+    return it->second;
+}
+
+void stop_server(cj::Context *c) {
+    std::shared_ptr<cj::Server> s;
+    {
+        std::lock_guard<std::mutex> g(g_srv_mu);
+        auto it = g_servers.find(c);
+        if (it == g_servers.end()) return;
+        s = it->second;
+        g_servers.erase(it);
+    }
+    s->drain();
+}
 }  // namespace
 
 extern "C" {
@@ -60,7 +90,10 @@ int cj_ctx_create(const cj_ctx_options *opts, cj_ctx **out) {
 }
 
 int cj_ctx_destroy(cj_ctx *ctx) {
-    return guard([&] { delete C(ctx); });
+    return guard([&] {
+        stop_server(C(ctx));
+        delete C(ctx);
+    });
 }
 
 int cj_ctx_device_info(cj_ctx *ctx, char *name, size_t name_cap, int32_t *num_cus) {
@@ -124,6 +157,54 @@ int cj_kernel_stats_reset(cj_ctx *ctx) {
     return guard([&] {
         if (!ctx) throw cj::InvalidError("null context");
         C(ctx)->kernel_stats_reset();
+    });
+}
+
+int cj_serve_start(cj_ctx *ctx, const cj_serve_options *opts) {
+    return guard([&] {
+        if (!ctx || !opts) throw cj::InvalidError("null argument");
+        cj::Context *c = C(ctx);
+        const int workers = std::max(1, std::min<int>(opts->workers, (int)c->streams.size()));
+        std::lock_guard<std::mutex> g(g_srv_mu);
+        if (g_servers.count(c)) throw cj::InvalidError("serving loop already started");
+        g_servers[c] = std::make_shared<cj::Server>(*c, workers, opts->admit_bytes);
+    });
+}
+
+int cj_submit(cj_ctx *ctx, const cj_plan_desc *plan, uint32_t flags, uint64_t *ticket) {
+    return guard([&] {
+        if (!ctx || !plan || !ticket) throw cj::InvalidError("null argument");
+        *ticket = server_of(ctx)->submit(*plan, flags);
+    });
+}
+
+int cj_wait(cj_ctx *ctx, uint64_t ticket, int64_t timeout_ms, cj_result *out, cj_timing *timing) {
+    if (!ctx || !out) {
+        g_err = "null argument";
+        return CJ_ERR_INVALID;
+    }
+    int st = CJ_OK;
+    const int rc = guard([&] {
+        std::memset(out, 0, sizeof *out);
+        std::string err;
+        st = server_of(ctx)->wait(ticket, timeout_ms, out, timing, &err);
+        if (st != CJ_OK && st != CJ_ERR_BUSY) g_err = err;
+        if (st == CJ_ERR_BUSY) g_err = "timed out";
+    });
+    return rc != CJ_OK ? rc : st;
+}
+
+int cj_serve_stop(cj_ctx *ctx) {
+    return guard([&] {
+        if (!ctx) throw cj::InvalidError("null context");
+        stop_server(C(ctx));
+    });
+}
+
+int cj_plan_footprint(const cj_plan_desc *plan, uint64_t *bytes) {
+    return guard([&] {
+        if (!plan || !bytes) throw cj::InvalidError("null argument");
+        *bytes = cj::estimate_plan_bytes(*plan);
     });
 }
 

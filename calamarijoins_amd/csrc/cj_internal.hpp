@@ -40,6 +40,9 @@ struct InvalidError : std::runtime_error {
 struct OomError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+struct BusyError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 #define CJ_HIP(expr)                                                                 \
     do {                                                                             \

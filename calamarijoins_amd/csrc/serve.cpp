@@ -1,0 +1,185 @@
+// serve.cpp -- the serving loop (serve.hpp).
+//
+// The reference runs one plan at a time on a global thread pool
+// (worker_pool.h:44-149, execute.cpp:311-325).  Here each worker thread owns
+// one of the context's HIP streams and runs Context::execute; admission is
+// by estimated device footprint so that concurrent plans never exceed the
+// budget, taking requests in arrival order but letting a later request that
+// fits pass one that does not (a request passed 8 times is admitted next).
+#include "serve.hpp"
+
+#include <algorithm>
+
+namespace cj {
+
+PlanCopy::PlanCopy(const cj_plan_desc &p) {
+    nodes.assign(p.nodes, p.nodes + p.num_nodes);
+    attrs.resize(p.num_nodes);
+    for (uint32_t i = 0; i < p.num_nodes; ++i) {
+        attrs[i].assign(p.nodes[i].output_attrs, p.nodes[i].output_attrs + p.nodes[i].num_output_attrs);
+        nodes[i].output_attrs = attrs[i].data();
+    }
+    inputs.assign(p.inputs, p.inputs + p.num_inputs);
+    cols.resize(p.num_inputs);
+    pages.resize(p.num_inputs);
+    for (uint32_t t = 0; t < p.num_inputs; ++t) {
+        const cj_table_desc &td = p.inputs[t];
+        cols[t].assign(td.columns, td.columns + td.num_columns);
+        pages[t].resize(td.num_columns);
+        for (uint32_t c = 0; c < td.num_columns; ++c) {
+            pages[t][c].assign(td.columns[c].pages, td.columns[c].pages + td.columns[c].num_pages);
+            cols[t][c].pages = pages[t][c].data();
+        }
+        inputs[t].columns = cols[t].data();
+    }
+    desc = p;
+    desc.nodes = nodes.data();
+    desc.inputs = inputs.data();
+}
+
+uint64_t estimate_plan_bytes(const cj_plan_desc &p) {
+    // rows a node produces: scans their table's rows, joins their probe side's
+    std::vector<uint64_t> rows(p.num_nodes, 0);
+    uint64_t bytes = 0;
+    for (uint32_t i = 0; i < p.num_nodes; ++i) {  // children precede parents in well-formed plans
+        const cj_node_desc &n = p.nodes[i];
+        if (n.kind == CJ_NODE_SCAN) {
+            if (n.base_table_id < p.num_inputs) {
+                const cj_table_desc &t = p.inputs[n.base_table_id];
+                rows[i] = t.num_rows;
+                // decoded columns (dense values + validity) when not cached
+                bytes += t.num_rows * (8ull + 1) * n.num_output_attrs;
+            }
+            continue;
+        }
+        const uint64_t l = n.left < p.num_nodes ? rows[n.left] : 0;
+        const uint64_t r = n.right < p.num_nodes ? rows[n.right] : 0;
+        const uint64_t b = std::min(l, r), pr = std::max(l, r);
+        rows[i] = pr;
+        // build structure (hash table or direct map), counts, match bitmap,
+        // pair lists, one row-id list per output slot
+        bytes += 40 * b + pr / 4 + 8 * pr + 4ull * pr * n.num_output_attrs;
+    }
+    // result pages of the root
+    if (p.root < p.num_nodes) bytes += rows[p.root] * 9ull * p.nodes[p.root].num_output_attrs;
+    return bytes + (64ull << 20);
+}
+
+Server::Server(Context &ctx, int workers, uint64_t admit_bytes) : ctx_(ctx), budget_(admit_bytes) {
+    if (workers < 1) workers = 1;
+    for (int i = 0; i < workers; ++i) threads_.emplace_back([this] { worker(); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        stop_ = true;
+    }
+    cv_work_.notify_all();
+    for (auto &t : threads_) t.join();
+    for (auto &kv : all_)
+        if (kv.second->done && kv.second->status == CJ_OK) ctx_.release(&kv.second->result);
+}
+
+uint64_t Server::submit(const cj_plan_desc &plan, uint32_t flags) {
+    auto r = std::make_unique<ServeRequest>();
+    r->plan = std::make_unique<PlanCopy>(plan);
+    r->flags = flags;
+    r->est_bytes = estimate_plan_bytes(r->plan->desc);
+    r->t_submit = std::chrono::steady_clock::now();
+    uint64_t t;
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        if (stop_) throw BusyError("server is stopping");
+        t = r->ticket = next_ticket_++;
+        pending_.push_back(r.get());
+        all_[t] = std::move(r);
+    }
+    cv_work_.notify_one();
+    return t;
+}
+
+ServeRequest *Server::pick_locked() {
+    for (size_t i = 0; i < pending_.size(); ++i) {
+        ServeRequest *r = pending_[i];
+        const bool fits = admitted_ + r->est_bytes <= budget_ || running_ == 0;
+        if (fits) {
+            for (size_t k = 0; k < i; ++k) ++pending_[k]->skipped;
+            pending_.erase(pending_.begin() + (long)i);
+            return r;
+        }
+        if (r->skipped >= 8) break;  // it goes next: do not pass it again
+    }
+    return nullptr;
+}
+
+void Server::worker() {
+    std::unique_lock<std::mutex> g(mu_);
+    for (;;) {
+        ServeRequest *r = nullptr;
+        cv_work_.wait(g, [&] { return (stop_ && pending_.empty()) || (r = pick_locked()) != nullptr; });
+        if (!r) return;
+        admitted_ += r->est_bytes;
+        ++running_;
+        peak_admitted_bytes = std::max(peak_admitted_bytes, admitted_);
+        peak_running = std::max(peak_running, running_);
+        r->t_start = std::chrono::steady_clock::now();
+        g.unlock();
+        int status = CJ_OK;
+        std::string err;
+        try {
+            ctx_.execute(r->plan->desc, r->flags, &r->result, &r->timing);
+        } catch (const InvalidError &e) {
+            status = CJ_ERR_INVALID, err = e.what();
+        } catch (const UnsupportedError &e) {
+            status = CJ_ERR_UNSUPPORTED, err = e.what();
+        } catch (const OomError &e) {
+            status = CJ_ERR_OOM, err = e.what();
+        } catch (const DeviceError &e) {
+            status = CJ_ERR_DEVICE, err = e.what();
+        } catch (const std::bad_alloc &) {
+            status = CJ_ERR_OOM, err = "host allocation failed";
+        } catch (const std::exception &e) {
+            status = CJ_ERR_INVALID, err = e.what();
+        }
+        g.lock();
+        r->t_done = std::chrono::steady_clock::now();
+        admitted_ -= r->est_bytes;
+        --running_;
+        r->status = status;
+        r->error = err;
+        r->done = true;
+        cv_done_.notify_all();
+        cv_work_.notify_all();  // memory came back: another request may fit
+    }
+}
+
+int Server::wait(uint64_t ticket, int64_t timeout_ms, cj_result *out, cj_timing *timing, std::string *err) {
+    std::unique_lock<std::mutex> g(mu_);
+    auto it = all_.find(ticket);
+    if (it == all_.end()) throw InvalidError("unknown or already collected ticket");
+    ServeRequest *r = it->second.get();
+    auto ready = [&] { return r->done; };
+    if (timeout_ms < 0) cv_done_.wait(g, ready);
+    else if (!cv_done_.wait_for(g, std::chrono::milliseconds(timeout_ms), ready)) return CJ_ERR_BUSY;
+    const int status = r->status;
+    if (status == CJ_OK) {
+        *out = r->result;
+        if (timing) {
+            *timing = r->timing;
+            timing->queue_ms = std::chrono::duration<double, std::milli>(r->t_start - r->t_submit).count();
+            timing->service_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_start).count();
+        }
+    } else if (err) {
+        *err = r->error;
+    }
+    all_.erase(it);  // the caller owns the result now
+    return status;
+}
+
+void Server::drain() {
+    std::unique_lock<std::mutex> g(mu_);
+    cv_done_.wait(g, [&] { return pending_.empty() && running_ == 0; });
+}
+
+}  // namespace cj

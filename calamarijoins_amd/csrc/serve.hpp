@@ -1,0 +1,82 @@
+// serve.hpp -- serving loop: plans submitted as they arrive are admitted by
+// their estimated device footprint within a fixed budget and run
+// concurrently, one worker per HIP stream, against the context's resident
+// tables.  While one plan waits on a host read-back (the row count of a
+// join), the kernels of the others keep the GPU busy.
+#pragma once
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace cj {
+
+// Owning copy of a cj_plan_desc (descriptor arrays only: the pages stay the
+// caller's and must outlive the request).
+struct PlanCopy {
+    std::vector<cj_node_desc> nodes;
+    std::vector<std::vector<cj_attr>> attrs;
+    std::vector<cj_table_desc> inputs;
+    std::vector<std::vector<cj_column_desc>> cols;
+    std::vector<std::vector<std::vector<const void *>>> pages;
+    cj_plan_desc desc{};
+    explicit PlanCopy(const cj_plan_desc &p);
+};
+
+struct ServeRequest {
+    uint64_t ticket = 0;
+    std::unique_ptr<PlanCopy> plan;
+    uint32_t flags = 0;
+    uint64_t est_bytes = 0;
+    uint32_t skipped = 0;  // times a later request was admitted first
+    std::chrono::steady_clock::time_point t_submit, t_start, t_done;
+    bool done = false;
+    int status = 0;
+    std::string error;
+    cj_result result{};
+    cj_timing timing{};
+};
+
+// Device bytes a plan may hold at its peak: build structures, match and
+// row-id lists, decoded columns and output pages, from the table sizes
+// (joins are assumed not to grow their probe side, as with key / foreign-key
+// joins).
+uint64_t estimate_plan_bytes(const cj_plan_desc &p);
+
+class Server {
+  public:
+    Server(Context &ctx, int workers, uint64_t admit_bytes);
+    ~Server();
+    uint64_t submit(const cj_plan_desc &plan, uint32_t flags);
+    // CJ_OK with the result, CJ_ERR_BUSY on timeout (the request stays), or
+    // the request's own error status.  A finished request is handed out once.
+    int wait(uint64_t ticket, int64_t timeout_ms, cj_result *out, cj_timing *timing, std::string *err);
+    void drain();
+
+    uint64_t peak_admitted_bytes = 0;
+    int peak_running = 0;
+
+  private:
+    void worker();
+    ServeRequest *pick_locked();
+
+    Context &ctx_;
+    uint64_t budget_;
+    std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<ServeRequest *> pending_;
+    std::map<uint64_t, std::unique_ptr<ServeRequest>> all_;
+    uint64_t admitted_ = 0;
+    int running_ = 0;
+    bool stop_ = false;
+    uint64_t next_ticket_ = 1;
+    std::vector<std::thread> threads_;
+};
+
+}  // namespace cj

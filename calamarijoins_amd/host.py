@@ -140,6 +140,39 @@ class Engine:
                                       C.byref(tm) if timing else None))
         return EngineResult(self, res, tm if timing else None)
 
+    # ---- serving loop (cj_serve_start / cj_submit / cj_wait / cj_serve_stop)
+    def serve_start(self, workers: int, admit_bytes: int):
+        abi.check(self.lib.cj_serve_start(self.ctx, C.byref(abi.ServeOptions(workers, admit_bytes))))
+        self._inflight = {}
+
+    def submit(self, plan: Plan, flags: int = 0) -> int:
+        t = C.c_uint64()
+        abi.check(self.lib.cj_submit(self.ctx, C.byref(plan.desc()), flags, C.byref(t)))
+        self._inflight[t.value] = plan  # the plan's pages must outlive the request
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        """The result of a submitted plan (EngineResult with timing), or None
+        when timeout_ms passes first."""
+        res = abi.Result()
+        tm = abi.Timing()
+        rc = self.lib.cj_wait(self.ctx, ticket, timeout_ms, C.byref(res), C.byref(tm))
+        if rc == abi.CJ_ERR_BUSY and timeout_ms >= 0:
+            return None
+        self._inflight.pop(ticket, None)
+        abi.check(rc)
+        return EngineResult(self, res, tm)
+
+    def serve_stop(self):
+        abi.check(self.lib.cj_serve_stop(self.ctx))
+        self._inflight = {}
+
+    @staticmethod
+    def footprint(plan: Plan) -> int:
+        b = C.c_uint64()
+        abi.check(abi.load_library().cj_plan_footprint(C.byref(plan.desc()), C.byref(b)))
+        return b.value
+
     def kernel_stats(self):
         arr = (abi.KernelStat * 128)()
         n = C.c_uint32()

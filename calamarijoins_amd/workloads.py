@@ -216,3 +216,91 @@ def c2_plan(build: Table, probe: Table) -> Plan:
     p.root = p.new_join_node(True, 0, 1, 0, 0,
                              [(1, "INT32"), (2, "INT64"), (4, "INT32"), (5, "INT64"), (6, "FP64")])
     return p
+
+
+# ---------------------------------------------------------------- C5 stream
+C5_TABLES = 12
+
+
+def c5_tables(seed: int = 1005, scale: float = 1.0, threads: int = 0, keep_lo: float = 0.01, keep_hi: float = 0.40):
+    """Twelve tables with candidate sizes log-uniform in [0.1M, 36M] (times
+    `scale`), each pre-filtered to a seeded keep_lo-keep_hi (1-40 %) of its rows.  Columns:
+    0 primary key (unique shuffled INT32 over the candidate domain), 1-2
+    foreign keys into two other tables (Zipf(1.0) over the target's key
+    ranks, so the join keeps the target's filter fraction), 3 INT64, 4 FP64,
+    5 VARCHAR (5-60 printable ASCII).  Returns (tables, meta) where meta[t] =
+    {"rows", "domain", "keep", "refs": (target of fk1, target of fk2)}."""
+    sizes, keeps, refs = [], [], []
+    for t in range(C5_TABLES):
+        u = _unit(seed, 1000 + t)
+        sizes.append(max(1000, int(round(1e5 * (360.0 ** u) * scale))))
+        keeps.append(keep_lo + (keep_hi - keep_lo) * _unit(seed, 2000 + t))
+        a = int(_unit(seed, 3000 + t) * (C5_TABLES - 1))
+        b = int(_unit(seed, 4000 + t) * (C5_TABLES - 2))
+        ra = (t + 1 + a) % C5_TABLES
+        others = [x for x in range(C5_TABLES) if x not in (t, ra)]
+        refs.append((ra, others[b % len(others)]))
+    tables, meta = [], []
+    for t in range(C5_TABLES):
+        cols = [col(GEN_PERM, "INT32", 1, 0.0, domain=sizes[t], perm_stream=100 + t)]
+        for k, r in enumerate(refs[t]):
+            cols.append(col(GEN_ZIPF_FK, "INT32", 2 + k, 0.0, domain=sizes[r], perm_stream=100 + r, zipf_s=1.0,
+                            miss_frac=0.0, miss_lo=0, miss_hi=1))
+        cols += [col(GEN_I64_FULL, "INT64", 5, 0.02), col(GEN_F64_SCALED, "FP64", 6, 0.02, scale=1e6),
+                 col(GEN_VARCHAR, "VARCHAR", 7, 0.02, lo=5, hi=60)]
+        tb = generate(sizes[t], seed + 17 * t, cols, keep_frac=keeps[t], threads=threads)
+        tables.append(tb)
+        meta.append({"rows": tb.num_rows, "domain": sizes[t], "keep": keeps[t], "refs": refs[t]})
+    return tables, meta
+
+
+def c5_plan(tables, meta, plan_seed: int) -> Plan:
+    """A random left-deep plan of 3-8 key / foreign-key joins over the C5
+    tables: start at a random table, then repeatedly join a table that a
+    foreign key already in the intermediate references (not yet joined).
+    The intermediate keeps every key column; the root outputs up to six
+    payload and key columns of the joined tables."""
+    njoins = 3 + int(_unit(plan_seed, 1) * 6)
+    start = int(_unit(plan_seed, 2) * C5_TABLES)
+    p = Plan()
+    for tb in tables:
+        p.new_input(tb)
+    key_attrs = [(0, "INT32"), (1, "INT32"), (2, "INT32")]
+    pay = [(3, "INT64"), (4, "FP64"), (5, "VARCHAR")]
+    # intermediate layout: per joined table, its three keys then one payload
+    def scan(t):
+        pc = pay[int(_unit(plan_seed, 100 + t) * 3)]
+        return p.new_scan_node(t, key_attrs + [pc]), pc[1]
+
+    node, ptype = scan(start)
+    layout = [(start, "pk", "INT32"), (start, "fk1", "INT32"), (start, "fk2", "INT32"), (start, "pay", ptype)]
+    joined = [start]
+    for j in range(njoins):
+        cands = []
+        for i, (t, what, _) in enumerate(layout):
+            if what in ("fk1", "fk2"):
+                target = meta[t]["refs"][0 if what == "fk1" else 1]
+                if target not in joined:
+                    cands.append((i, target))
+        if not cands:
+            break
+        i, target = cands[int(_unit(plan_seed, 200 + j) * len(cands))]
+        right, rtype = scan(target)
+        rlay = [(target, "pk", "INT32"), (target, "fk1", "INT32"), (target, "fk2", "INT32"), (target, "pay", rtype)]
+        both = layout + rlay
+        last = j == njoins - 1 or len(joined) + 1 == C5_TABLES
+        if last:
+            pays = [k for k, x in enumerate(both) if x[1] == "pay"][-5:]
+            outs = [(k, both[k][2]) for k in pays] + [(i, "INT32")]
+            p.root = p.new_join_node(False, node, right, i, 0, outs)
+            return p
+        node = p.new_join_node(False, node, right, i, 0, [(k, x[2]) for k, x in enumerate(both)])
+        layout = both
+        joined.append(target)
+    # ran out of joinable tables: close with the last join's columns
+    pays = [k for k, x in enumerate(layout) if x[1] == "pay"][-5:]
+    if p.nodes[-1]["kind"] == abi.CJ_NODE_JOIN:
+        p.nodes[-1]["attrs"] = [(k, layout[k][2]) for k in pays] + [(0, "INT32")]
+        p.root = len(p.nodes) - 1
+        return p
+    raise ValueError("plan with no join")

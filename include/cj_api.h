@@ -105,6 +105,8 @@ typedef struct cj_timing {
     double h2d_ms;      /* page upload of non-resident inputs              */
     double d2h_ms;      /* result page download (host output only)         */
     uint64_t bytes_alg; /* algorithmic bytes: input pages read + output written */
+    double queue_ms;    /* served plans: submit .. admitted (0 for cj_execute) */
+    double service_ms;  /* served plans: admitted .. result ready              */
 } cj_timing;
 
 /* A result column. Host output: pages[i] are host pointers valid until
@@ -176,6 +178,27 @@ typedef struct cj_kernel_stat {
 } cj_kernel_stat;
 int cj_kernel_stats(cj_ctx *ctx, cj_kernel_stat *out, uint32_t cap, uint32_t *count);
 int cj_kernel_stats_reset(cj_ctx *ctx);
+
+/* ---- serving loop (no counterpart in the reference, which runs one plan
+ *      at a time: execute.cpp:311-325).  Plans submitted as they arrive are
+ *      admitted by estimated device footprint within admit_bytes and run
+ *      concurrently, one worker per stream (at most num_streams of the
+ *      context).  The plan's descriptor arrays are copied at submit; its
+ *      pages must stay valid until cj_wait returns. ------------------------ */
+typedef struct cj_serve_options {
+    int32_t workers;      /* concurrent plans (<= the context's num_streams)  */
+    uint64_t admit_bytes; /* footprint budget of the admitted plans           */
+} cj_serve_options;
+int cj_serve_start(cj_ctx *ctx, const cj_serve_options *opts);
+int cj_submit(cj_ctx *ctx, const cj_plan_desc *plan, uint32_t flags, uint64_t *ticket);
+/* CJ_OK and the result (owned by the caller), CJ_ERR_BUSY when timeout_ms
+ * (>= 0) passes first, or the plan's own error.  timeout_ms < 0 waits. */
+int cj_wait(cj_ctx *ctx, uint64_t ticket, int64_t timeout_ms, cj_result *out, cj_timing *timing);
+/* Finishes every submitted plan, then stops the workers.  Results not yet
+ * collected are released. */
+int cj_serve_stop(cj_ctx *ctx);
+/* Estimated device footprint the server charges for a plan. */
+int cj_plan_footprint(const cj_plan_desc *plan, uint64_t *bytes);
 
 #ifdef __cplusplus
 }

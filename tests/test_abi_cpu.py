@@ -111,3 +111,21 @@ def test_cpp_api_header_compiles_with_reference_layout():
         exe = os.path.join(d, "t")
         subprocess.run(["g++", "-std=c++17", "-I", os.path.join(REPO, "include"), src, "-o", exe], check=True)
         assert subprocess.run([exe]).returncode == 0
+
+
+def test_plan_footprint_grows_with_tables():
+    """The serving loop's admission estimate (host only)."""
+    from calamarijoins_amd import workloads
+    from calamarijoins_amd.host import Engine
+    small, big = workloads.c2_tables(build_rows=1000, probe_rows=10_000), workloads.c2_tables(build_rows=10_000,
+                                                                                               probe_rows=100_000)
+    fs = Engine.footprint(workloads.c2_plan(*small))
+    fb = Engine.footprint(workloads.c2_plan(*big))
+    assert 0 < fs < fb
+
+
+def test_serve_calls_fail_cleanly_without_context():
+    lib = abi.load_library()
+    t = C.c_uint64()
+    assert lib.cj_serve_stop(None) == abi.CJ_ERR_INVALID
+    assert lib.cj_submit(None, None, 0, C.byref(t)) == abi.CJ_ERR_INVALID

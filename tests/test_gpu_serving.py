@@ -1,0 +1,86 @@
+"""Serving loop (cj_serve_start / cj_submit / cj_wait): plans run concurrently
+in the mix give the same row multisets as each plan run alone, and the
+first plans are checked against the oracle."""
+import numpy as np
+import pytest
+
+import cjtest
+from calamarijoins_amd import abi, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c5_small():
+    tables, meta = workloads.c5_tables(scale=0.01, keep_lo=0.5, keep_hi=0.95)
+    plans = [workloads.c5_plan(tables, meta, 5000 + k) for k in range(12)]
+    b, p = workloads.c2_tables(seed=91, build_rows=100_000, probe_rows=800_000)
+    plans.append(workloads.c2_plan(b, p))
+    return tables, plans, (b, p)
+
+
+def hashes(res):
+    return cjtest.result_row_hashes(res)
+
+
+def test_gpu_served_results_equal_alone(engine, c5_small):
+    tables, plans, _ = c5_small
+    alone = []
+    for pl in plans:
+        r = engine.execute(pl)
+        alone.append((r.num_rows, hashes(r)))
+        r.release()
+    assert sum(n for n, _ in alone) > 0
+    for k in range(3):  # the oracle pins the first plans
+        exp = cjtest.OracleRun(plans[k], threads=8)
+        assert alone[k][0] == exp.num_rows
+        assert np.array_equal(alone[k][1], cjtest.result_row_hashes(exp))
+    engine.serve_start(4, 64 << 30)
+    try:
+        order = [k % len(plans) for k in range(2 * len(plans))]
+        order = order[::2] + order[1::2]
+        tickets = [(k, engine.submit(plans[k])) for k in order]
+        for k, t in tickets:
+            r = engine.wait(t)
+            assert r.num_rows == alone[k][0], (k, r.num_rows, alone[k][0])
+            assert np.array_equal(hashes(r), alone[k][1]), k
+            assert r.timing.service_ms > 0 and r.timing.queue_ms >= 0
+            r.release()
+    finally:
+        engine.serve_stop()
+
+
+def test_gpu_served_tiny_budget_runs_one_at_a_time(engine, c5_small):
+    """A budget below every plan's footprint still admits one plan when
+    nothing runs, so the stream completes."""
+    _, plans, _ = c5_small
+    engine.serve_start(4, 1)
+    try:
+        ts = [(k, engine.submit(plans[k], abi.CJ_EXEC_OUTPUT_DEVICE)) for k in range(4)]
+        for k, t in ts:
+            r = engine.wait(t)
+            r.release()
+    finally:
+        engine.serve_stop()
+
+
+def test_gpu_served_error_and_timeout(engine, c5_small):
+    tables, plans, _ = c5_small
+    bad = cjtest.Plan()
+    t = cjtest.table_from_rows(["INT32", "INT64"], [[1, 2]])
+    bad.new_input(t)
+    bad.new_scan_node(0, [(0, "INT32"), (1, "INT64")])
+    bad.new_scan_node(0, [(0, "INT32"), (1, "INT64")])
+    bad.root = bad.new_join_node(True, 0, 1, 1, 1, [(0, "INT32")])  # INT64 key
+    engine.serve_start(2, 64 << 30)
+    try:
+        tb = engine.submit(bad)
+        with pytest.raises(abi.CJError):
+            engine.wait(tb)
+        tg = engine.submit(plans[0])
+        r = None
+        while r is None:
+            r = engine.wait(tg, timeout_ms=50)
+        r.release()
+    finally:
+        engine.serve_stop()
