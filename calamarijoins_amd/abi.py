@@ -83,6 +83,17 @@ class ServeOptions(C.Structure):
     _fields_ = [("workers", C.c_int32), ("admit_bytes", C.c_uint64)]
 
 
+CJ_BATCH_AUTOTUNE = 16
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("plans_batched", C.c_uint32), ("plans_single", C.c_uint32), ("device_ms", C.c_double),
+        ("total_ms", C.c_double), ("output_rows", C.c_uint64), ("load_factor", C.c_uint32), ("grid", C.c_uint32),
+        ("tuned", C.c_int32), ("graph_replayed", C.c_int32),
+    ]
+
+
 class ResultColumn(C.Structure):
     _fields_ = [
         ("type", C.c_int32),
@@ -125,7 +136,7 @@ EXPORTS = [
     "cj_table_register", "cj_table_unregister",
     "cj_execute", "cj_result_release", "cj_result_to_host",
     "cj_kernel_stats", "cj_kernel_stats_reset",
-    "cj_serve_start", "cj_submit", "cj_wait", "cj_serve_stop", "cj_plan_footprint",
+    "cj_serve_start", "cj_submit", "cj_wait", "cj_serve_stop", "cj_plan_footprint", "cj_execute_batch",
 ]
 
 _lib = None
@@ -159,6 +170,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "cj_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, P(Result), P(Timing)]),
         "cj_serve_stop": (C.c_int, [C.c_void_p]),
         "cj_plan_footprint": (C.c_int, [P(PlanDesc), P(C.c_uint64)]),
+        "cj_execute_batch": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, C.c_uint32, P(Result), P(BatchStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

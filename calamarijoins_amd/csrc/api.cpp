@@ -201,6 +201,15 @@ int cj_serve_stop(cj_ctx *ctx) {
     });
 }
 
+int cj_execute_batch(cj_ctx *ctx, const cj_plan_desc *plans, uint32_t n, uint32_t flags, cj_result *out,
+                     cj_batch_stats *stats) {
+    return guard([&] {
+        if (!ctx || (n && (!plans || !out))) throw cj::InvalidError("null argument");
+        CJ_HIP(hipSetDevice(C(ctx)->device));
+        C(ctx)->execute_batch(plans, n, flags, out, stats);
+    });
+}
+
 int cj_plan_footprint(const cj_plan_desc *plan, uint64_t *bytes) {
     return guard([&] {
         if (!plan || !bytes) throw cj::InvalidError("null argument");

@@ -183,6 +183,7 @@ Context::Context(const cj_ctx_options &o) : opts(o), pool(0) {
 Context::~Context() {
     hipSetDevice(device);
     hipDeviceSynchronize();
+    release_batch_templates();
     for (auto &kv : tables)
         for (auto &c : kv.second->cols) free_column(*c);
     tables.clear();

@@ -101,6 +101,12 @@ struct ExecStream {
     bool busy = false;
 };
 
+struct BatchTemplate;  // batch.cpp
+struct BatchConfig {
+    uint32_t load_factor;  // directory slots per build row (power-of-two rounded)
+    uint32_t grid;         // persistent workgroups per batch launch
+};
+
 class Context {
   public:
     explicit Context(const cj_ctx_options &o);
@@ -118,6 +124,11 @@ class Context {
     void unregister_table(uint64_t id);
 
     void execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing);
+    // many independent single-join plans in shared launches (batch.cpp)
+    void execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flags, cj_result *out, cj_batch_stats *stats);
+    void release_batch_templates();
+    std::map<uint64_t, BatchConfig> batch_tuning;                       // template class -> tuned config
+    std::map<uint64_t, std::shared_ptr<BatchTemplate>> batch_templates;  // workspaces + phase-1 graphs
     void release(cj_result *res);
     void to_host(cj_result *res);
 

@@ -167,6 +167,18 @@ class Engine:
         abi.check(self.lib.cj_serve_stop(self.ctx))
         self._inflight = {}
 
+    def execute_batch(self, plans, flags: int = 0):
+        """Run many plans through cj_execute_batch; returns ([EngineResult],
+        abi.BatchStats)."""
+        n = len(plans)
+        descs = (abi.PlanDesc * max(n, 1))()
+        for i, p in enumerate(plans):
+            descs[i] = p.desc()
+        res = (abi.Result * max(n, 1))()
+        st = abi.BatchStats()
+        abi.check(self.lib.cj_execute_batch(self.ctx, descs, n, flags, res, C.byref(st)))
+        return [EngineResult(self, res[i], None) for i in range(n)], st
+
     @staticmethod
     def footprint(plan: Plan) -> int:
         b = C.c_uint64()

@@ -304,3 +304,39 @@ def c5_plan(tables, meta, plan_seed: int) -> Plan:
         p.root = len(p.nodes) - 1
         return p
     raise ValueError("plan with no join")
+
+
+# ------------------------------------------------------------- C4 batch
+def c4_plan_tables(k: int, scale: float = 1.0, threads: int = 0):
+    """Plan k of the batch (seed 2000 + k): build rows log-uniform in
+    [1e3, 2e6], probe rows log-uniform in [1e4, 2e7] (times scale), key domain
+    [0, build rows).  Build keys are uniform over the domain (repeats are
+    allowed, so matches fan out); probe keys are uniform or Zipf(0.8) over
+    the domain by a per-plan coin.  Each side has a NULL rate U[0, 0.10] on
+    its key and one full-range INT32 payload.  (Zipf on both sides would make
+    a hot key's output quadratic in its frequency.)"""
+    seed = 2000 + k
+    nb = max(8, int(round(1e3 * (2000.0 ** _unit(seed, 1)) * scale)))
+    npr = max(8, int(round(1e4 * (2000.0 ** _unit(seed, 2)) * scale)))
+    zipf = _unit(seed, 3) < 0.5
+    nb_null, np_null = 0.10 * _unit(seed, 4), 0.10 * _unit(seed, 5)
+    b = generate(nb, seed, [col(GEN_UNIFORM, "INT32", 1, nb_null, lo=0, hi=nb), col(GEN_I32_FULL, "INT32", 2)],
+                 threads=threads)
+    if zipf:
+        pk = col(GEN_ZIPF_FK, "INT32", 11, np_null, domain=nb, perm_stream=60, zipf_s=0.8, miss_frac=0.0,
+                 miss_lo=0, miss_hi=1)
+    else:
+        pk = col(GEN_UNIFORM, "INT32", 11, np_null, lo=0, hi=nb)
+    p = generate(npr, seed + 1, [pk, col(GEN_I32_FULL, "INT32", 12)], threads=threads)
+    return b, p
+
+
+def c4_plan(build: Table, probe: Table) -> Plan:
+    """[build key, build payload, probe payload]."""
+    p = Plan()
+    p.new_input(build)
+    p.new_input(probe)
+    p.new_scan_node(0, [(0, "INT32"), (1, "INT32")])
+    p.new_scan_node(1, [(0, "INT32"), (1, "INT32")])
+    p.root = p.new_join_node(True, 0, 1, 0, 0, [(0, "INT32"), (1, "INT32"), (3, "INT32")])
+    return p

@@ -200,6 +200,32 @@ int cj_serve_stop(cj_ctx *ctx);
 /* Estimated device footprint the server charges for a plan. */
 int cj_plan_footprint(const cj_plan_desc *plan, uint64_t *bytes);
 
+/* ---- batched execution of many independent plans (no counterpart in the
+ *      reference).  Plans that are one join of two resident base-table scans
+ *      with INT32 keys and fixed-width outputs run together: every join phase
+ *      is one persistent-kernel launch over a device work list covering all
+ *      of them, the first phase sequence is a hipGraph replayed while the
+ *      batch shape repeats, and the result sizes are read back once.  Other
+ *      plans run one at a time.  out[i] is plan i's result (each released
+ *      with cj_result_release); results equal cj_execute's. */
+enum {
+    CJ_BATCH_AUTOTUNE = 16 /* time the candidate configurations on the first
+                            * batch of a template class and keep the best */
+};
+typedef struct cj_batch_stats {
+    uint32_t plans_batched;  /* plans run in the shared launches           */
+    uint32_t plans_single;   /* plans run one at a time                     */
+    double device_ms;        /* shared launches, first kernel .. last page  */
+    double total_ms;         /* the whole call                              */
+    uint64_t output_rows;    /* rows of the batched plans                   */
+    uint32_t load_factor;    /* directory slots per build row (chosen)      */
+    uint32_t grid;           /* persistent workgroups per launch (chosen)   */
+    int32_t tuned;           /* this call ran the autotuner                 */
+    int32_t graph_replayed;  /* the phase-1 graph of an earlier call ran    */
+} cj_batch_stats;
+int cj_execute_batch(cj_ctx *ctx, const cj_plan_desc *plans, uint32_t n, uint32_t flags, cj_result *out,
+                     cj_batch_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

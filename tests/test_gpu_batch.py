@@ -1,0 +1,107 @@
+"""Batched execution (cj_execute_batch): every plan's result equals its
+per-plan cj_execute result (row multiset), including plans that run one at a
+time (non-resident inputs, nested-loop builds), a repeated batch that
+replays the phase graph, and the autotuned configuration.  The first plans
+are pinned to the oracle."""
+import numpy as np
+import pytest
+
+import cjtest
+from calamarijoins_amd import abi, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c4_small(engine):
+    pairs = [workloads.c4_plan_tables(k, scale=0.01) for k in range(24)]
+    ids = []
+    for b, p in pairs:
+        ids += [engine.register(b), engine.register(p)]
+    plans = [workloads.c4_plan(b, p) for b, p in pairs]
+    alone = []
+    for pl in plans:
+        r = engine.execute(pl)
+        alone.append((r.num_rows, cjtest.result_row_hashes(r)))
+        r.release()
+    yield pairs, plans, alone
+    for t in ids:
+        engine.unregister(t)
+
+
+def check(results, alone, idx):
+    for r, k in zip(results, idx):
+        assert r.num_rows == alone[k][0], (k, r.num_rows, alone[k][0])
+        assert np.array_equal(cjtest.result_row_hashes(r), alone[k][1]), k
+        r.release()
+
+
+def test_gpu_batch_oracle_pins_per_plan(c4_small):
+    _, plans, alone = c4_small
+    assert sum(n for n, _ in alone) > 0
+    for k in range(3):
+        exp = cjtest.OracleRun(plans[k], threads=8)
+        assert alone[k][0] == exp.num_rows
+        assert np.array_equal(alone[k][1], cjtest.result_row_hashes(exp))
+
+
+def test_gpu_batch_equals_per_plan(engine, c4_small):
+    _, plans, alone = c4_small
+    res, st = engine.execute_batch(plans)
+    assert st.plans_batched == len(plans) and st.plans_single == 0
+    assert st.output_rows == sum(n for n, _ in alone)
+    check(res, alone, range(len(plans)))
+    # same batch again: the phase graph replays, results unchanged
+    res, st = engine.execute_batch(plans)
+    assert st.graph_replayed == 1
+    check(res, alone, range(len(plans)))
+
+
+def test_gpu_batch_device_output_and_order(engine, c4_small):
+    _, plans, alone = c4_small
+    idx = list(range(len(plans)))[::-1][:10]
+    res, st = engine.execute_batch([plans[k] for k in idx], abi.CJ_EXEC_OUTPUT_DEVICE)
+    assert st.plans_batched == len(idx)
+    for r in res:
+        r.to_host()
+    check(res, alone, idx)
+
+
+def test_gpu_batch_mixed_with_single_plans(engine, c4_small):
+    """A non-resident input and a nested-loop build run one at a time
+    inside the same call."""
+    _, plans, alone = c4_small
+    b, p = workloads.c2_tables(seed=93, build_rows=5_000, probe_rows=40_000)
+    c2 = workloads.c2_plan(b, p)
+    nl_b = cjtest.table_from_rows(["INT32", "INT32"], [[i, 10 * i] for i in range(5)])
+    nl_p = cjtest.table_from_rows(["INT32", "INT32"], [[i % 7, i] for i in range(100)])
+    nl = workloads.c4_plan(nl_b, nl_p)
+    extra = []
+    for pl in (c2, nl):
+        r = engine.execute(pl)
+        extra.append((r.num_rows, cjtest.result_row_hashes(r)))
+        r.release()
+    assert extra[1][0] == sum(1 for i in range(100) if i % 7 < 5)
+    all_alone = alone[:6] + extra
+    mix = plans[:3] + [c2] + plans[3:6] + [nl]
+    order = [0, 1, 2, 6, 3, 4, 5, 7]
+    res, st = engine.execute_batch(mix)
+    assert st.plans_batched == 6 and st.plans_single == 2
+    check(res, all_alone, order)
+
+
+def test_gpu_batch_autotune(engine, c4_small):
+    _, plans, alone = c4_small
+    sub = plans[4:20]
+    res, st = engine.execute_batch(sub, abi.CJ_BATCH_AUTOTUNE)
+    assert st.tuned == 1 and st.load_factor in (1, 2, 4) and st.grid in (1024, 2048, 4096)
+    check(res, alone, range(4, 20))
+    lf, grid = st.load_factor, st.grid
+    res, st = engine.execute_batch(sub, abi.CJ_BATCH_AUTOTUNE)
+    assert st.tuned == 0 and (st.load_factor, st.grid) == (lf, grid)
+    check(res, alone, range(4, 20))
+
+
+def test_gpu_batch_empty(engine):
+    res, st = engine.execute_batch([])
+    assert res == [] and st.plans_batched == 0 and st.plans_single == 0
