@@ -1,21 +1,31 @@
-// batch.hip -- persistent work-list kernels of the batched join (batch.hpp).
+// batch.hip -- persistent work-list kernels of the batched radix join
+// (batch.hpp).
 //
-// The per-plan algorithm is the reference's (select build side, radix
-// partition the build keys, build each partition's slice of the unchained
-// directory with Bloom tags, probe, materialize: hashtable.h:197-246,405-505,
-// hash_join.h:156-244, materialize.h:85-186); what changes is that one
-// launch serves every plan of the batch.  A workgroup takes global units
-// blockIdx.x, blockIdx.x + gridDim.x, ... and finds each unit's plan by a
-// binary search over the plans' unit prefix (a few hundred entries, cached).
+// The per-plan algorithm is the reference's (build side = fewer rows,
+// partition the build keys by hash bits, build each partition's slice of an
+// unchained directory, probe, materialize: hashtable.h:197-246,405-505,
+// hash_join.h:156-244, materialize.h:85-186).  What changes: the probe side
+// is partitioned too, output values travel with the keys, a build partition
+// is probed from LDS, and one launch serves every plan of the batch.  A
+// workgroup takes global units blockIdx.x, blockIdx.x + gridDim.x, ... and
+// finds each unit's plan by a binary search over the plans' unit prefix.
 #include "batch.hpp"
 #include "probe8.hpp"
 
 namespace cj {
 
-__device__ __forceinline__ uint64_t bslot_of(uint64_t h, uint32_t log_slots) { return h >> (64 - log_slots); }
+namespace {
 
-// Largest k < n with start[k] <= g (plans with no units share a start).
-__device__ __forceinline__ uint32_t find_unit_owner(const uint32_t *start, uint32_t n, uint32_t g) {
+constexpr uint32_t kSlots = 1u << kBatchLogPart;
+constexpr uint32_t kSlotMask = kSlots - 1;
+
+__device__ __forceinline__ uint64_t bslot_of(int32_t key, uint32_t log_slots) {
+    return hash_key(key) >> (64 - log_slots);
+}
+
+// Largest k < n with start[k] <= g (entries with no units share a start).
+template <class T>
+__device__ __forceinline__ uint32_t find_owner(const T *start, uint32_t n, T g) {
     uint32_t lo = 0, hi = n - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
@@ -25,20 +35,64 @@ __device__ __forceinline__ uint32_t find_unit_owner(const uint32_t *start, uint3
     return lo;
 }
 
-// ------------------------------------------------ radix partition of builds
+// Carried column c of side `sd` for the thread's kRPT rows (row0 + kWG j,
+// those set in `valid`) into plan-local tuples pos[j] - plan_base.
+__device__ __forceinline__ void carry8(const BSide &sd, uint32_t c, uint32_t valid, uint64_t row0,
+                                       const uint64_t pos[kRPT], uint64_t plan_base) {
+    if (sd.car_w[c] == 4) {
+        // the decode pass folded NULL into INT32_MIN
+        const CJ_G int32_t *src = glb(reinterpret_cast<const int32_t *>(sd.src[c]));
+        CJ_G int32_t *dst = glb(reinterpret_cast<int32_t *>(sd.car[c]));
+        int32_t v[kRPT];
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j)
+            if ((valid >> j) & 1u) v[j] = src[row0 + kWG * j];
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j)
+            if ((valid >> j) & 1u) dst[pos[j] - plan_base] = v[j];
+    } else {
+        const CJ_G uint64_t *src = glb(reinterpret_cast<const uint64_t *>(sd.src[c]));
+        CJ_G uint64_t *dst = glb(reinterpret_cast<uint64_t *>(sd.car[c]));
+        CJ_G uint8_t *dv = glb(sd.car_vld[c]);
+        const CJ_G uint32_t *sv = glb(sd.src_valid[c]);
+        uint64_t v[kRPT];
+        uint32_t vb = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j)
+            if ((valid >> j) & 1u) {
+                v[j] = src[row0 + kWG * j];
+                vb |= (uint32_t)valid_bit(sv, row0 + kWG * j) << j;
+            }
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j)
+            if ((valid >> j) & 1u) {
+                dst[pos[j] - plan_base] = v[j];
+                dv[pos[j] - plan_base] = (uint8_t)((vb >> j) & 1u);
+            }
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------ partition histogram
 // Partition of a key = top bits of its directory slot (a plan's partitions
-// own contiguous slot ranges).  NULL build keys are not inserted
-// (build_columnar, hashtable.h:439-478).
-__global__ __launch_bounds__(kWG) void k_batch_part_count(BatchArgs a, uint32_t units) {
+// own contiguous slot ranges).  NULL keys take no part in a join
+// (hashtable.h:439-478, hash_join.h:156-244).
+template <bool PROBE>
+__global__ __launch_bounds__(kWG) void k_bx_part_count(BatchArgs a, uint32_t units, const BPlan *__restrict__ plans,
+                                                       const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ part_start) {
     extern __shared__ uint32_t hist[];  // max_parts
     __shared__ uint32_t lds32[32];
+    uint64_t *count = PROBE ? a.pcount : a.bcount;
     for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
-        const uint32_t k = find_unit_owner(a.bpage_start, a.nplans, g);
-        const BPlan pl = a.plans[k];
-        const uint32_t P = 1u << (pl.log_slots - pl.log_part);
+        const uint32_t k = find_owner(pstart, a.nplans, g);
+        const BPlan &pl = plans[k];
+        const BSide &sd = PROBE ? pl.p : pl.b;
+        const uint32_t ls = pl.log_slots;
+        const uint32_t P = 1u << (ls - kBatchLogPart);
         for (uint32_t i = threadIdx.x; i < P; i += kWG) hist[i] = 0;
         __syncthreads();
-        const uint8_t *p = pl.bk_pages + (size_t)(g - a.bpage_start[k]) * kPage;
+        const CJ_G uint8_t *p = glb(sd.k_pages) + (size_t)(g - pstart[k]) * kPage;
         const uint32_t n = ld16(p), nv = ld16(p + 2);
         uint32_t running = 0;
         for (uint32_t t0 = 0; t0 < n; t0 += kTile) {
@@ -46,29 +100,39 @@ __global__ __launch_bounds__(kWG) void k_batch_part_count(BatchArgs a, uint32_t 
             const uint32_t valid = load_keys8(p, n, nv, n == nv, t0, &running, key, lds32);
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j)
-                if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(hash_key(key[j]), pl.log_slots) >> pl.log_part], 1u);
+                if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(key[j], ls) >> kBatchLogPart], 1u);
         }
         __syncthreads();
-        const uint32_t pb = a.part_start[k];
+        const uint32_t pb = part_start[k];
         for (uint32_t i = threadIdx.x; i < P; i += kWG)
-            if (hist[i]) atomicAdd((unsigned long long *)&a.part_count[pb + i], (unsigned long long)hist[i]);
+            if (hist[i]) atomicAdd((unsigned long long *)&count[pb + i], (unsigned long long)hist[i]);
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kWG) void k_batch_part_scatter(BatchArgs a, uint32_t units) {
+// ------------------------------------------------------ partition scatter
+// Keys go to their partition's range of the side's key array; carried
+// values to the same plan-local position of the carried arrays.
+template <bool PROBE>
+__global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t units, const BPlan *__restrict__ plans,
+                                                         const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ part_start,
+                                                         const uint64_t *__restrict__ off) {
     extern __shared__ uint32_t sm[];  // hist[max_parts] then cursors (u64)
     uint32_t *hist = sm;
     uint64_t *cur = reinterpret_cast<uint64_t *>(sm + ((a.max_parts + 1) & ~1u));
     __shared__ uint32_t lds32[32];
+    uint64_t *gcur = PROBE ? a.pcur : a.bcur;
+    int32_t *keys_out = PROBE ? a.pkeys : a.bstage_keys;
     for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
-        const uint32_t k = find_unit_owner(a.bpage_start, a.nplans, g);
-        const BPlan pl = a.plans[k];
-        const uint32_t P = 1u << (pl.log_slots - pl.log_part);
-        const uint32_t page = g - a.bpage_start[k];
+        const uint32_t k = find_owner(pstart, a.nplans, g);
+        const BPlan &pl = plans[k];
+        const BSide &sd = PROBE ? pl.p : pl.b;
+        const uint32_t ls = pl.log_slots;
+        const uint32_t P = 1u << (ls - kBatchLogPart);
+        const uint32_t page = g - pstart[k];
         for (uint32_t i = threadIdx.x; i < P; i += kWG) hist[i] = 0;
         __syncthreads();
-        const uint8_t *p = pl.bk_pages + (size_t)page * kPage;
+        const CJ_G uint8_t *p = glb(sd.k_pages) + (size_t)page * kPage;
         const uint32_t n = ld16(p), nv = ld16(p + 2);
         uint32_t running = 0;
         for (uint32_t t0 = 0; t0 < n; t0 += kTile) {
@@ -76,336 +140,586 @@ __global__ __launch_bounds__(kWG) void k_batch_part_scatter(BatchArgs a, uint32_
             const uint32_t valid = load_keys8(p, n, nv, n == nv, t0, &running, key, lds32);
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j)
-                if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(hash_key(key[j]), pl.log_slots) >> pl.log_part], 1u);
+                if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(key[j], ls) >> kBatchLogPart], 1u);
         }
         __syncthreads();
-        const uint32_t pb = a.part_start[k];
+        const uint32_t pb = part_start[k];
+        const uint64_t plan_base = off[pb];
         for (uint32_t i = threadIdx.x; i < P; i += kWG)
-            cur[i] = hist[i] ? atomicAdd((unsigned long long *)&a.part_cur[pb + i], (unsigned long long)hist[i]) : 0;
+            cur[i] = hist[i] ? atomicAdd((unsigned long long *)&gcur[pb + i], (unsigned long long)hist[i]) : 0;
         __syncthreads();
-        const uint64_t r0 = pl.bk_rs[page];
+        const uint64_t r0 = glb(sd.k_rs)[page];
+        const uint32_t ncar = sd.ncar;
         running = 0;
         for (uint32_t t0 = 0; t0 < n; t0 += kTile) {
             int32_t key[kRPT];
             const uint32_t valid = load_keys8(p, n, nv, n == nv, t0, &running, key, lds32);
-#pragma unroll
-            for (uint32_t j = 0; j < kRPT; ++j)
-                if ((valid >> j) & 1u) {
-                    const uint32_t part = (uint32_t)(bslot_of(hash_key(key[j]), pl.log_slots) >> pl.log_part);
-                    const uint64_t pos = atomicAdd((unsigned long long *)&cur[part], 1ull);
-                    a.stage_keys[pos] = key[j];
-                    a.stage_rids[pos] = (uint32_t)(r0 + t0 + kWG * j + threadIdx.x);
-                }
-        }
-        __syncthreads();
-    }
-}
-
-// --------------------------------------------- per-partition build in LDS
-// build_partition (hashtable.h:197-246): count per slot, exclusive prefix,
-// place keys/row ids, OR Bloom tags, write (end << 16) | tag with the end as
-// a global tuple index.
-__global__ __launch_bounds__(kWG) void k_batch_build(BatchArgs a, uint32_t nparts) {
-    extern __shared__ uint32_t sm[];  // cnt[4096], tag[4096]
-    __shared__ uint32_t lds4[4];
-    for (uint32_t g = blockIdx.x; g < nparts; g += gridDim.x) {
-        const uint32_t k = find_unit_owner(a.part_start, a.nplans, g);
-        const BPlan pl = a.plans[k];
-        const uint32_t S = 1u << pl.log_part, smask = S - 1;
-        uint32_t *cnt = sm, *tag = sm + S;
-        const uint64_t t_a = a.part_off[g], t_e = a.part_off[g + 1];
-        for (uint32_t i = threadIdx.x; i < S; i += kWG) {
-            cnt[i] = 0;
-            tag[i] = 0;
-        }
-        __syncthreads();
-        for (uint64_t i = t_a + threadIdx.x; i < t_e; i += kWG) {
-            const uint64_t h = hash_key(a.stage_keys[i]);
-            const uint32_t ls = (uint32_t)bslot_of(h, pl.log_slots) & smask;
-            atomicAdd(&cnt[ls], 1u);
-            atomicOr(&tag[ls], (uint32_t)c_bloom.t[h & 0x7FF]);
-        }
-        __syncthreads();
-        const uint32_t per = (S + kWG - 1) / kWG;
-        const uint32_t s0 = threadIdx.x * per;
-        uint32_t local = 0;
-        for (uint32_t i = 0; i < per; ++i)
-            if (s0 + i < S) local += cnt[s0 + i];
-        uint32_t tot;
-        uint32_t pre = block_excl_scan<uint32_t>(local, &tot, lds4);
-        for (uint32_t i = 0; i < per; ++i)
-            if (s0 + i < S) {
-                const uint32_t c = cnt[s0 + i];
-                cnt[s0 + i] = pre;
-                pre += c;
-            }
-        __syncthreads();
-        for (uint64_t i = t_a + threadIdx.x; i < t_e; i += kWG) {
-            const int32_t key = a.stage_keys[i];
-            const uint32_t ls = (uint32_t)bslot_of(hash_key(key), pl.log_slots) & smask;
-            const uint64_t pos = t_a + atomicAdd(&cnt[ls], 1u);
-            a.keys[pos] = key;
-            a.rids[pos] = a.stage_rids[i];
-        }
-        __syncthreads();
-        // directory ends are plan-local tuple indices (the plan's tuples start
-        // at tb), so slot 0 of every plan starts at its own first tuple
-        const uint64_t tb = a.part_off[a.part_start[k]];
-        uint64_t *d = a.dir + pl.dir_base + (uint64_t)(g - a.part_start[k]) * S;
-        for (uint32_t i = threadIdx.x; i < S; i += kWG)
-            d[i] = ((t_a - tb + cnt[i]) << 16) | (uint64_t)(tag[i] & 0xFFFF);
-        __syncthreads();
-    }
-}
-
-// ------------------------------------------------------------------ probe
-__global__ __launch_bounds__(kWG) void k_batch_probe_count(BatchArgs a, uint32_t units) {
-    __shared__ uint32_t lds32[32];
-    __shared__ uint64_t red[4];
-    const NLKeys nl{};
-    const DirectTable dt{};
-    for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
-        const uint32_t k = find_unit_owner(a.ppage_start, a.nplans, g);
-        const BPlan pl = a.plans[k];
-        const uint64_t tb = a.part_off[a.part_start[k]];
-        const HashTableDev ht{a.dir + pl.dir_base, a.keys + tb, a.rids + tb, (int)pl.log_slots, 0};
-        const uint8_t *p = pl.pk_pages + (size_t)(g - a.ppage_start[k]) * kPage;
-        const uint32_t n = ld16(p), nv = ld16(p + 2);
-        uint32_t running = 0;
-        uint64_t mine = 0;
-        for (uint32_t t0 = 0; t0 < n; t0 += kTile) {
-            int32_t key[kRPT];
-            const uint32_t valid = load_keys8(p, n, nv, n == nv, t0, &running, key, lds32);
-            uint32_t m[kRPT], s[kRPT], e[kRPT];
-            lookup8<kModeHash>(valid, key, ht, nl, dt, m, s, e);
-#pragma unroll
-            for (uint32_t j = 0; j < kRPT; ++j) mine += m[j];
-        }
-        uint64_t tot;
-        block_excl_scan<uint64_t>(mine, &tot, red);
-        if (threadIdx.x == 0) a.pcount[g] = tot;
-        __syncthreads();
-    }
-}
-
-// Exclusive prefix (row order t0 + 256 j + t) of 8 per-row counts.
-__device__ __forceinline__ uint32_t batch_scan8(const uint32_t cnt[kRPT], uint32_t pre[kRPT], uint32_t *lds32) {
-    const uint32_t w = wave_id(), lane = lane_id();
-    uint32_t inc[kRPT];
-#pragma unroll
-    for (uint32_t j = 0; j < kRPT; ++j) {
-        inc[j] = wave_incl_scan<uint32_t>(cnt[j]);
-        if (lane == 63) lds32[4 * j + w] = inc[j];
-    }
-    __syncthreads();
-    uint32_t run = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kRPT; ++j) {
-#pragma unroll
-        for (uint32_t ww = 0; ww < 4; ++ww) {
-            if (ww == w) pre[j] = run + inc[j] - cnt[j];
-            run += lds32[4 * j + ww];
-        }
-    }
-    __syncthreads();
-    return run;
-}
-
-__global__ __launch_bounds__(kWG) void k_batch_probe_write(BatchArgs a, uint32_t units) {
-    __shared__ uint32_t lds32[32];
-    const NLKeys nl{};
-    const DirectTable dt{};
-    for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
-        uint64_t base = a.poff[g];
-        const uint64_t end = a.poff[g + 1];
-        if (base == end) continue;
-        const uint32_t k = find_unit_owner(a.ppage_start, a.nplans, g);
-        const BPlan pl = a.plans[k];
-        const uint64_t tb = a.part_off[a.part_start[k]];
-        const HashTableDev ht{a.dir + pl.dir_base, a.keys + tb, a.rids + tb, (int)pl.log_slots, 0};
-        const uint32_t page = g - a.ppage_start[k];
-        const uint8_t *p = pl.pk_pages + (size_t)page * kPage;
-        const uint32_t n = ld16(p), nv = ld16(p + 2);
-        const uint64_t r0 = pl.pk_rs[page];
-        uint32_t running = 0;
-        for (uint32_t t0 = 0; t0 < n && base < end; t0 += kTile) {
-            int32_t key[kRPT];
-            const uint32_t valid = load_keys8(p, n, nv, n == nv, t0, &running, key, lds32);
-            uint32_t m[kRPT], s[kRPT], e[kRPT], pre[kRPT];
-            lookup8<kModeHash>(valid, key, ht, nl, dt, m, s, e);
-            const uint32_t tot = batch_scan8(m, pre, lds32);
+            uint64_t pos[kRPT];
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j) {
-                if (!m[j]) continue;
-                const uint32_t prid = (uint32_t)(r0 + t0 + kWG * j + threadIdx.x);
-                uint64_t o = base + pre[j];
-                for (uint32_t i = s[j]; i < e[j]; ++i)
-                    if (ht.keys[i] == key[j]) {
-                        if (o < end) {
-                            a.out_b[o] = ht.rids[i];
-                            a.out_p[o] = prid;
-                        }
-                        ++o;
-                    }
+                pos[j] = 0;
+                if ((valid >> j) & 1u)
+                    pos[j] = atomicAdd((unsigned long long *)&cur[bslot_of(key[j], ls) >> kBatchLogPart], 1ull);
             }
-            base += tot;
+            // per carried column: all loads, then all stores (a store between
+            // two loads would serialize them on memory latency)
+            for (uint32_t c = 0; c < ncar; ++c) carry8(sd, c, valid, r0 + t0 + threadIdx.x, pos, plan_base);
+#pragma unroll
+            for (uint32_t j = 0; j < kRPT; ++j)
+                if ((valid >> j) & 1u) keys_out[pos[j]] = key[j];
         }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------- per-partition build (slot order)
+// build_partition (hashtable.h:197-246): count per slot, exclusive prefix,
+// place the tuples; the directory keeps each slot's partition-local end.
+__global__ __launch_bounds__(kWG) void k_bx_build(BatchArgs a, const BPlan *__restrict__ plans,
+                                                  const uint32_t *__restrict__ part_start, const uint64_t *__restrict__ boff,
+                                                  const int32_t *__restrict__ bstage_keys) {
+    __shared__ uint32_t cnt[kSlots];
+    __shared__ uint32_t lds4[4];
+    for (uint32_t g = blockIdx.x; g < a.nparts; g += gridDim.x) {
+        const uint32_t k = find_owner(part_start, a.nplans, g);
+        const BPlan &pl = plans[k];
+        const uint32_t ls = pl.log_slots;
+        const uint64_t t_a = boff[g], t_e = boff[g + 1];
+        const uint64_t plan_base = boff[part_start[k]];
+        for (uint32_t i = threadIdx.x; i < kSlots; i += kWG) cnt[i] = 0;
+        __syncthreads();
+        for (uint64_t i = t_a + threadIdx.x; i < t_e; i += kWG)
+            atomicAdd(&cnt[(uint32_t)bslot_of(bstage_keys[i], ls) & kSlotMask], 1u);
+        __syncthreads();
+        constexpr uint32_t per = kSlots / kWG;
+        const uint32_t s0 = threadIdx.x * per;
+        uint32_t local = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < per; ++i) local += cnt[s0 + i];
+        uint32_t tot;
+        uint32_t pre = block_excl_scan<uint32_t>(local, &tot, lds4);
+#pragma unroll
+        for (uint32_t i = 0; i < per; ++i) {
+            const uint32_t c = cnt[s0 + i];
+            cnt[s0 + i] = pre;
+            pre += c;
+        }
+        __syncthreads();
+        const uint32_t ncar = pl.b.ncar;
+        for (uint64_t i = t_a + threadIdx.x; i < t_e; i += kWG) {
+            const int32_t key = bstage_keys[i];
+            const uint32_t pos = atomicAdd(&cnt[(uint32_t)bslot_of(key, ls) & kSlotMask], 1u);
+            a.bkeys[t_a + pos] = key;
+            const uint64_t src = i - plan_base, dst = t_a + pos - plan_base;
+            for (uint32_t c = 0; c < ncar; ++c) {
+                if (pl.b.car_w[c] == 4) {
+                    glb(reinterpret_cast<int32_t *>(pl.bslot_car[c]))[dst] =
+                        glb(reinterpret_cast<const int32_t *>(pl.b.car[c]))[src];
+                } else {
+                    glb(reinterpret_cast<uint64_t *>(pl.bslot_car[c]))[dst] =
+                        glb(reinterpret_cast<const uint64_t *>(pl.b.car[c]))[src];
+                    glb(pl.bslot_vld[c])[dst] = glb(pl.b.car_vld[c])[src];
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t *d = a.dir + pl.dir_base + (uint64_t)(g - part_start[k]) * kSlots;
+        for (uint32_t i = threadIdx.x; i < kSlots; i += kWG) d[i] = cnt[i];
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- join units
+// A probe partition is cut into chunks of kBatchChunk tuples; a partition
+// whose build side is empty has no units (it cannot match).
+__global__ void k_bx_units(BatchArgs a) {
+    const uint32_t g = blockIdx.x * kWG + threadIdx.x;
+    if (g >= a.nparts) return;
+    const uint64_t np = a.poff[g + 1] - a.poff[g];
+    const bool build = a.boff[g + 1] > a.boff[g];
+    a.ucount[g] = build ? (np + kBatchChunk - 1) / kBatchChunk : 0;
+}
+
+// ------------------------------------------------------------------ join
+namespace {
+
+// The build partition a join unit probes: in LDS (ends as u16, keys and
+// carried values) when it holds at most lds_cap tuples, else read in place.
+struct BuildPart {
+    const uint16_t *l_ends;
+    const uint8_t *l_dyn;
+    const uint32_t *g_ends;
+    const int32_t *g_keys;  // bkeys + t_a
+    uint64_t g_car0;        // plan-local index of the partition's first tuple
+    uint32_t cap;           // LDS layout: keys[cap], values of each carried column, validity bytes
+    uint32_t ncar;
+    uint32_t w8;            // bit c: carried column c is 8 bytes wide
+    // byte offset of carried column c's values / validity bytes in LDS
+    __device__ uint32_t coff(uint32_t c) const {
+        return cap * (4 + 4 * c + 4 * __popc(w8 & ((1u << c) - 1)));
+    }
+    __device__ uint32_t voff(uint32_t c) const {
+        return cap * (4 + 4 * ncar + 4 * __popc(w8)) + cap * __popc(w8 & ((1u << c) - 1));
+    }
+};
+
+template <bool LDS>
+__device__ __forceinline__ uint32_t bp_end(const BuildPart &bp, uint32_t s) {
+    if constexpr (LDS) return bp.l_ends[s];
+    else return bp.g_ends[s];
+}
+template <bool LDS>
+__device__ __forceinline__ int32_t bp_key(const BuildPart &bp, uint32_t t) {
+    if constexpr (LDS) return reinterpret_cast<const int32_t *>(bp.l_dyn)[t];
+    else return bp.g_keys[t];
+}
+
+constexpr uint32_t kMatchBuf = kBatchMatchBuf;
+
+// Writes output rows base .. base + cnt - 1 from the expanded matches
+// (mrow: the probe tuple's offset in the tile, mt: the build tuple), one
+// column at a time so every store is coalesced.
+template <bool LDS>
+__device__ __forceinline__ void bx_emit(const BPlan &pl, const BOut *bo, const BuildPart &bp, uint64_t base,
+                                        uint32_t cnt, const uint16_t *mrow, const uint32_t *mt, uint64_t pt0) {
+    constexpr uint32_t Q = kMatchBuf / kWG;  // matches per thread (all loads, then all stores)
+    const uint32_t nout = bo->nout;
+    for (uint32_t c = 0; c < nout; ++c) {
+        const uint32_t src = bo->src[c], cj = bo->j[c], w = bo->w[c];
+        uint8_t *dst = bo->dense[c];
+        if (w == 4) {
+            const CJ_G int32_t *gb = glb(reinterpret_cast<const int32_t *>(pl.bslot_car[cj])) + bp.g_car0;
+            const int32_t *lb = reinterpret_cast<const int32_t *>(bp.l_dyn + bp.coff(cj));
+            const CJ_G int32_t *gp = glb(reinterpret_cast<const int32_t *>(pl.p.car[cj])) + pt0;
+            CJ_G int32_t *out = glb(reinterpret_cast<int32_t *>(dst));
+            int32_t v[Q];
+#pragma unroll
+            for (uint32_t q = 0; q < Q; ++q) {
+                const uint32_t i = q * kWG + threadIdx.x;
+                if (i >= cnt) continue;
+                if (src == 0) {
+                    v[q] = bp_key<LDS>(bp, mt[i]);
+                } else if (src == 1) {
+                    if constexpr (LDS) v[q] = lb[mt[i]];
+                    else v[q] = gb[mt[i]];
+                } else {
+                    v[q] = gp[mrow[i]];
+                }
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < Q; ++q) {
+                const uint32_t i = q * kWG + threadIdx.x;
+                if (i < cnt) out[base + i] = v[q];
+            }
+        } else {
+            const CJ_G uint64_t *gb = glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[cj])) + bp.g_car0;
+            const CJ_G uint8_t *gbv = glb(pl.bslot_vld[cj]) + bp.g_car0;
+            const uint64_t *lb = reinterpret_cast<const uint64_t *>(bp.l_dyn + bp.coff(cj));
+            const uint8_t *lbv = bp.l_dyn + bp.voff(cj);
+            const CJ_G uint64_t *gp = glb(reinterpret_cast<const uint64_t *>(pl.p.car[cj])) + pt0;
+            const CJ_G uint8_t *gpv = glb(pl.p.car_vld[cj]) + pt0;
+            CJ_G uint8_t *dv = glb(bo->dvld[c]);
+            CJ_G uint64_t *out = glb(reinterpret_cast<uint64_t *>(dst));
+            uint64_t v[Q];
+            uint32_t vb = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < Q; ++q) {
+                const uint32_t i = q * kWG + threadIdx.x;
+                if (i >= cnt) continue;
+                uint8_t b;
+                if (src == 1) {
+                    const uint32_t t = mt[i];
+                    if constexpr (LDS) v[q] = lb[t], b = lbv[t];
+                    else v[q] = gb[t], b = gbv[t];
+                } else {
+                    const uint32_t r = mrow[i];
+                    v[q] = gp[r], b = gpv[r];
+                }
+                vb |= (uint32_t)(b != 0) << q;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < Q; ++q) {
+                const uint32_t i = q * kWG + threadIdx.x;
+                if (i >= cnt) continue;
+                out[base + i] = v[q];
+                dv[base + i] = (uint8_t)((vb >> q) & 1u);
+            }
+        }
+    }
+}
+
+// Probe tuples [p0, p1) of the unit, kRPT per thread per tile: returns the
+// thread's match count (count pass) or writes the matches (write pass).
+template <bool WRITE, bool LDS, uint32_t RPT>
+__device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &pl, const BOut *bo,
+                                             const BuildPart &bp, uint64_t p0, uint64_t p1, uint64_t pbase,
+                                             uint64_t obase, uint32_t *lds32, uint16_t *mrow, uint32_t *mt) {
+    uint64_t mine = 0, base = obase;
+    const uint32_t ls = pl.log_slots;
+    for (uint64_t i0 = p0; i0 < p1; i0 += kWG * RPT) {
+        int32_t key[RPT];
+        uint32_t s[RPT], e[RPT], m[RPT];
+#pragma unroll
+        for (uint32_t j = 0; j < RPT; ++j) {
+            const uint64_t idx = i0 + kWG * j + threadIdx.x;
+            key[j] = pkeys[idx < p1 ? idx : p1 - 1];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < RPT; ++j) {
+            const uint32_t sl = (uint32_t)bslot_of(key[j], ls) & kSlotMask;
+            const uint32_t hi = bp_end<LDS>(bp, sl);
+            const uint32_t lo = sl ? bp_end<LDS>(bp, sl - 1) : 0u;
+            const bool in = i0 + kWG * j + threadIdx.x < p1;
+            s[j] = in ? lo : 0u;
+            e[j] = in ? hi : 0u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < RPT; ++j) {
+            uint32_t c = 0;
+            for (uint32_t t = s[j]; t < e[j]; ++t) c += bp_key<LDS>(bp, t) == key[j];
+            m[j] = c;
+        }
+        if constexpr (!WRITE) {
+#pragma unroll
+            for (uint32_t j = 0; j < RPT; ++j) mine += m[j];
+        } else {
+            // exclusive prefix in tuple order i0 + 256 j + t
+            const uint32_t w = wave_id(), lane = lane_id();
+            uint32_t inc[RPT], pre[RPT];
+#pragma unroll
+            for (uint32_t j = 0; j < RPT; ++j) {
+                inc[j] = wave_incl_scan<uint32_t>(m[j]);
+                if (lane == 63) lds32[4 * j + w] = inc[j];
+            }
+            __syncthreads();
+            uint32_t run = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < RPT; ++j)
+#pragma unroll
+                for (uint32_t ww = 0; ww < 4; ++ww) {
+                    if (ww == w) pre[j] = run + inc[j] - m[j];
+                    run += lds32[4 * j + ww];
+                }
+            __syncthreads();
+            // expand the tile's matches into LDS a window at a time, then emit them
+            for (uint32_t w0 = 0; w0 < run; w0 += kMatchBuf) {
+                const uint32_t w1 = run - w0 < kMatchBuf ? run : w0 + kMatchBuf;
+#pragma unroll
+                for (uint32_t j = 0; j < RPT; ++j) {
+                    if (!m[j] || pre[j] >= w1 || pre[j] + m[j] <= w0) continue;
+                    uint32_t q = pre[j];
+                    for (uint32_t t = s[j]; t < e[j] && q < w1; ++t) {
+                        if (bp_key<LDS>(bp, t) != key[j]) continue;
+                        if (q >= w0) {
+                            mrow[q - w0] = (uint16_t)(kWG * j + threadIdx.x);
+                            mt[q - w0] = t;
+                        }
+                        ++q;
+                    }
+                }
+                __syncthreads();
+                bx_emit<LDS>(pl, bo, bp, base + w0, w1 - w0, mrow, mt, i0 - pbase);
+                __syncthreads();
+            }
+            base += run;
+        }
+    }
+    return mine;
+}
+
+}  // namespace
+
+template <bool WRITE>
+__global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__restrict__ outs, const BPlan *__restrict__ plans,
+                                                 const uint32_t *__restrict__ part_start, const uint64_t *__restrict__ uoff,
+                                                 const uint64_t *__restrict__ boff, const uint64_t *__restrict__ poff,
+                                                 const uint64_t *__restrict__ moff, const int32_t *__restrict__ pkeys,
+                                                 const int32_t *__restrict__ bkeys, const uint32_t *__restrict__ dir) {
+    extern __shared__ __align__(16) uint8_t dyn[];  // keys, carried build values
+    __shared__ uint16_t ends[kSlots];
+    __shared__ uint32_t lds32[32];
+    __shared__ uint64_t red[4];
+    __shared__ uint16_t mrow[WRITE ? kMatchBuf : 1];
+    __shared__ uint32_t mt[WRITE ? kMatchBuf : 1];
+    const uint64_t total = uoff[a.nparts];
+    for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
+        const uint32_t g = find_owner<uint64_t>(uoff, a.nparts, u);
+        const uint32_t k = find_owner(part_start, a.nplans, g);
+        const BPlan &pl = plans[k];
+        const uint64_t t_a = boff[g], t_e = boff[g + 1];
+        const uint32_t nb = (uint32_t)(t_e - t_a);
+        const uint64_t bbase = boff[part_start[k]], pbase = poff[part_start[k]];
+        const uint64_t p0 = poff[g] + (u - uoff[g]) * kBatchChunk;
+        const uint64_t pe = poff[g + 1];
+        const uint64_t p1 = p0 + kBatchChunk < pe ? p0 + kBatchChunk : pe;
+        BuildPart bp;
+        bp.l_ends = ends;
+        bp.l_dyn = dyn;
+        bp.g_ends = dir + pl.dir_base + (uint64_t)(g - part_start[k]) * kSlots;
+        bp.g_keys = bkeys + t_a;
+        bp.g_car0 = t_a - bbase;
+        const uint32_t ncar = pl.b.ncar;
+        bp.cap = a.lds_cap;
+        bp.ncar = ncar;
+        bp.w8 = 0;
+        for (uint32_t c = 0; c < ncar; ++c) bp.w8 |= (pl.b.car_w[c] == 8 ? 1u : 0u) << c;
+        uint64_t obase = 0;
+        if (WRITE) obase = moff[u] - moff[uoff[part_start[k]]];
+        uint64_t mine;
+        if (nb <= a.lds_cap) {
+            for (uint32_t i = threadIdx.x; i < kSlots; i += kWG) ends[i] = (uint16_t)bp.g_ends[i];
+            for (uint32_t i = threadIdx.x; i < nb; i += kWG) reinterpret_cast<int32_t *>(dyn)[i] = bp.g_keys[i];
+            if (WRITE)
+                for (uint32_t c = 0; c < ncar; ++c) {
+                    if (pl.b.car_w[c] == 4) {
+                        const CJ_G int32_t *src = glb(reinterpret_cast<const int32_t *>(pl.bslot_car[c])) + bp.g_car0;
+                        for (uint32_t i = threadIdx.x; i < nb; i += kWG)
+                            reinterpret_cast<int32_t *>(dyn + bp.coff(c))[i] = src[i];
+                    } else {
+                        const CJ_G uint64_t *src = glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[c])) + bp.g_car0;
+                        const CJ_G uint8_t *vs = glb(pl.bslot_vld[c]) + bp.g_car0;
+                        for (uint32_t i = threadIdx.x; i < nb; i += kWG) {
+                            reinterpret_cast<uint64_t *>(dyn + bp.coff(c))[i] = src[i];
+                            dyn[bp.voff(c) + i] = vs[i];
+                        }
+                    }
+                }
+            __syncthreads();
+            mine = bx_probe<WRITE, true, WRITE ? 4u : kRPT>(pkeys, pl, outs + k, bp, p0, p1, pbase, obase, lds32, mrow, mt);
+        } else {
+            mine = bx_probe<WRITE, false, WRITE ? 4u : kRPT>(pkeys, pl, outs + k, bp, p0, p1, pbase, obase, lds32, mrow, mt);
+        }
+        if (!WRITE) {
+            uint64_t tot;
+            block_excl_scan<uint64_t>(mine, &tot, red);
+            if (threadIdx.x == 0) a.mcount[u] = tot;
+        }
+        __syncthreads();
     }
 }
 
 // ------------------------------------------------------ decode (dense copy)
-__global__ __launch_bounds__(kWG) void k_batch_decode(const BDecode *jobs, const uint32_t *page_start, uint32_t njobs,
-                                                      uint32_t units) {
+// INT32 NULLs are written as INT32_MIN (the value the output writer reads
+// as NULL); 8-byte columns also set a validity bitmap.  A thread takes 8
+// consecutive rows (4 for 8-byte values): one bitmap byte (nibble), one
+// block scan for the value ranks, all page loads before the dense stores.
+template <uint32_t W>
+__device__ __forceinline__ void bx_decode_page(const BDecode &d, const CJ_G uint8_t *p, uint64_t base,
+                                               uint32_t *lds4) {
+    constexpr uint32_t RPT = W == 4 ? 8 : 4;
+    const uint32_t n = ld16(p), nv = ld16(p + 2);
+    const CJ_G uint8_t *bm = p + kPage - ((n + 7) >> 3);
+    const bool dense = n == nv;
+    const uint32_t blast = n ? (n - 1) >> 3 : 0, vlast = nv ? nv - 1 : 0;
+    uint32_t running = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += kWG * RPT) {
+        const uint32_t i0 = c0 + RPT * threadIdx.x;
+        const uint32_t in = i0 >= n ? 0u : (n - i0 >= RPT ? (1u << RPT) - 1 : (1u << (n - i0)) - 1);
+        uint32_t bits = in;
+        if (!dense) {
+            const uint32_t byte = bm[(i0 >> 3) < blast ? (i0 >> 3) : blast];
+            bits &= W == 4 ? byte : (byte >> (i0 & 4)) & 15u;
+        }
+        uint32_t tot;
+        const uint32_t pre = running + block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);
+        if constexpr (W == 4) {
+            int32_t v[RPT];
+#pragma unroll
+            for (uint32_t q = 0; q < RPT; ++q) {
+                const uint32_t vi = pre + __popc(bits & ((1u << q) - 1));
+                const int32_t x = ld32(p + 4 + 4 * (size_t)(vi < nv ? vi : vlast));
+                v[q] = ((bits >> q) & 1u) && vi < nv ? x : kI32Null;
+            }
+            CJ_G int32_t *out = glb(static_cast<int32_t *>(d.values)) + base + i0;
+#pragma unroll
+            for (uint32_t q = 0; q < RPT; ++q)
+                if ((in >> q) & 1u) out[q] = v[q];
+        } else {
+            uint64_t v[RPT];
+            uint32_t ok = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < RPT; ++q) {
+                const uint32_t vi = pre + __popc(bits & ((1u << q) - 1));
+                v[q] = ld64(p + 8 + 8 * (size_t)(vi < nv ? vi : vlast));
+                ok |= (((bits >> q) & 1u) && vi < nv ? 1u : 0u) << q;
+            }
+            CJ_G uint64_t *out = glb(static_cast<uint64_t *>(d.values)) + base + i0;
+#pragma unroll
+            for (uint32_t q = 0; q < RPT; ++q)
+                if ((in >> q) & 1u) out[q] = (ok >> q) & 1u ? v[q] : 0;
+            if (ok) {
+                const uint64_t r = base + i0;
+                const uint32_t sh = (uint32_t)(r & 31);
+                CJ_G uint32_t *vw = glb(d.valid) + (r >> 5);
+                __hip_atomic_fetch_or(vw, ok << sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (sh + RPT > 32 && (ok >> (32 - sh)))
+                    __hip_atomic_fetch_or(vw + 1, ok >> (32 - sh), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        running += tot;
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_bx_decode(const BDecode *__restrict__ jobs, const uint32_t *__restrict__ page_start, uint32_t njobs,
+                                                   uint32_t units) {
     __shared__ uint32_t lds4[4];
     for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
-        const uint32_t j = find_unit_owner(page_start, njobs, g);
+        const uint32_t j = find_owner(page_start, njobs, g);
         const BDecode d = jobs[j];
         const uint32_t pg = g - page_start[j];
-        const uint8_t *p = d.pages + (size_t)pg * kPage;
-        const uint32_t n = ld16(p), nv = ld16(p + 2);
-        const uint64_t base = d.rs[pg];
-        const uint8_t *bm = p + kPage - ((n + 7) >> 3);
-        const bool dense = n == nv;
-        const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
-        uint32_t running = 0;
-        for (uint32_t t0 = 0; t0 < n; t0 += kWG) {
-            const uint32_t r = t0 + threadIdx.x;
-            const bool inr = r < n;
-            const uint32_t rc = inr ? r : nlast;
-            const bool ok = inr && (dense || page_bit(bm, rc));
-            uint64_t m;
-            uint32_t tot;
-            const uint32_t vi = running + tile_rank(ok, &m, &tot, lds4);
-            const bool rd = ok && vi < nv;
-            const uint32_t vc = vi < nv ? vi : vlast;
-            if (d.width == 4) {
-                const int32_t v = ld32(p + 4 + 4 * (size_t)vc);
-                if (inr) static_cast<int32_t *>(d.values)[base + r] = rd ? v : 0;
-            } else {
-                const uint64_t v = ld64(p + 8 + 8 * (size_t)vc);
-                if (inr) static_cast<uint64_t *>(d.values)[base + r] = rd ? v : 0;
-            }
-            if (lane_id() == 0) or_mask_bits(d.valid, base + t0 + 64 * wave_id(), m);
-            running += tot;
-        }
+        const CJ_G uint8_t *p = glb(d.pages) + (size_t)pg * kPage;
+        const uint64_t base = glb(d.rs)[pg];
+        if (d.width == 4) bx_decode_page<4>(d, p, base, lds4);
+        else bx_decode_page<8>(d, p, base, lds4);
         __syncthreads();
     }
 }
 
-// ------------------------------------------------------------ materialize
+// -------------------------------------------------------------------- pack
 // Output page k of a column holds result rows [k R, (k + 1) R), R = 1984
-// (INT32) or 1007 (8-byte): gather by row id, rank the non-NULL rows.
-__global__ __launch_bounds__(kWG) void k_batch_materialize(const BMat *cols, const uint32_t *page_start,
-                                                           uint32_t ncols, uint32_t units) {
+// (INT32) or 1007 (8-byte).  A thread takes 8 consecutive rows (4 for 8-byte
+// values): loads them, ranks the non-NULL ones with one block scan, writes
+// the values and its bitmap byte (nibble).  INT32_MIN is written as NULL
+// (page_builders.h:123).
+__global__ __launch_bounds__(kWG) void k_bx_pack(const BPack *__restrict__ cols, const uint32_t *__restrict__ page_start, uint32_t ncols,
+                                                 uint32_t units) {
     __shared__ uint32_t lds4[4];
     for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
-        const uint32_t c = find_unit_owner(page_start, ncols, g);
-        const BMat m = cols[c];
+        const uint32_t c = find_owner(page_start, ncols, g);
+        const BPack m = cols[c];
         const uint32_t R = m.width == 4 ? kRowsI32 : kRowsI64;
         const uint64_t pg = g - page_start[c];
         const uint64_t r0 = pg * R;
         const uint32_t cnt = (uint32_t)(m.rows - r0 < R ? m.rows - r0 : R);
-        uint8_t *p = m.out + pg * kPage;
+        CJ_G uint8_t *p = glb(m.out) + pg * kPage;
         const uint32_t nbytes = (cnt + 7) >> 3;
-        uint8_t *bm = p + kPage - nbytes;
-        uint32_t running = 0;
-        for (uint32_t t0 = 0; t0 < cnt; t0 += kWG) {
-            const uint32_t i = t0 + threadIdx.x;
-            const uint64_t ri = r0 + (i < cnt ? i : cnt - 1);
-            const uint32_t rid = m.ids[ri];
-            bool ok = i < cnt && valid_bit(m.valid, rid);
-            uint64_t v;
-            if (m.width == 4) {
-                const int32_t x = static_cast<const int32_t *>(m.values)[rid];
-                if (m.is_int32 && x == kI32Null) ok = false;  // page_builders.h:123
-                v = (uint32_t)x;
+        CJ_G uint8_t *bm = p + kPage - nbytes;
+        uint32_t tot;
+        if (m.width == 4) {
+            const uint32_t i0 = 8 * threadIdx.x;
+            const CJ_G int32_t *src = glb(reinterpret_cast<const int32_t *>(m.dense)) + r0;
+            int32_t v[8];
+            if (i0 + 8 <= cnt) {
+                typedef int32_t v4i __attribute__((ext_vector_type(4)));
+                const v4i a = *reinterpret_cast<const CJ_G v4i *>(src + i0);
+                const v4i b = *reinterpret_cast<const CJ_G v4i *>(src + i0 + 4);
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) v[q] = a[q], v[4 + q] = b[q];
             } else {
-                v = static_cast<const uint64_t *>(m.values)[rid];
+#pragma unroll
+                for (uint32_t q = 0; q < 8; ++q) v[q] = i0 + q < cnt ? src[i0 + q] : kI32Null;
             }
-            uint64_t mk;
-            uint32_t tot;
-            const uint32_t vi = running + tile_rank(ok, &mk, &tot, lds4);
-            if (ok) {
-                if (m.width == 4) *reinterpret_cast<uint32_t *>(p + 4 + 4 * (size_t)vi) = (uint32_t)v;
-                else *reinterpret_cast<uint64_t *>(p + 8 + 8 * (size_t)vi) = v;
+            uint32_t bits = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) bits |= (v[q] != kI32Null ? 1u : 0u) << q;
+            uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);
+            CJ_G uint32_t *vals = reinterpret_cast<CJ_G uint32_t *>(p + 4);
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q)
+                if ((bits >> q) & 1u) vals[o++] = (uint32_t)v[q];
+            if (threadIdx.x < nbytes) bm[threadIdx.x] = (uint8_t)bits;
+        } else {
+            const uint32_t i0 = 4 * threadIdx.x;
+            const CJ_G uint64_t *src = glb(reinterpret_cast<const uint64_t *>(m.dense)) + r0;
+            const CJ_G uint8_t *sv = glb(m.dvld) + r0;
+            uint64_t v[4];
+            uint32_t bits = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                const uint32_t i = i0 + q < cnt ? i0 + q : cnt - 1;
+                v[q] = src[i];
+                bits |= (i0 + q < cnt && sv[i] ? 1u : 0u) << q;
             }
-            const uint32_t lane = lane_id();
-            if (lane < 8) {
-                const uint32_t b = ((t0 + 64 * wave_id()) >> 3) + lane;
-                if (b < nbytes) bm[b] = (uint8_t)(mk >> (8 * lane));
-            }
-            running += tot;
+            uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);
+            CJ_G uint64_t *vals = reinterpret_cast<CJ_G uint64_t *>(p + 8);
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q)
+                if ((bits >> q) & 1u) vals[o++] = v[q];
+            const uint32_t hi = __shfl_down(bits, 1);  // rows i0 + 4 .. i0 + 7 (0 past cnt)
+            if (!(threadIdx.x & 1) && (threadIdx.x >> 1) < nbytes) bm[threadIdx.x >> 1] = (uint8_t)(bits | (hi << 4));
         }
         if (threadIdx.x == 0) {
-            *reinterpret_cast<uint16_t *>(p) = (uint16_t)cnt;
-            *reinterpret_cast<uint16_t *>(p + 2) = (uint16_t)running;
+            *reinterpret_cast<CJ_G uint16_t *>(p) = (uint16_t)cnt;
+            *reinterpret_cast<CJ_G uint16_t *>(p + 2) = (uint16_t)tot;
         }
         __syncthreads();
     }
 }
 
-// -------------------------------------------------------------- launchers
-static unsigned pgrid(uint32_t units, uint32_t grid) { return units < grid ? units : grid; }
+__global__ void k_bx_totals(BatchArgs a, uint64_t *totals) {
+    const uint32_t k = blockIdx.x * kWG + threadIdx.x;
+    if (k < a.nplans) totals[k] = a.moff[a.uoff[a.part_start[k + 1]]] - a.moff[a.uoff[a.part_start[k]]];
+}
 
-void batch_part_count(const BatchArgs &a, uint32_t units, uint32_t grid, hipStream_t s) {
+__global__ void k_copy_u64(const uint64_t *src, uint64_t *dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kWG) dst[i] = src[i];
+}
+
+// -------------------------------------------------------------- launchers
+static unsigned pgrid(uint64_t units, uint32_t grid) { return (unsigned)(units < grid ? units : grid); }
+
+void batch_part_count(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s) {
     if (!units) return;
-    k_batch_part_count<<<pgrid(units, grid), kWG, a.max_parts * sizeof(uint32_t), s>>>(a, units);
+    const size_t sh = a.max_parts * sizeof(uint32_t);
+    if (probe) k_bx_part_count<true><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.ppage_start, a.part_start);
+    else k_bx_part_count<false><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.bpage_start, a.part_start);
     CJ_HIP(hipGetLastError());
 }
 
-void batch_part_scatter(const BatchArgs &a, uint32_t units, uint32_t grid, hipStream_t s) {
+void batch_part_scatter(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s) {
     if (!units) return;
     const size_t sh = ((a.max_parts + 1) & ~1u) * sizeof(uint32_t) + a.max_parts * sizeof(uint64_t);
-    k_batch_part_scatter<<<pgrid(units, grid), kWG, sh, s>>>(a, units);
+    if (probe) k_bx_part_scatter<true><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.ppage_start, a.part_start,
+                                                                     a.poff);
+    else k_bx_part_scatter<false><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.bpage_start, a.part_start,
+                                                                      a.boff);
     CJ_HIP(hipGetLastError());
 }
 
-void batch_build(const BatchArgs &a, uint32_t nparts, uint32_t grid, hipStream_t s) {
-    if (!nparts) return;
-    const size_t sh = 2 * sizeof(uint32_t) << kBatchMaxLogPart;
-    k_batch_build<<<pgrid(nparts, grid), kWG, sh, s>>>(a, nparts);
+void batch_build(const BatchArgs &a, uint32_t grid, hipStream_t s) {
+    if (!a.nparts) return;
+    k_bx_build<<<pgrid(a.nparts, grid), kWG, 0, s>>>(a, a.plans, a.part_start, a.boff, a.bstage_keys);
     CJ_HIP(hipGetLastError());
 }
 
-void batch_probe_count(const BatchArgs &a, uint32_t units, uint32_t grid, hipStream_t s) {
-    if (!units) return;
-    k_batch_probe_count<<<pgrid(units, grid), kWG, 0, s>>>(a, units);
+void batch_units(const BatchArgs &a, hipStream_t s) {
+    if (!a.nparts) return;
+    k_bx_units<<<(a.nparts + kWG - 1) / kWG, kWG, 0, s>>>(a);
     CJ_HIP(hipGetLastError());
 }
 
-void batch_probe_write(const BatchArgs &a, uint32_t units, uint32_t grid, hipStream_t s) {
-    if (!units) return;
-    k_batch_probe_write<<<pgrid(units, grid), kWG, 0, s>>>(a, units);
+void batch_join_count(const BatchArgs &a, uint32_t grid, hipStream_t s) {
+    if (!a.max_units) return;
+    k_bx_join<false><<<pgrid(a.max_units, grid), kWG, a.lds_bytes, s>>>(a, nullptr, a.plans, a.part_start, a.uoff, a.boff,
+                                                                        a.poff, a.moff, a.pkeys, a.bkeys, a.dir);
+    CJ_HIP(hipGetLastError());
+}
+
+void batch_join_write(const BatchArgs &a, const BOut *outs, uint32_t grid, hipStream_t s) {
+    if (!a.max_units) return;
+    k_bx_join<true><<<pgrid(a.max_units, grid), kWG, a.lds_bytes, s>>>(a, outs, a.plans, a.part_start, a.uoff, a.boff,
+                                                                       a.poff, a.moff, a.pkeys, a.bkeys, a.dir);
     CJ_HIP(hipGetLastError());
 }
 
 void batch_decode(const BDecode *jobs, const uint32_t *page_start, uint32_t njobs, uint32_t units, uint32_t grid,
                   hipStream_t s) {
     if (!units || !njobs) return;
-    k_batch_decode<<<pgrid(units, grid), kWG, 0, s>>>(jobs, page_start, njobs, units);
+    k_bx_decode<<<pgrid(units, grid), kWG, 0, s>>>(jobs, page_start, njobs, units);
     CJ_HIP(hipGetLastError());
 }
 
-void batch_materialize(const BMat *cols, const uint32_t *page_start, uint32_t ncols, uint32_t units, uint32_t grid,
-                       hipStream_t s) {
+void batch_pack(const BPack *cols, const uint32_t *page_start, uint32_t ncols, uint32_t units, uint32_t grid,
+                hipStream_t s) {
     if (!units || !ncols) return;
-    k_batch_materialize<<<pgrid(units, grid), kWG, 0, s>>>(cols, page_start, ncols, units);
+    k_bx_pack<<<pgrid(units, grid), kWG, 0, s>>>(cols, page_start, ncols, units);
     CJ_HIP(hipGetLastError());
 }
 
-__global__ void k_batch_totals(const uint64_t *poff, const uint32_t *ppage_start, uint32_t n, uint64_t *totals) {
-    const uint32_t k = blockIdx.x * kWG + threadIdx.x;
-    if (k < n) totals[k] = poff[ppage_start[k + 1]] - poff[ppage_start[k]];
-}
-
-void batch_totals(const uint64_t *poff, const uint32_t *ppage_start, uint32_t n, uint64_t *totals, hipStream_t s) {
-    if (!n) return;
-    k_batch_totals<<<(n + kWG - 1) / kWG, kWG, 0, s>>>(poff, ppage_start, n, totals);
+void batch_totals(const BatchArgs &a, uint64_t *totals, hipStream_t s) {
+    if (!a.nplans) return;
+    k_bx_totals<<<(a.nplans + kWG - 1) / kWG, kWG, 0, s>>>(a, totals);
     CJ_HIP(hipGetLastError());
-}
-
-__global__ void k_copy_u64(const uint64_t *src, uint64_t *dst, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kWG) dst[i] = src[i];
 }
 
 void batch_copy_u64(const uint64_t *src, uint64_t *dst, uint64_t n, hipStream_t s) {

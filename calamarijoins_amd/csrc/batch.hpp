@@ -1,28 +1,58 @@
 // batch.hpp -- batched execution of many independent single-join plans.
 //
-// Every phase of the join (radix partition histogram and scatter, LDS hash
-// build, probe count, pair write, column decode, page materialization) is
-// one launch of a persistent kernel that walks a device work list covering
-// all plans: global unit g (a key page, a partition, an output page) maps to
-// (plan, local unit) through a per-plan prefix array.  Per-plan geometry
-// lives in a device descriptor array, and all per-plan buffers are slices of
-// global arrays, so one global scan yields every plan's offsets.  The two
-// phase sequences (before and after the read-back of the per-plan result
-// sizes) are captured into hipGraphs once per batch template and replayed.
+// A radix join over all plans at once.  Both sides of every plan are
+// partitioned by the top bits of the key hash; a build partition covers 2^12
+// directory slots, so its tuples (a few thousand) fit in LDS.  The output
+// columns' values travel with the keys through the partition pass
+// ("carried" columns), so no phase gathers by row id: the probe side streams
+// through HBM and every lookup is an LDS access.  Phases:
+//
+//   decode carried columns -> partition histogram and scatter of both sides
+//   -> per-partition build (slot order, partition-local ends) -> join units
+//   (probe chunks of a partition) -> match count per unit -> [read-back of
+//   per-plan result sizes] -> match write (dense output columns) -> pack
+//   into 8 KiB pages.
+//
+// Every phase is one launch of a persistent kernel that walks a work list
+// covering all plans: global unit g (a key page, a partition, a probe chunk,
+// an output page) maps to (plan, local unit) through a prefix array.  The
+// phase sequence before the read-back is a hipGraph captured once per batch
+// template and replayed.
 #pragma once
 #include "cj_internal.hpp"
 
 namespace cj {
 
-// One plan as the batch kernels see it.  The build side is already chosen.
+constexpr uint32_t kBatchMaxCarry = 4;                // carried columns per side
+constexpr uint32_t kBatchMaxOut = 8;                  // output columns per plan
+constexpr uint32_t kBatchLogPart = 12;                // slots per partition = 4096
+constexpr uint64_t kBatchMaxBuildRows = 1ull << 22;   // larger builds run one plan at a time
+constexpr uint32_t kBatchChunk = 8192;                // probe tuples per join unit
+constexpr uint32_t kBatchLdsBytes = 64 * 1024;        // LDS of a join workgroup
+constexpr uint32_t kBatchMatchBuf = 2048;             // matches a join workgroup expands at a time
+
+// One side of one plan.  Carried column j holds, in partition order, the
+// value of the source column for each tuple: INT32 with NULL folded to
+// INT32_MIN (the output writer turns INT32_MIN into NULL either way), 8-byte
+// values with a validity byte beside them.
+struct BSide {
+    const uint8_t *k_pages;  // key column pages
+    const uint64_t *k_rs;    // their row starts
+    uint32_t ncar;
+    uint32_t car_w[kBatchMaxCarry];             // 4 or 8
+    const uint8_t *src[kBatchMaxCarry];         // dense decoded source column (by row)
+    const uint32_t *src_valid[kBatchMaxCarry];  // its validity bitmap (by row)
+    uint8_t *car[kBatchMaxCarry];               // carried values, plan-local tuple index
+    uint8_t *car_vld[kBatchMaxCarry];           // validity bytes (8-byte columns only)
+};
+
 struct BPlan {
-    const uint8_t *bk_pages;  // build key column
-    const uint64_t *bk_rs;    // its row starts
-    const uint8_t *pk_pages;  // probe key column
-    const uint64_t *pk_rs;
-    uint32_t log_slots;       // directory = 2^log_slots entries
-    uint32_t log_part;        // slots per partition = 2^log_part
-    uint64_t dir_base;        // first directory entry of the plan
+    BSide b, p;
+    uint32_t log_slots;  // directory = 2^log_slots slots, partitions of 2^12
+    uint64_t dir_base;   // first directory entry of the plan
+    // build side in slot order (written by the build phase)
+    uint8_t *bslot_car[kBatchMaxCarry];
+    uint8_t *bslot_vld[kBatchMaxCarry];
 };
 
 struct BatchArgs {
@@ -32,17 +62,35 @@ struct BatchArgs {
     const uint32_t *ppage_start;  // [nplans + 1] prefix of probe key pages
     const uint32_t *part_start;   // [nplans + 1] prefix of partitions
     uint32_t max_parts;           // largest partition count of a plan (LDS histogram)
-    uint64_t *part_count;         // [nparts + 1]
-    uint64_t *part_cur;           // scatter cursors (copy of the partition offsets)
-    const uint64_t *part_off;     // [nparts + 1] exclusive scan of part_count
-    int32_t *stage_keys;          // partitioned (key, row) tuples, all plans
-    uint32_t *stage_rids;
-    uint64_t *dir;                // directories, all plans
-    int32_t *keys;                // tuples in slot order, all plans
-    uint32_t *rids;
-    uint64_t *pcount;             // [nppages + 1] matches per probe key page
-    const uint64_t *poff;         // [nppages + 1] exclusive scan of pcount
-    uint32_t *out_b, *out_p;      // (build row, probe row) pairs, all plans
+    uint32_t nparts;
+    // build side
+    uint64_t *bcount, *bcur;      // [nparts + 1]
+    const uint64_t *boff;         // exclusive scan of bcount
+    int32_t *bstage_keys;         // partition order
+    int32_t *bkeys;               // slot order
+    uint32_t *dir;                // partition-local end of each slot
+    // probe side
+    uint64_t *pcount, *pcur;
+    const uint64_t *poff;
+    int32_t *pkeys;               // partition order
+    // join units
+    uint64_t *ucount;             // [nparts + 1] units per partition
+    const uint64_t *uoff;         // exclusive scan (uoff[nparts] = units)
+    uint32_t max_units;
+    uint64_t *mcount;             // [max_units + 1] matches per unit
+    const uint64_t *moff;         // exclusive scan
+    uint32_t lds_cap;             // build tuples a join workgroup holds in LDS
+    uint32_t lds_bytes;
+};
+
+// Output columns of one plan (known after the read-back).
+struct BOut {
+    uint32_t nout;
+    uint8_t src[kBatchMaxOut];  // 0 key, 1 build carried, 2 probe carried
+    uint8_t j[kBatchMaxOut];    // carried column index
+    uint8_t w[kBatchMaxOut];    // 4 or 8
+    uint8_t *dense[kBatchMaxOut];
+    uint8_t *dvld[kBatchMaxOut];  // 8-byte columns
 };
 
 // Dense decode of a base column (values + validity bitmap).
@@ -54,32 +102,28 @@ struct BDecode {
     uint32_t *valid;  // zeroed
 };
 
-// One output column of one plan.
-struct BMat {
-    const uint32_t *ids;  // the plan's build or probe rows, one per result row
+// One output column of one plan: dense values -> pages.
+struct BPack {
+    const uint8_t *dense;
+    const uint8_t *dvld;
     uint64_t rows;
-    const void *values;   // dense decoded source column
-    const uint32_t *valid;
     int32_t width;
-    int32_t is_int32;
-    uint8_t *out;         // page block
+    uint8_t *out;
 };
 
-constexpr uint32_t kBatchMaxLogPart = 12;             // partitions of <= 4096 slots (LDS build)
-constexpr uint64_t kBatchMaxBuildRows = 1ull << 22;   // larger builds run one plan at a time
-
 // Each launcher takes the persistent grid size (workgroups).
-void batch_part_count(const BatchArgs &a, uint32_t units, uint32_t grid, hipStream_t s);
-void batch_part_scatter(const BatchArgs &a, uint32_t units, uint32_t grid, hipStream_t s);
-void batch_build(const BatchArgs &a, uint32_t nparts, uint32_t grid, hipStream_t s);
-void batch_probe_count(const BatchArgs &a, uint32_t units, uint32_t grid, hipStream_t s);
-void batch_probe_write(const BatchArgs &a, uint32_t units, uint32_t grid, hipStream_t s);
+void batch_part_count(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s);
+void batch_part_scatter(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s);
+void batch_build(const BatchArgs &a, uint32_t grid, hipStream_t s);
+void batch_units(const BatchArgs &a, hipStream_t s);
+void batch_join_count(const BatchArgs &a, uint32_t grid, hipStream_t s);
+void batch_join_write(const BatchArgs &a, const BOut *outs, uint32_t grid, hipStream_t s);
 void batch_decode(const BDecode *jobs, const uint32_t *page_start, uint32_t njobs, uint32_t units, uint32_t grid,
                   hipStream_t s);
-void batch_materialize(const BMat *cols, const uint32_t *page_start, uint32_t ncols, uint32_t units, uint32_t grid,
-                       hipStream_t s);
-// totals[k] = poff[ppage_start[k + 1]] - poff[ppage_start[k]]
-void batch_totals(const uint64_t *poff, const uint32_t *ppage_start, uint32_t n, uint64_t *totals, hipStream_t s);
+void batch_pack(const BPack *cols, const uint32_t *page_start, uint32_t ncols, uint32_t units, uint32_t grid,
+                hipStream_t s);
+// totals[k] = matches of plan k's units
+void batch_totals(const BatchArgs &a, uint64_t *totals, hipStream_t s);
 void batch_copy_u64(const uint64_t *src, uint64_t *dst, uint64_t n, hipStream_t s);
 
 }  // namespace cj

@@ -1,15 +1,17 @@
-// batch_host.cpp -- host side of the batched join (batch.hpp): plan analysis,
-// the per-template device workspace, hipGraph capture of the first phase
-// sequence, one read-back of the per-plan result sizes, the write phase and
-// the results; plus the autotuner that picks the hash load factor and the
-// persistent grid size per batch template class.
+// batch_host.cpp -- host side of the batched radix join (batch.hpp): plan
+// analysis, the per-template device workspace, hipGraph capture of the
+// first phase sequence, one read-back of the per-plan result sizes, the
+// write and pack phase and the results; plus the autotuner that picks the
+// hash load factor and the persistent grid size per batch template class.
 //
 // A plan joins in the batch when it is one join of two base-table scans of
-// resident tables, with INT32 keys, fixed-width output columns and a build
-// side of at least 8 rows (smaller builds take the nested loop, which the
-// per-plan executor runs); every other plan runs through Context::execute.
-// The build side is the side with fewer rows, ties left
-// (join_setup.h:108-136), as in the per-plan executor.
+// resident tables, with INT32 keys, fixed-width output columns (at most 8,
+// at most 4 distinct non-key columns per side) and a build side of 8 to 2^22
+// rows (smaller builds take the nested loop, which the per-plan executor
+// runs); every other plan runs through Context::execute.  The build side is
+// the side with fewer rows, ties left (join_setup.h:108-136), as in the
+// per-plan executor.  An output column that is a join key column of either
+// side is the matched key itself.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -27,9 +29,10 @@ namespace {
 struct PlanInfo {
     bool eligible = false;
     DevColumn *bk = nullptr, *pk = nullptr;
+    std::vector<DevColumn *> bcar, pcar;  // carried columns per side
     struct Out {
-        bool from_build;
-        DevColumn *col;
+        uint8_t src;  // 0 key, 1 build carried, 2 probe carried
+        uint8_t j;
         int32_t type;
     };
     std::vector<Out> outs;
@@ -56,13 +59,12 @@ PlanInfo analyse(Context &ctx, const cj_plan_desc &p) {
     const cj_node_desc &B = build_left ? L : R, &P = build_left ? R : L;
     const cj_table_desc &bt = build_left ? lt : rt, &ptt = build_left ? rt : lt;
     if (bt.num_rows < 8) return pi;  // nested loop (execute.cpp:236-238)
-    // <= 2^24 directory slots at the largest load factor: <= 4096 partitions,
-    // whose histogram and cursors fit the partition kernels' LDS
     if (bt.num_rows > kBatchMaxBuildRows) return pi;
     const uint64_t ba = build_left ? j.left_attr : j.right_attr, pa = build_left ? j.right_attr : j.left_attr;
     pi.bk = col_of(B, bt, ba);
     pi.pk = col_of(P, ptt, pa);
     if (!pi.bk || !pi.pk || pi.bk->type != CJ_INT32 || pi.pk->type != CJ_INT32) return pi;
+    if (j.num_output_attrs > kBatchMaxOut) return pi;
     const uint64_t lw = L.num_output_attrs;
     for (uint32_t a = 0; a < j.num_output_attrs; ++a) {
         const uint64_t ci = j.output_attrs[a].index;
@@ -73,7 +75,19 @@ PlanInfo analyse(Context &ctx, const cj_plan_desc &p) {
         const cj_table_desc &st = from_left ? lt : rt;
         DevColumn *c = col_of(S, st, from_left ? ci : ci - lw);
         if (!c || c->type != ty) return pi;
-        pi.outs.push_back({from_left == build_left, c, ty});
+        const bool from_build = from_left == build_left;
+        if (c == (from_build ? pi.bk : pi.pk)) {
+            pi.outs.push_back({0, 0, ty});
+            continue;
+        }
+        auto &car = from_build ? pi.bcar : pi.pcar;
+        auto it = std::find(car.begin(), car.end(), c);
+        if (it == car.end()) {
+            if (car.size() == kBatchMaxCarry) return pi;
+            car.push_back(c);
+            it = car.end() - 1;
+        }
+        pi.outs.push_back({(uint8_t)(from_build ? 1 : 2), (uint8_t)(it - car.begin()), ty});
     }
     pi.eligible = true;
     return pi;
@@ -85,6 +99,8 @@ uint32_t ceil_log2(uint64_t x) {
     return l;
 }
 
+uint32_t width_of(int32_t type) { return type == CJ_INT32 ? 4u : 8u; }
+
 double ms_since(std::chrono::steady_clock::time_point t) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
@@ -94,7 +110,6 @@ double ms_since(std::chrono::steady_clock::time_point t) {
 // Device workspace of one batch template: sized by its unit counts, reused
 // (and its phase-1 graph replayed) while a batch of the same shape repeats.
 struct BatchTemplate {
-    uint64_t key_hash = 0;
     std::vector<void *> blocks;
     BatchArgs args{};
     BPlan *d_plans = nullptr;
@@ -103,14 +118,15 @@ struct BatchTemplate {
     uint32_t *d_job_start = nullptr;
     uint64_t *scan_tmp = nullptr;
     size_t scan_elems = 0;
-    uint64_t *totals = nullptr;    // [n]
+    uint64_t *totals = nullptr;  // [n]
     uint32_t *valid_all = nullptr;
     uint64_t valid_words = 0;
     std::vector<void *> dec_values;
     std::vector<uint32_t *> dec_valid;
+    std::vector<BPlan> hplans;  // device pointers of the per-plan arrays
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    uint32_t n = 0, nb = 0, np = 0, nparts = 0, njobs = 0, ndpages = 0, grid = 0;
+    uint32_t njobs = 0, ndpages = 0;
     void release(DevicePool &pool) {
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
@@ -121,9 +137,8 @@ struct BatchTemplate {
     }
 };
 
-static double run_batch(Context &ctx, const cj_plan_desc *plans, const std::vector<uint32_t> &batched,
-                        const std::vector<PlanInfo> &info, const BatchConfig &cfg, uint32_t flags, cj_result *out,
-                        cj_batch_stats *bs);
+static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, const std::vector<PlanInfo> &info,
+                        const BatchConfig &cfg, uint32_t flags, cj_result *out, cj_batch_stats *bs);
 
 void Context::execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flags, cj_result *out,
                             cj_batch_stats *stats) {
@@ -166,11 +181,11 @@ void Context::execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flag
         }
     }
     if (!tuned && (flags & CJ_BATCH_AUTOTUNE)) {
-        const BatchConfig cands[] = {{1, 1024}, {2, 1024}, {2, 2048}, {2, 4096}, {4, 2048}};
+        const BatchConfig cands[] = {{2, 1024}, {2, 2048}, {2, 4096}, {4, 2048}, {4, 4096}};
         double best = 1e30;
         for (const BatchConfig &c : cands) {
             std::vector<cj_result> tmp(n);
-            double ms = run_batch(*this, plans, batched, info, c, flags | CJ_EXEC_OUTPUT_DEVICE, tmp.data(), nullptr);
+            double ms = run_batch(*this, batched, info, c, flags | CJ_EXEC_OUTPUT_DEVICE, tmp.data(), nullptr);
             for (uint32_t i : batched) release(&tmp[i]);
             if (ms < best) {
                 best = ms;
@@ -181,20 +196,16 @@ void Context::execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flag
         batch_tuning[cls] = cfg;
         bs.tuned = 1;
     }
-    bs.device_ms = run_batch(*this, plans, batched, info, cfg, flags, out, &bs);
+    bs.device_ms = run_batch(*this, batched, info, cfg, flags, out, &bs);
     bs.load_factor = cfg.load_factor;
     bs.grid = cfg.grid;
     bs.total_ms = ms_since(t_start);
 }
 
-static double run_batch(Context &ctx, const cj_plan_desc *plans, const std::vector<uint32_t> &batched,
-                        const std::vector<PlanInfo> &info, const BatchConfig &cfg, uint32_t flags, cj_result *out,
-                        cj_batch_stats *bs) {
-    (void)plans;
+static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, const std::vector<PlanInfo> &info,
+                        const BatchConfig &cfg, uint32_t flags, cj_result *out, cj_batch_stats *bs) {
     DevicePool &pool = ctx.pool;
     PinnedPool &pinned = ctx.pinned;
-    std::mutex &mu = ctx.mu;
-    auto &batch_templates = ctx.batch_templates;
     const uint32_t n = (uint32_t)batched.size();
     hipStream_t s = ctx.acquire_stream();
     struct Rel_ {
@@ -203,100 +214,141 @@ static double run_batch(Context &ctx, const cj_plan_desc *plans, const std::vect
         ~Rel_() { c.release_stream(s); }
     } rel_guard{ctx, s};
     CJ_HIP(hipSetDevice(ctx.device));
-    // ---- host descriptors
-    std::vector<BPlan> hp(n);
+    // ---- shape of the batch
     std::vector<uint32_t> starts(3 * (n + 1), 0);
     uint32_t *bps = starts.data(), *pps = bps + (n + 1), *pts = pps + (n + 1);
-    uint64_t dir_total = 0, tup_total = 0;
-    uint32_t max_parts = 1;
+    std::vector<uint32_t> log_slots(n);
+    uint64_t dir_total = 0, sum_nb = 0, sum_np = 0;
+    uint32_t max_parts = 1, max_tuple_bytes = 4;
     std::vector<DevColumn *> dec_cols;
     std::map<DevColumn *, uint32_t> dec_index;
+    uint64_t key = 1469598103934665603ull;
+    auto mixk = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
     for (uint32_t k = 0; k < n; ++k) {
         const PlanInfo &pi = info[batched[k]];
-        BPlan &b = hp[k];
-        b.bk_pages = pi.bk->pages;
-        b.bk_rs = pi.bk->row_start;
-        b.pk_pages = pi.pk->pages;
-        b.pk_rs = pi.pk->row_start;
-        const uint64_t want = std::max<uint64_t>(2048, (uint64_t)cfg.load_factor * pi.bk->num_rows);
-        b.log_slots = std::min<uint32_t>(ceil_log2(want), 30);
-        b.log_part = std::min(b.log_slots, kBatchMaxLogPart);
-        b.dir_base = dir_total;
-        dir_total += 1ull << b.log_slots;
-        tup_total += pi.bk->num_rows;
-        const uint32_t parts = 1u << (b.log_slots - b.log_part);
+        const uint64_t want = std::max<uint64_t>(1ull << kBatchLogPart, (uint64_t)cfg.load_factor * pi.bk->num_rows);
+        log_slots[k] = std::min<uint32_t>(ceil_log2(want), 30);
+        const uint32_t parts = 1u << (log_slots[k] - kBatchLogPart);
         max_parts = std::max(max_parts, parts);
+        dir_total += 1ull << log_slots[k];
+        sum_nb += pi.bk->num_rows;
+        sum_np += pi.pk->num_rows;
         bps[k + 1] = bps[k] + pi.bk->num_pages;
         pps[k + 1] = pps[k] + pi.pk->num_pages;
         pts[k + 1] = pts[k] + parts;
-        for (const auto &o : pi.outs)
-            if (!dec_index.count(o.col)) {
-                dec_index[o.col] = (uint32_t)dec_cols.size();
-                dec_cols.push_back(o.col);
-            }
+        uint32_t tb = 4;
+        for (DevColumn *c : pi.bcar) tb += width_of(c->type) + (c->type == CJ_INT32 ? 0 : 1);
+        max_tuple_bytes = std::max(max_tuple_bytes, tb);
+        for (const auto *side : {&pi.bcar, &pi.pcar})
+            for (DevColumn *c : *side)
+                if (!dec_index.count(c)) {
+                    dec_index[c] = (uint32_t)dec_cols.size();
+                    dec_cols.push_back(c);
+                }
+        mixk(pi.bk->num_rows), mixk(pi.pk->num_rows), mixk(pi.bk->num_pages), mixk(pi.pk->num_pages);
+        mixk(log_slots[k]), mixk(pi.bcar.size() | pi.pcar.size() << 8);
+        for (DevColumn *c : pi.bcar) mixk(c->type);
+        for (DevColumn *c : pi.pcar) mixk(c->type);
     }
     const uint32_t nbp = bps[n], npp = pps[n], nparts = pts[n];
+    const uint64_t max_units64 = nparts + sum_np / kBatchChunk + 1;
+    if (max_units64 > 0xffffffffull) throw UnsupportedError("batch too large");
+    const uint32_t max_units = (uint32_t)max_units64;
+    // LDS of a join workgroup: slot ends (u16), the match window (u16 + u32), then the partition
+    const uint32_t dyn_budget = kBatchLdsBytes - (2u << kBatchLogPart) - 6 * kBatchMatchBuf - 512;
+    const uint32_t lds_cap = std::min<uint32_t>((dyn_budget / max_tuple_bytes) & ~15u, 65535u & ~15u);
     std::vector<BDecode> jobs(dec_cols.size());
     std::vector<uint32_t> job_start(dec_cols.size() + 1, 0);
     uint64_t valid_words = 0, val_bytes = 0;
     for (size_t j = 0; j < dec_cols.size(); ++j) {
         job_start[j + 1] = job_start[j] + dec_cols[j]->num_pages;
         valid_words += dec_cols[j]->num_rows / 32 + 2;
-        val_bytes += (dec_cols[j]->num_rows + 2) * (dec_cols[j]->type == CJ_INT32 ? 4 : 8);
+        val_bytes += ((dec_cols[j]->num_rows + 2) * width_of(dec_cols[j]->type) + 15) & ~15ull;
+        mixk(dec_cols[j]->num_rows), mixk(dec_cols[j]->type);
     }
+    mixk(n), mixk(cfg.grid), mixk(cfg.load_factor), mixk(dec_cols.size());
     // ---- the template workspace (reused while the batch shape repeats)
-    uint64_t key = 1469598103934665603ull;
-    auto mixk = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
-    mixk(n), mixk(nbp), mixk(npp), mixk(nparts), mixk(dir_total), mixk(tup_total), mixk(val_bytes), mixk(cfg.grid);
-    mixk(max_parts), mixk(dec_cols.size()), mixk(job_start.back());
-    for (size_t j = 0; j < dec_cols.size(); ++j) mixk(dec_cols[j]->num_rows), mixk(dec_cols[j]->type);
     std::shared_ptr<BatchTemplate> tpl;
     {
-        std::lock_guard<std::mutex> g(mu);
-        auto it = batch_templates.find(key);
-        if (it != batch_templates.end()) {
+        std::lock_guard<std::mutex> g(ctx.mu);
+        auto it = ctx.batch_templates.find(key);
+        if (it != ctx.batch_templates.end()) {
             tpl = it->second;
-            batch_templates.erase(it);  // in use: not shared with a concurrent batch
+            ctx.batch_templates.erase(it);  // in use: not shared with a concurrent batch
         }
     }
     bool fresh = false;
     if (!tpl) {
         fresh = true;
         tpl = std::make_shared<BatchTemplate>();
-        tpl->key_hash = key;
         auto blk = [&](size_t bytes) {
             void *p = pool.alloc(bytes + 64);
             tpl->blocks.push_back(p);
-            return p;
+            return static_cast<uint8_t *>(p);
         };
         BatchArgs &a = tpl->args;
-        tpl->d_plans = static_cast<BPlan *>(blk(n * sizeof(BPlan)));
-        tpl->d_starts = static_cast<uint32_t *>(blk(starts.size() * 4));
-        a.part_count = static_cast<uint64_t *>(blk((nparts + 1) * 8));
-        uint64_t *part_off = static_cast<uint64_t *>(blk((nparts + 1) * 8));
-        a.part_off = part_off;
-        a.part_cur = static_cast<uint64_t *>(blk((nparts + 1) * 8));
-        a.stage_keys = static_cast<int32_t *>(blk((tup_total + 1) * 4));
-        a.stage_rids = static_cast<uint32_t *>(blk((tup_total + 1) * 4));
-        a.dir = static_cast<uint64_t *>(blk(dir_total * 8));
-        a.keys = static_cast<int32_t *>(blk((tup_total + 1) * 4));
-        a.rids = static_cast<uint32_t *>(blk((tup_total + 1) * 4));
-        a.pcount = static_cast<uint64_t *>(blk((npp + 1) * 8));
-        a.poff = static_cast<uint64_t *>(blk((npp + 1) * 8));
-        tpl->totals = static_cast<uint64_t *>(blk(n * 8));
-        tpl->scan_elems = scan_tmp_elems(std::max<uint64_t>(std::max(nparts, npp), 1));
-        tpl->scan_tmp = static_cast<uint64_t *>(blk(tpl->scan_elems * 8));
-        tpl->d_jobs = static_cast<BDecode *>(blk(std::max<size_t>(jobs.size(), 1) * sizeof(BDecode)));
-        tpl->d_job_start = static_cast<uint32_t *>(blk(job_start.size() * 4));
-        tpl->valid_all = static_cast<uint32_t *>(blk(std::max<uint64_t>(valid_words, 1) * 4));
+        tpl->d_plans = reinterpret_cast<BPlan *>(blk(n * sizeof(BPlan)));
+        tpl->d_starts = reinterpret_cast<uint32_t *>(blk(starts.size() * 4));
+        a.bcount = reinterpret_cast<uint64_t *>(blk((nparts + 1) * 8));
+        a.bcur = reinterpret_cast<uint64_t *>(blk((nparts + 1) * 8));
+        a.boff = reinterpret_cast<uint64_t *>(blk((nparts + 1) * 8));
+        a.bstage_keys = reinterpret_cast<int32_t *>(blk((sum_nb + 1) * 4));
+        a.bkeys = reinterpret_cast<int32_t *>(blk((sum_nb + 1) * 4));
+        a.dir = reinterpret_cast<uint32_t *>(blk(dir_total * 4));
+        a.pcount = reinterpret_cast<uint64_t *>(blk((nparts + 1) * 8));
+        a.pcur = reinterpret_cast<uint64_t *>(blk((nparts + 1) * 8));
+        a.poff = reinterpret_cast<uint64_t *>(blk((nparts + 1) * 8));
+        a.pkeys = reinterpret_cast<int32_t *>(blk((sum_np + 1) * 4));
+        a.ucount = reinterpret_cast<uint64_t *>(blk((nparts + 1) * 8));
+        a.uoff = reinterpret_cast<uint64_t *>(blk((nparts + 1) * 8));
+        a.mcount = reinterpret_cast<uint64_t *>(blk(((uint64_t)max_units + 1) * 8));
+        a.moff = reinterpret_cast<uint64_t *>(blk(((uint64_t)max_units + 1) * 8));
+        tpl->totals = reinterpret_cast<uint64_t *>(blk(n * 8));
+        tpl->scan_elems = scan_tmp_elems(std::max<uint64_t>(std::max<uint64_t>(nparts, max_units), 1));
+        tpl->scan_tmp = reinterpret_cast<uint64_t *>(blk(tpl->scan_elems * 8));
+        tpl->d_jobs = reinterpret_cast<BDecode *>(blk(std::max<size_t>(jobs.size(), 1) * sizeof(BDecode)));
+        tpl->d_job_start = reinterpret_cast<uint32_t *>(blk(job_start.size() * 4));
+        tpl->valid_all = reinterpret_cast<uint32_t *>(blk(std::max<uint64_t>(valid_words, 1) * 4));
         tpl->valid_words = valid_words;
-        uint8_t *vals = static_cast<uint8_t *>(blk(std::max<uint64_t>(val_bytes, 8)));
+        uint8_t *vals = blk(std::max<uint64_t>(val_bytes, 16));
         uint64_t vo = 0, wo = 0;
         for (size_t j = 0; j < dec_cols.size(); ++j) {
             tpl->dec_values.push_back(vals + vo);
             tpl->dec_valid.push_back(tpl->valid_all + wo);
-            vo += ((dec_cols[j]->num_rows + 2) * (dec_cols[j]->type == CJ_INT32 ? 4 : 8) + 15) & ~15ull;
+            vo += ((dec_cols[j]->num_rows + 2) * width_of(dec_cols[j]->type) + 15) & ~15ull;
             wo += dec_cols[j]->num_rows / 32 + 2;
+        }
+        // per-plan carried arrays (plan-local tuple index, sized by the side's rows)
+        tpl->hplans.resize(n);
+        uint64_t dir_base = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            const PlanInfo &pi = info[batched[k]];
+            BPlan &b = tpl->hplans[k];
+            std::memset(&b, 0, sizeof b);
+            b.log_slots = log_slots[k];
+            b.dir_base = dir_base;
+            dir_base += 1ull << log_slots[k];
+            auto side = [&](BSide &sd, const DevColumn *kc, const std::vector<DevColumn *> &car, bool build) {
+                sd.k_pages = kc->pages;
+                sd.k_rs = kc->row_start;
+                sd.ncar = (uint32_t)car.size();
+                const uint64_t rows = kc->num_rows + 1;
+                for (size_t c = 0; c < car.size(); ++c) {
+                    const uint32_t w = width_of(car[c]->type);
+                    const uint32_t j = dec_index.at(car[c]);
+                    sd.car_w[c] = w;
+                    sd.src[c] = static_cast<const uint8_t *>(tpl->dec_values[j]);
+                    sd.src_valid[c] = tpl->dec_valid[j];
+                    sd.car[c] = blk(rows * w);
+                    if (w == 8) sd.car_vld[c] = blk(rows);
+                    if (build) {
+                        b.bslot_car[c] = blk(rows * w);
+                        if (w == 8) b.bslot_vld[c] = blk(rows);
+                    }
+                }
+            };
+            side(b.b, pi.bk, pi.bcar, true);
+            side(b.p, pi.pk, pi.pcar, false);
         }
         a.plans = tpl->d_plans;
         a.nplans = n;
@@ -304,19 +356,41 @@ static double run_batch(Context &ctx, const cj_plan_desc *plans, const std::vect
         a.ppage_start = tpl->d_starts + (n + 1);
         a.part_start = tpl->d_starts + 2 * (n + 1);
         a.max_parts = max_parts;
-        tpl->n = n, tpl->nb = nbp, tpl->np = npp, tpl->nparts = nparts;
-        tpl->njobs = (uint32_t)jobs.size(), tpl->ndpages = job_start.back(), tpl->grid = cfg.grid;
+        a.nparts = nparts;
+        a.max_units = max_units;
+        a.lds_cap = lds_cap;
+        a.lds_bytes = lds_cap * max_tuple_bytes;
+        tpl->njobs = (uint32_t)jobs.size();
+        tpl->ndpages = job_start.back();
     }
     for (size_t j = 0; j < dec_cols.size(); ++j)
-        jobs[j] = BDecode{dec_cols[j]->pages, dec_cols[j]->row_start, dec_cols[j]->type == CJ_INT32 ? 4 : 8,
+        jobs[j] = BDecode{dec_cols[j]->pages, dec_cols[j]->row_start, (int32_t)width_of(dec_cols[j]->type),
                           tpl->dec_values[j], tpl->dec_valid[j]};
+    // key pages of a reused template may belong to other tables of the same shape
+    for (uint32_t k = 0; k < n; ++k) {
+        const PlanInfo &pi = info[batched[k]];
+        BPlan &b = tpl->hplans[k];
+        b.b.k_pages = pi.bk->pages, b.b.k_rs = pi.bk->row_start;
+        b.p.k_pages = pi.pk->pages, b.p.k_rs = pi.pk->row_start;
+        for (size_t c = 0; c < pi.bcar.size(); ++c) {
+            const uint32_t j = dec_index.at(pi.bcar[c]);
+            b.b.src[c] = static_cast<const uint8_t *>(tpl->dec_values[j]), b.b.src_valid[c] = tpl->dec_valid[j];
+        }
+        for (size_t c = 0; c < pi.pcar.size(); ++c) {
+            const uint32_t j = dec_index.at(pi.pcar[c]);
+            b.p.src[c] = static_cast<const uint8_t *>(tpl->dec_values[j]), b.p.src_valid[c] = tpl->dec_valid[j];
+        }
+    }
     // ---- upload the descriptors (pinned staging; the copies finish before the graph runs)
-    const size_t up_bytes = n * sizeof(BPlan) + starts.size() * 4 + jobs.size() * sizeof(BDecode) + job_start.size() * 4;
-    uint8_t *stage = static_cast<uint8_t *>(pinned.alloc(up_bytes + 64));
+    const size_t up_bytes =
+        n * sizeof(BPlan) + starts.size() * 4 + jobs.size() * sizeof(BDecode) + job_start.size() * 4 + 64;
+    uint8_t *stage = static_cast<uint8_t *>(pinned.alloc(up_bytes));
     struct PinGuard {
         PinnedPool &p;
         void *b;
-        ~PinGuard() { p.free(b); }
+        ~PinGuard() {
+            if (b) p.free(b);
+        }
     } pg{pinned, stage};
     size_t o = 0;
     auto up = [&](void *dst, const void *src, size_t bytes) {
@@ -325,46 +399,52 @@ static double run_batch(Context &ctx, const cj_plan_desc *plans, const std::vect
         CJ_HIP(hipMemcpyAsync(dst, stage + o, bytes, hipMemcpyHostToDevice, s));
         o += bytes;
     };
-    up(tpl->d_plans, hp.data(), n * sizeof(BPlan));
+    up(tpl->d_plans, tpl->hplans.data(), n * sizeof(BPlan));
     up(tpl->d_starts, starts.data(), starts.size() * 4);
     up(tpl->d_jobs, jobs.data(), jobs.size() * sizeof(BDecode));
     up(tpl->d_job_start, job_start.data(), job_start.size() * 4);
-    // ---- phase 1: partition, build, probe count, decode (captured once)
+    // ---- phase 1: decode, partition both sides, build, units, match counts (captured once)
     BatchTemplate &T = *tpl;
+    BatchArgs &A = T.args;
     auto phase1 = [&](hipStream_t ps) {
-        fill_u64(T.args.part_count, 0, nparts + 1, ps);
-        batch_part_count(T.args, nbp, cfg.grid, ps);
-        scan_exclusive_u64(T.args.part_count, const_cast<uint64_t *>(T.args.part_off), nparts, ps, T.scan_tmp,
-                           T.scan_elems);
-        batch_copy_u64(T.args.part_off, T.args.part_cur, nparts + 1, ps);
-        batch_part_scatter(T.args, nbp, cfg.grid, ps);
-        batch_build(T.args, nparts, cfg.grid, ps);
-        batch_probe_count(T.args, npp, cfg.grid, ps);
-        scan_exclusive_u64(T.args.pcount, const_cast<uint64_t *>(T.args.poff), npp, ps, T.scan_tmp, T.scan_elems);
-        batch_totals(T.args.poff, T.args.ppage_start, n, T.totals, ps);
+        fill_u64(A.bcount, 0, nparts + 1, ps);
+        fill_u64(A.pcount, 0, nparts + 1, ps);
+        fill_u64(A.mcount, 0, (uint64_t)max_units + 1, ps);
         fill_u32(T.valid_all, 0, T.valid_words, ps);
         batch_decode(T.d_jobs, T.d_job_start, T.njobs, T.ndpages, cfg.grid, ps);
+        batch_part_count(A, false, nbp, cfg.grid, ps);
+        scan_exclusive_u64(A.bcount, const_cast<uint64_t *>(A.boff), nparts, ps, T.scan_tmp, T.scan_elems);
+        batch_copy_u64(A.boff, A.bcur, nparts + 1, ps);
+        batch_part_scatter(A, false, nbp, cfg.grid, ps);
+        batch_part_count(A, true, npp, cfg.grid, ps);
+        scan_exclusive_u64(A.pcount, const_cast<uint64_t *>(A.poff), nparts, ps, T.scan_tmp, T.scan_elems);
+        batch_copy_u64(A.poff, A.pcur, nparts + 1, ps);
+        batch_part_scatter(A, true, npp, cfg.grid, ps);
+        batch_build(A, cfg.grid, ps);
+        batch_units(A, ps);
+        scan_exclusive_u64(A.ucount, const_cast<uint64_t *>(A.uoff), nparts, ps, T.scan_tmp, T.scan_elems);
+        batch_join_count(A, cfg.grid, ps);
+        scan_exclusive_u64(A.mcount, const_cast<uint64_t *>(A.moff), max_units, ps, T.scan_tmp, T.scan_elems);
+        batch_totals(A, T.totals, ps);
         CJ_HIP(hipGetLastError());
     };
-    hipEvent_t e0, e1, e2;
+    hipEvent_t e0, e1;
     CJ_HIP(hipEventCreate(&e0));
     CJ_HIP(hipEventCreate(&e1));
-    CJ_HIP(hipEventCreate(&e2));
     struct EvGuard {
-        hipEvent_t a, b, c;
+        hipEvent_t a, b;
         ~EvGuard() {
             hipEventDestroy(a);
             hipEventDestroy(b);
-            hipEventDestroy(c);
         }
-    } eg{e0, e1, e2};
+    } eg{e0, e1};
     CJ_HIP(hipEventRecord(e0, s));
     if (fresh || !T.exec) {
         CJ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
             phase1(s);
         } catch (...) {
-            hipGraph_t g;
+            hipGraph_t g = nullptr;
             hipStreamEndCapture(s, &g);
             if (g) hipGraphDestroy(g);
             throw;
@@ -377,10 +457,9 @@ static double run_batch(Context &ctx, const cj_plan_desc *plans, const std::vect
     std::vector<uint64_t> totals(n);
     CJ_HIP(hipMemcpyAsync(totals.data(), T.totals, n * 8, hipMemcpyDeviceToHost, s));
     CJ_HIP(hipStreamSynchronize(s));
-    uint64_t pairs = 0;
-    for (uint64_t t : totals) pairs += t;
-    if (pairs > 0xffffffffull) throw UnsupportedError("batch result exceeds 2^32 rows");
-    // ---- phase 2: pairs, then the output pages of every plan and column
+    uint64_t out_rows = 0;
+    for (uint64_t t : totals) out_rows += t;
+    // ---- phase 2: dense output columns, then their pages
     std::vector<void *> tmp_blocks;
     struct TmpGuard {
         DevicePool &p;
@@ -392,50 +471,69 @@ static double run_batch(Context &ctx, const cj_plan_desc *plans, const std::vect
     auto tblk = [&](size_t bytes) {
         void *p = pool.alloc(bytes + 64);
         tmp_blocks.push_back(p);
-        return p;
+        return static_cast<uint8_t *>(p);
     };
-    T.args.out_b = static_cast<uint32_t *>(tblk((pairs + 1) * 4));
-    T.args.out_p = static_cast<uint32_t *>(tblk((pairs + 1) * 4));
-    std::vector<BMat> mats;
-    std::vector<uint32_t> mat_start(1, 0);
+    uint64_t dense_bytes = 0;
+    for (uint32_t k = 0; k < n; ++k)
+        for (const auto &oc : info[batched[k]].outs)
+            dense_bytes += ((totals[k] * (width_of(oc.type) + (oc.type == CJ_INT32 ? 0 : 1))) + 15) & ~15ull;
+    uint8_t *dense = tblk(std::max<uint64_t>(dense_bytes, 16));
+    std::vector<BOut> bouts(n);
+    std::vector<BPack> packs;
+    std::vector<uint32_t> pack_start(1, 0);
     std::vector<std::vector<std::pair<uint8_t *, uint32_t>>> out_blocks(n);
-    uint64_t pair_base = 0, out_bytes_pages = 0;
-    for (uint32_t k = 0; k < n; ++k) {
-        const PlanInfo &pi = info[batched[k]];
-        for (const auto &oc : pi.outs) {
-            const uint32_t R = oc.type == CJ_INT32 ? kRowsI32 : kRowsI64;
-            const uint32_t np = (uint32_t)((totals[k] + R - 1) / R);
-            uint8_t *blk = np ? static_cast<uint8_t *>(pool.alloc((size_t)np * kPage + 16)) : nullptr;
-            out_blocks[k].push_back({blk, np});
-            if (!np) continue;
-            const uint32_t j = dec_index.at(oc.col);
-            mats.push_back(BMat{(oc.from_build ? T.args.out_b : T.args.out_p) + pair_base, totals[k],
-                                T.dec_values[j], T.dec_valid[j], oc.type == CJ_INT32 ? 4 : 8, oc.type == CJ_INT32,
-                                blk});
-            mat_start.push_back(mat_start.back() + np);
-            out_bytes_pages += (uint64_t)np * kPage;
-        }
-        pair_base += totals[k];
-    }
+    uint64_t dpos = 0;
     try {
-        BMat *d_mats = static_cast<BMat *>(tblk(std::max<size_t>(mats.size(), 1) * sizeof(BMat)));
-        uint32_t *d_ms = static_cast<uint32_t *>(tblk(mat_start.size() * 4));
-        uint8_t *stage2 = static_cast<uint8_t *>(pinned.alloc(mats.size() * sizeof(BMat) + mat_start.size() * 4 + 64));
+        for (uint32_t k = 0; k < n; ++k) {
+            const PlanInfo &pi = info[batched[k]];
+            BOut &bo = bouts[k];
+            std::memset(&bo, 0, sizeof bo);
+            bo.nout = (uint32_t)pi.outs.size();
+            for (size_t c = 0; c < pi.outs.size(); ++c) {
+                const auto &oc = pi.outs[c];
+                const uint32_t w = width_of(oc.type);
+                bo.src[c] = oc.src, bo.j[c] = oc.j, bo.w[c] = (uint8_t)w;
+                bo.dense[c] = dense + dpos;
+                dpos += totals[k] * w;
+                if (w == 8) {
+                    bo.dvld[c] = dense + dpos;
+                    dpos += totals[k];
+                }
+                dpos = (dpos + 15) & ~15ull;
+                const uint32_t R = w == 4 ? kRowsI32 : kRowsI64;
+                const uint32_t np = (uint32_t)((totals[k] + R - 1) / R);
+                uint8_t *blk = np ? static_cast<uint8_t *>(pool.alloc((size_t)np * kPage + 16)) : nullptr;
+                out_blocks[k].push_back({blk, np});
+                if (!np) continue;
+                packs.push_back(BPack{bo.dense[c], bo.dvld[c], totals[k], (int32_t)w, blk});
+                pack_start.push_back(pack_start.back() + np);
+            }
+        }
+        BOut *d_outs = reinterpret_cast<BOut *>(tblk(n * sizeof(BOut)));
+        BPack *d_packs = reinterpret_cast<BPack *>(tblk(std::max<size_t>(packs.size(), 1) * sizeof(BPack)));
+        uint32_t *d_ps = reinterpret_cast<uint32_t *>(tblk(pack_start.size() * 4));
+        const size_t b2 = n * sizeof(BOut) + packs.size() * sizeof(BPack) + pack_start.size() * 4 + 64;
+        uint8_t *stage2 = static_cast<uint8_t *>(pinned.alloc(b2));
         PinGuard pg2{pinned, stage2};
-        std::memcpy(stage2, mats.data(), mats.size() * sizeof(BMat));
-        std::memcpy(stage2 + mats.size() * sizeof(BMat), mat_start.data(), mat_start.size() * 4);
-        if (!mats.empty())
-            CJ_HIP(hipMemcpyAsync(d_mats, stage2, mats.size() * sizeof(BMat), hipMemcpyHostToDevice, s));
-        CJ_HIP(hipMemcpyAsync(d_ms, stage2 + mats.size() * sizeof(BMat), mat_start.size() * 4,
-                              hipMemcpyHostToDevice, s));
-        if (pairs) batch_probe_write(T.args, npp, cfg.grid, s);
-        batch_materialize(d_mats, d_ms, (uint32_t)mats.size(), mat_start.back(), cfg.grid, s);
+        size_t o2 = 0;
+        auto up2 = [&](void *dst, const void *src, size_t bytes) {
+            if (!bytes) return;
+            std::memcpy(stage2 + o2, src, bytes);
+            CJ_HIP(hipMemcpyAsync(dst, stage2 + o2, bytes, hipMemcpyHostToDevice, s));
+            o2 += bytes;
+        };
+        up2(d_outs, bouts.data(), n * sizeof(BOut));
+        up2(d_packs, packs.data(), packs.size() * sizeof(BPack));
+        up2(d_ps, pack_start.data(), pack_start.size() * 4);
+        if (out_rows) batch_join_write(A, d_outs, cfg.grid, s);
+        batch_pack(d_packs, d_ps, (uint32_t)packs.size(), pack_start.back(), cfg.grid, s);
         CJ_HIP(hipEventRecord(e1, s));
         CJ_HIP(hipStreamSynchronize(s));
     } catch (...) {
         hipStreamSynchronize(s);
         for (auto &v : out_blocks)
-            for (auto &b : v) pool.free(b.first);
+            for (auto &b : v)
+                if (b.first) pool.free(b.first);
         throw;
     }
     float dev_ms = 0;
@@ -460,27 +558,27 @@ static double run_batch(Context &ctx, const cj_plan_desc *plans, const std::vect
         if (!(flags & CJ_EXEC_OUTPUT_DEVICE)) {
             ctx.to_host(&r);
             for (size_t c = 0; c < impl->cols.size(); ++c) {
-                pool.free(impl->dev_pages[c]);
+                if (impl->dev_pages[c]) pool.free(impl->dev_pages[c]);
                 impl->dev_pages[c] = nullptr;
                 impl->cols[c].device_pages = nullptr;
             }
         }
     }
     if (bs) {
-        bs->output_rows = pairs;
+        bs->output_rows = out_rows;
         bs->graph_replayed = fresh ? 0 : 1;
     }
     {
-        std::lock_guard<std::mutex> g(mu);
-        auto &slot = batch_templates[key];
+        std::lock_guard<std::mutex> g(ctx.mu);
+        auto &slot = ctx.batch_templates[key];
         if (slot) slot->release(pool);
         slot = tpl;
         // keep the workspace of a few recent shapes only
-        while (batch_templates.size() > 4) {
-            auto it = batch_templates.begin();
+        while (ctx.batch_templates.size() > 4) {
+            auto it = ctx.batch_templates.begin();
             if (it->first == key) ++it;
             it->second->release(pool);
-            batch_templates.erase(it);
+            ctx.batch_templates.erase(it);
         }
     }
     return dev_ms;

@@ -13,14 +13,32 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l ? (~0ull >> (64 - l)) : 0ull;
 }
 
+// Global address space.  A pointer read from device memory (a descriptor
+// array) is generic, and accesses through generic pointers compile to flat_*
+// instructions, which also count against the LDS wait counter: every LDS wait
+// would then wait for outstanding HBM traffic.  glb() gives such a pointer
+// the global address space, so its accesses are global_* instructions.
+#define CJ_G __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ CJ_G T *glb(T *p) {
+    return (CJ_G T *)p;
+}
+
 __device__ __forceinline__ uint16_t ld16(const uint8_t *p) { return *reinterpret_cast<const uint16_t *>(p); }
 __device__ __forceinline__ int32_t ld32(const uint8_t *p) { return *reinterpret_cast<const int32_t *>(p); }
 __device__ __forceinline__ uint64_t ld64(const uint8_t *p) { return *reinterpret_cast<const uint64_t *>(p); }
+__device__ __forceinline__ uint16_t ld16(const CJ_G uint8_t *p) { return *reinterpret_cast<const CJ_G uint16_t *>(p); }
+__device__ __forceinline__ int32_t ld32(const CJ_G uint8_t *p) { return *reinterpret_cast<const CJ_G int32_t *>(p); }
+__device__ __forceinline__ uint64_t ld64(const CJ_G uint8_t *p) { return *reinterpret_cast<const CJ_G uint64_t *>(p); }
 
 // Bit r of a page's NULL bitmap (LSB first, stored in the last bytes).
-__device__ __forceinline__ bool page_bit(const uint8_t *bm, uint32_t r) { return (bm[r >> 3] >> (r & 7)) & 1u; }
+template <class P>
+__device__ __forceinline__ bool page_bit(P bm, uint32_t r) {
+    return (bm[r >> 3] >> (r & 7)) & 1u;
+}
 
-__device__ __forceinline__ bool valid_bit(const uint32_t *bits, uint64_t row) {
+template <class P>
+__device__ __forceinline__ bool valid_bit(P bits, uint64_t row) {
     return (bits[row >> 5] >> (row & 31)) & 1u;
 }
 

@@ -46,8 +46,9 @@ __device__ __forceinline__ uint32_t rank_slots(uint32_t pred, uint32_t rank[S], 
 // Keys of this thread's 8 rows (t0 + 256 j + t) of a key page.  Bit j of
 // the returned mask: the row exists and its key is not NULL.  Loads are
 // unconditional with clamped in-page addresses (no branch around a load), so
-// all 8 are in flight together.
-__device__ __forceinline__ uint32_t load_keys8(const uint8_t *p, uint32_t n, uint32_t nv, bool dense, uint32_t t0,
+// all 8 are in flight together.  P: a generic or a global (glb) page pointer.
+template <class P>
+__device__ __forceinline__ uint32_t load_keys8(P p, uint32_t n, uint32_t nv, bool dense, uint32_t t0,
                                                uint32_t *running, int32_t key[kRPT], uint32_t *lds32) {
     const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
     uint32_t valid = 0;
@@ -59,7 +60,7 @@ __device__ __forceinline__ uint32_t load_keys8(const uint8_t *p, uint32_t n, uin
             if (vi[j] < n) valid |= 1u << j;
         }
     } else {
-        const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+        const P bm = p + kPage - ((n + 7) >> 3);
         uint32_t byte[kRPT];
 #pragma unroll
         for (uint32_t j = 0; j < kRPT; ++j) {

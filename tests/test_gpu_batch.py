@@ -105,3 +105,67 @@ def test_gpu_batch_autotune(engine, c4_small):
 def test_gpu_batch_empty(engine):
     res, st = engine.execute_batch([])
     assert res == [] and st.plans_batched == 0 and st.plans_single == 0
+
+
+@pytest.fixture(scope="module")
+def wide_plans(engine):
+    """Plans whose outputs carry 8-byte columns with NULLs on both sides, key
+    columns of either side, a column twice, a build partition too large for
+    LDS (two keys with 8000 rows each), and a plan with more output columns
+    than the batch writes (it runs one at a time)."""
+    b, p = workloads.c2_tables(seed=95, build_rows=60_000, probe_rows=500_000)
+    c2 = workloads.c2_plan(b, p)
+    # keys of both sides, 8-byte columns of both sides, a column twice, build_left=False
+    mix = cjtest.Plan()
+    mix.new_input(p)
+    mix.new_input(b)
+    mix.new_scan_node(0, [(3, "FP64"), (0, "INT32"), (2, "INT64")])
+    mix.new_scan_node(1, [(2, "INT64"), (0, "INT32"), (1, "INT32")])
+    mix.root = mix.new_join_node(False, 0, 1, 1, 1, [(4, "INT32"), (1, "INT32"), (0, "FP64"), (3, "INT64"),
+                                                      (5, "INT32"), (2, "INT64"), (0, "FP64")])
+    # duplicate-heavy build: key i % 2 over 16000 rows (partitions above the LDS capacity)
+    dup_b = workloads.generate(16_000, 77, [workloads.col(workloads.GEN_UNIFORM, "INT32", 1, 0.0, lo=0, hi=2),
+                                            workloads.col(workloads.GEN_I64_FULL, "INT64", 2, 0.1)])
+    dup_p = workloads.generate(20_000, 78, [workloads.col(workloads.GEN_UNIFORM, "INT32", 3, 0.1, lo=0, hi=500),
+                                        workloads.col(workloads.GEN_I32_FULL, "INT32", 4, 0.1)])
+    dup = cjtest.Plan()
+    dup.new_input(dup_b)
+    dup.new_input(dup_p)
+    dup.new_scan_node(0, [(0, "INT32"), (1, "INT64")])
+    dup.new_scan_node(1, [(0, "INT32"), (1, "INT32")])
+    dup.root = dup.new_join_node(True, 0, 1, 0, 0, [(1, "INT64"), (3, "INT32"), (2, "INT32")])
+    # nine output columns: more than the batch writes (runs one at a time)
+    nine = cjtest.Plan()
+    nine.new_input(b)
+    nine.new_input(p)
+    nine.new_scan_node(0, [(0, "INT32")])
+    nine.new_scan_node(1, [(0, "INT32"), (1, "INT32"), (2, "INT64"), (3, "FP64")])
+    nine.root = nine.new_join_node(True, 0, 1, 0, 0, [(1, "INT32"), (2, "INT32"), (3, "INT64"), (4, "FP64"),
+                                                      (2, "INT32"), (3, "INT64"), (4, "FP64"), (1, "INT32"),
+                                                      (0, "INT32")])
+    ids = [engine.register(t) for t in (b, p, dup_b, dup_p)]
+    plans = [c2, mix, dup, nine]
+    alone = []
+    for pl in plans:
+        r = engine.execute(pl)
+        alone.append((r.num_rows, cjtest.result_row_hashes(r)))
+        r.release()
+    yield plans, alone
+    for t in ids:
+        engine.unregister(t)
+
+
+def test_gpu_batch_wide_and_duplicate_plans(engine, wide_plans):
+    plans, alone = wide_plans
+    assert alone[2][0] > 100_000  # every probe key 0 or 1 meets about 8000 build rows
+    for k in (1, 2):
+        exp = cjtest.OracleRun(plans[k], threads=8)
+        assert alone[k][0] == exp.num_rows
+        assert np.array_equal(alone[k][1], cjtest.result_row_hashes(exp)), k
+    res, st = engine.execute_batch(plans)
+    assert st.plans_batched == 3 and st.plans_single == 1
+    check(res, alone, range(len(plans)))
+    res, st = engine.execute_batch(plans, abi.CJ_EXEC_OUTPUT_DEVICE)
+    for r in res:
+        r.to_host()
+    check(res, alone, range(len(plans)))
