@@ -275,13 +275,17 @@ __device__ __forceinline__ int32_t bp_key(const BuildPart &bp, uint32_t t) {
 }
 
 constexpr uint32_t kMatchBuf = kBatchMatchBuf;
+// A match in the LDS window: (probe row in the tile) << 22 | build tuple of the partition.
+constexpr uint32_t kMatchTBits = 22;
+constexpr uint32_t kMatchT = (1u << kMatchTBits) - 1;
+static_assert(kBatchMaxBuildRows <= (1ull << kMatchTBits), "build tuple index must fit the match entry");
 
 // Writes output rows base .. base + cnt - 1 from the expanded matches
-// (mrow: the probe tuple's offset in the tile, mt: the build tuple), one
-// column at a time so every store is coalesced.
+// (mbuf: probe row in the tile and build tuple), one column at a time so
+// every store is coalesced.
 template <bool LDS>
 __device__ __forceinline__ void bx_emit(const BPlan &pl, const BOut *bo, const BuildPart &bp, uint64_t base,
-                                        uint32_t cnt, const uint16_t *mrow, const uint32_t *mt, uint64_t pt0) {
+                                        uint32_t cnt, const uint32_t *mbuf, uint64_t pt0) {
     constexpr uint32_t Q = kMatchBuf / kWG;  // matches per thread (all loads, then all stores)
     const uint32_t nout = bo->nout;
     for (uint32_t c = 0; c < nout; ++c) {
@@ -298,12 +302,12 @@ __device__ __forceinline__ void bx_emit(const BPlan &pl, const BOut *bo, const B
                 const uint32_t i = q * kWG + threadIdx.x;
                 if (i >= cnt) continue;
                 if (src == 0) {
-                    v[q] = bp_key<LDS>(bp, mt[i]);
+                    v[q] = bp_key<LDS>(bp, mbuf[i] & kMatchT);
                 } else if (src == 1) {
-                    if constexpr (LDS) v[q] = lb[mt[i]];
-                    else v[q] = gb[mt[i]];
+                    if constexpr (LDS) v[q] = lb[mbuf[i] & kMatchT];
+                    else v[q] = gb[mbuf[i] & kMatchT];
                 } else {
-                    v[q] = gp[mrow[i]];
+                    v[q] = gp[mbuf[i] >> kMatchTBits];
                 }
             }
 #pragma unroll
@@ -328,11 +332,11 @@ __device__ __forceinline__ void bx_emit(const BPlan &pl, const BOut *bo, const B
                 if (i >= cnt) continue;
                 uint8_t b;
                 if (src == 1) {
-                    const uint32_t t = mt[i];
+                    const uint32_t t = mbuf[i] & kMatchT;
                     if constexpr (LDS) v[q] = lb[t], b = lbv[t];
                     else v[q] = gb[t], b = gbv[t];
                 } else {
-                    const uint32_t r = mrow[i];
+                    const uint32_t r = mbuf[i] >> kMatchTBits;
                     v[q] = gp[r], b = gpv[r];
                 }
                 vb |= (uint32_t)(b != 0) << q;
@@ -353,7 +357,8 @@ __device__ __forceinline__ void bx_emit(const BPlan &pl, const BOut *bo, const B
 template <bool WRITE, bool LDS, uint32_t RPT>
 __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &pl, const BOut *bo,
                                              const BuildPart &bp, uint64_t p0, uint64_t p1, uint64_t pbase,
-                                             uint64_t obase, uint32_t *lds32, uint16_t *mrow, uint32_t *mt) {
+                                             uint64_t obase, uint32_t *lds32, uint32_t *mbuf) {
+    static_assert(!WRITE || kWG * RPT <= (1u << (32 - kMatchTBits)), "tile row must fit the match entry");
     uint64_t mine = 0, base = obase;
     const uint32_t ls = pl.log_slots;
     for (uint64_t i0 = p0; i0 < p1; i0 += kWG * RPT) {
@@ -373,11 +378,21 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
             s[j] = in ? lo : 0u;
             e[j] = in ? hi : 0u;
         }
+        uint32_t f0[RPT], f1[RPT];  // first two matching build tuples (write pass)
 #pragma unroll
         for (uint32_t j = 0; j < RPT; ++j) {
-            uint32_t c = 0;
-            for (uint32_t t = s[j]; t < e[j]; ++t) c += bp_key<LDS>(bp, t) == key[j];
+            uint32_t c = 0, a0 = 0, a1 = 0;
+            for (uint32_t t = s[j]; t < e[j]; ++t) {
+                const bool hit = bp_key<LDS>(bp, t) == key[j];
+                if (WRITE && hit) {
+                    a1 = c == 1 ? t : a1;
+                    a0 = c == 0 ? t : a0;
+                }
+                c += hit;
+            }
             m[j] = c;
+            f0[j] = a0;
+            f1[j] = a1;
         }
         if constexpr (!WRITE) {
 #pragma unroll
@@ -407,18 +422,22 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
 #pragma unroll
                 for (uint32_t j = 0; j < RPT; ++j) {
                     if (!m[j] || pre[j] >= w1 || pre[j] + m[j] <= w0) continue;
+                    const uint32_t row = (kWG * j + threadIdx.x) << kMatchTBits;
+                    if (m[j] <= 2 && pre[j] >= w0 && pre[j] + m[j] <= w1) {
+                        // the usual case: the recorded matches, all inside the window
+                        mbuf[pre[j] - w0] = row | f0[j];
+                        if (m[j] == 2) mbuf[pre[j] + 1 - w0] = row | f1[j];
+                        continue;
+                    }
                     uint32_t q = pre[j];
                     for (uint32_t t = s[j]; t < e[j] && q < w1; ++t) {
                         if (bp_key<LDS>(bp, t) != key[j]) continue;
-                        if (q >= w0) {
-                            mrow[q - w0] = (uint16_t)(kWG * j + threadIdx.x);
-                            mt[q - w0] = t;
-                        }
+                        if (q >= w0) mbuf[q - w0] = row | t;
                         ++q;
                     }
                 }
                 __syncthreads();
-                bx_emit<LDS>(pl, bo, bp, base + w0, w1 - w0, mrow, mt, i0 - pbase);
+                bx_emit<LDS>(pl, bo, bp, base + w0, w1 - w0, mbuf, i0 - pbase);
                 __syncthreads();
             }
             base += run;
@@ -428,6 +447,24 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
 }
 
 }  // namespace
+
+// dst[0 .. n) = src[0 .. n), eight loads per thread in flight before the stores.
+template <class T, class S>
+__device__ __forceinline__ void stage_lds(T *dst, S src, uint32_t n) {
+    for (uint32_t i0 = 0; i0 < n; i0 += 8 * kWG) {
+        T v[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q) {
+            const uint32_t i = i0 + q * kWG + threadIdx.x;
+            if (i < n) v[q] = src[i];
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q) {
+            const uint32_t i = i0 + q * kWG + threadIdx.x;
+            if (i < n) dst[i] = v[q];
+        }
+    }
+}
 
 template <bool WRITE>
 __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__restrict__ outs, const BPlan *__restrict__ plans,
@@ -439,8 +476,7 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
     __shared__ uint16_t ends[kSlots];
     __shared__ uint32_t lds32[32];
     __shared__ uint64_t red[4];
-    __shared__ uint16_t mrow[WRITE ? kMatchBuf : 1];
-    __shared__ uint32_t mt[WRITE ? kMatchBuf : 1];
+    __shared__ uint32_t mbuf[WRITE ? kMatchBuf : 1];
     const uint64_t total = uoff[a.nparts];
     for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
         const uint32_t g = find_owner<uint64_t>(uoff, a.nparts, u);
@@ -467,27 +503,30 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
         if (WRITE) obase = moff[u] - moff[uoff[part_start[k]]];
         uint64_t mine;
         if (nb <= a.lds_cap) {
-            for (uint32_t i = threadIdx.x; i < kSlots; i += kWG) ends[i] = (uint16_t)bp.g_ends[i];
-            for (uint32_t i = threadIdx.x; i < nb; i += kWG) reinterpret_cast<int32_t *>(dyn)[i] = bp.g_keys[i];
+            // stage the partition: each batch of loads is issued before its LDS stores
+            {
+                uint32_t ev[kSlots / kWG];
+#pragma unroll
+                for (uint32_t q = 0; q < kSlots / kWG; ++q) ev[q] = bp.g_ends[q * kWG + threadIdx.x];
+#pragma unroll
+                for (uint32_t q = 0; q < kSlots / kWG; ++q) ends[q * kWG + threadIdx.x] = (uint16_t)ev[q];
+            }
+            stage_lds<int32_t>(reinterpret_cast<int32_t *>(dyn), bp.g_keys, nb);
             if (WRITE)
                 for (uint32_t c = 0; c < ncar; ++c) {
                     if (pl.b.car_w[c] == 4) {
-                        const CJ_G int32_t *src = glb(reinterpret_cast<const int32_t *>(pl.bslot_car[c])) + bp.g_car0;
-                        for (uint32_t i = threadIdx.x; i < nb; i += kWG)
-                            reinterpret_cast<int32_t *>(dyn + bp.coff(c))[i] = src[i];
+                        stage_lds<int32_t>(reinterpret_cast<int32_t *>(dyn + bp.coff(c)),
+                                           glb(reinterpret_cast<const int32_t *>(pl.bslot_car[c])) + bp.g_car0, nb);
                     } else {
-                        const CJ_G uint64_t *src = glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[c])) + bp.g_car0;
-                        const CJ_G uint8_t *vs = glb(pl.bslot_vld[c]) + bp.g_car0;
-                        for (uint32_t i = threadIdx.x; i < nb; i += kWG) {
-                            reinterpret_cast<uint64_t *>(dyn + bp.coff(c))[i] = src[i];
-                            dyn[bp.voff(c) + i] = vs[i];
-                        }
+                        stage_lds<uint64_t>(reinterpret_cast<uint64_t *>(dyn + bp.coff(c)),
+                                            glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[c])) + bp.g_car0, nb);
+                        stage_lds<uint8_t>(dyn + bp.voff(c), glb(pl.bslot_vld[c]) + bp.g_car0, nb);
                     }
                 }
             __syncthreads();
-            mine = bx_probe<WRITE, true, WRITE ? 4u : kRPT>(pkeys, pl, outs + k, bp, p0, p1, pbase, obase, lds32, mrow, mt);
+            mine = bx_probe<WRITE, true, WRITE ? 4u : kRPT>(pkeys, pl, outs + k, bp, p0, p1, pbase, obase, lds32, mbuf);
         } else {
-            mine = bx_probe<WRITE, false, WRITE ? 4u : kRPT>(pkeys, pl, outs + k, bp, p0, p1, pbase, obase, lds32, mrow, mt);
+            mine = bx_probe<WRITE, false, WRITE ? 4u : kRPT>(pkeys, pl, outs + k, bp, p0, p1, pbase, obase, lds32, mbuf);
         }
         if (!WRITE) {
             uint64_t tot;

@@ -254,8 +254,8 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
     const uint64_t max_units64 = nparts + sum_np / kBatchChunk + 1;
     if (max_units64 > 0xffffffffull) throw UnsupportedError("batch too large");
     const uint32_t max_units = (uint32_t)max_units64;
-    // LDS of a join workgroup: slot ends (u16), the match window (u16 + u32), then the partition
-    const uint32_t dyn_budget = kBatchLdsBytes - (2u << kBatchLogPart) - 6 * kBatchMatchBuf - 512;
+    // LDS of a join workgroup: slot ends (u16), the match window (u32), then the partition
+    const uint32_t dyn_budget = kBatchLdsBytes - (2u << kBatchLogPart) - 4 * kBatchMatchBuf - 512;
     const uint32_t lds_cap = std::min<uint32_t>((dyn_budget / max_tuple_bytes) & ~15u, 65535u & ~15u);
     std::vector<BDecode> jobs(dec_cols.size());
     std::vector<uint32_t> job_start(dec_cols.size() + 1, 0);

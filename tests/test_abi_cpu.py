@@ -129,3 +129,27 @@ def test_serve_calls_fail_cleanly_without_context():
     t = C.c_uint64()
     assert lib.cj_serve_stop(None) == abi.CJ_ERR_INVALID
     assert lib.cj_submit(None, None, 0, C.byref(t)) == abi.CJ_ERR_INVALID
+
+
+def test_batch_call_fails_cleanly_without_context():
+    lib = abi.load_library()
+    st = abi.BatchStats()
+    assert lib.cj_execute_batch(None, None, 0, 0, None, C.byref(st)) == abi.CJ_ERR_INVALID
+    assert "null" in lib.cj_last_error().decode()
+
+
+def test_c4_generator_follows_the_plan_seeds():
+    """Plan k of the batch workload uses seed 2000 + k: sizes inside the
+    log-uniform ranges (times the scale), keys inside [0, build rows)."""
+    from calamarijoins_amd import workloads
+    import cjtest
+    for k in (0, 1, 7):
+        b, p = workloads.c4_plan_tables(k, scale=0.01)
+        b2, _ = workloads.c4_plan_tables(k, scale=0.01)
+        assert b.num_rows == b2.num_rows
+        assert 8 <= b.num_rows <= 20_000 and 100 <= p.num_rows <= 200_000
+        kb, vb = cjtest.decode_column_arrays(abi.CJ_INT32, b.columns[0].page_ptrs(), b.columns[0].num_pages)
+        kp, vp = cjtest.decode_column_arrays(abi.CJ_INT32, p.columns[0].page_ptrs(), p.columns[0].num_pages)
+        assert len(kb) == b.num_rows and len(kp) == p.num_rows
+        assert kb[vb].min() >= 0 and kb[vb].max() < b.num_rows
+        assert kp[vp].min() >= 0 and kp[vp].max() < b.num_rows
