@@ -137,6 +137,7 @@ EXPORTS = [
     "cj_execute", "cj_result_release", "cj_result_to_host",
     "cj_kernel_stats", "cj_kernel_stats_reset",
     "cj_serve_start", "cj_submit", "cj_wait", "cj_serve_stop", "cj_plan_footprint", "cj_execute_batch",
+    "cj_tbl_write", "cj_tbl_open", "cj_tbl_load", "cj_tbl_close", "cj_tbl_cache_name",
 ]
 
 _lib = None
@@ -171,6 +172,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "cj_serve_stop": (C.c_int, [C.c_void_p]),
         "cj_plan_footprint": (C.c_int, [P(PlanDesc), P(C.c_uint64)]),
         "cj_execute_batch": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, C.c_uint32, P(Result), P(BatchStats)]),
+        "cj_tbl_write": (C.c_int, [C.c_char_p, P(TableDesc)]),
+        "cj_tbl_open": (C.c_int, [C.c_char_p, P(C.c_void_p), P(TableDesc)]),
+        "cj_tbl_load": (C.c_int, [C.c_void_p, C.c_char_p, P(C.c_void_p), P(TableDesc), P(C.c_uint64)]),
+        "cj_tbl_close": (C.c_int, [C.c_void_p]),
+        "cj_tbl_cache_name": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

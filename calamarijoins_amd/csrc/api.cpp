@@ -3,6 +3,7 @@
 // negative status plus a per-thread message for cj_last_error().
 #include "engine.hpp"
 #include "serve.hpp"
+#include "tbl.hpp"
 
 #include <cstring>
 #include <map>
@@ -10,6 +11,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 
 namespace {
 thread_local std::string g_err;
@@ -71,6 +73,10 @@ void stop_server(cj::Context *c) {
     s->drain();
 }
 }  // namespace
+
+struct cj_tbl {
+    std::shared_ptr<cj::TblFile> f;
+};
 
 extern "C" {
 
@@ -207,6 +213,62 @@ int cj_execute_batch(cj_ctx *ctx, const cj_plan_desc *plans, uint32_t n, uint32_
         if (!ctx || (n && (!plans || !out))) throw cj::InvalidError("null argument");
         CJ_HIP(hipSetDevice(C(ctx)->device));
         C(ctx)->execute_batch(plans, n, flags, out, stats);
+    });
+}
+
+int cj_tbl_write(const char *path, const cj_table_desc *table) {
+    return guard([&] {
+        if (!path || !table) throw cj::InvalidError("null argument");
+        cj::tbl_write(path, *table);
+    });
+}
+
+int cj_tbl_open(const char *path, cj_tbl **out, cj_table_desc *desc) {
+    if (out) *out = nullptr;
+    return guard([&] {
+        if (!path || !out || !desc) throw cj::InvalidError("null argument");
+        auto h = std::make_unique<cj_tbl>();
+        h->f = cj::TblFile::open(path, /*populate*/ true);
+        *desc = h->f->desc();
+        *out = h.release();
+    });
+}
+
+int cj_tbl_load(cj_ctx *ctx, const char *path, cj_tbl **out, cj_table_desc *desc, uint64_t *table_id) {
+    if (out) *out = nullptr;
+    return guard([&] {
+        if (!ctx || !path || !out || !desc || !table_id) throw cj::InvalidError("null argument");
+        cj::Context *c = C(ctx);
+        CJ_HIP(hipSetDevice(c->device));
+        auto h = std::make_unique<cj_tbl>();
+        // the mapping gives the host page addresses plans use; the pages
+        // themselves go from the file to HBM through pinned buffers
+        h->f = cj::TblFile::open(path, /*populate*/ false);
+        const int threads = c->opts.host_threads > 0
+                                ? c->opts.host_threads
+                                : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+        std::shared_ptr<cj::TblFile> f = h->f;
+        *table_id = c->register_table(
+            f->desc(),
+            [f, threads](uint32_t col, uint32_t p0, uint32_t np, uint8_t *dst) {
+                cj::tbl_read_pages(*f, col, p0, np, dst, threads);
+            },
+            f);
+        *desc = f->desc();
+        *out = h.release();
+    });
+}
+
+int cj_tbl_close(cj_tbl *t) {
+    return guard([&] { delete t; });
+}
+
+int cj_tbl_cache_name(const char *table, const char *filter_text, char *out, size_t cap) {
+    return guard([&] {
+        if (!table || !out || !cap) throw cj::InvalidError("null argument");
+        const std::string n = cj::tbl_cache_name(table, filter_text);
+        if (n.size() + 1 > cap) throw cj::InvalidError("name buffer too small");
+        std::memcpy(out, n.c_str(), n.size() + 1);
     });
 }
 

@@ -254,33 +254,20 @@ static void validate_page(const uint8_t *p, int32_t type, uint32_t idx, uint16_t
 
 // Upload a column's pages (gathered through pinned staging, 32 MiB chunks,
 // double-buffered) and its row index.
-void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s) {
+void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s,
+                            const std::function<void(uint32_t p0, uint32_t np, uint8_t *dst)> &fill) {
     c.type = d.type;
     c.num_pages = d.num_pages;
+    if (!fill)
+        for (uint32_t i = 0; i < d.num_pages; ++i)
+            if (!d.pages[i]) throw InvalidError("null page pointer");
     // host index: row_start (first row of each page) and byte_start (first
-    // bitmap byte of each page in the page index), both length num_pages + 1
+    // bitmap byte of each page in the page index), both length num_pages + 1,
+    // built from the page headers as the pages pass through the staging buffer
     std::vector<uint64_t> rs(2 * (size_t)d.num_pages + 2);
     uint64_t *bs = rs.data() + d.num_pages + 1;
     uint64_t run = 0, bytes = 0;
     uint16_t prev = 0;
-    for (uint32_t i = 0; i < d.num_pages; ++i) {
-        if (!d.pages[i]) throw InvalidError("null page pointer");
-        const uint8_t *p = static_cast<const uint8_t *>(d.pages[i]);
-        validate_page(p, d.type, i, prev);
-        std::memcpy(&prev, p, 2);
-        rs[i] = run;
-        bs[i] = bytes;
-        run += host_rows_of(p);
-        if (prev < 0xfffe) {
-            bytes += (prev + 7u) / 8u;
-            c.max_page_rows = std::max<uint32_t>(c.max_page_rows, prev);
-        }
-    }
-    if (run > 0xffffffffull) throw UnsupportedError("column has more than 2^32 rows");
-    rs[d.num_pages] = run;
-    bs[d.num_pages] = bytes;
-    c.num_rows = run;
-    c.bitmap_bytes = bytes;
     c.pages = static_cast<uint8_t *>(pool.alloc((size_t)d.num_pages * kPage + 16));
     c.row_start = dalloc<uint64_t>(2 * (size_t)d.num_pages + 2);
     c.byte_start = nullptr;  // lives inside the row_start block
@@ -293,37 +280,67 @@ void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s
     CJ_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     CJ_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     bool pending[2] = {false, false};
+    struct Cleanup {
+        Context &ctx;
+        hipStream_t s;
+        hipEvent_t *ev;
+        uint8_t **stage;
+        ~Cleanup() {
+            hipStreamSynchronize(s);
+            hipEventDestroy(ev[0]);
+            hipEventDestroy(ev[1]);
+            ctx.pinned.free(stage[0]);
+            ctx.pinned.free(stage[1]);
+        }
+    } cleanup{*this, s, ev, stage};
     int nthreads = opts.host_threads > 0 ? opts.host_threads : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     for (uint32_t p0 = 0, k = 0; p0 < d.num_pages; p0 += chunk, k ^= 1) {
         uint32_t np = std::min(chunk, d.num_pages - p0);
         if (pending[k]) CJ_HIP(hipEventSynchronize(ev[k]));
         uint8_t *dst = stage[k];
-        auto copy_range = [&](uint32_t a, uint32_t b) {
-            for (uint32_t i = a; i < b; ++i) std::memcpy(dst + (size_t)i * kPage, d.pages[p0 + i], kPage);
-        };
-        if (np >= 512 && nthreads > 1) {
-            std::vector<std::thread> th;
-            uint32_t per = (np + nthreads - 1) / nthreads;
-            for (int t = 0; t < nthreads; ++t) {
-                uint32_t a = t * per, b = std::min(np, a + per);
-                if (a < b) th.emplace_back(copy_range, a, b);
-            }
-            for (auto &x : th) x.join();
+        if (fill) {
+            fill(p0, np, dst);
         } else {
-            copy_range(0, np);
+            auto copy_range = [&](uint32_t a, uint32_t b) {
+                for (uint32_t i = a; i < b; ++i) std::memcpy(dst + (size_t)i * kPage, d.pages[p0 + i], kPage);
+            };
+            if (np >= 512 && nthreads > 1) {
+                std::vector<std::thread> th;
+                uint32_t per = (np + nthreads - 1) / nthreads;
+                for (int t = 0; t < nthreads; ++t) {
+                    uint32_t a = t * per, b = std::min(np, a + per);
+                    if (a < b) th.emplace_back(copy_range, a, b);
+                }
+                for (auto &x : th) x.join();
+            } else {
+                copy_range(0, np);
+            }
+        }
+        for (uint32_t i = 0; i < np; ++i) {
+            const uint8_t *p = dst + (size_t)i * kPage;
+            validate_page(p, d.type, p0 + i, prev);
+            std::memcpy(&prev, p, 2);
+            rs[p0 + i] = run;
+            bs[p0 + i] = bytes;
+            run += host_rows_of(p);
+            if (prev < 0xfffe) {
+                bytes += (prev + 7u) / 8u;
+                c.max_page_rows = std::max<uint32_t>(c.max_page_rows, prev);
+            }
         }
         CJ_HIP(hipMemcpyAsync(c.pages + (size_t)p0 * kPage, dst, (size_t)np * kPage, hipMemcpyHostToDevice, s));
         CJ_HIP(hipEventRecord(ev[k], s));
         pending[k] = true;
     }
+    if (run > 0xffffffffull) throw UnsupportedError("column has more than 2^32 rows");
+    rs[d.num_pages] = run;
+    bs[d.num_pages] = bytes;
+    c.num_rows = run;
+    c.bitmap_bytes = bytes;
     CJ_HIP(hipStreamSynchronize(s));
     std::memcpy(stage[0], rs.data(), rs.size() * 8);
     CJ_HIP(hipMemcpyAsync(c.row_start, stage[0], rs.size() * 8, hipMemcpyHostToDevice, s));
     CJ_HIP(hipStreamSynchronize(s));
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
-    pinned.free(stage[0]);
-    pinned.free(stage[1]);
     c.byte_start = c.row_start + d.num_pages + 1;
     c.loaded = true;
 }
@@ -340,17 +357,22 @@ void Context::ensure_decoded(DevColumn &c, hipStream_t s) {
     else decode_fixed(c.pages, c.num_pages, c.row_start, (int)w, c.values, c.valid, s);
 }
 
-uint64_t Context::register_table(const cj_table_desc &t) {
+uint64_t Context::register_table(const cj_table_desc &t, const PageFill &fill, std::shared_ptr<void> keepalive) {
     hipStream_t s = acquire_stream();
     auto tab = std::make_unique<DevTable>();
+    tab->keepalive = std::move(keepalive);
     try {
         tab->resident = true;
         tab->num_rows = t.num_rows;
         for (uint32_t i = 0; i < t.num_columns; ++i) {
             auto c = std::make_unique<DevColumn>();
-            upload_column(*c, t.columns[i], s);
-            tab->host_pages.emplace_back(t.columns[i].pages, t.columns[i].pages + t.columns[i].num_pages);
             tab->cols.push_back(std::move(c));
+            if (fill)
+                upload_column(*tab->cols.back(), t.columns[i], s,
+                              [&](uint32_t p0, uint32_t np, uint8_t *dst) { fill(i, p0, np, dst); });
+            else
+                upload_column(*tab->cols.back(), t.columns[i], s);
+            tab->host_pages.emplace_back(t.columns[i].pages, t.columns[i].pages + t.columns[i].num_pages);
         }
     } catch (...) {
         for (auto &c : tab->cols) free_column(*c);
@@ -472,10 +494,9 @@ class Exec {
         DevColumn *dc = ctx.find_resident(d);
         if (!dc) {
             auto t0 = std::chrono::steady_clock::now();
-            auto own = std::make_unique<DevColumn>();
-            ctx.upload_column(*own, d, st);
-            dc = own.get();
-            own_cols.push_back(std::move(own));
+            own_cols.push_back(std::make_unique<DevColumn>());  // owned first: freed if the upload throws
+            dc = own_cols.back().get();
+            ctx.upload_column(*dc, d, st);
             h2d_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
         // every column of a table must describe the same rows: row ids of one

@@ -7,6 +7,7 @@
 
 #include <atomic>
 #include <condition_variable>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -74,6 +75,7 @@ struct DevTable {
     uint64_t num_rows = 0;
     std::vector<std::unique_ptr<DevColumn>> cols;
     std::vector<std::vector<const void *>> host_pages;  // identity of each column
+    std::shared_ptr<void> keepalive;                     // e.g. the mapping of a .tbl file
 };
 
 // Per-kernel hipEvent timing (enabled by cj_ctx_options::kernel_timing).
@@ -120,7 +122,10 @@ class Context {
     int num_cus = 0;
     size_t max_lds = 65536;  // LDS bytes one workgroup may use
 
-    uint64_t register_table(const cj_table_desc &t);
+    // Fills dst with pages [p0, p0 + np) of column `col`; an empty function
+    // copies them from the host page pointers of the descriptor.
+    using PageFill = std::function<void(uint32_t col, uint32_t p0, uint32_t np, uint8_t *dst)>;
+    uint64_t register_table(const cj_table_desc &t, const PageFill &fill = {}, std::shared_ptr<void> keepalive = {});
     void unregister_table(uint64_t id);
 
     void execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing);
@@ -148,7 +153,8 @@ class Context {
     }
     void ensure_index(DevColumn &c, hipStream_t s);
     void ensure_decoded(DevColumn &c, hipStream_t s);
-    void upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s);
+    void upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s,
+                       const std::function<void(uint32_t p0, uint32_t np, uint8_t *dst)> &fill = {});
     void free_column(DevColumn &c);
 
     std::mutex mu;  // registry + stats

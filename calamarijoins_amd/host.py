@@ -57,6 +57,57 @@ class Table:
         return sum(self.columns[i].num_pages * PAGE for i in col_indices)
 
 
+class TblTable(Table):
+    """A .tbl cache file mapped read-only (cj_tbl_open, like the harness's
+    Table::from_cache) or also streamed into HBM as a resident table
+    (cj_tbl_load).  Column pages are numpy views of the mapping; close()
+    releases it (unregister a loaded table first)."""
+
+    def __init__(self, handle: C.c_void_p, desc: abi.TableDesc, table_id=None, engine=None):
+        cols = []
+        for c in range(desc.num_columns):
+            cd = desc.columns[c]
+            if cd.num_pages:
+                buf = (C.c_uint8 * (cd.num_pages * PAGE)).from_address(cd.pages[0])
+                arr = np.ctypeslib.as_array(buf)
+            else:
+                arr = np.zeros(0, dtype=np.uint8)
+            cols.append(Column(cd.type, arr, owner=self))
+        super().__init__(desc.num_rows, cols)
+        self.handle = handle
+        self.table_id = table_id
+        self.engine = engine
+
+    def close(self):
+        if self.handle:
+            if self.table_id is not None and self.engine is not None and self.engine.ctx:
+                self.engine.unregister(self.table_id)
+                self.table_id = None
+            abi.check(abi.load_library().cj_tbl_close(self.handle))
+            self.handle = None
+
+
+def write_tbl(path: str, table: Table) -> None:
+    """DumpTable::dump equivalent (cj_tbl_write)."""
+    abi.check(abi.load_library().cj_tbl_write(str(path).encode(), C.byref(table.desc())))
+
+
+def read_tbl(path: str) -> TblTable:
+    """Table::from_cache equivalent (cj_tbl_open): pages stay in the file mapping."""
+    h = C.c_void_p()
+    d = abi.TableDesc()
+    abi.check(abi.load_library().cj_tbl_open(str(path).encode(), C.byref(h), C.byref(d)))
+    return TblTable(h, d)
+
+
+def tbl_cache_name(table: str, filter_text=None) -> str:
+    """The harness's cache file name of a scan: <table>_<FNV-1a 64 of the filter text>.tbl."""
+    out = C.create_string_buffer(512)
+    abi.check(abi.load_library().cj_tbl_cache_name(table.encode(), filter_text.encode() if filter_text else None,
+                                                    out, 512))
+    return out.value.decode()
+
+
 class Plan:
     def __init__(self):
         self.nodes = []
@@ -132,6 +183,14 @@ class Engine:
 
     def unregister(self, tid: int):
         abi.check(self.lib.cj_table_unregister(self.ctx, tid))
+
+    def load_tbl(self, path: str) -> TblTable:
+        """Stream a .tbl file into HBM as a resident table (cj_tbl_load)."""
+        h = C.c_void_p()
+        d = abi.TableDesc()
+        tid = C.c_uint64()
+        abi.check(self.lib.cj_tbl_load(self.ctx, str(path).encode(), C.byref(h), C.byref(d), C.byref(tid)))
+        return TblTable(h, d, tid.value, self)
 
     def execute(self, plan: Plan, flags: int = 0, timing: bool = False):
         res = abi.Result()

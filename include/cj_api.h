@@ -226,6 +226,30 @@ typedef struct cj_batch_stats {
 int cj_execute_batch(cj_ctx *ctx, const cj_plan_desc *plans, uint32_t n, uint32_t flags, cj_result *out,
                      cj_batch_stats *stats);
 
+/* ---- .tbl table cache files: replace the harness's DumpTable::dump and
+ *      Table::from_cache (reference include/data_access/table.h:30-37,
+ *      208-221; src/build_table.cpp:140-197).  Layout: one 8192-byte header
+ *      page {u64 num_rows; u64 num_cols; i32 types[16]; u64 num_pages[16];
+ *      zero filler}, then the 8 KiB pages of each column, column after column.
+ *      At most 16 columns. ------------------------------------------------- */
+typedef struct cj_tbl cj_tbl;
+/* Write `table` to `path` (through a temporary file renamed over it). */
+int cj_tbl_write(const char *path, const cj_table_desc *table);
+/* Map `path` read-only: desc's page pointers point into the mapping (valid
+ * until cj_tbl_close), as Table::from_cache does.  Size, column count and
+ * types are checked against the header. */
+int cj_tbl_open(const char *path, cj_tbl **out, cj_table_desc *desc);
+/* Open `path` and stream its pages from the file into HBM as a resident
+ * table (cj_table_register without a host copy).  Plans use desc (pointers
+ * into the mapping); the mapping lives until both cj_tbl_close and
+ * cj_table_unregister(table_id) happened. */
+int cj_tbl_load(cj_ctx *ctx, const char *path, cj_tbl **out, cj_table_desc *desc, uint64_t *table_id);
+int cj_tbl_close(cj_tbl *t);
+/* The harness's cache file name of a scan: "<table>_<FNV-1a 64 of the
+ * filter text, decimal>.tbl", filter_text NULL = "No filter"
+ * (reference tests/read_sql.cpp:1154-1158, include/data_access/table.h:15-27). */
+int cj_tbl_cache_name(const char *table, const char *filter_text, char *out, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif
