@@ -85,6 +85,16 @@ class ServeOptions(C.Structure):
 
 CJ_BATCH_AUTOTUNE = 16
 
+CJ_F_CMP, CJ_F_AND, CJ_F_OR, CJ_F_NOT = range(4)
+(CJ_OP_EQ, CJ_OP_NEQ, CJ_OP_LT, CJ_OP_GT, CJ_OP_LEQ, CJ_OP_GEQ, CJ_OP_LIKE, CJ_OP_NOT_LIKE, CJ_OP_IS_NULL,
+ CJ_OP_IS_NOT_NULL) = range(10)
+CJ_LIT_NONE, CJ_LIT_INT, CJ_LIT_FLOAT, CJ_LIT_STR = range(4)
+
+
+class FilterNode(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("op", C.c_int32), ("column", C.c_uint32), ("lit_kind", C.c_int32),
+                ("ival", C.c_int64), ("fval", C.c_double), ("sval", C.c_char_p), ("slen", C.c_uint32)]
+
 
 class BatchStats(C.Structure):
     _fields_ = [
@@ -137,7 +147,7 @@ EXPORTS = [
     "cj_execute", "cj_result_release", "cj_result_to_host",
     "cj_kernel_stats", "cj_kernel_stats_reset",
     "cj_serve_start", "cj_submit", "cj_wait", "cj_serve_stop", "cj_plan_footprint", "cj_execute_batch",
-    "cj_tbl_write", "cj_tbl_open", "cj_tbl_load", "cj_tbl_close", "cj_tbl_cache_name",
+    "cj_tbl_write", "cj_tbl_open", "cj_tbl_load", "cj_tbl_close", "cj_tbl_cache_name", "cj_filter",
 ]
 
 _lib = None
@@ -173,6 +183,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "cj_plan_footprint": (C.c_int, [P(PlanDesc), P(C.c_uint64)]),
         "cj_execute_batch": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, C.c_uint32, P(Result), P(BatchStats)]),
         "cj_tbl_write": (C.c_int, [C.c_char_p, P(TableDesc)]),
+        "cj_filter": (C.c_int, [C.c_void_p, P(TableDesc), P(FilterNode), C.c_uint32, C.c_uint32, P(Result),
+                                P(Timing)]),
         "cj_tbl_open": (C.c_int, [C.c_char_p, P(C.c_void_p), P(TableDesc)]),
         "cj_tbl_load": (C.c_int, [C.c_void_p, C.c_char_p, P(C.c_void_p), P(TableDesc), P(C.c_uint64)]),
         "cj_tbl_close": (C.c_int, [C.c_void_p]),

@@ -216,6 +216,15 @@ int cj_execute_batch(cj_ctx *ctx, const cj_plan_desc *plans, uint32_t n, uint32_
     });
 }
 
+int cj_filter(cj_ctx *ctx, const cj_table_desc *table, const cj_filter_node *program, uint32_t num_nodes,
+              uint32_t flags, cj_result *out, cj_timing *timing) {
+    return guard([&] {
+        if (!ctx || !table || !program || !out) throw cj::InvalidError("null argument");
+        CJ_HIP(hipSetDevice(C(ctx)->device));
+        C(ctx)->filter(*table, program, num_nodes, flags, out, timing);
+    });
+}
+
 int cj_tbl_write(const char *path, const cj_table_desc *table) {
     return guard([&] {
         if (!path || !table) throw cj::InvalidError("null argument");

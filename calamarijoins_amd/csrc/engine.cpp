@@ -18,6 +18,7 @@
 #include "engine.hpp"
 #include "fused.hpp"
 #include "pagemat.hpp"
+#include "filter.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -682,6 +683,13 @@ class Exec {
     void materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
                           const std::vector<uint32_t> &remap, uint64_t total, const uint32_t *ob, const uint32_t *op,
                           cj_result *out);
+    // pages of base column c gathered by row ids (rows of them); int32_quirk:
+    // INT32_MIN values are written as NULL (result page builders only)
+    std::pair<uint8_t *, uint32_t> materialize_column(DevColumn *c, int32_t type, const uint32_t *rids, uint64_t rows,
+                                                      bool int32_quirk);
+    // filter of input table 0: the rows passing the program, every column
+    void filter_root(const std::vector<FNode> &prog, const std::vector<uint32_t> &used,
+                     const std::vector<uint8_t> &lits);
 
     // output placement
     bool device_output = false;
@@ -1140,32 +1148,93 @@ void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, u
             final_rids[key] = rids;
         }
         DevColumn *c = column(S.slots[cr.slot].table, cr.col);
-        ColView v = view(c);
-        uint8_t *pages;
-        uint32_t np;
-        if (type == CJ_VARCHAR) {
-            uint64_t chunks = (total + kVarcharChunk - 1) / kVarcharChunk;
-            uint64_t *cp = tmp<uint64_t>(chunks + 1);
-            uint64_t *co = tmp<uint64_t>(chunks + 1);
-            VarcharSource vs{c->pages, static_cast<const uint64_t *>(v.values), v.valid, rids};
-            {
-                TimedScope ts(timer, "varchar_plan", 12 * total);
-                varchar_plan(vs, total, cp, st);
-            }
-            scan(cp, co, chunks);
-            uint64_t npg = read_u64(co + chunks);
-            np = (uint32_t)npg;
-            pages = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
-            TimedScope ts(timer, "varchar_write", (uint64_t)np * kPage);
-            varchar_write(vs, total, co, pages, st);
-        } else {
-            uint32_t R = type == CJ_INT32 ? kRowsI32 : kRowsI64;
-            np = (uint32_t)((total + R - 1) / R);
-            pages = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
-            FixedSource fs{type == CJ_INT32 ? 4 : 8, type == CJ_INT32, v.values, v.valid, rids};
-            TimedScope ts(timer, type == CJ_INT32 ? "materialize_i32" : "materialize_i64", fixed_mat_bytes(total, type));
-            materialize_fixed(fs, total, R, pages, st);
+        auto [pages, np] = materialize_column(c, type, rids, total, true);
+        out_cols.push_back({pages, {np, type}});
+    }
+}
+
+std::pair<uint8_t *, uint32_t> Exec::materialize_column(DevColumn *c, int32_t type, const uint32_t *rids,
+                                                        uint64_t total, bool int32_quirk) {
+    ColView v = view(c);
+    uint8_t *pages;
+    uint32_t np;
+    if (type == CJ_VARCHAR) {
+        uint64_t chunks = (total + kVarcharChunk - 1) / kVarcharChunk;
+        uint64_t *cp = tmp<uint64_t>(chunks + 1);
+        uint64_t *co = tmp<uint64_t>(chunks + 1);
+        VarcharSource vs{c->pages, static_cast<const uint64_t *>(v.values), v.valid, rids};
+        {
+            TimedScope ts(timer, "varchar_plan", 12 * total);
+            varchar_plan(vs, total, cp, st);
         }
+        scan(cp, co, chunks);
+        uint64_t npg = read_u64(co + chunks);
+        np = (uint32_t)npg;
+        pages = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
+        TimedScope ts(timer, "varchar_write", (uint64_t)np * kPage);
+        varchar_write(vs, total, co, pages, st);
+    } else {
+        uint32_t R = type == CJ_INT32 ? kRowsI32 : kRowsI64;
+        np = (uint32_t)((total + R - 1) / R);
+        pages = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
+        FixedSource fs{type == CJ_INT32 ? 4 : 8, type == CJ_INT32 && int32_quirk, v.values, v.valid, rids};
+        TimedScope ts(timer, type == CJ_INT32 ? "materialize_i32" : "materialize_i64", fixed_mat_bytes(total, type));
+        materialize_fixed(fs, total, R, pages, st);
+    }
+    return {pages, np};
+}
+
+void Exec::filter_root(const std::vector<FNode> &prog, const std::vector<uint32_t> &used,
+                       const std::vector<uint8_t> &lits) {
+    const cj_table_desc &t = plan.inputs[0];
+    uint64_t rows = 0;
+    if (t.num_columns) rows = column(0, 0)->num_rows;
+    for (uint32_t c = 1; c < t.num_columns; ++c) column(0, c);  // same row count checked there
+    // the referenced columns, decoded
+    std::vector<FCol> fc(used.size());
+    for (size_t i = 0; i < used.size(); ++i) {
+        DevColumn *c = column(0, used[i]);
+        ColView v = view(c);
+        fc[i] = FCol{c->type, v.values, v.valid, c->pages,
+                     c->type == CJ_VARCHAR ? static_cast<const uint64_t *>(v.values) : nullptr};
+    }
+    FNode *d_prog = tmp<FNode>(prog.size());
+    FCol *d_cols = tmp<FCol>(std::max<size_t>(fc.size(), 1));
+    uint8_t *d_lits = tmp<uint8_t>(std::max<size_t>(lits.size(), 8));
+    CJ_HIP(hipMemcpyAsync(d_prog, prog.data(), prog.size() * sizeof(FNode), hipMemcpyHostToDevice, st));
+    if (!fc.empty()) CJ_HIP(hipMemcpyAsync(d_cols, fc.data(), fc.size() * sizeof(FCol), hipMemcpyHostToDevice, st));
+    if (!lits.empty()) CJ_HIP(hipMemcpyAsync(d_lits, lits.data(), lits.size(), hipMemcpyHostToDevice, st));
+    uint64_t total = 0;
+    uint32_t *rids = nullptr;
+    if (rows) {
+        uint32_t *sel = tmp<uint32_t>(filter_sel_words(rows));
+        {
+            uint64_t bytes = 0;
+            for (uint32_t u : used) bytes += (uint64_t)column(0, u)->num_pages * kPage;
+            TimedScope ts(timer, "filter_eval", bytes);
+            filter_eval(d_prog, (uint32_t)prog.size(), d_cols, d_lits, rows, sel, st);
+        }
+        const uint64_t chunks = (rows + kSelChunk - 1) / kSelChunk;
+        uint64_t *cnt = tmp<uint64_t>(chunks + 1);
+        uint64_t *off = tmp<uint64_t>(chunks + 1);
+        sel_count(sel, rows, cnt, st);
+        scan(cnt, off, chunks);
+        total = read_u64(off + chunks);
+        rids = tmp<uint32_t>(total + 1);
+        TimedScope ts(timer, "filter_rids", rows / 8 + 4 * total);
+        sel_rids(sel, rows, off, rids, st);
+    }
+    out_rows = total;
+    // a filtered base table keeps its values: no INT32_MIN quirk
+    // (from_inner_to_column, build_table.cpp:322-360)
+    for (uint32_t c = 0; c < t.num_columns; ++c) {
+        const int32_t type = t.columns[c].type;
+        out_types.push_back(type);
+        if (total == 0) {
+            out_cols.push_back({nullptr, {0u, type}});
+            continue;
+        }
+        auto [pages, np] = materialize_column(column(0, c), type, rids, total, false);
         out_cols.push_back({pages, {np, type}});
     }
 }
@@ -1194,23 +1263,23 @@ static uint64_t host_payload_bytes(const uint8_t *pages, uint32_t np, int32_t ty
     return b;
 }
 
-void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing) {
+// Runs one execution on a stream of the context: `prepare` brings the inputs
+// in (not device time), `body` fills the Exec's output columns between the
+// two timing events; then the result is assembled (device pages, or host
+// pages through pinned memory) and the timing filled.
+static void run_exec(Context &ctx, const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing,
+                     const std::function<void(Exec &)> &prepare, const std::function<void(Exec &)> &body) {
     auto t_start = std::chrono::steady_clock::now();
-    std::memset(out, 0, sizeof *out);
-    if (plan.root >= plan.num_nodes) throw InvalidError("plan root out of range");
-    if (plan.nodes[plan.root].kind != CJ_NODE_JOIN) throw UnsupportedError("plan root must be a join node");
-    for (uint32_t i = 0; i < plan.num_inputs; ++i)
-        for (uint32_t c = 0; c < plan.inputs[i].num_columns; ++c)
-            if (plan.inputs[i].columns[c].type < CJ_INT32 || plan.inputs[i].columns[c].type > CJ_VARCHAR)
-                throw InvalidError("bad column type");
-    hipStream_t s = acquire_stream();
+    DevicePool &pool = ctx.pool;
+    PinnedPool &pinned = ctx.pinned;
+    hipStream_t s = ctx.acquire_stream();
     struct Rel_ {
         Context &c;
         hipStream_t s;
         ~Rel_() { c.release_stream(s); }
-    } rel_guard{*this, s};
-    CJ_HIP(hipSetDevice(device));
-    Exec ex(*this, plan, s);
+    } rel_guard{ctx, s};
+    CJ_HIP(hipSetDevice(ctx.device));
+    Exec ex(ctx, plan, s);
     ex.device_output = (flags & CJ_EXEC_OUTPUT_DEVICE) != 0;
     ex.flags = flags;
     hipEvent_t e0, e1, e2;
@@ -1219,16 +1288,9 @@ void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, 
     CJ_HIP(hipEventCreate(&e2));
     auto *impl = new ResultImpl();
     try {
-        // inputs first (uploads of non-resident columns are not device time)
-        for (uint32_t i = 0; i < plan.num_nodes; ++i) {
-            const cj_node_desc &nd = plan.nodes[i];
-            if (nd.kind != CJ_NODE_SCAN || nd.base_table_id >= plan.num_inputs) continue;
-            for (uint32_t a = 0; a < nd.num_output_attrs; ++a)
-                if (nd.output_attrs[a].index < plan.inputs[nd.base_table_id].num_columns)
-                    ex.column((uint32_t)nd.base_table_id, (uint32_t)nd.output_attrs[a].index);
-        }
+        prepare(ex);
         CJ_HIP(hipEventRecord(e0, s));
-        ex.join(plan.nodes[plan.root], true, 0, out);
+        body(ex);
         CJ_HIP(hipEventRecord(e1, s));
         out->num_rows = ex.out_rows;
         out->num_columns = (uint32_t)ex.out_cols.size();
@@ -1316,8 +1378,8 @@ void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, 
         throw;
     }
     if (ex.timer.on) {
-        std::lock_guard<std::mutex> g(mu);
-        ex.timer.collect(kstats);
+        std::lock_guard<std::mutex> g(ctx.mu);
+        ex.timer.collect(ctx.kstats);
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -1326,6 +1388,96 @@ void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, 
         auto ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
         timing->total_execution_ms = (int64_t)ms;
     }
+}
+
+void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing) {
+    std::memset(out, 0, sizeof *out);
+    if (plan.root >= plan.num_nodes) throw InvalidError("plan root out of range");
+    if (plan.nodes[plan.root].kind != CJ_NODE_JOIN) throw UnsupportedError("plan root must be a join node");
+    for (uint32_t i = 0; i < plan.num_inputs; ++i)
+        for (uint32_t c = 0; c < plan.inputs[i].num_columns; ++c)
+            if (plan.inputs[i].columns[c].type < CJ_INT32 || plan.inputs[i].columns[c].type > CJ_VARCHAR)
+                throw InvalidError("bad column type");
+    run_exec(
+        *this, plan, flags, out, timing,
+        [&](Exec &ex) {
+            // inputs first (uploads of non-resident columns are not device time)
+            for (uint32_t i = 0; i < plan.num_nodes; ++i) {
+                const cj_node_desc &nd = plan.nodes[i];
+                if (nd.kind != CJ_NODE_SCAN || nd.base_table_id >= plan.num_inputs) continue;
+                for (uint32_t a = 0; a < nd.num_output_attrs; ++a)
+                    if (nd.output_attrs[a].index < plan.inputs[nd.base_table_id].num_columns)
+                        ex.column((uint32_t)nd.base_table_id, (uint32_t)nd.output_attrs[a].index);
+            }
+        },
+        [&](Exec &ex) { ex.join(plan.nodes[plan.root], true, 0, out); });
+}
+
+void Context::filter(const cj_table_desc &t, const cj_filter_node *nodes, uint32_t n, uint32_t flags, cj_result *out,
+                     cj_timing *timing) {
+    std::memset(out, 0, sizeof *out);
+    if (n == 0 || n > kFilterMaxNodes) throw InvalidError("a filter program has 1 to 64 nodes");
+    if (t.num_columns && !t.columns) throw InvalidError("null column array");
+    for (uint32_t c = 0; c < t.num_columns; ++c)
+        if (t.columns[c].type < CJ_INT32 || t.columns[c].type > CJ_VARCHAR) throw InvalidError("bad column type");
+    std::vector<FNode> prog(n);
+    std::vector<uint32_t> used;
+    std::map<uint32_t, uint32_t> slot;
+    std::vector<uint8_t> lits;
+    uint32_t depth = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const cj_filter_node &x = nodes[i];
+        FNode &f = prog[i];
+        std::memset(&f, 0, sizeof f);
+        f.kind = x.kind;
+        if (x.kind == CJ_F_CMP) {
+            if (x.column >= t.num_columns) throw InvalidError("filter column out of range");
+            const int32_t type = t.columns[x.column].type;
+            if (x.op < CJ_OP_EQ || x.op > CJ_OP_IS_NOT_NULL) throw InvalidError("bad filter operator");
+            const bool nulltest = x.op == CJ_OP_IS_NULL || x.op == CJ_OP_IS_NOT_NULL;
+            const bool like = x.op == CJ_OP_LIKE || x.op == CJ_OP_NOT_LIKE;
+            if (!nulltest) {
+                // the literal types the reference's column comparisons read (statement.cpp:47-164)
+                const int32_t want = type == CJ_VARCHAR ? CJ_LIT_STR : (type == CJ_FP64 ? CJ_LIT_FLOAT : CJ_LIT_INT);
+                if (x.lit_kind != want) throw InvalidError("filter literal type does not match its column");
+                if (like && type != CJ_VARCHAR) throw InvalidError("LIKE needs a VARCHAR column");
+                if (x.lit_kind == CJ_LIT_STR) {
+                    if (x.slen && !x.sval) throw InvalidError("null string literal");
+                    if (like && std::memchr(x.sval, '\n', x.slen))
+                        throw UnsupportedError("LIKE pattern with a newline");
+                    f.soff = (uint32_t)lits.size();
+                    f.slen = x.slen;
+                    lits.insert(lits.end(), x.sval, x.sval + x.slen);
+                }
+            }
+            f.op = x.op;
+            f.lit_kind = x.lit_kind;
+            f.ival = x.ival;
+            f.fval = x.fval;
+            auto it = slot.find(x.column);
+            if (it == slot.end()) {
+                it = slot.emplace(x.column, (uint32_t)used.size()).first;
+                used.push_back(x.column);
+            }
+            f.col = it->second;
+            if (++depth > kFilterMaxDepth) throw InvalidError("filter program nests deeper than 32");
+        } else if (x.kind == CJ_F_NOT) {
+            if (depth < 1) throw InvalidError("NOT without an operand");
+        } else if (x.kind == CJ_F_AND || x.kind == CJ_F_OR) {
+            if (depth < 2) throw InvalidError("AND / OR without two operands");
+            --depth;
+        } else {
+            throw InvalidError("bad filter node kind");
+        }
+    }
+    if (depth != 1) throw InvalidError("a filter program leaves exactly one result");
+    const cj_plan_desc plan{0, nullptr, 1, &t, 0};
+    run_exec(
+        *this, plan, flags, out, timing,
+        [&](Exec &ex) {
+            for (uint32_t c = 0; c < t.num_columns; ++c) ex.column(0, c);
+        },
+        [&](Exec &ex) { ex.filter_root(prog, used, lits); });
 }
 
 void Context::release(cj_result *res) {

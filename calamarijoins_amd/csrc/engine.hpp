@@ -129,6 +129,9 @@ class Context {
     void unregister_table(uint64_t id);
 
     void execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing);
+    // rows of one table passing a filter program (cj_filter)
+    void filter(const cj_table_desc &t, const cj_filter_node *prog, uint32_t n, uint32_t flags, cj_result *out,
+                cj_timing *timing);
     // many independent single-join plans in shared launches (batch.cpp)
     void execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flags, cj_result *out, cj_batch_stats *stats);
     void release_batch_templates();

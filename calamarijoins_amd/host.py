@@ -184,6 +184,18 @@ class Engine:
     def unregister(self, tid: int):
         abi.check(self.lib.cj_table_unregister(self.ctx, tid))
 
+    def filter(self, table: Table, program, flags: int = 0, timing: bool = False):
+        """Rows of `table` that pass `program` (a list of abi.FilterNode in
+        postfix order, e.g. from job.filter_program), every column, in row
+        order (cj_filter)."""
+        n = len(program)
+        arr = (abi.FilterNode * max(n, 1))(*program)
+        res = abi.Result()
+        tm = abi.Timing()
+        abi.check(self.lib.cj_filter(self.ctx, C.byref(table.desc()), arr, n, flags, C.byref(res),
+                                     C.byref(tm) if timing else None))
+        return EngineResult(self, res, tm if timing else None)
+
     def load_tbl(self, path: str) -> TblTable:
         """Stream a .tbl file into HBM as a resident table (cj_tbl_load)."""
         h = C.c_void_p()

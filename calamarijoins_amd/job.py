@@ -96,11 +96,26 @@ class Comparison:
             v = str(self.value)
         return f"{' ' * indent}{self.column} {self.op} {v}"
 
-    def eval(self, record) -> bool:
+    def eval(self, record, types=None) -> bool:
         """Comparison::eval(record) (src/statement.cpp:166-249): NULL fails
         every comparison; numbers compare as numbers, strings as strings,
-        LIKE with % and _ over the whole string."""
+        LIKE with % and _ over the whole string.  With the column types
+        given, the column-wise evaluation the reference's scans use
+        (statement.cpp:47-164): an INT32 column compares with the literal cast
+        to int32, INT64 exactly, FP64 as doubles, VARCHAR byte-wise."""
         x = record[self.column]
+        if types is not None and x is not None and self.op not in ("IS NULL", "IS NOT NULL", "LIKE", "NOT LIKE"):
+            t = types[self.column]
+            if t == INT32:
+                a, b = int(x), ((int(self.value) + 2 ** 31) % 2 ** 32) - 2 ** 31
+            elif t == INT64:
+                a, b = int(x), int(self.value)
+            elif t == FP64:
+                a, b = float(x), float(self.value)
+            else:
+                a = x.encode("utf-8") if isinstance(x, str) else x
+                b = self.value.encode("utf-8") if isinstance(self.value, str) else self.value
+            return {"=": a == b, "!=": a != b, "<": a < b, ">": a > b, "<=": a <= b, ">=": a >= b}[self.op]
         if self.op == "IS NULL":
             return x is None
         if self.op == "IS NOT NULL":
@@ -108,8 +123,7 @@ class Comparison:
         if self.op in ("LIKE", "NOT LIKE"):
             if not isinstance(x, (str, bytes)) or not isinstance(self.value, str):
                 return False
-            s = x.decode("latin-1") if isinstance(x, bytes) else x
-            m = like_match(s, self.value)
+            m = like_match(x, self.value)
             return m if self.op == "LIKE" else not m
         if x is None:
             return False
@@ -117,8 +131,8 @@ class Comparison:
         if num(x) and num(self.value):
             a, b = float(x), float(self.value)
         elif isinstance(x, (str, bytes)) and isinstance(self.value, str):
-            a = x.encode("latin-1") if isinstance(x, str) else x
-            b = self.value.encode("latin-1")
+            a = x.encode("utf-8") if isinstance(x, str) else x
+            b = self.value.encode("utf-8")
         else:
             return False
         return {"=": a == b, "!=": a != b, "<": a < b, ">": a > b, "<=": a <= b, ">=": a >= b}[self.op]
@@ -134,19 +148,78 @@ class Logical:
         s = f"{' ' * indent}[{self.op}]\n" + "".join(c.pretty(indent + 2) + "\n" for c in self.children)
         return s[:-1] if self.children else s
 
-    def eval(self, record) -> bool:
+    def eval(self, record, types=None) -> bool:
         if self.op == "AND":
-            return all(c.eval(record) for c in self.children)
+            return all(c.eval(record, types) for c in self.children)
         if self.op == "OR":
-            return any(c.eval(record) for c in self.children)
-        return len(self.children) == 1 and not self.children[0].eval(record)
+            return any(c.eval(record, types) for c in self.children)
+        return len(self.children) == 1 and not self.children[0].eval(record, types)
 
 
-def like_match(s: str, pattern: str) -> bool:
-    """SQL LIKE over the whole string: % any run, _ one character, every
-    other character literal (statement.h: like_match)."""
+def _text(v) -> str:
+    b = v if isinstance(v, bytes) else v.encode("utf-8", "surrogateescape")
+    return b.decode("utf-8", "surrogateescape")
+
+
+def like_match(s, pattern) -> bool:
+    """SQL LIKE over the whole string as the reference's regular expression
+    reads it (statement.h: like_match, RE2 with default options): % any run
+    of characters, _ one UTF-8 character, neither matching a newline, every
+    other character literal."""
+    s, pattern = _text(s), _text(pattern)
     rx = "".join(".*" if c == "%" else "." if c == "_" else re.escape(c) for c in pattern)
-    return re.fullmatch(rx, s, flags=re.S) is not None
+    return re.fullmatch(rx, s) is not None
+
+
+_OPS = {"=": abi.CJ_OP_EQ, "!=": abi.CJ_OP_NEQ, "<": abi.CJ_OP_LT, ">": abi.CJ_OP_GT, "<=": abi.CJ_OP_LEQ,
+        ">=": abi.CJ_OP_GEQ, "LIKE": abi.CJ_OP_LIKE, "NOT LIKE": abi.CJ_OP_NOT_LIKE, "IS NULL": abi.CJ_OP_IS_NULL,
+        "IS NOT NULL": abi.CJ_OP_IS_NOT_NULL}
+
+
+def filter_program(stmt, types) -> list:
+    """A statement as the postfix program of cj_filter (children before their
+    operator); `types` are the table's column types.  Literal types follow
+    the reference's column comparisons: integers for INT32 / INT64 columns,
+    floats for FP64, strings for VARCHAR."""
+    out, keep = [], []
+
+    def rec(s):
+        if isinstance(s, Logical):
+            for c in s.children:
+                rec(c)
+            kind = {"AND": abi.CJ_F_AND, "OR": abi.CJ_F_OR, "NOT": abi.CJ_F_NOT}[s.op]
+            out.append(abi.FilterNode(kind, 0, 0, abi.CJ_LIT_NONE, 0, 0.0, None, 0))
+            return
+        t = types[s.column]
+        n = abi.FilterNode(abi.CJ_F_CMP, _OPS[s.op], s.column, abi.CJ_LIT_NONE, 0, 0.0, None, 0)
+        if s.op not in ("IS NULL", "IS NOT NULL"):
+            v = s.value
+            if t == VARCHAR:
+                if not isinstance(v, str):
+                    raise JobError("a VARCHAR column compares with a string literal")
+                b = v.encode("utf-8")
+                keep.append(b)
+                n.lit_kind, n.sval, n.slen = abi.CJ_LIT_STR, b, len(b)
+            elif t == FP64:
+                if not isinstance(v, float):
+                    raise JobError("an FP64 column compares with a float literal")
+                n.lit_kind, n.fval = abi.CJ_LIT_FLOAT, v
+            else:
+                if not isinstance(v, int):
+                    raise JobError("an integer column compares with an integer literal")
+                n.lit_kind, n.ival = abi.CJ_LIT_INT, v
+        out.append(n)
+
+    rec(stmt)
+    return _Program(out, keep)
+
+
+class _Program(list):
+    """Filter nodes plus the string literals they point at (kept alive)."""
+
+    def __init__(self, nodes, keep):
+        super().__init__(nodes)
+        self.keep = keep
 
 
 # ----------------------------------------------------------------- SQL subset

@@ -226,6 +226,39 @@ typedef struct cj_batch_stats {
 int cj_execute_batch(cj_ctx *ctx, const cj_plan_desc *plans, uint32_t n, uint32_t flags, cj_result *out,
                      cj_batch_stats *stats);
 
+/* ---- filters on the GPU: replace the filter step of the harness's scans,
+ *      Table::from_csv(attributes, path, filter) (reference
+ *      src/build_table.cpp:201-360) with its Statement evaluation
+ *      (include/data_model/statement.h, src/statement.cpp:47-249,
+ *      include/data_model/inner_column.h).  The filter is a postfix program
+ *      of comparison and logical nodes over the table's columns; the result
+ *      holds the rows that pass, in their original order, with every column
+ *      of the table.  Column-wise semantics of the reference: a comparison is
+ *      false on NULL; IS NULL is its own test; NOT inverts (NOT over a NULL
+ *      comparison is true); INT32 columns compare with the integer literal
+ *      cast to int32; INT64 with the integer literal; FP64 with the float
+ *      literal; VARCHAR byte-wise with the string literal, LIKE / NOT LIKE
+ *      with % and _ as the reference's regular expression reads them (_ is
+ *      one UTF-8 character, neither matches a newline). ------------------- */
+enum { CJ_F_CMP = 0, CJ_F_AND = 1, CJ_F_OR = 2, CJ_F_NOT = 3 };
+enum {
+    CJ_OP_EQ = 0, CJ_OP_NEQ = 1, CJ_OP_LT = 2, CJ_OP_GT = 3, CJ_OP_LEQ = 4, CJ_OP_GEQ = 5,
+    CJ_OP_LIKE = 6, CJ_OP_NOT_LIKE = 7, CJ_OP_IS_NULL = 8, CJ_OP_IS_NOT_NULL = 9
+};  /* Comparison::Op numbering (statement.h) */
+enum { CJ_LIT_NONE = 0, CJ_LIT_INT = 1, CJ_LIT_FLOAT = 2, CJ_LIT_STR = 3 };
+typedef struct cj_filter_node {
+    int32_t kind;      /* CJ_F_*; AND / OR pop two results, NOT one       */
+    int32_t op;        /* CJ_OP_* (CJ_F_CMP)                               */
+    uint32_t column;   /* column of the table (CJ_F_CMP)                   */
+    int32_t lit_kind;  /* CJ_LIT_*                                         */
+    int64_t ival;
+    double fval;
+    const char *sval;  /* string literal bytes (need not end in NUL)       */
+    uint32_t slen;
+} cj_filter_node;
+int cj_filter(cj_ctx *ctx, const cj_table_desc *table, const cj_filter_node *program, uint32_t num_nodes,
+              uint32_t flags, cj_result *out, cj_timing *timing);
+
 /* ---- .tbl table cache files: replace the harness's DumpTable::dump and
  *      Table::from_cache (reference include/data_access/table.h:30-37,
  *      208-221; src/build_table.cpp:140-197).  Layout: one 8192-byte header
