@@ -255,7 +255,32 @@ int cj_gen_table_create(uint64_t candidate_rows, double keep_frac, uint64_t seed
             });
             uint8_t *pages = nullptr;
             uint64_t np = 0;
-            if (g.kind == CJ_GEN_VARCHAR) {
+            if (g.kind == CJ_GEN_VARCHAR_VOCAB) {
+                // words of the vocabulary, each ended by a NUL byte
+                std::vector<std::pair<uint64_t, uint32_t>> words;
+                if (!g.vocab || !g.vocab_bytes) return CJ_ERR_INVALID;
+                for (uint64_t a = 0, b = 0; b < g.vocab_bytes; ++b)
+                    if (g.vocab[b] == '\0') {
+                        words.push_back({a, (uint32_t)(b - a)});
+                        a = b + 1;
+                    }
+                if (words.empty()) return CJ_ERR_INVALID;
+                std::vector<uint32_t> pick(n);
+                std::vector<uint64_t> offs(n + 1, 0);
+                parallel_for(n, threads, [&](uint64_t a, uint64_t b) {
+                    for (uint64_t r = a; r < b; ++r) pick[r] = (uint32_t)(u_at(seed, sv, kept[r]) % words.size());
+                });
+                for (uint64_t r = 0; r < n; ++r) offs[r + 1] = offs[r] + (valid[r] ? words[pick[r]].second : 0);
+                std::vector<char> bytes(offs[n] + 8);
+                parallel_for(n, threads, [&](uint64_t a, uint64_t b) {
+                    for (uint64_t r = a; r < b; ++r)
+                        if (valid[r]) std::memcpy(bytes.data() + offs[r], g.vocab + words[pick[r]].first, words[pick[r]].second);
+                });
+                uint64_t maxp = n + offs[n] / (kPageSz - 8) + 4;
+                pages = alloc_pages(maxp);
+                int rc = cj_pages_encode_varchar(n, offs.data(), bytes.data(), valid.data(), pages, maxp, &np);
+                if (rc) { std::free(pages); throw std::runtime_error("varchar encode overflow"); }
+            } else if (g.kind == CJ_GEN_VARCHAR) {
                 std::vector<uint64_t> offs(n + 1, 0);
                 std::vector<uint32_t> lens(n);
                 parallel_for(n, threads, [&](uint64_t a, uint64_t b) {

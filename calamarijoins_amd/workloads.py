@@ -17,7 +17,8 @@ import numpy as np
 from . import abi
 from .host import PAGE, Column, Plan, Table
 
-GEN_PERM, GEN_UNIFORM, GEN_ZIPF_FK, GEN_I32_FULL, GEN_I64_FULL, GEN_F64_SCALED, GEN_VARCHAR, GEN_UNIFORM_FK = range(8)
+GEN_PERM, GEN_UNIFORM, GEN_ZIPF_FK, GEN_I32_FULL, GEN_I64_FULL, GEN_F64_SCALED, GEN_VARCHAR, GEN_UNIFORM_FK, \
+    GEN_VARCHAR_VOCAB = range(9)
 
 
 class ColGen(C.Structure):
@@ -26,6 +27,7 @@ class ColGen(C.Structure):
         ("domain", C.c_uint64), ("perm_stream", C.c_uint64), ("zipf_s", C.c_double), ("miss_frac", C.c_double),
         ("lo", C.c_int64), ("hi", C.c_int64), ("miss_lo", C.c_int64), ("miss_hi", C.c_int64),
         ("scale", C.c_double), ("long_frac", C.c_double), ("long_lo", C.c_int64), ("long_hi", C.c_int64),
+        ("vocab", C.c_char_p), ("vocab_bytes", C.c_uint64),
     ]
 
 
@@ -68,11 +70,17 @@ class _Owned:
 
 
 def col(kind, type_name, stream, null_frac=0.0, **kw) -> ColGen:
+    """A column generator; for GEN_VARCHAR_VOCAB pass words=[str, ...]."""
     g = ColGen()
     g.kind = kind
     g.type = abi.TYPE_NAMES[type_name]
     g.stream = stream
     g.null_frac = null_frac
+    words = kw.pop("words", None)
+    if words is not None:
+        blob = b"".join(w.encode("utf-8") + b"\0" for w in words)
+        g._blob = blob  # kept alive with the generator
+        g.vocab, g.vocab_bytes = blob, len(blob)
     for k, v in kw.items():
         setattr(g, k, v)
     return g

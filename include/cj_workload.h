@@ -30,7 +30,8 @@ enum {
     CJ_GEN_F64_SCALED = 5,/* FP64 = (u53 / 2^53) * scale                          */
     CJ_GEN_VARCHAR = 6,   /* printable ASCII, length U[lo, hi]; w.p. long_frac a
                              long string of length U[long_lo, long_hi]            */
-    CJ_GEN_UNIFORM_FK = 7 /* INT32 = perm(uniform rank in [0, domain))            */
+    CJ_GEN_UNIFORM_FK = 7,/* INT32 = perm(uniform rank in [0, domain))            */
+    CJ_GEN_VARCHAR_VOCAB = 8 /* VARCHAR word of `vocab`, chosen uniformly          */
 };
 
 typedef struct cj_colgen {
@@ -47,6 +48,8 @@ typedef struct cj_colgen {
     double scale;
     double long_frac;
     int64_t long_lo, long_hi;
+    const char *vocab;     /* VARCHAR_VOCAB: words, each ended by a NUL byte  */
+    uint64_t vocab_bytes;
 } cj_colgen;
 
 /* A generated table: page blocks owned by the library. */
