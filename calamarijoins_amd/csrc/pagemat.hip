@@ -215,132 +215,179 @@ void key_page_out_pages(const uint64_t *counts, uint32_t pages, uint64_t *p4, ui
 }
 
 // --------------------------------------------------------- build-side pages
-constexpr uint32_t kMaxPagesPerKeyPage = 3;  // <= 2047 matches: 2 INT32 or 3 8-byte pages
-
-// Dword `d` of a record held as NQ 16-byte chunks (d is uniform).
+// Dword `d` of a record held as NQ 16-byte chunks (d is uniform): the chunk,
+// then its 8-byte half, then a shift, so that few uniform lane masks stay
+// live across the row loop.
 template <int NQ>
 __device__ __forceinline__ uint32_t rec_dword(const uint4 (&r)[NQ], uint32_t d) {
-    uint32_t x = 0;
+    uint32_t x = r[0].x, y = r[0].y, z = r[0].z, w = r[0].w;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        x = d == 4u * q + 0 ? r[q].x : x;
-        x = d == 4u * q + 1 ? r[q].y : x;
-        x = d == 4u * q + 2 ? r[q].z : x;
-        x = d == 4u * q + 3 ? r[q].w : x;
+    for (int q = 1; q < NQ; ++q) {
+        const bool here = (d >> 2) == (uint32_t)q;
+        x = here ? r[q].x : x;
+        y = here ? r[q].y : y;
+        z = here ? r[q].z : z;
+        w = here ? r[q].w : w;
     }
-    return x;
+    const uint64_t h = (d & 2) ? (((uint64_t)w << 32) | z) : (((uint64_t)y << 32) | x);
+    return (uint32_t)(h >> (32 * (d & 1)));
 }
 
 constexpr uint32_t kBuildColsPerWave = 4;  // build columns one launch writes
 constexpr uint32_t kBuildBatch = 8;        // 64-row chunks whose loads are issued together
-constexpr uint32_t kObmWords = kCap4 / 32 + 1;
 
-// One build column's part of a 64-row chunk: the chunk's matched rows are
-// result rows [mbase, mbase + tm) of the key page (ko per lane).  The page
-// capacity is a compile-time constant, so the page arithmetic is a multiply
-// and a shift rather than a division.
-template <int NQ, int W>
-__device__ __forceinline__ void emit_col(const BuildPageCol &col, const uint4 (&rec)[NQ], uint32_t fl_dw,
-                                         uint64_t first, bool matched, uint64_t mm, uint32_t ko, uint32_t mbase,
-                                         uint32_t tm, uint32_t *nn_l, uint32_t *ps,
-                                         uint32_t (*obm)[kObmWords], uint32_t lane, uint64_t lt) {
-    // nn_l: values so far, ps[q]: values before page q (the wave's LDS slice;
-    // written by lane 0, read back by the wave in order)
-    const uint32_t nn = *nn_l;
-    constexpr uint32_t cap = W == 4 ? kCap4 : kCap8;
-    const uint32_t lo = rec_dword<NQ>(rec, col.rec_offset >> 2);
-    const uint32_t hi = W == 8 ? rec_dword<NQ>(rec, (col.rec_offset >> 2) + 1) : 0u;
-    const uint32_t fl = rec_dword<NQ>(rec, fl_dw);
-    bool ok = matched && ((fl >> col.rec_bit) & 1u);
-    if (W == 4 && col.is_int32 && (int32_t)lo == kI32Null) ok = false;  // page_builders.h:123
-    const uint64_t om = __ballot(ok);
-    // a page of this column starts inside the chunk: record the values before it
-    const uint32_t b = (mbase + cap - 1) / cap * cap;
-    if (b < mbase + tm) {
-        const uint64_t before = __ballot(matched && ko < b);
-        const uint32_t v = nn + (uint32_t)__popcll(om & before);
-        const uint32_t pi = b / cap;
-        if (lane == 0 && pi < kMaxPagesPerKeyPage) ps[pi] = v;
-        __builtin_amdgcn_wave_barrier();
-    }
-    if (ok) {
-        const uint32_t pp = ko / cap;
-        const uint32_t base = ps[pp < kMaxPagesPerKeyPage ? pp : 0];
-        const uint32_t x = nn + (uint32_t)__popcll(om & lt) - base;
-        uint8_t *page = col.out + (first + pp) * kPage;
-        if (W == 4) *reinterpret_cast<uint32_t *>(page + 4 + 4 * (size_t)x) = lo;
-        else *reinterpret_cast<uint64_t *>(page + 8 + 8 * (size_t)x) = ((uint64_t)hi << 32) | lo;
-    }
-    // output bitmap: result rows [mbase, mbase + tm) carry the packed ok bits;
-    // split at a page boundary
-    const uint64_t pk = pack_selected(matched, ok, mm);
-    if (lane == 0) {
-        const uint32_t p0 = mbase / cap, r0 = mbase - p0 * cap;
-        const uint32_t n0 = cap - r0 < tm ? cap - r0 : tm;
-        if (p0 < kMaxPagesPerKeyPage) or_bits_at(obm[p0], r0, pk, n0);
-        if (n0 < tm && p0 + 1 < kMaxPagesPerKeyPage) or_bits_at(obm[p0 + 1], 0, pk >> n0, tm - n0);
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) *nn_l = nn + (uint32_t)__popcll(om);
-    __builtin_amdgcn_wave_barrier();
+// Output state of one column while a wave walks one key page: result rows
+// fill the column's pages in order.  The page being filled starts qoff bytes
+// after the column's first page and holds pos rows; the key page's non-NULL
+// values so far number nn, pb of them on earlier pages (all wave-uniform).
+// Lane j holds validity word j of the page (rows 32j .. 32j + 31; a page has
+// at most 62 words) in bw.
+struct OutCol {
+    uint32_t nn, pb, qoff, pos;
+    uint32_t bw;
+};
+
+// OR the validity bits pk (bit i: page row pos + i) into the page's words.
+__device__ __forceinline__ void or_bits(uint32_t &bw, uint64_t pk, uint32_t pos, uint32_t lane) {
+    const uint32_t w0 = pos >> 5, sh = pos & 31;
+    const uint32_t lo = (uint32_t)(pk << sh);
+    const uint32_t mid = (uint32_t)(pk >> (32 - sh));
+    const uint32_t top = sh ? (uint32_t)(pk >> (64 - sh)) : 0u;
+    const uint32_t e = lane - w0;
+    bw |= e == 0 ? lo : (e == 1 ? mid : (e == 2 ? top : 0u));
 }
 
-// One wave per key page, no barriers.  For each batch of 8 chunks of 64 rows:
-// bitmap bytes (sparse pages), keys, presence words and records are loaded
-// with clamped addresses so every load of a phase is in flight together;
-// then, chunk by chunk, ballots give each matched row its result row within
-// the key page (ko) and each column's non-NULL values their slot.  Result
-// row ko goes to output page ko / cap of the key page; the values before
-// each page start are kept per column (ps) as the boundary passes.
-template <int NQ>
+// Finish the page being filled (rows result rows, nvals values): header and
+// validity bitmap (at the page end, one byte per 8 rows); start the next.
+__device__ __forceinline__ void close_page(OutCol &o, CJ_G uint8_t *base, uint32_t rows, uint32_t nvals,
+                                           uint32_t lane) {
+    CJ_G uint8_t *page = base + o.qoff;
+    // an opaque row count keeps the compiler from hoisting this rare path's
+    // lane masks out of the row loop, where they would crowd the scalar
+    // registers
+    asm volatile("" : "+v"(rows));
+    const uint32_t nbytes = (rows + 7) >> 3;
+    if (4 * lane < nbytes) {
+        CJ_G uint8_t *dst = page + kPage - nbytes + 4 * lane;
+        const uint32_t k = nbytes - 4 * lane;
+        dst[0] = (uint8_t)o.bw;
+        if (k > 1) dst[1] = (uint8_t)(o.bw >> 8);
+        if (k > 2) dst[2] = (uint8_t)(o.bw >> 16);
+        if (k > 3) dst[3] = (uint8_t)(o.bw >> 24);
+    }
+    if (lane == 0) *reinterpret_cast<CJ_G uint32_t *>(page) = rows | (nvals << 16);
+    o.qoff += kPage;
+    o.pos = 0;
+    o.bw = 0;
+}
+
+// Where one column's fields sit in a direct-map record.
+struct ColPick {
+    uint32_t lo_dw;  // dword of the value (the low dword of an 8-byte value)
+    uint32_t bit;    // validity bit in the flags dword
+    bool w8;         // 8-byte value
+    bool i32;        // INT32: INT32_MIN is NULL in a result
+};
+
+// One column, one 64-row chunk: the matched lanes (mask mm; lane rank among
+// them `rank`, tm in all; dst: the lane's slot in the matched-first order)
+// with a non-NULL value (ok; value words lo, hi) write it at the column's
+// next value slot, and every matched lane appends its validity bit; a page
+// that fills inside the chunk is finished.
+__device__ __forceinline__ void emit_col(bool w8, CJ_G uint8_t *base, uint32_t lo, uint32_t hi, bool ok, uint64_t mm,
+                                         bool matched, uint32_t rank, uint32_t tm, uint32_t dst, OutCol &o,
+                                         uint32_t lane, uint64_t lt) {
+    const uint32_t cap = w8 ? kCap8 : kCap4;
+    const uint64_t om = __ballot(ok);
+    const uint32_t room = cap - o.pos;  // >= 1: a full page is finished at once
+    const bool cross = tm >= room;
+    uint32_t nb = 0;  // values before the next page
+    if (cross) nb = o.nn + (uint32_t)__popcll(om & __ballot(matched && rank < room));
+    if (ok) {
+        const bool next = cross && rank >= room;
+        const uint32_t x = o.nn + (uint32_t)__popcll(om & lt) - (next ? nb : o.pb);
+        const uint32_t off = o.qoff + (next ? kPage : 0u);
+        if (w8) *reinterpret_cast<CJ_G uint64_t *>(base + (off + 8 + 8 * x)) = ((uint64_t)hi << 32) | lo;
+        else *reinterpret_cast<CJ_G uint32_t *>(base + (off + 4 + 4 * x)) = lo;
+    }
+    uint64_t pk;  // validity bits of the chunk's result rows
+    if (om == mm) {
+        pk = tm == 64 ? ~0ull : (1ull << tm) - 1ull;
+    } else {
+        const int got = __builtin_amdgcn_ds_permute((int)(dst << 2), ok ? 1 : 0);
+        pk = __ballot(lane < tm && got != 0);
+    }
+    if (!cross) {
+        or_bits(o.bw, pk, o.pos, lane);
+        o.pos += tm;
+    } else {
+        or_bits(o.bw, room < 64 ? pk & ((1ull << room) - 1ull) : pk, o.pos, lane);
+        close_page(o, base, cap, nb - o.pb, lane);
+        o.pb = nb;
+        if (tm > room) {
+            or_bits(o.bw, pk >> room, 0, lane);
+            o.pos = tm - room;
+        }
+    }
+    o.nn += (uint32_t)__popcll(om);
+}
+
+// One wave per key page, no LDS.  For each batch of 8 chunks of 64 rows:
+// bitmap bytes (sparse pages), keys, match words and records are loaded with
+// clamped addresses so every load of a phase is in flight together; then,
+// chunk by chunk, ballots give each matched row its result row within the
+// key page and each column's non-NULL values their slot.  NC (2 or 4) bounds
+// the columns of the launch.
+template <int NQ, int NC>
 __global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a, uint32_t c_lo) {
-    __shared__ uint32_t obm_all[kWG / 64][kBuildColsPerWave][kMaxPagesPerKeyPage][kObmWords];
-    __shared__ uint32_t nn_all[kWG / 64][kBuildColsPerWave];
-    __shared__ uint32_t ps_all[kWG / 64][kBuildColsPerWave][kMaxPagesPerKeyPage];
+    // chunks per batch: fewer for wide records and many columns, whose
+    // registers would not fit
+    constexpr uint32_t kB = NQ * NC <= 4 ? kBuildBatch : kBuildBatch / 2;
     const uint32_t lane = lane_id();
     const uint32_t pg = blockIdx.x * (kWG / 64) + wave_id();
     if (pg >= a.key.num_pages) return;
-    const uint32_t nc = a.ncols - c_lo < kBuildColsPerWave ? a.ncols - c_lo : kBuildColsPerWave;
-    const uint8_t *p = a.key.pages + (size_t)pg * kPage;
+    const uint32_t nc = a.ncols - c_lo < (uint32_t)NC ? a.ncols - c_lo : (uint32_t)NC;
+    const CJ_G uint8_t *p = glb(a.key.pages) + (size_t)pg * kPage;
     const uint32_t n = ld16(p), nv = ld16(p + 2);
     const bool dense = n == nv;
-    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
-    const int32_t *keys = reinterpret_cast<const int32_t *>(p + 4);
-    uint32_t(*obm)[kMaxPagesPerKeyPage][kObmWords] = obm_all[wave_id()];
-    for (uint32_t i = lane; i < kBuildColsPerWave * kMaxPagesPerKeyPage * kObmWords; i += 64)
-        (&obm[0][0][0])[i] = 0;
-    uint32_t *nn = nn_all[wave_id()];
-    uint32_t(*ps)[kMaxPagesPerKeyPage] = ps_all[wave_id()];
-    if (lane < kBuildColsPerWave) nn[lane] = 0;
-    if (lane < kBuildColsPerWave * kMaxPagesPerKeyPage) (&ps[0][0])[lane] = 0;
-    __builtin_amdgcn_wave_barrier();
-    const uint64_t first4 = a.p4_off[pg], first8 = a.p8_off[pg];
+    const CJ_G uint8_t *bm = p + kPage - ((n + 7) >> 3);
+    const CJ_G int32_t *keys = reinterpret_cast<const CJ_G int32_t *>(p + 4);
     const uint64_t r0 = a.key.row_start[pg];
     const uint32_t *mw = a.match_bits + (r0 >> 5);
     const uint32_t msh = (uint32_t)(r0 & 31);
     const uint64_t lt = lanemask_lt();
     const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
     const uint32_t fl_dw = a.dt.flags_offset >> 2;
-    uint32_t vbase = 0, mbase = 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kBuildBatch) {
+    CJ_G uint8_t *base[NC];
+    ColPick pick[NC];
+    OutCol oc[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        const BuildPageCol &col = a.cols[c_lo + (k < (int)nc ? k : 0)];
+        base[k] = glb(col.out) + (col.width == 8 ? a.p8_off[pg] : a.p4_off[pg]) * kPage;
+        pick[k] = ColPick{col.rec_offset >> 2, col.rec_bit, col.width == 8, col.is_int32 != 0};
+        oc[k] = OutCol{0, 0, 0, 0, 0u};
+    }
+    uint32_t vbase = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kB) {
         uint32_t vbits = 0;
-        uint32_t vi[kBuildBatch];
+        uint32_t vi[kB];
         if (dense) {
 #pragma unroll
-            for (uint32_t c = 0; c < kBuildBatch; ++c) {
+            for (uint32_t c = 0; c < kB; ++c) {
                 const uint32_t lr = c0 + 64 * c + lane;
                 vi[c] = lr;
                 if (lr < n) vbits |= 1u << c;
             }
         } else {
-            uint32_t byte[kBuildBatch];
+            uint32_t byte[kB];
 #pragma unroll
-            for (uint32_t c = 0; c < kBuildBatch; ++c) {
+            for (uint32_t c = 0; c < kB; ++c) {
                 const uint32_t lr = c0 + 64 * c + lane;
                 byte[c] = bm[(lr < n ? lr : nlast) >> 3];
             }
 #pragma unroll
-            for (uint32_t c = 0; c < kBuildBatch; ++c) {
+            for (uint32_t c = 0; c < kB; ++c) {
                 const uint32_t lr = c0 + 64 * c + lane;
                 const bool v = lr < n && ((byte[c] >> (lr & 7)) & 1u);
                 const uint64_t vm = __ballot(v);
@@ -349,76 +396,83 @@ __global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a, uint32_t 
                 if (v) vbits |= 1u << c;
             }
         }
-        int32_t key[kBuildBatch];
+        int32_t key[kB];
 #pragma unroll
-        for (uint32_t c = 0; c < kBuildBatch; ++c) {
+        for (uint32_t c = 0; c < kB; ++c) {
             key[c] = keys[vi[c] < nv ? vi[c] : vlast];
             if (vi[c] >= nv) vbits &= ~(1u << c);
         }
         // matched rows come from the count pass's match bitmap (coalesced
         // words) instead of a second random presence lookup per row
-        uint32_t mword[kBuildBatch];
+        uint32_t mword[kB];
 #pragma unroll
-        for (uint32_t c = 0; c < kBuildBatch; ++c) {
+        for (uint32_t c = 0; c < kB; ++c) {
             const uint32_t lr = c0 + 64 * c + lane;
             mword[c] = mw[(msh + (lr < n ? lr : nlast)) >> 5];
         }
-        uint32_t slot[kBuildBatch], mbits = 0;
-        uint4 rec[kBuildBatch][NQ];
+        uint32_t mbits = 0;
+        uint4 rec[kB][NQ];
 #pragma unroll
-        for (uint32_t c = 0; c < kBuildBatch; ++c) {
+        for (uint32_t c = 0; c < kB; ++c) {
             const uint32_t lr = c0 + 64 * c + lane;
             const int64_t off = (int64_t)key[c] - (int64_t)a.dt.min_key;
             const bool m = ((vbits >> c) & 1u) && ((mword[c] >> ((msh + lr) & 31)) & 1u) && off >= 0 &&
                            (uint64_t)off < a.dt.range;
-            slot[c] = m ? (uint32_t)off : 0u;
+            const uint32_t slot = m ? (uint32_t)off : 0u;
             if (m) mbits |= 1u << c;
-            const uint4 *r = reinterpret_cast<const uint4 *>(a.dt.records + (uint64_t)slot[c] * a.dt.stride);
+            const uint4 *r = reinterpret_cast<const uint4 *>(a.dt.records + (uint64_t)slot * a.dt.stride);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) rec[c][q] = r[q];
         }
+        // the columns' value words and validity out of the records, so the
+        // records' registers are free before the emit loop
+        uint32_t vlo[NC][kB], vhi[NC][kB], okb[NC];
 #pragma unroll
-        for (uint32_t c = 0; c < kBuildBatch; ++c) {
+        for (int k = 0; k < NC; ++k) okb[k] = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < kB; ++c) {
+            const uint32_t fl = rec_dword<NQ>(rec[c], fl_dw);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const uint32_t lo = rec_dword<NQ>(rec[c], pick[k].lo_dw);
+                vlo[k][c] = lo;
+                vhi[k][c] = pick[k].w8 ? rec_dword<NQ>(rec[c], pick[k].lo_dw + 1) : 0u;
+                const bool ok = ((mbits >> c) & 1u) && ((fl >> pick[k].bit) & 1u) &&
+                                !(pick[k].i32 && (int32_t)lo == kI32Null);  // page_builders.h:123
+                okb[k] |= (uint32_t)ok << c;
+            }
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < kB; ++c) {
             const bool matched = (mbits >> c) & 1u;
             const uint64_t mm = __ballot(matched);
             if (!mm) continue;
-            const uint32_t ko = mbase + (uint32_t)__popcll(mm & lt);
+            const uint32_t rank = (uint32_t)__popcll(mm & lt);
             const uint32_t tm = (uint32_t)__popcll(mm);
-#pragma unroll 1
-            for (uint32_t k = 0; k < nc; ++k) {
-                const BuildPageCol &col = a.cols[c_lo + k];
-                if (col.width == 4)
-                    emit_col<NQ, 4>(col, rec[c], fl_dw, first4, matched, mm, ko, mbase, tm, &nn[k], ps[k], obm[k], lane, lt);
-                else
-                    emit_col<NQ, 8>(col, rec[c], fl_dw, first8, matched, mm, ko, mbase, tm, &nn[k], ps[k], obm[k], lane, lt);
+            const uint32_t dst = matched ? rank : tm + (lane - rank);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                if (k >= (int)nc) break;
+                emit_col(pick[k].w8, base[k], vlo[k][c], vhi[k][c], (okb[k] >> c) & 1u, mm, matched, rank, tm, dst,
+                         oc[k], lane, lt);
             }
-            mbase += tm;
         }
     }
-    __builtin_amdgcn_wave_barrier();
-    // headers and bitmaps
 #pragma unroll
-    for (uint32_t k = 0; k < kBuildColsPerWave; ++k) {
-        if (k >= nc) break;
-        const BuildPageCol &col = a.cols[c_lo + k];
-        const uint32_t cap = col.width == 4 ? kCap4 : kCap8;
-        const uint32_t np = (mbase + cap - 1) / cap;
-#pragma unroll
-        for (uint32_t pp = 0; pp < kMaxPagesPerKeyPage; ++pp) {
-            if (pp >= np) break;
-            uint8_t *page = col.out + ((col.width == 4 ? first4 : first8) + pp) * kPage;
-            const uint32_t rows = mbase - pp * cap < cap ? mbase - pp * cap : cap;
-            const uint32_t nb = (rows + 7) >> 3;
-            const uint8_t *src = reinterpret_cast<const uint8_t *>(obm[k][pp]);
-            for (uint32_t b = lane; b < nb; b += 64) page[kPage - nb + b] = src[b];
-            if (lane == 0) {
-                uint32_t end = nn[k];
-#pragma unroll
-                for (uint32_t q = 1; q < kMaxPagesPerKeyPage; ++q) end = pp + 1 == q && q < np ? ps[k][q] : end;
-                *reinterpret_cast<uint16_t *>(page) = (uint16_t)rows;
-                *reinterpret_cast<uint16_t *>(page + 2) = (uint16_t)(end - ps[k][pp]);
-            }
-        }
+    for (int k = 0; k < NC; ++k) {
+        if (k >= (int)nc) break;
+        if (oc[k].pos) close_page(oc[k], base[k], oc[k].pos, oc[k].nn - oc[k].pb, lane);
+    }
+}
+
+template <int NC>
+static void launch_build_pages(const BuildPagesArgs &a, uint32_t c_lo, uint32_t grid, hipStream_t s) {
+    switch (a.dt.stride / 16) {
+        case 1: k_build_pages<1, NC><<<grid, kWG, 0, s>>>(a, c_lo); break;
+        case 2: k_build_pages<2, NC><<<grid, kWG, 0, s>>>(a, c_lo); break;
+        case 3: k_build_pages<3, NC><<<grid, kWG, 0, s>>>(a, c_lo); break;
+        case 4: k_build_pages<4, NC><<<grid, kWG, 0, s>>>(a, c_lo); break;
+        default: k_build_pages<5, NC><<<grid, kWG, 0, s>>>(a, c_lo); break;
     }
 }
 
@@ -429,13 +483,8 @@ void build_pages(const BuildPagesArgs &a, hipStream_t s) {
     const uint32_t per = kWG / 64;
     const uint32_t grid = (units + per - 1) / per;
     for (uint32_t c_lo = 0; c_lo < a.ncols; c_lo += kBuildColsPerWave) {
-        switch (a.dt.stride / 16) {
-            case 1: k_build_pages<1><<<grid, kWG, 0, s>>>(a, c_lo); break;
-            case 2: k_build_pages<2><<<grid, kWG, 0, s>>>(a, c_lo); break;
-            case 3: k_build_pages<3><<<grid, kWG, 0, s>>>(a, c_lo); break;
-            case 4: k_build_pages<4><<<grid, kWG, 0, s>>>(a, c_lo); break;
-            default: k_build_pages<5><<<grid, kWG, 0, s>>>(a, c_lo); break;
-        }
+        if (a.ncols - c_lo <= 2) launch_build_pages<2>(a, c_lo, grid, s);
+        else launch_build_pages<kBuildColsPerWave>(a, c_lo, grid, s);
     }
     CJ_HIP(hipGetLastError());
 }
