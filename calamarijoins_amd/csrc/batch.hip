@@ -448,7 +448,8 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
 
 }  // namespace
 
-// dst[0 .. n) = src[0 .. n), eight loads per thread in flight before the stores.
+// dst[0 .. n) = src[0 .. n), eight loads per thread in flight before the
+// stores (clamped addresses: a load under a condition is awaited alone).
 template <class T, class S>
 __device__ __forceinline__ void stage_lds(T *dst, S src, uint32_t n) {
     for (uint32_t i0 = 0; i0 < n; i0 += 8 * kWG) {
@@ -456,7 +457,7 @@ __device__ __forceinline__ void stage_lds(T *dst, S src, uint32_t n) {
 #pragma unroll
         for (uint32_t q = 0; q < 8; ++q) {
             const uint32_t i = i0 + q * kWG + threadIdx.x;
-            if (i < n) v[q] = src[i];
+            v[q] = src[i < n ? i : n - 1];
         }
 #pragma unroll
         for (uint32_t q = 0; q < 8; ++q) {
@@ -667,7 +668,7 @@ __global__ __launch_bounds__(kWG) void k_bx_pack(const BPack *__restrict__ cols,
             for (uint32_t q = 0; q < 4; ++q) {
                 const uint32_t i = i0 + q < cnt ? i0 + q : cnt - 1;
                 v[q] = src[i];
-                bits |= (i0 + q < cnt && sv[i] ? 1u : 0u) << q;
+                bits |= (uint32_t)((i0 + q < cnt) & (sv[i] != 0)) << q;
             }
             uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);
             CJ_G uint64_t *vals = reinterpret_cast<CJ_G uint64_t *>(p + 8);

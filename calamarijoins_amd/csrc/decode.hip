@@ -43,10 +43,11 @@ constexpr uint64_t kScanBlock = (uint64_t)kWG * kScanItems;
 __global__ void k_scan_block_sums(const uint64_t *in, uint64_t n, uint64_t *sums) {
     __shared__ uint64_t lds[4];
     uint64_t base = blockIdx.x * kScanBlock + (uint64_t)threadIdx.x * kScanItems;
-    uint64_t s = 0;
+    uint64_t v[kScanItems], s = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i)
-        if (base + i < n) s += in[base + i];
+    for (int i = 0; i < kScanItems; ++i) v[i] = in[base + i < n ? base + i : n - 1];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) s += base + i < n ? v[i] : 0;
     uint64_t tot;
     block_excl_scan<uint64_t>(s, &tot, lds);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -58,8 +59,10 @@ __global__ void k_scan_block(const uint64_t *in, uint64_t *out, uint64_t n, cons
     uint64_t v[kScanItems];
     uint64_t s = 0;
 #pragma unroll
+    for (int i = 0; i < kScanItems; ++i) v[i] = in[base + i < n ? base + i : n - 1];
+#pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-        v[i] = (base + i < n) ? in[base + i] : 0;
+        v[i] = base + i < n ? v[i] : 0;
         s += v[i];
     }
     uint64_t tot;

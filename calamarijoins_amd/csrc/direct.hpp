@@ -40,6 +40,7 @@ struct DirectTable {
     const uint8_t *records;
     uint32_t stride;
     uint32_t flags_offset;
+    uint32_t full;  // every key of the range is present: no presence lookup
 };
 
 // mm[0] = min, mm[1] = max of the non-NULL keys (INT32_MAX, INT32_MIN when

@@ -937,7 +937,9 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     const uint64_t range = (uint64_t)((int64_t)hi - (int64_t)lo) + 1;
     if (range > a.max_range) return false;
     if (h[2] != h[3]) return false;  // duplicate keys
-    *dt = DirectTable{lo, range, a.presence, a.records, a.stride, a.flags_offset};
+    // keys that fill their range (primary keys 0..n-1) match every in-range
+    // probe key, so the probes skip the bitmap
+    *dt = DirectTable{lo, range, a.presence, a.records, a.stride, a.flags_offset, h[3] == range ? 1u : 0u};
     return true;
 }
 

@@ -136,17 +136,26 @@ __global__ __launch_bounds__(kWG) void k_count_match_direct(ProbeArgs a, uint64_
                     if (v) vbits |= 1u << c;
                 }
             }
+            // all key loads first, unconditionally (a load under a
+            // short-circuit condition is issued and awaited one at a time)
+            int32_t key[kCountBatch];
+#pragma unroll
+            for (uint32_t c = 0; c < kCountBatch; ++c) key[c] = keys[vi[c] < nv ? vi[c] : vlast];
             uint32_t slot[kCountBatch], word[kCountBatch], in = 0;
 #pragma unroll
             for (uint32_t c = 0; c < kCountBatch; ++c) {
-                const int32_t key = keys[vi[c] < nv ? vi[c] : vlast];
-                const int64_t off = (int64_t)key - (int64_t)dt.min_key;
-                const bool ok = ((vbits >> c) & 1u) && vi[c] < nv && off >= 0 && (uint64_t)off < dt.range;
+                const int64_t off = (int64_t)key[c] - (int64_t)dt.min_key;
+                const bool ok = ((vbits >> c) & 1u) & (vi[c] < nv) & (off >= 0) & ((uint64_t)off < dt.range);
                 slot[c] = ok ? (uint32_t)off : 0u;
-                if (ok) in |= 1u << c;
+                in |= (uint32_t)ok << c;
             }
+            if (dt.full) {
 #pragma unroll
-            for (uint32_t c = 0; c < kCountBatch; ++c) word[c] = dt.presence[slot[c] >> 5];
+                for (uint32_t c = 0; c < kCountBatch; ++c) word[c] = ~0u;
+            } else {
+#pragma unroll
+                for (uint32_t c = 0; c < kCountBatch; ++c) word[c] = dt.presence[slot[c] >> 5];
+            }
 #pragma unroll
             for (uint32_t c = 0; c < kCountBatch; ++c) {
                 const bool m = ((in >> c) & 1u) && ((word[c] >> (slot[c] & 31)) & 1u);

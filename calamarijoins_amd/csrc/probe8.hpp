@@ -101,7 +101,7 @@ __device__ __forceinline__ void lookup8(uint32_t valid, const int32_t key[kRPT],
             const bool in = ((valid >> j) & 1u) && off >= 0 && (uint64_t)off < dt.range;
             s[j] = in ? (uint32_t)off : 0u;
             e[j] = in;
-            w[j] = dt.presence[s[j] >> 5];
+            w[j] = dt.full ? ~0u : dt.presence[s[j] >> 5];
         }
 #pragma unroll
         for (uint32_t j = 0; j < kRPT; ++j) {
