@@ -533,14 +533,15 @@ constexpr uint32_t kWaveMaxRows = 4096;  // pages k_probe_filter_wave handles
 constexpr uint32_t kWaveBatch = 16;      // 64-row chunks whose loads are issued together
 
 // One wave per payload page of at most kWaveMaxRows rows, no barriers: for
-// each batch of 16 chunks of 64 rows, the page-bitmap bytes and match words
+// each batch of 16 (8-byte values: 8) chunks of 64 rows, the page-bitmap bytes and match words
 // are loaded, value indices come from ballots of the validity bits, all
 // value loads are issued, then matched values are stored compacted and the
-// output bitmap is assembled in the wave's LDS slice.
+// output bitmap is assembled in registers (lane j holds words j and 64 + j).
 template <int W>
-__global__ __launch_bounds__(kWG) void k_probe_filter_wave(FilterArgs a) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8))) void k_probe_filter_wave(FilterArgs a) {
     using T = typename std::conditional<W == 4, uint32_t, uint64_t>::type;
-    __shared__ uint32_t obm_all[kWG / 64][kWaveMaxRows / 32];
+    // 8-byte values: half the batch, so that 8 waves fit a SIMD
+    constexpr uint32_t kB = W == 4 ? kWaveBatch : kWaveBatch / 2;
     const uint32_t lane = lane_id();
     const uint32_t q = blockIdx.x * (kWG / 64) + wave_id();
     if (q >= a.num_pages) return;
@@ -557,36 +558,36 @@ __global__ __launch_bounds__(kWG) void k_probe_filter_wave(FilterArgs a) {
     T *ovals = reinterpret_cast<T *>(out + (W == 4 ? 4 : 8));
     const uint32_t *mw = a.match_bits + (rs >> 5);
     const uint32_t msh = (uint32_t)(rs & 31);
-    uint32_t *obm = obm_all[wave_id()];
-    for (uint32_t i = lane; i < (n + 31) / 32; i += 64) obm[i] = 0;
-    __builtin_amdgcn_wave_barrier();
+    // lane j: words j and 64 + j of the output validity bitmap (a page of
+    // this kernel has at most 4096 rows)
+    uint32_t bw = 0, bw1 = 0;
     const uint64_t lt = lanemask_lt();
     uint32_t vbase = 0, mbase = 0, obase = 0;
     // every load below is unconditional with a clamped in-page address, so
     // the compiler issues a whole batch before the first wait
     const uint32_t nlast = n - 1, vlast = nv ? nv - 1 : 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kWaveBatch) {
+    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kB) {
         uint32_t vbits = 0, mbits = 0;
-        uint32_t byte[kWaveBatch], word[kWaveBatch];
+        uint32_t byte[kB], word[kB];
 #pragma unroll
-        for (uint32_t c = 0; c < kWaveBatch; ++c) {
+        for (uint32_t c = 0; c < kB; ++c) {
             const uint32_t lr = c0 + 64 * c + lane;
             const uint32_t lc = lr < n ? lr : nlast;
             byte[c] = bm[lc >> 3];
             word[c] = mw[(msh + lc) >> 5];
         }
 #pragma unroll
-        for (uint32_t c = 0; c < kWaveBatch; ++c) {
+        for (uint32_t c = 0; c < kB; ++c) {
             const uint32_t lr = c0 + 64 * c + lane;
             const uint32_t lc = lr < n ? lr : nlast;
             const bool in = lr < n;
             if (in && (dense || ((byte[c] >> (lc & 7)) & 1u))) vbits |= 1u << c;
             if (in && ((word[c] >> ((msh + lc) & 31)) & 1u)) mbits |= 1u << c;
         }
-        T v[kWaveBatch];
+        T v[kB];
         uint32_t okbits = 0;
 #pragma unroll
-        for (uint32_t c = 0; c < kWaveBatch; ++c) {
+        for (uint32_t c = 0; c < kB; ++c) {
             const uint64_t vm = __ballot((vbits >> c) & 1u);
             const uint32_t vi = vbase + (uint32_t)__popcll(vm & lt);
             vbase += (uint32_t)__popcll(vm);
@@ -594,25 +595,50 @@ __global__ __launch_bounds__(kWG) void k_probe_filter_wave(FilterArgs a) {
             if ((((vbits & mbits) >> c) & 1u) && vi < nv) okbits |= 1u << c;
         }
 #pragma unroll
-        for (uint32_t c = 0; c < kWaveBatch; ++c) {
+        for (uint32_t c = 0; c < kB; ++c) {
             bool ok = (okbits >> c) & 1u;
             if (W == 4 && a.is_int32 && (int32_t)(uint32_t)v[c] == kI32Null) ok = false;  // page_builders.h:123
             const uint64_t om = __ballot(ok);
-            const uint64_t mm = __ballot((mbits >> c) & 1u);
-            if (ok) {
-                const uint32_t so = obase + (uint32_t)__popcll(om & lt);
-                const uint32_t ko = mbase + (uint32_t)__popcll(mm & lt);
-                ovals[so] = v[c];
-                atomicOr(&obm[ko >> 5], 1u << (ko & 31));
+            const bool matched = (mbits >> c) & 1u;
+            const uint64_t mm = __ballot(matched);
+            if (ok) ovals[obase + (uint32_t)__popcll(om & lt)] = v[c];
+            // validity bits of the chunk's result rows (matched lanes in
+            // order), OR-ed into the lanes' bitmap words
+            const uint32_t tm = (uint32_t)__popcll(mm);
+            uint64_t pk;
+            if (om == mm) {
+                pk = tm == 64 ? ~0ull : (1ull << tm) - 1ull;
+            } else {
+                const uint32_t rank = (uint32_t)__popcll(mm & lt);
+                const uint32_t dst = matched ? rank : tm + (lane - rank);
+                const int got = __builtin_amdgcn_ds_permute((int)(dst << 2), ok ? 1 : 0);
+                pk = __ballot(lane < tm && got != 0);
+            }
+            if (tm) {
+                or_bits(bw, pk, mbase, lane);
+                if (mbase + tm > 2048) {
+                    if (mbase >= 2048) or_bits(bw1, pk, mbase - 2048, lane);
+                    else or_bits(bw1, pk >> (2048 - mbase), 0, lane);
+                }
             }
             obase += (uint32_t)__popcll(om);
-            mbase += (uint32_t)__popcll(mm);
+            mbase += tm;
         }
     }
-    __builtin_amdgcn_wave_barrier();
     const uint32_t nbo = (mbase + 7) >> 3;
-    const uint8_t *ob = reinterpret_cast<const uint8_t *>(obm);
-    for (uint32_t b = lane; b < nbo; b += 64) out[kPage - nbo + b] = ob[b];
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t j = lane + 64 * h;
+        const uint32_t word = h ? bw1 : bw;
+        if (4 * j < nbo) {
+            uint8_t *dst = out + kPage - nbo + 4 * j;
+            const uint32_t k = nbo - 4 * j;
+            dst[0] = (uint8_t)word;
+            if (k > 1) dst[1] = (uint8_t)(word >> 8);
+            if (k > 2) dst[2] = (uint8_t)(word >> 16);
+            if (k > 3) dst[3] = (uint8_t)(word >> 24);
+        }
+    }
     if (lane == 0) {
         *reinterpret_cast<uint16_t *>(out) = (uint16_t)mbase;
         *reinterpret_cast<uint16_t *>(out + 2) = (uint16_t)obase;
