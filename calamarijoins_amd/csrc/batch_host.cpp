@@ -455,8 +455,13 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
     CJ_HIP(hipGraphLaunch(T.exec, s));
     // ---- one read-back: the result size of every plan
     std::vector<uint64_t> totals(n);
-    CJ_HIP(hipMemcpyAsync(totals.data(), T.totals, n * 8, hipMemcpyDeviceToHost, s));
-    CJ_HIP(hipStreamSynchronize(s));
+    {
+        uint64_t *h = static_cast<uint64_t *>(pinned.alloc(n * 8));  // a pageable copy is staged
+        PinGuard pt{pinned, h};
+        CJ_HIP(hipMemcpyAsync(h, T.totals, n * 8, hipMemcpyDeviceToHost, s));
+        CJ_HIP(hipStreamSynchronize(s));
+        std::copy(h, h + n, totals.begin());
+    }
     uint64_t out_rows = 0;
     for (uint64_t t : totals) out_rows += t;
     // ---- phase 2: dense output columns, then their pages

@@ -35,8 +35,10 @@ void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ scans
-// Exclusive scan of u64, three phases (block sums, scan of sums, block
-// scan).  out has n + 1 entries; out[n] is the total.  In-place allowed.
+// Exclusive scan of u64: block sums, then each block's scan offset by the
+// sum of the block sums before it (read by the block itself up to
+// kScanDirect blocks, else from a recursive scan of the sums).  out has
+// n + 1 entries; out[n] is the total.  In-place allowed.
 constexpr int kScanItems = 8;
 constexpr uint64_t kScanBlock = (uint64_t)kWG * kScanItems;
 
@@ -53,8 +55,20 @@ __global__ void k_scan_block_sums(const uint64_t *in, uint64_t n, uint64_t *sums
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
+constexpr uint64_t kScanDirect = 8 * kWG;
+
+// RAW: block_off holds the unscanned block sums.
+template <bool RAW>
 __global__ void k_scan_block(const uint64_t *in, uint64_t *out, uint64_t n, const uint64_t *block_off) {
     __shared__ uint64_t lds[4];
+    uint64_t boff = 0;
+    if (RAW) {
+        uint64_t p = 0;
+        for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kWG) p += block_off[i];
+        block_excl_scan<uint64_t>(p, &boff, lds);
+    } else if (block_off) {
+        boff = block_off[blockIdx.x];
+    }
     uint64_t base = blockIdx.x * kScanBlock + (uint64_t)threadIdx.x * kScanItems;
     uint64_t v[kScanItems];
     uint64_t s = 0;
@@ -67,7 +81,7 @@ __global__ void k_scan_block(const uint64_t *in, uint64_t *out, uint64_t n, cons
     }
     uint64_t tot;
     uint64_t pre = block_excl_scan<uint64_t>(s, &tot, lds);
-    uint64_t off = (block_off ? block_off[blockIdx.x] : 0) + pre;
+    uint64_t off = boff + pre;
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         if (base + i < n) out[base + i] = off;
@@ -94,7 +108,7 @@ void scan_exclusive_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream
     }
     uint64_t nb = (n + kScanBlock - 1) / kScanBlock;
     if (nb == 1) {
-        k_scan_block<<<1, kWG, 0, s>>>(in, out, n, nullptr);
+        k_scan_block<false><<<1, kWG, 0, s>>>(in, out, n, nullptr);
         CJ_HIP(hipGetLastError());
         return;
     }
@@ -102,8 +116,13 @@ void scan_exclusive_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream
     uint64_t *sums = tmp, *sums_off = tmp + nb + 1;
     k_scan_block_sums<<<(unsigned)nb, kWG, 0, s>>>(in, n, sums);
     CJ_HIP(hipGetLastError());
+    if (nb <= kScanDirect) {
+        k_scan_block<true><<<(unsigned)nb, kWG, 0, s>>>(in, out, n, sums);
+        CJ_HIP(hipGetLastError());
+        return;
+    }
     scan_exclusive_u64(sums, sums_off, nb, s, tmp + 2 * (nb + 1), tmp_elems - 2 * (nb + 1));
-    k_scan_block<<<(unsigned)nb, kWG, 0, s>>>(in, out, n, sums_off);
+    k_scan_block<false><<<(unsigned)nb, kWG, 0, s>>>(in, out, n, sums_off);
     CJ_HIP(hipGetLastError());
 }
 

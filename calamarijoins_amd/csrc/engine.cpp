@@ -466,6 +466,7 @@ class Exec {
     ~Exec() {
         for (void *p : temps) ctx.pool.free(p);
         for (auto &c : own_cols) ctx.free_column(*c);
+        if (rb) ctx.pinned.free(rb);
     }
 
     Context &ctx;
@@ -563,24 +564,34 @@ class Exec {
         uint64_t *sc = scan_scratch(n);
         scan_exclusive_u64(in, out, n, st, sc, scratch_elems);
     }
+    // Pinned host words for counter read-backs (a copy into pageable memory
+    // is staged and takes tens of microseconds).
+    static constexpr size_t kReadbackWords = 256;
+    uint64_t *rb = nullptr;
+    uint64_t *readback() {
+        if (!rb) rb = static_cast<uint64_t *>(ctx.pinned.alloc(kReadbackWords * 8));
+        return rb;
+    }
     uint64_t read_u64(const uint64_t *dev) {
-        uint64_t v = 0;
-        CJ_HIP(hipMemcpyAsync(&v, dev, 8, hipMemcpyDeviceToHost, st));
+        uint64_t *h = readback();
+        CJ_HIP(hipMemcpyAsync(h, dev, 8, hipMemcpyDeviceToHost, st));
         CJ_HIP(hipStreamSynchronize(st));
-        return v;
+        return h[0];
     }
     // Several device counters with one read-back.
     std::vector<uint64_t> read_u64s(const std::vector<const uint64_t *> &ptrs) {
         std::vector<uint64_t> v(ptrs.size());
+        uint64_t *h = ptrs.size() <= kReadbackWords ? readback() : v.data();
         for (size_t i = 0; i < ptrs.size(); i += 16) {
             U64List l{};
             l.n = (uint32_t)std::min<size_t>(16, ptrs.size() - i);
             for (uint32_t k = 0; k < l.n; ++k) l.p[k] = ptrs[i + k];
             uint64_t *d = tmp<uint64_t>(16);
             gather_u64s(l, d, st);
-            CJ_HIP(hipMemcpyAsync(v.data() + i, d, 8 * l.n, hipMemcpyDeviceToHost, st));
+            CJ_HIP(hipMemcpyAsync(h + i, d, 8 * l.n, hipMemcpyDeviceToHost, st));
         }
         CJ_HIP(hipStreamSynchronize(st));
+        if (h != v.data()) std::copy(h, h + ptrs.size(), v.begin());
         return v;
     }
 
@@ -929,7 +940,7 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
         TimedScope ts(timer, "direct_build", kc->num_rows * (4ull + a.stride + a.stride) + kc->num_rows / 8);
         direct_build(a, st);
     }
-    uint32_t h[4];
+    uint32_t *h = reinterpret_cast<uint32_t *>(readback());
     CJ_HIP(hipMemcpyAsync(h, st4, 16, hipMemcpyDeviceToHost, st));
     CJ_HIP(hipStreamSynchronize(st));
     const int32_t lo = (int32_t)h[0], hi = (int32_t)h[1];
