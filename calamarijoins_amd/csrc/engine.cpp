@@ -467,6 +467,7 @@ class Exec {
         for (void *p : temps) ctx.pool.free(p);
         for (auto &c : own_cols) ctx.free_column(*c);
         if (rb) ctx.pinned.free(rb);
+        if (rb_event) (void)hipEventDestroy(rb_event);
     }
 
     Context &ctx;
@@ -568,9 +569,23 @@ class Exec {
     // is staged and takes tens of microseconds).
     static constexpr size_t kReadbackWords = 256;
     uint64_t *rb = nullptr;
+    hipEvent_t rb_event = nullptr;  // a read-back's copy is done (the stream may go on)
     uint64_t *readback() {
         if (!rb) rb = static_cast<uint64_t *>(ctx.pinned.alloc(kReadbackWords * 8));
         return rb;
+    }
+    // First output page of each key page for 4- and 8-byte build columns
+    // (exclusive scans of ceil(matches / page capacity)), for counts `cnt`.
+    const uint64_t *po_cnt = nullptr;
+    uint64_t *po4 = nullptr, *po8 = nullptr;
+    void page_offsets(const uint64_t *cnt, uint32_t kp) {
+        uint64_t *p4 = tmp<uint64_t>(kp + 1), *p8 = tmp<uint64_t>(kp + 1);
+        po4 = tmp<uint64_t>(kp + 1);
+        po8 = tmp<uint64_t>(kp + 1);
+        key_page_out_pages(cnt, kp, p4, p8, st);
+        scan(p4, po4, kp);
+        scan(p8, po8, kp);
+        po_cnt = cnt;
     }
     uint64_t read_u64(const uint64_t *dev) {
         uint64_t *h = readback();
@@ -781,7 +796,20 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
             else probe_count(pk, use_nl ? nullptr : &ht, nl, cnt, st);
         }
         scan(cnt, off, units);
-        total = read_u64(off + units);
+        // the page-aligned writers' output page offsets depend only on the
+        // counts: queue them before the read-back so the device works while
+        // the host waits
+        if (aligned) {
+            uint64_t *h = readback();
+            CJ_HIP(hipMemcpyAsync(h, off + units, 8, hipMemcpyDeviceToHost, st));
+            if (!rb_event) CJ_HIP(hipEventCreateWithFlags(&rb_event, hipEventDisableTiming));
+            CJ_HIP(hipEventRecord(rb_event, st));
+            page_offsets(cnt, units);
+            CJ_HIP(hipEventSynchronize(rb_event));
+            total = h[0];
+        } else {
+            total = read_u64(off + units);
+        }
         if (total > 0xffffffffull) throw UnsupportedError("join result exceeds 2^32 rows");
         // dense matching (at least half the probe rows) keeps the filtered
         // pages at least half full on average; sparser results take the
@@ -1063,13 +1091,9 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
     for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
         if (remap[i] >= bw) continue;
         if (!ba.p4_off) {
-            uint64_t *p4 = tmp<uint64_t>(kp + 1), *p8 = tmp<uint64_t>(kp + 1);
-            uint64_t *o4 = tmp<uint64_t>(kp + 1), *o8 = tmp<uint64_t>(kp + 1);
-            key_page_out_pages(cnt, kp, p4, p8, st);
-            scan(p4, o4, kp);
-            scan(p8, o8, kp);
-            ba.p4_off = o4;
-            ba.p8_off = o8;
+            if (po_cnt != cnt) page_offsets(cnt, kp);
+            ba.p4_off = po4;
+            ba.p8_off = po8;
         }
         const int32_t type = n.output_attrs[i].type;
         const uint32_t width = type == CJ_INT32 ? 4 : 8;
