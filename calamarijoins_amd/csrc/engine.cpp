@@ -826,7 +826,10 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
             if (need_b) ob = tmp<uint32_t>(total);
             if (need_p) op = tmp<uint32_t>(total);
             TimedScope ts(timer, use_nl ? "nl_probe_write" : "probe_write", key_bytes(pk) + total * (4ull * (need_b + need_p)));
-            probe_write(pk, use_nl ? nullptr : &ht, nl, off, ob, op, st);
+            // a columnar probe side was counted per key page by the 8-row
+            // lookups; its pairs are written by the same kernels
+            if (pk.mode == KEY_COLUMNAR) probe_write_columnar(fa, off, ob, op, st);
+            else probe_write(pk, use_nl ? nullptr : &ht, nl, off, ob, op, st);
         }
     }
 
