@@ -162,6 +162,7 @@ __global__ __launch_bounds__(kWG) void k_direct_scatter(DirectBuildArgs a) {
                 rec_put<NQ>(q, fd.offset + 4, (uint32_t)(v >> 32));
             }
         }
+        if (a.rid_offset >= 0) rec_put<NQ>(q, (uint32_t)a.rid_offset, (uint32_t)i);
         rec_put<NQ>(q, a.flags_offset, flags);
         uint4 *dst = reinterpret_cast<uint4 *>(a.records + slot * a.stride);
 #pragma unroll

@@ -30,6 +30,7 @@ struct DirectBuildArgs {
     uint32_t *partials;     // 2 * kDirectParts scratch words
     uint32_t nfields;
     DirectField fields[kDirectMaxFields];
+    int32_t rid_offset;  // >= 0: the build row id is stored at this record offset
 };
 
 // The table as the probe sees it.
@@ -41,6 +42,7 @@ struct DirectTable {
     uint32_t stride;
     uint32_t flags_offset;
     uint32_t full;  // every key of the range is present: no presence lookup
+    int32_t rid_offset;  // >= 0: record offset of the build row id (joins whose output lists row ids)
 };
 
 // mm[0] = min, mm[1] = max of the non-NULL keys (INT32_MAX, INT32_MIN when

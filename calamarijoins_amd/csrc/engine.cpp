@@ -702,7 +702,7 @@ class Exec {
                       const std::vector<uint32_t> &remap, const ProbeArgs &a, const uint64_t *cnt,
                       const uint32_t *match_bits, uint64_t total);
     bool try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint32_t bw,
-                    const std::vector<uint32_t> &remap, DirectTable *dt);
+                    const std::vector<uint32_t> &remap, DirectTable *dt, bool rids);
     const uint16_t *page_index(DevColumn *c);
     uint32_t flags = 0;
     std::vector<std::pair<uint32_t, uint32_t>> direct_field;  // per output attr: record offset, flag bit
@@ -763,14 +763,18 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
             TimedScope ts(timer, "nl_collect", 0);
             nl_collect(bk, nl, st);
             fa.mode = kModeNL;
-        } else if (!(fused && B.columnar && try_direct(n, B, battr, bw, remap, &fa.dt))) {
+        } else if (B.columnar && ((fused && try_direct(n, B, battr, bw, remap, &fa.dt, false)) ||
+                                  (!fused && pk.mode == KEY_COLUMNAR &&
+                                   try_direct(n, B, battr, bw, remap, &fa.dt, true)))) {
+            // a direct map: with the projected build payloads for the fused
+            // root writers, or with the build row ids for pair lists
+            fa.mode = kModeDirect;
+        } else {
             // tuple capacity: the rows the key pages actually hold (the
             // declared num_rows only sizes the directory)
             uint64_t cap = B.rows;
             if (B.columnar) cap = column(B.slots[0].table, B.cols[battr].col)->num_rows;
             ht = build_hash(bk, B.rows, cap);
-        } else {
-            fa.mode = kModeDirect;
         }
         fa.key = pk;
         fa.ht = ht;
@@ -920,15 +924,16 @@ const uint16_t *Exec::page_index(DevColumn *c) {
 // keys are unique and span at most ~2x its rows.  Returns false (and the
 // caller builds the hash table) when the keys are too sparse or repeat.
 bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint32_t bw,
-                      const std::vector<uint32_t> &remap, DirectTable *dt) {
+                      const std::vector<uint32_t> &remap, DirectTable *dt, bool rids) {
     if (flags & CJ_EXEC_NO_DIRECT) return false;
     DevColumn *kc = column(B.slots[0].table, B.cols[battr].col);
     if (kc->num_rows == 0) return false;
     // record: 8-byte fields, then 4-byte fields, then the u32 of NULL flags
     DirectBuildArgs a{};
     std::vector<uint32_t> build_attrs;
-    for (uint32_t i = 0; i < n.num_output_attrs; ++i)
-        if (remap[i] < bw) build_attrs.push_back(i);
+    if (!rids)
+        for (uint32_t i = 0; i < n.num_output_attrs; ++i)
+            if (remap[i] < bw) build_attrs.push_back(i);
     if (build_attrs.size() > kDirectMaxFields) return false;
     ColView kv = view(kc);
     direct_field.assign(n.num_output_attrs, {0u, 0u});
@@ -946,6 +951,11 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
             off += w;
         }
     a.nfields = (uint32_t)build_attrs.size();
+    a.rid_offset = -1;
+    if (rids) {
+        a.rid_offset = (int32_t)off;
+        off += 4;
+    }
     a.flags_offset = off;
     a.stride = (off + 4 + 15) & ~15u;  // whole 16-byte chunks: one vector load each
     a.keys = static_cast<const int32_t *>(kv.values);
@@ -954,6 +964,9 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     // the table is sized for the widest range the direct map accepts, so the
     // key range, the build and the duplicate check need one read-back
     a.max_range = 2 * kc->num_rows + 1024;
+    // a map of row ids is 16 bytes a slot with nothing to fill but the keyed
+    // rows: sparser keys (filtered primary keys) still take it
+    if (rids) a.max_range = std::min<uint64_t>(16 * kc->num_rows + 1024, uint64_t(1) << 25);
     uint32_t *st4 = tmp<uint32_t>(4);  // min, max, keyed rows, present slots (written by kernels)
     int32_t *mm = reinterpret_cast<int32_t *>(st4);
     {
@@ -981,7 +994,8 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     if (h[2] != h[3]) return false;  // duplicate keys
     // keys that fill their range (primary keys 0..n-1) match every in-range
     // probe key, so the probes skip the bitmap
-    *dt = DirectTable{lo, range, a.presence, a.records, a.stride, a.flags_offset, h[3] == range ? 1u : 0u};
+    *dt = DirectTable{lo, range, a.presence, a.records, a.stride, a.flags_offset, h[3] == range ? 1u : 0u,
+                      a.rid_offset};
     return true;
 }
 

@@ -174,6 +174,20 @@ __global__ __launch_bounds__(kWG) void k_probe_write8(ProbeArgs a, const uint64_
             uint32_t m[kRPT], s[kRPT], e[kRPT], pre[kRPT];
             lookup8<MODE>(valid, key, a.ht, nl, a.dt, m, s, e);
             const uint32_t tot = scan_slots8(m, pre, lds32);
+            // direct mode: the build id is the record slot, or the build row
+            // id the record carries (all 8 loads issued together; slot 0 for
+            // a key without a match)
+            uint32_t bid[kRPT];
+            if (MODE == kModeDirect) {
+#pragma unroll
+                for (uint32_t j = 0; j < kRPT; ++j) bid[j] = s[j];
+                if (a.dt.rid_offset >= 0) {
+#pragma unroll
+                    for (uint32_t j = 0; j < kRPT; ++j)
+                        bid[j] = *reinterpret_cast<const uint32_t *>(a.dt.records + (uint64_t)s[j] * a.dt.stride +
+                                                                     (uint32_t)a.dt.rid_offset);
+                }
+            }
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j) {
                 if (!m[j]) continue;
@@ -181,7 +195,7 @@ __global__ __launch_bounds__(kWG) void k_probe_write8(ProbeArgs a, const uint64_
                 uint64_t o = base + pre[j];
                 if (MODE == kModeDirect) {
                     if (o < end) {
-                        if (ob) ob[o] = s[j];
+                        if (ob) ob[o] = bid[j];
                         if (op) op[o] = prid;
                     }
                 } else if (MODE == kModeNL) {
