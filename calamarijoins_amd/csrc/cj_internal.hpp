@@ -109,7 +109,12 @@ struct KeySource {
     const uint32_t *rids;        // row-id list of the intermediate (nullptr = identity)
     uint64_t rows;               // rows of the intermediate
 };
-uint32_t key_source_units(const KeySource &k);  // work units (pages / row chunks)
+constexpr uint32_t kInterChunk = 2048;  // rows of a row-id list per work unit
+// work units (pages / row chunks)
+__host__ __device__ inline uint32_t key_source_units(const KeySource &k) {
+    if (k.mode == KEY_COLUMNAR) return k.num_pages;
+    return (uint32_t)((k.rows + kInterChunk - 1) / kInterChunk);
+}
 
 void build_partition_hist(const KeySource &k, const HashGeom &g, uint64_t *part_count, hipStream_t s);
 void build_partition_scatter(const KeySource &k, const HashGeom &g, uint64_t *part_cursor,

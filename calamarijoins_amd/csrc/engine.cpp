@@ -764,8 +764,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
             nl_collect(bk, nl, st);
             fa.mode = kModeNL;
         } else if (B.columnar && ((fused && try_direct(n, B, battr, bw, remap, &fa.dt, false)) ||
-                                  (!fused && pk.mode == KEY_COLUMNAR &&
-                                   try_direct(n, B, battr, bw, remap, &fa.dt, true)))) {
+                                  (!fused && try_direct(n, B, battr, bw, remap, &fa.dt, true)))) {
             // a direct map: with the projected build payloads for the fused
             // root writers, or with the build row ids for pair lists
             fa.mode = kModeDirect;
@@ -796,8 +795,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         {
             TimedScope ts(timer, use_nl ? "nl_probe_count" : "probe_count", key_bytes(pk));
             if (aligned) probe_count_match(fa, cnt, match_bits, st);
-            else if (pk.mode == KEY_COLUMNAR) probe_count_columnar(fa, cnt, st);
-            else probe_count(pk, use_nl ? nullptr : &ht, nl, cnt, st);
+            else probe_count_columnar(fa, cnt, st);
         }
         scan(cnt, off, units);
         // the page-aligned writers' output page offsets depend only on the
@@ -830,10 +828,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
             if (need_b) ob = tmp<uint32_t>(total);
             if (need_p) op = tmp<uint32_t>(total);
             TimedScope ts(timer, use_nl ? "nl_probe_write" : "probe_write", key_bytes(pk) + total * (4ull * (need_b + need_p)));
-            // a columnar probe side was counted per key page by the 8-row
-            // lookups; its pairs are written by the same kernels
-            if (pk.mode == KEY_COLUMNAR) probe_write_columnar(fa, off, ob, op, st);
-            else probe_write(pk, use_nl ? nullptr : &ht, nl, off, ob, op, st);
+            probe_write_columnar(fa, off, ob, op, st);
         }
     }
 

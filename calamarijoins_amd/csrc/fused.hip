@@ -89,24 +89,54 @@ void window_first(const uint32_t *probe_rows, uint64_t rows, uint32_t R, const u
 }
 
 // --------------------------------------------------------- counting pass
-template <int MODE>
+// Work unit pg of the probe side: a key page (columnar), or rows
+// [pg kInterChunk, ...) of an intermediate's row-id list (INTER).
+struct ProbeUnit {
+    const uint8_t *p;
+    uint32_t n, nv;
+    bool dense;
+    uint64_t r0;  // probe row of the unit's first row
+};
+template <bool INTER>
+__device__ __forceinline__ ProbeUnit probe_unit(const KeySource &k, uint32_t pg) {
+    ProbeUnit u{};
+    if (INTER) {
+        u.r0 = (uint64_t)pg * kInterChunk;
+        u.n = (uint32_t)(k.rows - u.r0 < kInterChunk ? k.rows - u.r0 : kInterChunk);
+        return u;
+    }
+    u.p = k.pages + (size_t)pg * kPage;
+    u.n = ld16(u.p);
+    u.nv = ld16(u.p + 2);
+    u.dense = u.n == u.nv;
+    u.r0 = k.row_start[pg];
+    return u;
+}
+template <bool INTER>
+__device__ __forceinline__ uint32_t unit_keys8(const KeySource &k, const ProbeUnit &u, uint32_t t0, uint32_t *running,
+                                               int32_t key[kRPT], uint32_t *lds32) {
+    if (INTER) return load_keys8_inter(k, u.r0, u.n, t0, key);
+    return load_keys8(u.p, u.n, u.nv, u.dense, t0, running, key, lds32);
+}
+
+template <int MODE, bool INTER>
 __global__ __launch_bounds__(kWG) void k_probe_count8(ProbeArgs a, uint64_t *counts) {
     __shared__ uint32_t lds32[32];
     __shared__ uint64_t red[4];
     NLKeys nl;
     if (MODE == kModeNL) nl = *a.nl;
     const KeySource &k = a.key;
-    // a workgroup handles several pages (grid-stride), so short pages do not
+    const uint32_t units = key_source_units(k);
+    // a workgroup handles several units (grid-stride), so short pages do not
     // leave the machine waiting on workgroup dispatch
-    for (uint32_t pg = blockIdx.x; pg < k.num_pages; pg += gridDim.x) {
-        const uint8_t *p = k.pages + (size_t)pg * kPage;
-        const uint32_t n = ld16(p), nv = ld16(p + 2);
-        const bool dense = n == nv;
+    for (uint32_t pg = blockIdx.x; pg < units; pg += gridDim.x) {
+        const ProbeUnit u = probe_unit<INTER>(k, pg);
+        const uint32_t n = u.n;
         uint32_t running = 0;
         uint64_t mine = 0;
         for (uint32_t t0 = 0; t0 < n; t0 += kTile) {
             int32_t key[kRPT];
-            uint32_t valid = load_keys8(p, n, nv, dense, t0, &running, key, lds32);
+            uint32_t valid = unit_keys8<INTER>(k, u, t0, &running, key, lds32);
             uint32_t m[kRPT], s[kRPT], e[kRPT];
             lookup8<MODE>(valid, key, a.ht, nl, a.dt, m, s, e);
 #pragma unroll
@@ -118,13 +148,19 @@ __global__ __launch_bounds__(kWG) void k_probe_count8(ProbeArgs a, uint64_t *cou
     }
 }
 
+template <bool INTER>
+static void launch_count8(const ProbeArgs &a, uint64_t *counts, uint32_t grid, hipStream_t s) {
+    if (a.mode == kModeNL) k_probe_count8<kModeNL, INTER><<<grid, kWG, 0, s>>>(a, counts);
+    else if (a.mode == kModeDirect) k_probe_count8<kModeDirect, INTER><<<grid, kWG, 0, s>>>(a, counts);
+    else k_probe_count8<kModeHash, INTER><<<grid, kWG, 0, s>>>(a, counts);
+}
+
 void probe_count_columnar(const ProbeArgs &a, uint64_t *counts, hipStream_t s) {
-    const uint32_t units = a.key.num_pages;
+    const uint32_t units = key_source_units(a.key);
     if (!units) return;
     const uint32_t grid = units < kCountGrid ? units : kCountGrid;
-    if (a.mode == kModeNL) k_probe_count8<kModeNL><<<grid, kWG, 0, s>>>(a, counts);
-    else if (a.mode == kModeDirect) k_probe_count8<kModeDirect><<<grid, kWG, 0, s>>>(a, counts);
-    else k_probe_count8<kModeHash><<<grid, kWG, 0, s>>>(a, counts);
+    if (a.key.mode == KEY_INTER) launch_count8<true>(a, counts, grid, s);
+    else launch_count8<false>(a, counts, grid, s);
     CJ_HIP(hipGetLastError());
 }
 
@@ -153,24 +189,24 @@ __device__ __forceinline__ uint32_t scan_slots8(const uint32_t cnt[kRPT], uint32
     return run;
 }
 
-template <int MODE>
+template <int MODE, bool INTER>
 __global__ __launch_bounds__(kWG) void k_probe_write8(ProbeArgs a, const uint64_t *off, uint32_t *ob, uint32_t *op) {
     __shared__ uint32_t lds32[32];
     NLKeys nl;
     if (MODE == kModeNL) nl = *a.nl;
     const KeySource &k = a.key;
-    for (uint32_t pg = blockIdx.x; pg < k.num_pages; pg += gridDim.x) {
+    const uint32_t units = key_source_units(k);
+    for (uint32_t pg = blockIdx.x; pg < units; pg += gridDim.x) {
         uint64_t base = off[pg];
         const uint64_t end = off[pg + 1];
         if (base == end) continue;
-        const uint8_t *p = k.pages + (size_t)pg * kPage;
-        const uint32_t n = ld16(p), nv = ld16(p + 2);
-        const bool dense = n == nv;
-        const uint64_t r0 = k.row_start[pg];
+        const ProbeUnit u = probe_unit<INTER>(k, pg);
+        const uint32_t n = u.n;
+        const uint64_t r0 = u.r0;
         uint32_t running = 0;
         for (uint32_t t0 = 0; t0 < n && base < end; t0 += kTile) {
             int32_t key[kRPT];
-            uint32_t valid = load_keys8(p, n, nv, dense, t0, &running, key, lds32);
+            uint32_t valid = unit_keys8<INTER>(k, u, t0, &running, key, lds32);
             uint32_t m[kRPT], s[kRPT], e[kRPT], pre[kRPT];
             lookup8<MODE>(valid, key, a.ht, nl, a.dt, m, s, e);
             const uint32_t tot = scan_slots8(m, pre, lds32);
@@ -223,14 +259,21 @@ __global__ __launch_bounds__(kWG) void k_probe_write8(ProbeArgs a, const uint64_
     }
 }
 
+template <bool INTER>
+static void launch_write8(const ProbeArgs &a, const uint64_t *off, uint32_t *ob, uint32_t *op, uint32_t grid,
+                          hipStream_t s) {
+    if (a.mode == kModeNL) k_probe_write8<kModeNL, INTER><<<grid, kWG, 0, s>>>(a, off, ob, op);
+    else if (a.mode == kModeDirect) k_probe_write8<kModeDirect, INTER><<<grid, kWG, 0, s>>>(a, off, ob, op);
+    else k_probe_write8<kModeHash, INTER><<<grid, kWG, 0, s>>>(a, off, ob, op);
+}
+
 void probe_write_columnar(const ProbeArgs &a, const uint64_t *off, uint32_t *build_ids, uint32_t *probe_rows,
                           hipStream_t s) {
-    const uint32_t units = a.key.num_pages;
+    const uint32_t units = key_source_units(a.key);
     if (!units) return;
     const uint32_t grid = units < kCountGrid ? units : kCountGrid;
-    if (a.mode == kModeNL) k_probe_write8<kModeNL><<<grid, kWG, 0, s>>>(a, off, build_ids, probe_rows);
-    else if (a.mode == kModeDirect) k_probe_write8<kModeDirect><<<grid, kWG, 0, s>>>(a, off, build_ids, probe_rows);
-    else k_probe_write8<kModeHash><<<grid, kWG, 0, s>>>(a, off, build_ids, probe_rows);
+    if (a.key.mode == KEY_INTER) launch_write8<true>(a, off, build_ids, probe_rows, grid, s);
+    else launch_write8<false>(a, off, build_ids, probe_rows, grid, s);
     CJ_HIP(hipGetLastError());
 }
 

@@ -20,11 +20,13 @@ struct ProbeArgs {
     DirectTable dt;
 };
 
-// Matches per key page (8 rows per thread).
+// Matches per work unit of the probe side (key page, or kInterChunk rows of
+// an intermediate), 8 rows per thread.
 void probe_count_columnar(const ProbeArgs &a, uint64_t *counts, hipStream_t s);
 // Writes, in probe-row order, the (build id, probe row) of every match at
-// off[page] onwards; build id = record slot in direct mode, build row id
-// otherwise.  Either output may be null.
+// off[unit] onwards; build id = record slot in direct mode (the build row id
+// when the records carry it), build row id otherwise.  Either output may be
+// null.
 void probe_write_columnar(const ProbeArgs &a, const uint64_t *off, uint32_t *build_ids, uint32_t *probe_rows,
                           hipStream_t s);
 

@@ -22,12 +22,7 @@
 
 namespace cj {
 
-constexpr uint32_t kInterChunk = 2048;  // rows of a row-id list per workgroup
 
-uint32_t key_source_units(const KeySource &k) {
-    if (k.mode == KEY_COLUMNAR) return k.num_pages;
-    return (uint32_t)((k.rows + kInterChunk - 1) / kInterChunk);
-}
 
 // Iterate the keys of one work unit in row order, 256 rows per tile.  For
 // each tile, `fn(in_range, ok, key, rid)` is called by every thread (so it

@@ -86,6 +86,34 @@ __device__ __forceinline__ uint32_t load_keys8(P p, uint32_t n, uint32_t nv, boo
     return valid;
 }
 
+// Keys of this thread's 8 rows (t0 + 256 j + t) of rows [a, a + n) of an
+// intermediate (n >= 1): row ids, validity words and values each loaded in
+// one batch.  Through an intermediate a NULL key and INT32_MIN are alike
+// (the reference's value_t), so both leave the row out of the mask.
+__device__ __forceinline__ uint32_t load_keys8_inter(const KeySource &k, uint64_t a, uint32_t n, uint32_t t0,
+                                                     int32_t key[kRPT]) {
+    uint64_t rid[kRPT];
+    uint32_t vw[kRPT];
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) {
+        const uint32_t r = t0 + kWG * j + threadIdx.x;
+        const uint64_t row = a + (r < n ? r : n - 1);
+        rid[j] = k.rids ? k.rids[row] : row;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) vw[j] = k.valid_bits[rid[j] >> 5];
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) key[j] = k.values[rid[j]];
+    uint32_t valid = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) {
+        const uint32_t r = t0 + kWG * j + threadIdx.x;
+        const bool ok = (r < n) & (((vw[j] >> (rid[j] & 31)) & 1u) != 0) & (key[j] != kI32Null);
+        valid |= (uint32_t)ok << j;
+    }
+    return valid;
+}
+
 // Lookups of 8 keys, all loads issued before any is used.  m[j] = matches of
 // key j; hash mode: [s[j], e[j]) is its directory range; direct mode: s[j]
 // is its record slot (0 when it has none).
@@ -98,10 +126,18 @@ __device__ __forceinline__ void lookup8(uint32_t valid, const int32_t key[kRPT],
 #pragma unroll
         for (uint32_t j = 0; j < kRPT; ++j) {
             const int64_t off = (int64_t)key[j] - (int64_t)dt.min_key;
-            const bool in = ((valid >> j) & 1u) && off >= 0 && (uint64_t)off < dt.range;
+            const bool in = ((valid >> j) & 1u) & (off >= 0) & ((uint64_t)off < dt.range);
             s[j] = in ? (uint32_t)off : 0u;
             e[j] = in;
-            w[j] = dt.full ? ~0u : dt.presence[s[j] >> 5];
+        }
+        // the presence loads in one batch (a load chosen per key by a
+        // condition is issued and awaited one at a time)
+        if (dt.full) {
+#pragma unroll
+            for (uint32_t j = 0; j < kRPT; ++j) w[j] = ~0u;
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kRPT; ++j) w[j] = dt.presence[s[j] >> 5];
         }
 #pragma unroll
         for (uint32_t j = 0; j < kRPT; ++j) {
