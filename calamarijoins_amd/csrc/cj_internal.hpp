@@ -129,10 +129,6 @@ struct NLKeys {  // tiny build side (< 8 rows): keys broadcast to every lane
     uint32_t rids[8];
 };
 void nl_collect(const KeySource &k, NLKeys *out, hipStream_t s);
-void probe_count(const KeySource &k, const HashTableDev *ht, const NLKeys *nl, uint64_t *counts,
-                 hipStream_t s);
-void probe_write(const KeySource &k, const HashTableDev *ht, const NLKeys *nl,
-                 const uint64_t *offsets, uint32_t *out_build, uint32_t *out_probe, hipStream_t s);
 
 // materialize.hip
 struct FixedSource {

@@ -230,92 +230,4 @@ void nl_collect(const KeySource &k, NLKeys *out, hipStream_t s) {
     CJ_HIP(hipGetLastError());
 }
 
-// ------------------------------------------------------------------ probe
-// Matches of one probe key.  With WRITE, the (build_rid, probe_rid) pairs
-// go to out[pos..]; either output array may be null (the reference's
-// LEFT_ONLY / RIGHT_ONLY collection modes, match_collector.h:31-35).
-template <bool NL, bool WRITE>
-__device__ __forceinline__ uint32_t probe_key(int32_t key, uint32_t prid, const HashTableDev &ht, const NLKeys &nl,
-                                              uint64_t pos, uint32_t *ob, uint32_t *op) {
-    uint32_t m = 0;
-    if (NL) {
-        for (int i = 0; i < nl.count; ++i)
-            if (nl.keys[i] == key) {
-                if (WRITE) {
-                    if (ob) ob[pos + m] = nl.rids[i];
-                    if (op) op[pos + m] = prid;
-                }
-                ++m;
-            }
-        return m;
-    }
-    uint64_t h = hash_key(key);
-    uint64_t slot = slot_of(h, ht.log_slots);
-    uint64_t e = ht.dir[slot];
-    uint32_t tag = c_bloom.t[h & 0x7FF];
-    if ((e & tag) != tag) return 0;
-    uint64_t end = e >> 16, start = slot ? (ht.dir[slot - 1] >> 16) : 0;
-    for (uint64_t i = start; i < end; ++i)
-        if (ht.keys[i] == key) {
-            if (WRITE) {
-                if (ob) ob[pos + m] = ht.rids[i];
-                if (op) op[pos + m] = prid;
-            }
-            ++m;
-        }
-    return m;
-}
-
-template <bool NL>
-__global__ __launch_bounds__(kWG) void k_probe_count(KeySource k, HashTableDev ht, const NLKeys *nlp,
-                                                     uint64_t *counts) {
-    __shared__ uint32_t lds4[4];
-    __shared__ uint64_t red[4];
-    NLKeys nl;
-    if (NL) nl = *nlp;
-    uint64_t mine = 0;
-    for_each_key(k, blockIdx.x, false, lds4, [&](bool, bool ok, int32_t key, uint32_t rid) {
-        if (ok) mine += probe_key<NL, false>(key, rid, ht, nl, 0, nullptr, nullptr);
-    });
-    uint64_t tot;
-    block_excl_scan<uint64_t>(mine, &tot, red);
-    if (threadIdx.x == 0) counts[blockIdx.x] = tot;
-}
-
-template <bool NL>
-__global__ __launch_bounds__(kWG) void k_probe_write(KeySource k, HashTableDev ht, const NLKeys *nlp,
-                                                     const uint64_t *offsets, uint32_t *ob, uint32_t *op) {
-    __shared__ uint32_t lds4[4];
-    __shared__ uint64_t red[4];
-    NLKeys nl;
-    if (NL) nl = *nlp;
-    uint64_t base = offsets[blockIdx.x];
-    for_each_key(k, blockIdx.x, false, lds4, [&](bool, bool ok, int32_t key, uint32_t rid) {
-        uint32_t m = ok ? probe_key<NL, false>(key, rid, ht, nl, 0, nullptr, nullptr) : 0;
-        uint64_t tot;
-        uint64_t pre = block_excl_scan<uint64_t>((uint64_t)m, &tot, red);
-        if (m) probe_key<NL, true>(key, rid, ht, nl, base + pre, ob, op);
-        base += tot;
-    });
-}
-
-void probe_count(const KeySource &k, const HashTableDev *ht, const NLKeys *nl, uint64_t *counts, hipStream_t s) {
-    uint32_t units = key_source_units(k);
-    if (!units) return;
-    HashTableDev h = ht ? *ht : HashTableDev{};
-    if (nl) k_probe_count<true><<<units, kWG, 0, s>>>(k, h, nl, counts);
-    else k_probe_count<false><<<units, kWG, 0, s>>>(k, h, nl, counts);
-    CJ_HIP(hipGetLastError());
-}
-
-void probe_write(const KeySource &k, const HashTableDev *ht, const NLKeys *nl, const uint64_t *offsets,
-                 uint32_t *out_build, uint32_t *out_probe, hipStream_t s) {
-    uint32_t units = key_source_units(k);
-    if (!units) return;
-    HashTableDev h = ht ? *ht : HashTableDev{};
-    if (nl) k_probe_write<true><<<units, kWG, 0, s>>>(k, h, nl, offsets, out_build, out_probe);
-    else k_probe_write<false><<<units, kWG, 0, s>>>(k, h, nl, offsets, out_build, out_probe);
-    CJ_HIP(hipGetLastError());
-}
-
 }  // namespace cj
