@@ -3,8 +3,11 @@
 // Page decode restates the page format the reference reads in
 // data_access/columnar_reader.h:120-201 (PageIndex::build: cumulative rows,
 // 0xffff counts one row, 0xfffe none) and :452-474 (bitmap + popcount to
-// find a value's slot), turned into one workgroup per page that writes a
-// dense, row-indexed column plus a validity bitmap.
+// find a value's slot), turned into one wave (fixed width) or workgroup
+// (VARCHAR) per page that writes a dense, row-indexed column plus a
+// validity bitmap.
+#include <type_traits>
+
 #include "cj_internal.hpp"
 #include "device_util.hpp"
 
@@ -141,44 +144,92 @@ void page_row_counts(const uint8_t *pages, uint32_t num_pages, uint64_t *counts,
 }
 
 // ------------------------------------------------------------ dense decode
-// One workgroup per page.  Dense pages (n_rows == n_vals) map row r to value
-// r; sparse pages rank the bitmap with a wave ballot + popcount.
+// One wave per page, no barriers.  For each batch of 8 chunks of 64 rows,
+// the bitmap bytes (sparse pages) and then the values are loaded with
+// clamped in-page addresses, all in flight together; value indices of a
+// sparse page come from ballots of the row bits.  Validity words wholly
+// inside a page whose first row is 32-aligned are stored; the others
+// (shared with a neighbouring page) are OR-ed with atomics.
+constexpr uint32_t kDecodeBatch = 8;
+
 template <int W>
-__global__ __launch_bounds__(kWG) void k_decode_fixed(const uint8_t *pages, const uint64_t *row_start,
-                                                      void *values, uint32_t *valid_bits) {
-    __shared__ uint32_t lds[4];
-    const uint32_t pg = blockIdx.x;
+__global__ __launch_bounds__(kWG) void k_decode_fixed(const uint8_t *pages, uint32_t num_pages,
+                                                      const uint64_t *row_start, void *values, uint32_t *valid_bits) {
+    using T = typename std::conditional<W == 4, uint32_t, uint64_t>::type;
+    const uint32_t lane = lane_id();
+    const uint32_t pg = blockIdx.x * (kWG / 64) + wave_id();
+    if (pg >= num_pages) return;
     const uint8_t *p = pages + (size_t)pg * kPage;
     const uint32_t n = ld16(p), nv = ld16(p + 2);
     const uint64_t base = row_start[pg];
-    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
-    const uint8_t *vals = p + (W == 4 ? 4 : 8);
     const bool dense = n == nv;
-    uint32_t running = 0;
-    for (uint32_t t0 = 0; t0 < n; t0 += kWG) {
-        uint32_t r = t0 + threadIdx.x;
-        bool inr = r < n;
-        bool ok = inr && (dense || page_bit(bm, r));
-        uint64_t m;
-        uint32_t tot;
-        uint32_t vi = running + tile_rank(ok, &m, &tot, lds);
-        // a bitmap with more set bits than n_vals is malformed: never read
-        // past the page's value area
-        bool rd = ok && vi < nv;
-        if (inr) {
-            if (W == 4) static_cast<int32_t *>(values)[base + r] = rd ? ld32(vals + 4 * (size_t)vi) : 0;
-            else static_cast<uint64_t *>(values)[base + r] = rd ? ld64(vals + 8 * (size_t)vi) : 0;
+    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+    const T *vals = reinterpret_cast<const T *>(p + (W == 4 ? 4 : 8));
+    T *out = static_cast<T *>(values) + base;
+    const uint64_t lt = lanemask_lt();
+    const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
+    const bool aligned = (base & 31) == 0;
+    uint32_t vbase = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kDecodeBatch) {
+        uint32_t okbits = 0;
+        uint32_t vi[kDecodeBatch];
+        if (dense) {
+#pragma unroll
+            for (uint32_t c = 0; c < kDecodeBatch; ++c) {
+                const uint32_t r = c0 + 64 * c + lane;
+                vi[c] = r;
+                okbits |= (uint32_t)(r < n) << c;
+            }
+        } else {
+            uint32_t byte[kDecodeBatch];
+#pragma unroll
+            for (uint32_t c = 0; c < kDecodeBatch; ++c) {
+                const uint32_t r = c0 + 64 * c + lane;
+                byte[c] = bm[(r < n ? r : nlast) >> 3];
+            }
+#pragma unroll
+            for (uint32_t c = 0; c < kDecodeBatch; ++c) {
+                const uint32_t r = c0 + 64 * c + lane;
+                const bool ok = (r < n) & (((byte[c] >> (r & 7)) & 1u) != 0);
+                const uint64_t vm = __ballot(ok);
+                vi[c] = vbase + (uint32_t)__popcll(vm & lt);
+                vbase += (uint32_t)__popcll(vm);
+                okbits |= (uint32_t)ok << c;
+            }
         }
-        if (lane_id() == 0) or_mask_bits(valid_bits, base + t0 + 64 * wave_id(), m);
-        running += tot;
+        T v[kDecodeBatch];
+#pragma unroll
+        for (uint32_t c = 0; c < kDecodeBatch; ++c) v[c] = vals[vi[c] < nv ? vi[c] : vlast];
+#pragma unroll
+        for (uint32_t c = 0; c < kDecodeBatch; ++c) {
+            const uint32_t r0 = c0 + 64 * c;
+            if (r0 >= n) break;
+            const uint32_t r = r0 + lane;
+            // a bitmap with more set bits than n_vals is malformed: never
+            // read past the page's value area
+            const bool rd = ((okbits >> c) & 1u) && vi[c] < nv;
+            if (r < n) out[r] = rd ? v[c] : T(0);
+            const uint64_t m = __ballot(rd);
+            if (aligned) {
+                if (lane < 2) {
+                    const uint32_t word = (uint32_t)(m >> (32 * lane));
+                    uint32_t *dst = valid_bits + ((base + r0) >> 5) + lane;
+                    if (r0 + 32 * lane + 32 <= n) *dst = word;
+                    else if (word) atomicOr(dst, word);
+                }
+            } else if (lane == 0) {
+                or_mask_bits(valid_bits, base + r0, m);
+            }
+        }
     }
 }
 
 void decode_fixed(const uint8_t *pages, uint32_t num_pages, const uint64_t *row_start, int width,
                   void *values, uint32_t *valid_bits, hipStream_t s) {
     if (!num_pages) return;
-    if (width == 4) k_decode_fixed<4><<<num_pages, kWG, 0, s>>>(pages, row_start, values, valid_bits);
-    else k_decode_fixed<8><<<num_pages, kWG, 0, s>>>(pages, row_start, values, valid_bits);
+    const uint32_t grid = (num_pages + kWG / 64 - 1) / (kWG / 64);
+    if (width == 4) k_decode_fixed<4><<<grid, kWG, 0, s>>>(pages, num_pages, row_start, values, valid_bits);
+    else k_decode_fixed<8><<<grid, kWG, 0, s>>>(pages, num_pages, row_start, values, valid_bits);
     CJ_HIP(hipGetLastError());
 }
 
