@@ -209,6 +209,104 @@ void probe_count_match(const ProbeArgs &a, uint64_t *counts, uint32_t *match_bit
     CJ_HIP(hipGetLastError());
 }
 
+// One wave per key page with matches: bitmap bytes (sparse pages), keys and
+// match words loaded in batches of 8 chunks of 64 rows, then the build ids
+// of the matched rows (one record load each), then the pairs, each chunk's
+// matched lanes writing a contiguous run.
+constexpr uint32_t kPairBatch = 8;
+__global__ __launch_bounds__(kWG) void k_direct_pairs(ProbeArgs a, const uint32_t *match_bits, const uint64_t *off,
+                                                      uint32_t *ob, uint32_t *op) {
+    const uint32_t lane = lane_id();
+    const KeySource &k = a.key;
+    const DirectTable &dt = a.dt;
+    const uint32_t pg = blockIdx.x * (kWG / 64) + wave_id();
+    if (pg >= k.num_pages) return;
+    uint64_t o = off[pg];
+    if (off[pg + 1] == o) return;
+    const uint8_t *p = k.pages + (size_t)pg * kPage;
+    const uint32_t n = ld16(p), nv = ld16(p + 2);
+    const bool dense = n == nv;
+    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
+    const int32_t *keys = reinterpret_cast<const int32_t *>(p + 4);
+    const uint64_t r0 = k.row_start[pg];
+    const uint32_t *mw = match_bits + (r0 >> 5);
+    const uint32_t msh = (uint32_t)(r0 & 31);
+    const uint64_t lt = lanemask_lt();
+    const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
+    uint32_t vbase = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kPairBatch) {
+        uint32_t vbits = 0, vi[kPairBatch];
+        if (dense) {
+#pragma unroll
+            for (uint32_t c = 0; c < kPairBatch; ++c) {
+                const uint32_t lr = c0 + 64 * c + lane;
+                vi[c] = lr;
+                vbits |= (uint32_t)(lr < n) << c;
+            }
+        } else {
+            uint32_t byte[kPairBatch];
+#pragma unroll
+            for (uint32_t c = 0; c < kPairBatch; ++c) {
+                const uint32_t lr = c0 + 64 * c + lane;
+                byte[c] = bm[(lr < n ? lr : nlast) >> 3];
+            }
+#pragma unroll
+            for (uint32_t c = 0; c < kPairBatch; ++c) {
+                const uint32_t lr = c0 + 64 * c + lane;
+                const bool v = (lr < n) & (((byte[c] >> (lr & 7)) & 1u) != 0);
+                const uint64_t vm = __ballot(v);
+                vi[c] = vbase + (uint32_t)__popcll(vm & lt);
+                vbase += (uint32_t)__popcll(vm);
+                vbits |= (uint32_t)v << c;
+            }
+        }
+        int32_t key[kPairBatch];
+        uint32_t word[kPairBatch];
+#pragma unroll
+        for (uint32_t c = 0; c < kPairBatch; ++c) {
+            const uint32_t lr = c0 + 64 * c + lane;
+            key[c] = keys[vi[c] < nv ? vi[c] : vlast];
+            word[c] = mw[(msh + (lr < n ? lr : nlast)) >> 5];
+        }
+        uint32_t mbits = 0, bid[kPairBatch];
+#pragma unroll
+        for (uint32_t c = 0; c < kPairBatch; ++c) {
+            const uint32_t lr = c0 + 64 * c + lane;
+            const int64_t so = (int64_t)key[c] - (int64_t)dt.min_key;
+            // a set match bit means the key is in range and present
+            const bool m = ((vbits >> c) & 1u) & (vi[c] < nv) & (((word[c] >> ((msh + lr) & 31)) & 1u) != 0) &
+                           (so >= 0) & ((uint64_t)so < dt.range);
+            bid[c] = m ? (uint32_t)so : 0u;
+            mbits |= (uint32_t)m << c;
+        }
+        if (dt.rid_offset >= 0) {
+#pragma unroll
+            for (uint32_t c = 0; c < kPairBatch; ++c)
+                bid[c] = *reinterpret_cast<const uint32_t *>(dt.records + (uint64_t)bid[c] * dt.stride +
+                                                             (uint32_t)dt.rid_offset);
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < kPairBatch; ++c) {
+            const bool m = (mbits >> c) & 1u;
+            const uint64_t mm = __ballot(m);
+            if (m) {
+                const uint64_t at = o + (uint64_t)__popcll(mm & lt);
+                if (ob) ob[at] = bid[c];
+                if (op) op[at] = (uint32_t)(r0 + c0 + 64 * c + lane);
+            }
+            o += (uint64_t)__popcll(mm);
+        }
+    }
+}
+
+void direct_pairs(const ProbeArgs &a, const uint32_t *match_bits, const uint64_t *off, uint32_t *ob, uint32_t *op,
+                  hipStream_t s) {
+    const uint32_t units = a.key.num_pages;
+    if (!units) return;
+    k_direct_pairs<<<(units + kWG / 64 - 1) / (kWG / 64), kWG, 0, s>>>(a, match_bits, off, ob, op);
+    CJ_HIP(hipGetLastError());
+}
+
 __global__ void k_key_page_out_pages(const uint64_t *counts, uint32_t pages, uint64_t *p4, uint64_t *p8) {
     const uint32_t i = blockIdx.x * kWG + threadIdx.x;
     if (i >= pages) return;

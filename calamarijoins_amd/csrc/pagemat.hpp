@@ -18,6 +18,14 @@ namespace cj {
 // for every probe row r with a match.
 void probe_count_match(const ProbeArgs &a, uint64_t *counts, uint32_t *match_bits, hipStream_t s);
 
+// Pair lists from the match bits of probe_count_match (direct mode, key
+// pages): the matched rows of key page pg, in row order, as (build id, probe
+// row) at off[pg] onwards; the build id is the build row id the direct map's
+// records carry (dt.rid_offset >= 0), else the record slot.  Either output
+// may be null.
+void direct_pairs(const ProbeArgs &a, const uint32_t *match_bits, const uint64_t *off, uint32_t *ob, uint32_t *op,
+                  hipStream_t s);
+
 // Output pages each key page produces for the build side:
 // p4[pg] = ceil(m / 1984), p8[pg] = ceil(m / 1007) for its m matches.
 void key_page_out_pages(const uint64_t *counts, uint32_t pages, uint64_t *p4, uint64_t *p8, hipStream_t s);
