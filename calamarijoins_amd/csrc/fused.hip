@@ -137,8 +137,8 @@ __global__ __launch_bounds__(kWG) void k_probe_count8(ProbeArgs a, uint64_t *cou
         for (uint32_t t0 = 0; t0 < n; t0 += kTile) {
             int32_t key[kRPT];
             uint32_t valid = unit_keys8<INTER>(k, u, t0, &running, key, lds32);
-            uint32_t m[kRPT], s[kRPT], e[kRPT];
-            lookup8<MODE>(valid, key, a.ht, nl, a.dt, m, s, e);
+            uint32_t m[kRPT], s[kRPT], e[kRPT], hit[kRPT];
+            lookup8<MODE>(valid, key, a.ht, nl, a.dt, m, s, e, hit);
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j) mine += m[j];
         }
@@ -207,13 +207,21 @@ __global__ __launch_bounds__(kWG) void k_probe_write8(ProbeArgs a, const uint64_
         for (uint32_t t0 = 0; t0 < n && base < end; t0 += kTile) {
             int32_t key[kRPT];
             uint32_t valid = unit_keys8<INTER>(k, u, t0, &running, key, lds32);
-            uint32_t m[kRPT], s[kRPT], e[kRPT], pre[kRPT];
-            lookup8<MODE>(valid, key, a.ht, nl, a.dt, m, s, e);
+            uint32_t m[kRPT], s[kRPT], e[kRPT], pre[kRPT], hit[kRPT];
+            lookup8<MODE>(valid, key, a.ht, nl, a.dt, m, s, e, hit);
             const uint32_t tot = scan_slots8(m, pre, lds32);
             // direct mode: the build id is the record slot, or the build row
             // id the record carries (all 8 loads issued together; slot 0 for
             // a key without a match)
-            uint32_t bid[kRPT];
+            uint32_t bid[kRPT], bid1[kRPT];
+            if (MODE == kModeHash) {
+                // row ids of the first two candidates, in one batch
+#pragma unroll
+                for (uint32_t j = 0; j < kRPT; ++j) {
+                    bid[j] = a.ht.rids[s[j]];
+                    bid1[j] = a.ht.rids[s[j] + 1 < e[j] ? s[j] + 1 : s[j]];
+                }
+            }
             if (MODE == kModeDirect) {
 #pragma unroll
                 for (uint32_t j = 0; j < kRPT; ++j) bid[j] = s[j];
@@ -244,7 +252,21 @@ __global__ __launch_bounds__(kWG) void k_probe_write8(ProbeArgs a, const uint64_
                             ++o;
                         }
                 } else {
-                    for (uint32_t i = s[j]; i < e[j]; ++i)
+                    if (hit[j] & 1u) {
+                        if (o < end) {
+                            if (ob) ob[o] = bid[j];
+                            if (op) op[o] = prid;
+                        }
+                        ++o;
+                    }
+                    if (hit[j] & 2u) {
+                        if (o < end) {
+                            if (ob) ob[o] = bid1[j];
+                            if (op) op[o] = prid;
+                        }
+                        ++o;
+                    }
+                    for (uint32_t i = s[j] + 2; i < e[j]; ++i)
                         if (a.ht.keys[i] == key[j]) {
                             if (o < end) {
                                 if (ob) ob[o] = a.ht.rids[i];

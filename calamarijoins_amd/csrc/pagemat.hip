@@ -43,7 +43,8 @@ __global__ __launch_bounds__(kWG) void k_probe_count_match(ProbeArgs a, uint64_t
             int32_t key[kRPT];
             uint32_t valid = load_keys8(p, n, nv, dense, t0, &running, key, lds32);
             uint32_t m[kRPT], s[kRPT], e[kRPT];
-            lookup8<MODE>(valid, key, a.ht, nl, a.dt, m, s, e);
+            uint32_t hit[kRPT];
+            lookup8<MODE>(valid, key, a.ht, nl, a.dt, m, s, e, hit);
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j) mine += m[j];
             const uint64_t g0 = r0 + t0;

@@ -115,12 +115,14 @@ __device__ __forceinline__ uint32_t load_keys8_inter(const KeySource &k, uint64_
 }
 
 // Lookups of 8 keys, all loads issued before any is used.  m[j] = matches of
-// key j; hash mode: [s[j], e[j]) is its directory range; direct mode: s[j]
-// is its record slot (0 when it has none).
+// key j; hash mode: [s[j], e[j]) is its directory range and bits 0 / 1 of
+// hit[j] say whether candidates s[j] / s[j] + 1 hold the key (the first two
+// candidates of every key are read in one batch; a longer range is walked
+// from s[j] + 2); direct mode: s[j] is its record slot (0 when it has none).
 template <int MODE>
 __device__ __forceinline__ void lookup8(uint32_t valid, const int32_t key[kRPT], const HashTableDev &ht,
                                         const NLKeys &nl, const DirectTable &dt, uint32_t m[kRPT], uint32_t s[kRPT],
-                                        uint32_t e[kRPT]) {
+                                        uint32_t e[kRPT], uint32_t hit[kRPT]) {
     if (MODE == kModeDirect) {
         uint32_t w[kRPT];
 #pragma unroll
@@ -166,19 +168,26 @@ __device__ __forceinline__ void lookup8(uint32_t valid, const int32_t key[kRPT],
         d[j] = ht.dir[slot[j]];
         dp[j] = ht.dir[slot[j] ? slot[j] - 1 : 0];
     }
-    int32_t k0[kRPT];
 #pragma unroll
     for (uint32_t j = 0; j < kRPT; ++j) {
-        const bool pass = ((valid >> j) & 1u) && (d[j] & tag[j]) == tag[j];
+        const bool pass = ((valid >> j) & 1u) & ((d[j] & tag[j]) == tag[j]);
         e[j] = pass ? (uint32_t)(d[j] >> 16) : 0;
         s[j] = pass && slot[j] ? (uint32_t)(dp[j] >> 16) : 0;
         if (s[j] >= e[j]) s[j] = e[j] = 0;
-        k0[j] = ht.keys[s[j]];  // first candidate (index 0 is always a tuple when the table is not empty)
+    }
+    // first two candidates (index 0 is always a tuple when the table is not
+    // empty; the second index is clamped to the first)
+    int32_t k0[kRPT], k1[kRPT];
+#pragma unroll
+    for (uint32_t j = 0; j < kRPT; ++j) {
+        k0[j] = ht.keys[s[j]];
+        k1[j] = ht.keys[s[j] + 1 < e[j] ? s[j] + 1 : s[j]];
     }
 #pragma unroll
     for (uint32_t j = 0; j < kRPT; ++j) {
-        uint32_t c = s[j] < e[j] && k0[j] == key[j];
-        for (uint32_t i = s[j] + 1; i < e[j]; ++i) c += ht.keys[i] == key[j];
+        hit[j] = (uint32_t)((s[j] < e[j]) & (k0[j] == key[j])) | ((uint32_t)((s[j] + 1 < e[j]) & (k1[j] == key[j])) << 1);
+        uint32_t c = (hit[j] & 1u) + (hit[j] >> 1);
+        for (uint32_t i = s[j] + 2; i < e[j]; ++i) c += ht.keys[i] == key[j];
         m[j] = c;
     }
 }
