@@ -189,6 +189,39 @@ __device__ __forceinline__ uint32_t scan_slots8(const uint32_t cnt[kRPT], uint32
     return run;
 }
 
+// Pairs of the candidates [from, to) of one key per lane that equal its key,
+// written at out onwards in candidate order by the whole wave, one lane's
+// range at a time (see count_rest).  Called by every lane.
+__device__ __forceinline__ void write_rest(const HashTableDev &ht, uint32_t from, uint32_t to, int32_t key,
+                                           uint64_t out, uint64_t end, uint32_t prid, uint32_t *ob, uint32_t *op) {
+    const uint32_t lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    uint64_t pend = __ballot(from < to);
+    while (pend) {
+        const int l = (int)__builtin_ctzll(pend);
+        const uint32_t lf = (uint32_t)__shfl((int)from, l, 64), lt_ = (uint32_t)__shfl((int)to, l, 64);
+        const int32_t lk = __shfl(key, l, 64);
+        const uint32_t lp = (uint32_t)__shfl((int)prid, l, 64);
+        uint64_t lo = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(out >> 32), l, 64) << 32) |
+                      (uint32_t)__shfl((int)(uint32_t)out, l, 64);
+        for (uint32_t i0 = lf; i0 < lt_; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            const uint32_t ic = i < lt_ ? i : lt_ - 1;
+            const bool eq = (i < lt_) & (ht.keys[ic] == lk);
+            const uint64_t em = __ballot(eq);
+            if (eq) {
+                const uint64_t at = lo + (uint64_t)__popcll(em & lt);
+                if (at < end) {
+                    if (ob) ob[at] = ht.rids[ic];
+                    if (op) op[at] = lp;
+                }
+            }
+            lo += (uint64_t)__popcll(em);
+        }
+        pend &= pend - 1;
+    }
+}
+
 template <int MODE, bool INTER>
 __global__ __launch_bounds__(kWG) void k_probe_write8(ProbeArgs a, const uint64_t *off, uint32_t *ob, uint32_t *op) {
     __shared__ uint32_t lds32[32];
@@ -266,15 +299,14 @@ __global__ __launch_bounds__(kWG) void k_probe_write8(ProbeArgs a, const uint64_
                         }
                         ++o;
                     }
-                    for (uint32_t i = s[j] + 2; i < e[j]; ++i)
-                        if (a.ht.keys[i] == key[j]) {
-                            if (o < end) {
-                                if (ob) ob[o] = a.ht.rids[i];
-                                if (op) op[o] = prid;
-                            }
-                            ++o;
-                        }
                 }
+            }
+            // the rest of every longer candidate range, by the whole wave
+            if (MODE == kModeHash) {
+#pragma unroll
+                for (uint32_t j = 0; j < kRPT; ++j)
+                    write_rest(a.ht, s[j] + 2, e[j], key[j], base + pre[j] + (hit[j] & 1u) + (hit[j] >> 1), end,
+                               (uint32_t)(r0 + t0 + kWG * j + threadIdx.x), ob, op);
             }
             base += tot;
         }

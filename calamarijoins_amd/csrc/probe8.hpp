@@ -114,11 +114,36 @@ __device__ __forceinline__ uint32_t load_keys8_inter(const KeySource &k, uint64_
     return valid;
 }
 
+// Candidates [from, to) of one key per lane that equal its key, counted by
+// the whole wave one lane's range at a time, 64 candidates a step: a skewed
+// build side gives a few keys thousands of candidates, which one lane would
+// walk alone while the rest of the machine waits.  Called by every lane.
+__device__ __forceinline__ uint32_t count_rest(const HashTableDev &ht, uint32_t from, uint32_t to, int32_t key) {
+    const uint32_t lane = lane_id();
+    uint32_t c = 0;
+    uint64_t pend = __ballot(from < to);
+    while (pend) {
+        const int l = (int)__builtin_ctzll(pend);
+        const uint32_t lf = (uint32_t)__shfl((int)from, l, 64), lt_ = (uint32_t)__shfl((int)to, l, 64);
+        const int32_t lk = __shfl(key, l, 64);
+        uint32_t cnt = 0;
+        for (uint32_t i0 = lf; i0 < lt_; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            const bool eq = (i < lt_) & (ht.keys[i < lt_ ? i : lt_ - 1] == lk);
+            cnt += (uint32_t)__popcll(__ballot(eq));
+        }
+        if (lane == (uint32_t)l) c += cnt;
+        pend &= pend - 1;
+    }
+    return c;
+}
+
 // Lookups of 8 keys, all loads issued before any is used.  m[j] = matches of
 // key j; hash mode: [s[j], e[j]) is its directory range and bits 0 / 1 of
 // hit[j] say whether candidates s[j] / s[j] + 1 hold the key (the first two
-// candidates of every key are read in one batch; a longer range is walked
-// from s[j] + 2); direct mode: s[j] is its record slot (0 when it has none).
+// candidates of every key are read in one batch; the rest of a longer range
+// is counted by the whole wave, see count_rest); direct mode: s[j] is its
+// record slot (0 when it has none).  Called by every lane of the wave.
 template <int MODE>
 __device__ __forceinline__ void lookup8(uint32_t valid, const int32_t key[kRPT], const HashTableDev &ht,
                                         const NLKeys &nl, const DirectTable &dt, uint32_t m[kRPT], uint32_t s[kRPT],
@@ -186,9 +211,7 @@ __device__ __forceinline__ void lookup8(uint32_t valid, const int32_t key[kRPT],
 #pragma unroll
     for (uint32_t j = 0; j < kRPT; ++j) {
         hit[j] = (uint32_t)((s[j] < e[j]) & (k0[j] == key[j])) | ((uint32_t)((s[j] + 1 < e[j]) & (k1[j] == key[j])) << 1);
-        uint32_t c = (hit[j] & 1u) + (hit[j] >> 1);
-        for (uint32_t i = s[j] + 2; i < e[j]; ++i) c += ht.keys[i] == key[j];
-        m[j] = c;
+        m[j] = (hit[j] & 1u) + (hit[j] >> 1) + count_rest(ht, s[j] + 2, e[j], key[j]);
     }
 }
 
