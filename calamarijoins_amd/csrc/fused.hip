@@ -190,13 +190,25 @@ __device__ __forceinline__ uint32_t scan_slots8(const uint32_t cnt[kRPT], uint32
 }
 
 // Pairs of the candidates [from, to) of one key per lane that equal its key,
-// written at out onwards in candidate order by the whole wave, one lane's
-// range at a time (see count_rest).  Called by every lane.
+// written at out onwards in candidate order: by its own lane for a short
+// range, else by the whole wave, one lane's range at a time (see
+// count_rest).  Called by every lane.
 __device__ __forceinline__ void write_rest(const HashTableDev &ht, uint32_t from, uint32_t to, int32_t key,
                                            uint64_t out, uint64_t end, uint32_t prid, uint32_t *ob, uint32_t *op) {
     const uint32_t lane = lane_id();
     const uint64_t lt = lanemask_lt();
-    uint64_t pend = __ballot(from < to);
+    if (to <= from + kShortRest) {
+        uint64_t o = out;
+        for (uint32_t i = from; i < to; ++i)
+            if (ht.keys[i] == key) {
+                if (o < end) {
+                    if (ob) ob[o] = ht.rids[i];
+                    if (op) op[o] = prid;
+                }
+                ++o;
+            }
+    }
+    uint64_t pend = __ballot(to > from + kShortRest);
     while (pend) {
         const int l = (int)__builtin_ctzll(pend);
         const uint32_t lf = (uint32_t)__shfl((int)from, l, 64), lt_ = (uint32_t)__shfl((int)to, l, 64);

@@ -114,14 +114,18 @@ __device__ __forceinline__ uint32_t load_keys8_inter(const KeySource &k, uint64_
     return valid;
 }
 
-// Candidates [from, to) of one key per lane that equal its key, counted by
-// the whole wave one lane's range at a time, 64 candidates a step: a skewed
+// Candidates [from, to) of one key per lane that equal its key.  A short
+// range (at most kShortRest) is walked by its own lane; longer ones by the
+// whole wave, one lane's range at a time, 64 candidates a step: a skewed
 // build side gives a few keys thousands of candidates, which one lane would
 // walk alone while the rest of the machine waits.  Called by every lane.
+constexpr uint32_t kShortRest = 8;
 __device__ __forceinline__ uint32_t count_rest(const HashTableDev &ht, uint32_t from, uint32_t to, int32_t key) {
     const uint32_t lane = lane_id();
     uint32_t c = 0;
-    uint64_t pend = __ballot(from < to);
+    if (to <= from + kShortRest)
+        for (uint32_t i = from; i < to; ++i) c += ht.keys[i] == key;
+    uint64_t pend = __ballot(to > from + kShortRest);
     while (pend) {
         const int l = (int)__builtin_ctzll(pend);
         const uint32_t lf = (uint32_t)__shfl((int)from, l, 64), lt_ = (uint32_t)__shfl((int)to, l, 64);
