@@ -134,35 +134,42 @@ void build_partition_scatter(const KeySource &k, const HashGeom &g, uint64_t *pa
 
 // -------------------------------------------- per-partition build in LDS
 // build_partition (hashtable.h:197-246): count per slot, exclusive prefix,
-// scatter keys/row ids, OR Bloom tags, write (end << 16) | tag.
-__global__ __launch_bounds__(kWG) void k_build_partitions(HashGeom g, const uint64_t *part_start,
-                                                          const uint64_t *part_end, const int32_t *tup_keys,
-                                                          const uint32_t *tup_rids, uint64_t *dir, int32_t *keys,
-                                                          uint32_t *rids) {
+// scatter keys/row ids, OR Bloom tags, write (end << 16) | tag.  1024
+// threads per partition: a hot key of a skewed build side fills one
+// partition, whose workgroup sets the kernel's length.
+constexpr uint32_t kBuildWG = 1024;
+
+__global__ __launch_bounds__(kBuildWG) void k_build_partitions(HashGeom g, const uint64_t *part_start,
+                                                               const uint64_t *part_end, const int32_t *tup_keys,
+                                                               const uint32_t *tup_rids, uint64_t *dir,
+                                                               int32_t *keys, uint32_t *rids) {
     extern __shared__ uint32_t sm[];
     const uint32_t S = 1u << g.log_part;
     uint32_t *cnt = sm, *tag = sm + S;
-    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t wsum[kBuildWG / 64];
     const uint32_t p = blockIdx.x;
     const uint64_t a = part_start[p], e = part_end[p];
-    for (uint32_t i = threadIdx.x; i < S; i += kWG) { cnt[i] = 0; tag[i] = 0; }
+    for (uint32_t i = threadIdx.x; i < S; i += kBuildWG) { cnt[i] = 0; tag[i] = 0; }
     __syncthreads();
     const uint32_t smask = S - 1;
-    for (uint64_t i = a + threadIdx.x; i < e; i += kWG) {
+    for (uint64_t i = a + threadIdx.x; i < e; i += kBuildWG) {
         uint64_t h = hash_key(tup_keys[i]);
         uint32_t ls = (uint32_t)slot_of(h, g.log_slots) & smask;
         atomicAdd(&cnt[ls], 1u);
         atomicOr(&tag[ls], (uint32_t)c_bloom.t[h & 0x7FF]);
     }
     __syncthreads();
-    // exclusive prefix over the S slot counts: each thread owns S/256 slots
-    const uint32_t per = (S + kWG - 1) / kWG;
+    // exclusive prefix over the S slot counts: each thread owns S/1024 slots
+    const uint32_t per = (S + kBuildWG - 1) / kBuildWG;
     const uint32_t s0 = threadIdx.x * per;
     uint32_t local = 0;
     for (uint32_t i = 0; i < per; ++i)
         if (s0 + i < S) local += cnt[s0 + i];
-    uint32_t tot;
-    uint32_t pre = block_excl_scan<uint32_t>(local, &tot, lds4);
+    const uint32_t inc = wave_incl_scan<uint32_t>(local);
+    if (lane_id() == 63) wsum[wave_id()] = inc;
+    __syncthreads();
+    uint32_t pre = inc - local;
+    for (uint32_t w = 0; w < wave_id(); ++w) pre += wsum[w];
     for (uint32_t i = 0; i < per; ++i)
         if (s0 + i < S) {
             uint32_t c = cnt[s0 + i];
@@ -170,7 +177,7 @@ __global__ __launch_bounds__(kWG) void k_build_partitions(HashGeom g, const uint
             pre += c;
         }
     __syncthreads();
-    for (uint64_t i = a + threadIdx.x; i < e; i += kWG) {
+    for (uint64_t i = a + threadIdx.x; i < e; i += kBuildWG) {
         int32_t key = tup_keys[i];
         uint32_t ls = (uint32_t)slot_of(hash_key(key), g.log_slots) & smask;
         uint64_t pos = a + atomicAdd(&cnt[ls], 1u);
@@ -179,14 +186,14 @@ __global__ __launch_bounds__(kWG) void k_build_partitions(HashGeom g, const uint
     }
     __syncthreads();
     uint64_t *d = dir + (uint64_t)p * S;
-    for (uint32_t i = threadIdx.x; i < S; i += kWG) d[i] = ((a + cnt[i]) << 16) | (uint64_t)(tag[i] & 0xFFFF);
+    for (uint32_t i = threadIdx.x; i < S; i += kBuildWG) d[i] = ((a + cnt[i]) << 16) | (uint64_t)(tag[i] & 0xFFFF);
 }
 
 void build_partitions(const HashGeom &g, const uint64_t *part_off, const int32_t *tup_keys, const uint32_t *tup_rids,
                       uint64_t *dir, int32_t *keys, uint32_t *rids, hipStream_t s) {
     size_t sh = 2 * (size_t)(1u << g.log_part) * sizeof(uint32_t);
-    k_build_partitions<<<g.num_partitions, kWG, sh, s>>>(g, part_off, part_off + 1, tup_keys, tup_rids, dir, keys,
-                                                         rids);
+    k_build_partitions<<<g.num_partitions, kBuildWG, sh, s>>>(g, part_off, part_off + 1, tup_keys, tup_rids, dir,
+                                                              keys, rids);
     CJ_HIP(hipGetLastError());
 }
 
