@@ -258,14 +258,12 @@ __global__ __launch_bounds__(kWG) void k_probe_write8(ProbeArgs a, const uint64_
             // direct mode: the build id is the record slot, or the build row
             // id the record carries (all 8 loads issued together; slot 0 for
             // a key without a match)
-            uint32_t bid[kRPT], bid1[kRPT];
+            uint32_t bid[kRPT];
             if (MODE == kModeHash) {
-                // row ids of the first two candidates, in one batch
+                // row ids of the first candidates, in one batch (a match on
+                // the second candidate is rarer and reads its own)
 #pragma unroll
-                for (uint32_t j = 0; j < kRPT; ++j) {
-                    bid[j] = a.ht.rids[s[j]];
-                    bid1[j] = a.ht.rids[s[j] + 1 < e[j] ? s[j] + 1 : s[j]];
-                }
+                for (uint32_t j = 0; j < kRPT; ++j) bid[j] = a.ht.rids[s[j]];
             }
             if (MODE == kModeDirect) {
 #pragma unroll
@@ -306,7 +304,7 @@ __global__ __launch_bounds__(kWG) void k_probe_write8(ProbeArgs a, const uint64_
                     }
                     if (hit[j] & 2u) {
                         if (o < end) {
-                            if (ob) ob[o] = bid1[j];
+                            if (ob) ob[o] = a.ht.rids[s[j] + 1];
                             if (op) op[o] = prid;
                         }
                         ++o;
