@@ -96,31 +96,6 @@ __device__ __forceinline__ uint32_t tile_rank(bool pred, uint64_t *mask_out, uin
     return pre + (uint32_t)__popcll(m & lanemask_lt());
 }
 
-// Bit k of the result is `bit` of the k-th lane (in lane order) whose `sel`
-// is set: the bits of the selected lanes, packed.  One ds_permute (each lane
-// pushes its bit to a distinct lane: selected lanes to their rank,
-// unselected ones after them) and one ballot.
-__device__ __forceinline__ uint64_t pack_selected(bool sel, bool bit, uint64_t sel_mask) {
-    const uint32_t lane = lane_id();
-    const uint32_t rank = (uint32_t)__popcll(sel_mask & lanemask_lt());
-    const uint32_t tm = (uint32_t)__popcll(sel_mask);
-    const uint32_t dst = sel ? rank : tm + (lane - rank);
-    const int got = __builtin_amdgcn_ds_permute((int)(dst << 2), bit ? 1 : 0);
-    return __ballot(lane < tm && got != 0);
-}
-
-// OR the low `nbits` (<= 64) bits of `bits` into an LDS bit array at bit
-// `pos`: at most three atomics without return, so the caller never waits.
-__device__ __forceinline__ void or_bits_at(uint32_t *words, uint32_t pos, uint64_t bits, uint32_t nbits) {
-    if (nbits < 64) bits &= (1ull << nbits) - 1ull;
-    if (!bits) return;
-    const uint32_t w = pos >> 5, o = pos & 31;
-    const uint64_t lo = bits << o;
-    atomicOr(&words[w], (uint32_t)lo);
-    if (o + nbits > 32) atomicOr(&words[w + 1], (uint32_t)(lo >> 32));
-    if (o + nbits > 64) atomicOr(&words[w + 2], (uint32_t)(bits >> (64 - o)));
-}
-
 // OR a wave's 64-row validity mask into a global bitmap starting at row g0.
 __device__ __forceinline__ void or_mask_bits(uint32_t *bits, uint64_t g0, uint64_t m) {
     if (!m) return;
