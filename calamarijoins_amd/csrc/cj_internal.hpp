@@ -147,9 +147,17 @@ struct VarcharSource {
     const uint32_t *valid_bits;
     const uint32_t *rids;
 };
-constexpr uint32_t kVarcharChunk = 2048;  // output rows planned per workgroup
-void varchar_plan(const VarcharSource &src, uint64_t rows, uint64_t *chunk_pages, hipStream_t s);
-void varchar_write(const VarcharSource &src, uint64_t rows, const uint64_t *chunk_page_off,
+constexpr uint32_t kVarcharChunk = 2048;  // most output rows planned per workgroup
+// Rows per chunk for a VARCHAR result of `rows` rows: 2048, halved (down to
+// 256) while that leaves fewer than 256 chunks, so small results still
+// spread over the machine (each chunk starts a new page).
+inline uint32_t varchar_chunk_rows(uint64_t rows) {
+    uint32_t c = kVarcharChunk;
+    while (c > 256 && (rows + c - 1) / c < 256) c /= 2;
+    return c;
+}
+void varchar_plan(const VarcharSource &src, uint64_t rows, uint32_t chunk, uint64_t *chunk_pages, hipStream_t s);
+void varchar_write(const VarcharSource &src, uint64_t rows, uint32_t chunk, const uint64_t *chunk_page_off,
                    uint8_t *out_pages, hipStream_t s);
 // Adds a result column's algorithmic bytes (non-NULL value bytes, VARCHAR
 // chars + 2 per value, bitmap bytes) to *acc.

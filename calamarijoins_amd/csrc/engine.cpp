@@ -1211,20 +1211,21 @@ std::pair<uint8_t *, uint32_t> Exec::materialize_column(DevColumn *c, int32_t ty
     uint8_t *pages;
     uint32_t np;
     if (type == CJ_VARCHAR) {
-        uint64_t chunks = (total + kVarcharChunk - 1) / kVarcharChunk;
+        const uint32_t chunk = varchar_chunk_rows(total);
+        uint64_t chunks = (total + chunk - 1) / chunk;
         uint64_t *cp = tmp<uint64_t>(chunks + 1);
         uint64_t *co = tmp<uint64_t>(chunks + 1);
         VarcharSource vs{c->pages, static_cast<const uint64_t *>(v.values), v.valid, rids};
         {
             TimedScope ts(timer, "varchar_plan", 12 * total);
-            varchar_plan(vs, total, cp, st);
+            varchar_plan(vs, total, chunk, cp, st);
         }
         scan(cp, co, chunks);
         uint64_t npg = read_u64(co + chunks);
         np = (uint32_t)npg;
         pages = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
         TimedScope ts(timer, "varchar_write", (uint64_t)np * kPage);
-        varchar_write(vs, total, co, pages, st);
+        varchar_write(vs, total, chunk, co, pages, st);
     } else {
         uint32_t R = type == CJ_INT32 ? kRowsI32 : kRowsI64;
         np = (uint32_t)((total + R - 1) / R);

@@ -8,9 +8,9 @@
 // output row j simply j / R: one workgroup per output page, a wave ballot
 // ranks the non-NULL rows, values are packed at the aligned offset and the
 // NULL bitmap is written at the page end.  VARCHAR pages are packed greedily
-// with the canonical capacity rule inside independent 2048-row chunks (plan
-// pass counts pages, a scan places them, the write pass fills them); long
-// strings are emitted as 0xffff/0xfffe page runs copied from the source,
+// with the canonical capacity rule inside independent chunks of up to 2048
+// rows (plan pass counts pages, a scan places them, the write pass fills
+// them); long strings are emitted as 0xffff/0xfffe page runs copied from the source,
 // as copy_long_string_pages does (page_builders.h:344-360).
 #include "cj_internal.hpp"
 #include "device_util.hpp"
@@ -204,10 +204,11 @@ __device__ uint32_t pack_chunk(const uint32_t *info, uint32_t cnt, uint32_t *row
     return prel;
 }
 
-__global__ __launch_bounds__(kWG) void k_varchar_plan(VarcharSource src, uint64_t rows, uint64_t *chunk_pages) {
+__global__ __launch_bounds__(kWG) void k_varchar_plan(VarcharSource src, uint64_t rows, uint32_t chunk,
+                                                      uint64_t *chunk_pages) {
     __shared__ uint32_t info[kVarcharChunk];
-    const uint64_t r0 = (uint64_t)blockIdx.x * kVarcharChunk;
-    const uint32_t cnt = (uint32_t)(rows - r0 < kVarcharChunk ? rows - r0 : kVarcharChunk);
+    const uint64_t r0 = (uint64_t)blockIdx.x * chunk;
+    const uint32_t cnt = (uint32_t)(rows - r0 < chunk ? rows - r0 : chunk);
     load_rows(src, r0, cnt, info, nullptr);
     __syncthreads();
     if (threadIdx.x >= 64) return;
@@ -215,14 +216,15 @@ __global__ __launch_bounds__(kWG) void k_varchar_plan(VarcharSource src, uint64_
     if (threadIdx.x == 0) chunk_pages[blockIdx.x] = pages;
 }
 
-void varchar_plan(const VarcharSource &src, uint64_t rows, uint64_t *chunk_pages, hipStream_t s) {
+void varchar_plan(const VarcharSource &src, uint64_t rows, uint32_t chunk, uint64_t *chunk_pages, hipStream_t s) {
     if (!rows) return;
-    unsigned chunks = (unsigned)((rows + kVarcharChunk - 1) / kVarcharChunk);
-    k_varchar_plan<<<chunks, kWG, 0, s>>>(src, rows, chunk_pages);
+    if (chunk == 0 || chunk > kVarcharChunk) throw InvalidError("bad VARCHAR chunk size");
+    unsigned chunks = (unsigned)((rows + chunk - 1) / chunk);
+    k_varchar_plan<<<chunks, kWG, 0, s>>>(src, rows, chunk, chunk_pages);
     CJ_HIP(hipGetLastError());
 }
 
-__global__ __launch_bounds__(kWG) void k_varchar_write(VarcharSource src, uint64_t rows,
+__global__ __launch_bounds__(kWG) void k_varchar_write(VarcharSource src, uint64_t rows, uint32_t chunk,
                                                        const uint64_t *chunk_page_off, uint8_t *out) {
     __shared__ uint32_t info[kVarcharChunk];
     __shared__ uint32_t row_page[kVarcharChunk];  // short rows: page-table index; long: output page (rel)
@@ -232,8 +234,8 @@ __global__ __launch_bounds__(kWG) void k_varchar_write(VarcharSource src, uint64
     __shared__ uint64_t rref[kVarcharChunk];
     __shared__ uint16_t long_rows[kVarcharChunk];
     __shared__ uint32_t n_ptab, n_long;
-    const uint64_t r0 = (uint64_t)blockIdx.x * kVarcharChunk;
-    const uint32_t cnt = (uint32_t)(rows - r0 < kVarcharChunk ? rows - r0 : kVarcharChunk);
+    const uint64_t r0 = (uint64_t)blockIdx.x * chunk;
+    const uint32_t cnt = (uint32_t)(rows - r0 < chunk ? rows - r0 : chunk);
     uint8_t *const base = out + chunk_page_off[blockIdx.x] * (uint64_t)kPage;
     load_rows(src, r0, cnt, info, rref);
     __syncthreads();
@@ -295,11 +297,12 @@ __global__ __launch_bounds__(kWG) void k_varchar_write(VarcharSource src, uint64
     }
 }
 
-void varchar_write(const VarcharSource &src, uint64_t rows, const uint64_t *chunk_page_off, uint8_t *out_pages,
-                   hipStream_t s) {
+void varchar_write(const VarcharSource &src, uint64_t rows, uint32_t chunk, const uint64_t *chunk_page_off,
+                   uint8_t *out_pages, hipStream_t s) {
     if (!rows) return;
-    unsigned chunks = (unsigned)((rows + kVarcharChunk - 1) / kVarcharChunk);
-    k_varchar_write<<<chunks, kWG, 0, s>>>(src, rows, chunk_page_off, out_pages);
+    if (chunk == 0 || chunk > kVarcharChunk) throw InvalidError("bad VARCHAR chunk size");
+    unsigned chunks = (unsigned)((rows + chunk - 1) / chunk);
+    k_varchar_write<<<chunks, kWG, 0, s>>>(src, rows, chunk, chunk_page_off, out_pages);
     CJ_HIP(hipGetLastError());
 }
 
