@@ -108,7 +108,20 @@ struct KeySource {
     const uint32_t *valid_bits;
     const uint32_t *rids;        // row-id list of the intermediate (nullptr = identity)
     uint64_t rows;               // rows of the intermediate
+    // values == nullptr: the base column is read in place from its pages
+    // (INT32, no NULLs, dense_rows rows on every page but the last)
+    const uint8_t *dense_pages;
+    uint32_t dense_rows;
 };
+
+// Value of row `rid` of an INT32 column without NULLs whose pages all hold
+// `rows_per_page` rows (the last may hold fewer): value rid % R of page
+// rid / R, at its aligned offset.
+__device__ __forceinline__ int32_t dense_page_value(const uint8_t *pages, uint32_t rows_per_page, uint32_t rid) {
+    const uint32_t pg = rid / rows_per_page;
+    const uint32_t i = rid - pg * rows_per_page;
+    return *reinterpret_cast<const int32_t *>(pages + (size_t)pg * kPage + 4 + 4 * (size_t)i);
+}
 constexpr uint32_t kInterChunk = 2048;  // rows of a row-id list per work unit
 // work units (pages / row chunks)
 __host__ __device__ inline uint32_t key_source_units(const KeySource &k) {
@@ -137,6 +150,8 @@ struct FixedSource {
     const void *values;        // dense decoded base column
     const uint32_t *valid_bits;
     const uint32_t *rids;      // final row ids into the base column (nullptr = identity)
+    const uint8_t *dense_pages;  // values == nullptr: an INT32 column read in place (KeySource)
+    uint32_t dense_rows;
 };
 void materialize_fixed(const FixedSource &src, uint64_t rows, uint32_t rows_per_page,
                        uint8_t *out_pages, hipStream_t s);

@@ -269,6 +269,10 @@ void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s
     uint64_t *bs = rs.data() + d.num_pages + 1;
     uint64_t run = 0, bytes = 0;
     uint16_t prev = 0;
+    // INT32 pages without NULLs, all of one row count but a shorter last
+    // one, are read in place instead of decoded (dense_rows)
+    bool dense = d.type == CJ_INT32 && d.num_pages > 0, dense_short = false;
+    uint32_t dense_rows = 0;
     c.pages = static_cast<uint8_t *>(pool.alloc((size_t)d.num_pages * kPage + 16));
     c.row_start = dalloc<uint64_t>(2 * (size_t)d.num_pages + 2);
     c.byte_start = nullptr;  // lives inside the row_start block
@@ -321,6 +325,12 @@ void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s
             const uint8_t *p = dst + (size_t)i * kPage;
             validate_page(p, d.type, p0 + i, prev);
             std::memcpy(&prev, p, 2);
+            uint16_t nvals;
+            std::memcpy(&nvals, p + 2, 2);
+            if (prev != nvals || prev == 0 || prev >= 0xfffe || dense_short) dense = false;
+            if (dense_rows == 0) dense_rows = prev;
+            else if (prev > dense_rows) dense = false;
+            else if (prev < dense_rows) dense_short = true;
             rs[p0 + i] = run;
             bs[p0 + i] = bytes;
             run += host_rows_of(p);
@@ -338,6 +348,7 @@ void Context::upload_column(DevColumn &c, const cj_column_desc &d, hipStream_t s
     bs[d.num_pages] = bytes;
     c.num_rows = run;
     c.bitmap_bytes = bytes;
+    c.dense_rows = dense ? dense_rows : 0;
     CJ_HIP(hipStreamSynchronize(s));
     std::memcpy(stage[0], rs.data(), rs.size() * 8);
     CJ_HIP(hipMemcpyAsync(c.row_start, stage[0], rs.size() * 8, hipMemcpyHostToDevice, s));
@@ -645,10 +656,15 @@ class Exec {
             k.num_pages = c->num_pages;
             k.row_start = c->row_start;
         } else {
-            ColView v = view(c);
             k.mode = KEY_INTER;
-            k.values = static_cast<const int32_t *>(v.values);
-            k.valid_bits = v.valid;
+            if (c->dense_rows) {  // read in place: no decode
+                k.dense_pages = c->pages;
+                k.dense_rows = c->dense_rows;
+            } else {
+                ColView v = view(c);
+                k.values = static_cast<const int32_t *>(v.values);
+                k.valid_bits = v.valid;
+            }
             k.rids = r.slots[cr.slot].rids;
             k.rows = r.rows;
         }
@@ -1207,7 +1223,9 @@ void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, u
 
 std::pair<uint8_t *, uint32_t> Exec::materialize_column(DevColumn *c, int32_t type, const uint32_t *rids,
                                                         uint64_t total, bool int32_quirk) {
-    ColView v = view(c);
+    const bool in_place = type == CJ_INT32 && c->type == CJ_INT32 && c->dense_rows;
+    ColView v;
+    if (!in_place) v = view(c);
     uint8_t *pages;
     uint32_t np;
     if (type == CJ_VARCHAR) {
@@ -1230,7 +1248,8 @@ std::pair<uint8_t *, uint32_t> Exec::materialize_column(DevColumn *c, int32_t ty
         uint32_t R = type == CJ_INT32 ? kRowsI32 : kRowsI64;
         np = (uint32_t)((total + R - 1) / R);
         pages = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
-        FixedSource fs{type == CJ_INT32 ? 4 : 8, type == CJ_INT32 && int32_quirk, v.values, v.valid, rids};
+        FixedSource fs{type == CJ_INT32 ? 4 : 8, type == CJ_INT32 && int32_quirk, v.values, v.valid, rids,
+                       in_place ? c->pages : nullptr, in_place ? c->dense_rows : 0u};
         TimedScope ts(timer, type == CJ_INT32 ? "materialize_i32" : "materialize_i64", fixed_mat_bytes(total, type));
         materialize_fixed(fs, total, R, pages, st);
     }

@@ -65,6 +65,8 @@ struct DevColumn {
     uint16_t *byte_prefix = nullptr;  // page index: non-NULL rows before each bitmap byte
     uint64_t bitmap_bytes = 0;        // sum over pages of ceil(rows / 8)
     uint32_t max_page_rows = 0;       // largest n_rows of a (non long-string) page
+    uint32_t dense_rows = 0;          // INT32 without NULLs, this many rows on every page but
+                                      // a shorter last one: rows are read in place (0: no)
     bool loaded = false;        // pages + row_start present
     std::mutex mu;              // guards lazy decode
 };

@@ -60,7 +60,8 @@ __device__ __forceinline__ void for_each_key(const KeySource &k, uint32_t unit, 
             int32_t key = kI32Null;
             if (inr) {
                 uint32_t rid = k.rids ? k.rids[j] : (uint32_t)j;
-                if (valid_bit(k.valid_bits, rid)) key = k.values[rid];
+                if (!k.values) key = dense_page_value(k.dense_pages, k.dense_rows, rid);
+                else if (valid_bit(k.valid_bits, rid)) key = k.values[rid];
             }
             bool ok = inr && (want_nulls || key != kI32Null);
             fn(inr, ok, key, (uint32_t)j);
@@ -222,7 +223,8 @@ __global__ void k_nl_collect(KeySource k, NLKeys *out) {
             uint64_t a = (uint64_t)u * kInterChunk, e = a + kInterChunk < k.rows ? a + kInterChunk : k.rows;
             for (uint64_t j = a; j < e && r.count < 8; ++j) {
                 uint32_t rid = k.rids ? k.rids[j] : (uint32_t)j;
-                int32_t key = valid_bit(k.valid_bits, rid) ? k.values[rid] : kI32Null;
+                int32_t key = !k.values ? dense_page_value(k.dense_pages, k.dense_rows, rid)
+                                        : (valid_bit(k.valid_bits, rid) ? k.values[rid] : kI32Null);
                 if (key == kI32Null) continue;
                 r.keys[r.count] = key;
                 r.rids[r.count] = (uint32_t)j;

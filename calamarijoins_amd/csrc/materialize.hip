@@ -35,9 +35,10 @@ __global__ __launch_bounds__(kWG) void k_mat_fixed(FixedSource src, uint64_t row
         uint64_t v = 0;
         if (i < cnt) {
             uint64_t rid = src.rids ? src.rids[r0 + i] : r0 + i;
-            ok = valid_bit(src.valid_bits, rid);
+            ok = !src.values || valid_bit(src.valid_bits, rid);
             if (W == 4) {
-                int32_t x = static_cast<const int32_t *>(src.values)[rid];
+                int32_t x = src.values ? static_cast<const int32_t *>(src.values)[rid]
+                                       : dense_page_value(src.dense_pages, src.dense_rows, (uint32_t)rid);
                 // value_t NULL sentinel: an INT32 value equal to INT32_MIN is
                 // emitted as NULL (page_builders.h:123, intermediate.h:58-62).
                 if (src.is_int32 && x == kI32Null) ok = false;

@@ -100,10 +100,18 @@ __device__ __forceinline__ uint32_t load_keys8_inter(const KeySource &k, uint64_
         const uint64_t row = a + (r < n ? r : n - 1);
         rid[j] = k.rids ? k.rids[row] : row;
     }
+    if (k.values) {
 #pragma unroll
-    for (uint32_t j = 0; j < kRPT; ++j) vw[j] = k.valid_bits[rid[j] >> 5];
+        for (uint32_t j = 0; j < kRPT; ++j) vw[j] = k.valid_bits[rid[j] >> 5];
 #pragma unroll
-    for (uint32_t j = 0; j < kRPT; ++j) key[j] = k.values[rid[j]];
+        for (uint32_t j = 0; j < kRPT; ++j) key[j] = k.values[rid[j]];
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j) {
+            vw[j] = ~0u;
+            key[j] = dense_page_value(k.dense_pages, k.dense_rows, (uint32_t)rid[j]);
+        }
+    }
     uint32_t valid = 0;
 #pragma unroll
     for (uint32_t j = 0; j < kRPT; ++j) {
