@@ -23,15 +23,31 @@
 namespace cj {
 
 // Each thread reduces 16 consecutive keys (four 16-byte loads, half a
-// validity word); a workgroup reduces through shuffles and LDS and writes
+// validity word) of a decoded column, or 16 rows of a column read in place
+// from its pages (rows kWG apart, so a wave's loads stay contiguous); a workgroup reduces through shuffles and LDS and writes
 // its partial to its own slot; one workgroup combines the partials.  (Many
 // workgroups updating one address with device atomics serialize.)
-__global__ __launch_bounds__(kWG) void k_key_minmax(const int32_t *keys, const uint32_t *valid, uint64_t n,
+__global__ __launch_bounds__(kWG) void k_key_minmax(const int32_t *keys, const uint32_t *valid,
+                                                    const uint8_t *key_pages, uint32_t key_rows, uint64_t n,
                                                     int32_t *part) {
     __shared__ int32_t red[2][kWG / 64];
     int32_t lo = INT32_MAX, hi = INT32_MIN;
     const uint64_t groups = (n + 15) / 16;
-    for (uint64_t g = blockIdx.x * (uint64_t)kWG + threadIdx.x; g < groups; g += (uint64_t)gridDim.x * kWG) {
+    // in place (every row keyed): a workgroup takes 16 * kWG rows, lane-contiguous
+    for (uint64_t c = blockIdx.x; !keys && c * 16 * kWG < n; c += gridDim.x) {
+        int32_t k[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint64_t r = c * 16 * kWG + (uint64_t)i * kWG + threadIdx.x;
+            k[i] = dense_page_value(key_pages, key_rows, (uint32_t)(r < n ? r : n - 1));
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            lo = min(lo, k[i]);
+            hi = max(hi, k[i]);
+        }
+    }
+    for (uint64_t g = blockIdx.x * (uint64_t)kWG + threadIdx.x; keys && g < groups; g += (uint64_t)gridDim.x * kWG) {
         const uint64_t r0 = g * 16;
         const uint32_t vw = valid[r0 >> 5] >> (r0 & 16);
         if (r0 + 16 <= n) {
@@ -97,12 +113,13 @@ __global__ __launch_bounds__(kWG) void k_minmax_final(const int32_t *part, uint3
     }
 }
 
-void key_minmax(const int32_t *keys, const uint32_t *valid, uint64_t n, int32_t *mm, int32_t *part,
+void key_minmax(const int32_t *keys, const uint32_t *valid, const uint8_t *key_pages, uint32_t key_rows, uint64_t n,
+                int32_t *mm, int32_t *part,
                 hipStream_t s) {
     const uint64_t groups = (n + 15) / 16;
     uint64_t g = (groups + kWG - 1) / kWG;
     g = g < kMinmaxParts ? g : kMinmaxParts;
-    if (g) k_key_minmax<<<(unsigned)g, kWG, 0, s>>>(keys, valid, n, part);
+    if (g) k_key_minmax<<<(unsigned)g, kWG, 0, s>>>(keys, valid, key_pages, key_rows, n, part);
     k_minmax_final<<<1, kWG, 0, s>>>(part, (uint32_t)g, mm);
     CJ_HIP(hipGetLastError());
 }
@@ -143,9 +160,10 @@ __global__ __launch_bounds__(kWG) void k_direct_scatter(DirectBuildArgs a) {
     const uint64_t range = direct_range(a);
     uint32_t cnt = 0;
     for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; range && i < a.rows; i += (uint64_t)gridDim.x * kWG) {
-        if (!valid_bit(a.key_valid, i)) continue;
+        if (a.keys && !valid_bit(a.key_valid, i)) continue;
         ++cnt;
-        const uint64_t slot = (uint64_t)((int64_t)a.keys[i] - (int64_t)a.minmax[0]);
+        const int32_t key = a.keys ? a.keys[i] : dense_page_value(a.key_pages, a.key_rows, (uint32_t)i);
+        const uint64_t slot = (uint64_t)((int64_t)key - (int64_t)a.minmax[0]);
         uint4 q[NQ];
 #pragma unroll
         for (int c = 0; c < NQ; ++c) q[c] = make_uint4(0, 0, 0, 0);

@@ -19,6 +19,8 @@ struct DirectField {
 struct DirectBuildArgs {
     const int32_t *keys;  // dense decoded build key column
     const uint32_t *key_valid;
+    const uint8_t *key_pages;  // keys == nullptr: the key column read in place (dense_page_value)
+    uint32_t key_rows;
     uint64_t rows;
     const int32_t *minmax;  // device [min, max] of the non-NULL keys (key_minmax)
     uint64_t max_range;     // capacity of presence / records, in slots
@@ -47,8 +49,8 @@ struct DirectTable {
 
 // mm[0] = min, mm[1] = max of the non-NULL keys (INT32_MAX, INT32_MIN when
 // there is none); `part` holds 2 * kMinmaxParts scratch ints.
-void key_minmax(const int32_t *keys, const uint32_t *valid, uint64_t n, int32_t *mm, int32_t *part,
-                hipStream_t s);
+void key_minmax(const int32_t *keys, const uint32_t *valid, const uint8_t *key_pages, uint32_t key_rows, uint64_t n,
+                int32_t *mm, int32_t *part, hipStream_t s);
 // Zero the records of the key range, scatter one record per build row, then
 // derive the presence bitmap from the records.  Nothing is written when the
 // key range exceeds max_range.  Keys are unique iff counters[0] == counters[1].

@@ -949,7 +949,8 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
         for (uint32_t i = 0; i < n.num_output_attrs; ++i)
             if (remap[i] < bw) build_attrs.push_back(i);
     if (build_attrs.size() > kDirectMaxFields) return false;
-    ColView kv = view(kc);
+    ColView kv;
+    if (!kc->dense_rows) kv = view(kc);  // else the keys are read in place
     direct_field.assign(n.num_output_attrs, {0u, 0u});
     uint32_t off = 0;
     for (int pass = 0; pass < 2; ++pass)
@@ -974,6 +975,8 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     a.stride = (off + 4 + 15) & ~15u;  // whole 16-byte chunks: one vector load each
     a.keys = static_cast<const int32_t *>(kv.values);
     a.key_valid = kv.valid;
+    a.key_pages = kc->dense_rows ? kc->pages : nullptr;
+    a.key_rows = kc->dense_rows;
     a.rows = kc->num_rows;
     // the table is sized for the widest range the direct map accepts, so the
     // key range, the build and the duplicate check need one read-back
@@ -985,7 +988,7 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     int32_t *mm = reinterpret_cast<int32_t *>(st4);
     {
         TimedScope ts(timer, "key_minmax", kc->num_rows * 4 + kc->num_rows / 8);
-        key_minmax(a.keys, a.key_valid, kc->num_rows, mm, tmp<int32_t>(2 * kMinmaxParts), st);
+        key_minmax(a.keys, a.key_valid, a.key_pages, a.key_rows, kc->num_rows, mm, tmp<int32_t>(2 * kMinmaxParts), st);
     }
     a.minmax = mm;
     a.counters = st4 + 2;
