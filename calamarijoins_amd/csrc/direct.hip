@@ -170,6 +170,11 @@ __global__ __launch_bounds__(kWG) void k_direct_scatter(DirectBuildArgs a) {
         uint32_t flags = 1u << kDirectPresentBit;
         for (uint32_t f = 0; f < a.nfields; ++f) {
             const DirectField &fd = a.fields[f];
+            if (!fd.values) {  // in place: no NULLs
+                flags |= 1u << f;
+                rec_put<NQ>(q, fd.offset, (uint32_t)dense_page_value(fd.pages, fd.dense_rows, (uint32_t)i));
+                continue;
+            }
             const bool ok = valid_bit(fd.valid_bits, i);
             flags |= (uint32_t)ok << f;
             if (fd.width == 4) {
@@ -248,8 +253,71 @@ __global__ __launch_bounds__(kWG) void k_direct_finish(DirectBuildArgs a, uint32
     }
 }
 
+// Row-id maps (no payload fields): the record of a slot is the build row id
+// alone (4 bytes), and the scatter sets the presence bits itself, so only the
+// presence words of the key range are cleared beforehand and counted after.
+__global__ __launch_bounds__(kWG) void k_rid_zero(DirectBuildArgs a) {
+    const uint64_t words = 2 * ((direct_range(a) + 63) / 64);
+    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * kWG)
+        a.presence[i] = 0;
+}
+
+__global__ __launch_bounds__(kWG) void k_rid_scatter(DirectBuildArgs a) {
+    __shared__ uint32_t red[kWG / 64];
+    const uint64_t range = direct_range(a);
+    uint32_t *rids = reinterpret_cast<uint32_t *>(a.records);
+    uint32_t cnt = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; range && i < a.rows; i += (uint64_t)gridDim.x * kWG) {
+        if (a.keys && !valid_bit(a.key_valid, i)) continue;
+        ++cnt;
+        const int32_t key = a.keys ? a.keys[i] : dense_page_value(a.key_pages, a.key_rows, (uint32_t)i);
+        const uint64_t slot = (uint64_t)((int64_t)key - (int64_t)a.minmax[0]);
+        atomicOr(&a.presence[slot >> 5], 1u << (slot & 31));
+        rids[slot] = (uint32_t)i;
+    }
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if (lane_id() == 0) red[wave_id()] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kWG / 64; ++w) t += red[w];
+        a.partials[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_rid_present(DirectBuildArgs a) {
+    __shared__ uint32_t red[kWG / 64];
+    const uint64_t words = 2 * ((direct_range(a) + 63) / 64);
+    uint32_t cnt = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * kWG)
+        cnt += (uint32_t)__popc(a.presence[i]);
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if (lane_id() == 0) red[wave_id()] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kWG / 64; ++w) t += red[w];
+        a.partials[kDirectParts + blockIdx.x] = t;
+    }
+}
+
+static void rid_build(const DirectBuildArgs &a, hipStream_t s) {
+    if (a.stride != 4 || a.rid_offset != 0) throw InvalidError("bad row-id map layout");
+    const uint64_t words = 2 * ((a.max_range + 63) / 64);
+    uint64_t wg = (words + kWG - 1) / kWG;
+    wg = wg < kDirectParts ? wg : kDirectParts;
+    k_rid_zero<<<(unsigned)wg, kWG, 0, s>>>(a);
+    uint64_t sg = (a.rows + kWG - 1) / kWG;
+    sg = sg < kDirectParts ? sg : kDirectParts;
+    k_rid_scatter<<<(unsigned)sg, kWG, 0, s>>>(a);
+    k_rid_present<<<(unsigned)wg, kWG, 0, s>>>(a);
+    k_direct_finish<<<1, kWG, 0, s>>>(a, (uint32_t)sg, (uint32_t)wg);
+    CJ_HIP(hipGetLastError());
+}
+
 void direct_build(const DirectBuildArgs &a, hipStream_t s) {
     if (!a.rows || !a.max_range) return;
+    if (a.nfields == 0 && a.rid_offset >= 0) return rid_build(a, s);
     if (a.stride % 16 || a.stride == 0 || a.stride > 80) throw InvalidError("bad direct-map record stride");
     const uint64_t chunks = a.max_range * (a.stride / 16);
     const uint64_t zg = (chunks + kWG - 1) / kWG;

@@ -14,6 +14,8 @@ struct DirectField {
     uint32_t offset;  // byte offset inside the record
     const void *values;  // dense decoded build column
     const uint32_t *valid_bits;
+    const uint8_t *pages;  // values == nullptr: an INT32 column read in place (dense_page_value)
+    uint32_t dense_rows;
 };
 
 struct DirectBuildArgs {
@@ -32,7 +34,8 @@ struct DirectBuildArgs {
     uint32_t *partials;     // 2 * kDirectParts scratch words
     uint32_t nfields;
     DirectField fields[kDirectMaxFields];
-    int32_t rid_offset;  // >= 0: the build row id is stored at this record offset
+    int32_t rid_offset;  // >= 0: the build row id is stored at this record offset; with no
+                         // fields the records are the row ids alone (stride 4, offset 0)
 };
 
 // The table as the probe sees it.
@@ -52,7 +55,8 @@ struct DirectTable {
 void key_minmax(const int32_t *keys, const uint32_t *valid, const uint8_t *key_pages, uint32_t key_rows, uint64_t n,
                 int32_t *mm, int32_t *part, hipStream_t s);
 // Zero the records of the key range, scatter one record per build row, then
-// derive the presence bitmap from the records.  Nothing is written when the
+// derive the presence bitmap from the records (a map of row ids alone: clear
+// the presence words, scatter row ids and presence bits, count the bits).  Nothing is written when the
 // key range exceeds max_range.  Keys are unique iff counters[0] == counters[1].
 void direct_build(const DirectBuildArgs &a, hipStream_t s);
 

@@ -960,8 +960,13 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
             uint32_t w = type == CJ_INT32 ? 4 : 8;
             if ((pass == 0) != (w == 8)) continue;
             const ColRef &cr = B.cols[remap[i]];
-            ColView v = view(column(B.slots[0].table, cr.col));
-            a.fields[f] = DirectField{(int32_t)w, off, v.values, v.valid};
+            DevColumn *fc = column(B.slots[0].table, cr.col);
+            if (type == CJ_INT32 && fc->dense_rows) {  // read in place
+                a.fields[f] = DirectField{4, off, nullptr, nullptr, fc->pages, fc->dense_rows};
+            } else {
+                ColView v = view(fc);
+                a.fields[f] = DirectField{(int32_t)w, off, v.values, v.valid, nullptr, 0u};
+            }
             direct_field[i] = {off, f};
             off += w;
         }
@@ -973,6 +978,10 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     }
     a.flags_offset = off;
     a.stride = (off + 4 + 15) & ~15u;  // whole 16-byte chunks: one vector load each
+    if (rids) {  // records are the build row ids alone
+        a.flags_offset = 0;
+        a.stride = 4;
+    }
     a.keys = static_cast<const int32_t *>(kv.values);
     a.key_valid = kv.valid;
     a.key_pages = kc->dense_rows ? kc->pages : nullptr;
@@ -998,7 +1007,8 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     a.records = tmp<uint8_t>(a.max_range * a.stride);
     {
         // zeroed records + one record per row + the bitmap pass over the records
-        TimedScope ts(timer, "direct_build", kc->num_rows * (4ull + a.stride + a.stride) + kc->num_rows / 8);
+        TimedScope ts(timer, "direct_build",
+                      rids ? kc->num_rows * 8ull : kc->num_rows * (4ull + a.stride + a.stride) + kc->num_rows / 8);
         direct_build(a, st);
     }
     uint32_t *h = reinterpret_cast<uint32_t *>(readback());
