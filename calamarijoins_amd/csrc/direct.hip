@@ -254,12 +254,14 @@ __global__ __launch_bounds__(kWG) void k_direct_finish(DirectBuildArgs a, uint32
 }
 
 // Row-id maps (no payload fields): the record of a slot is the build row id
-// alone (4 bytes), and the scatter sets the presence bits itself, so only the
-// presence words of the key range are cleared beforehand and counted after.
+// alone (4 bytes; ~0u marks an empty slot, a row id never reaches it), so
+// clearing, scattering and the presence pass move a quarter of the bytes of
+// the smallest field record.
 __global__ __launch_bounds__(kWG) void k_rid_zero(DirectBuildArgs a) {
-    const uint64_t words = 2 * ((direct_range(a) + 63) / 64);
-    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * kWG)
-        a.presence[i] = 0;
+    const uint64_t chunks = (direct_range(a) + 3) / 4;
+    uint4 *r = reinterpret_cast<uint4 *>(a.records);
+    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < chunks; i += (uint64_t)gridDim.x * kWG)
+        r[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
 }
 
 __global__ __launch_bounds__(kWG) void k_rid_scatter(DirectBuildArgs a) {
@@ -271,9 +273,7 @@ __global__ __launch_bounds__(kWG) void k_rid_scatter(DirectBuildArgs a) {
         if (a.keys && !valid_bit(a.key_valid, i)) continue;
         ++cnt;
         const int32_t key = a.keys ? a.keys[i] : dense_page_value(a.key_pages, a.key_rows, (uint32_t)i);
-        const uint64_t slot = (uint64_t)((int64_t)key - (int64_t)a.minmax[0]);
-        atomicOr(&a.presence[slot >> 5], 1u << (slot & 31));
-        rids[slot] = (uint32_t)i;
+        rids[(uint64_t)((int64_t)key - (int64_t)a.minmax[0])] = (uint32_t)i;
     }
     for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
     if (lane_id() == 0) red[wave_id()] = cnt;
@@ -285,14 +285,24 @@ __global__ __launch_bounds__(kWG) void k_rid_scatter(DirectBuildArgs a) {
     }
 }
 
+// 64 slots per wave step: ballot of the filled slots -> two presence words.
 __global__ __launch_bounds__(kWG) void k_rid_present(DirectBuildArgs a) {
     __shared__ uint32_t red[kWG / 64];
-    const uint64_t words = 2 * ((direct_range(a) + 63) / 64);
+    const uint64_t range = direct_range(a);
+    const uint32_t *rids = reinterpret_cast<const uint32_t *>(a.records);
+    const uint32_t lane = lane_id();
+    const uint64_t steps = (range + 63) / 64;
+    const uint64_t wave = blockIdx.x * (uint64_t)(kWG / 64) + wave_id();
+    const uint64_t waves = (uint64_t)gridDim.x * (kWG / 64);
     uint32_t cnt = 0;
-    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * kWG)
-        cnt += (uint32_t)__popc(a.presence[i]);
-    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
-    if (lane_id() == 0) red[wave_id()] = cnt;
+    for (uint64_t st = wave; st < steps; st += waves) {
+        const uint64_t slot = st * 64 + lane;
+        const uint32_t r = rids[slot < range ? slot : range - 1];
+        const uint64_t b = __ballot(slot < range && r != ~0u);
+        if (lane < 2) a.presence[2 * st + lane] = (uint32_t)(b >> (32 * lane));
+        cnt += (uint32_t)__popcll(b);
+    }
+    if (lane == 0) red[wave_id()] = cnt;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t = 0;
@@ -303,15 +313,16 @@ __global__ __launch_bounds__(kWG) void k_rid_present(DirectBuildArgs a) {
 
 static void rid_build(const DirectBuildArgs &a, hipStream_t s) {
     if (a.stride != 4 || a.rid_offset != 0) throw InvalidError("bad row-id map layout");
-    const uint64_t words = 2 * ((a.max_range + 63) / 64);
-    uint64_t wg = (words + kWG - 1) / kWG;
-    wg = wg < kDirectParts ? wg : kDirectParts;
-    k_rid_zero<<<(unsigned)wg, kWG, 0, s>>>(a);
+    const uint64_t zg = ((a.max_range + 3) / 4 + kWG - 1) / kWG;
+    k_rid_zero<<<(unsigned)(zg < 8192 ? zg : 8192), kWG, 0, s>>>(a);
     uint64_t sg = (a.rows + kWG - 1) / kWG;
     sg = sg < kDirectParts ? sg : kDirectParts;
     k_rid_scatter<<<(unsigned)sg, kWG, 0, s>>>(a);
-    k_rid_present<<<(unsigned)wg, kWG, 0, s>>>(a);
-    k_direct_finish<<<1, kWG, 0, s>>>(a, (uint32_t)sg, (uint32_t)wg);
+    const uint64_t steps = (a.max_range + 63) / 64;
+    uint64_t pg = (steps + kWG / 64 - 1) / (kWG / 64);
+    pg = pg < kDirectParts ? pg : kDirectParts;
+    k_rid_present<<<(unsigned)pg, kWG, 0, s>>>(a);
+    k_direct_finish<<<1, kWG, 0, s>>>(a, (uint32_t)sg, (uint32_t)pg);
     CJ_HIP(hipGetLastError());
 }
 

@@ -55,8 +55,8 @@ struct DirectTable {
 void key_minmax(const int32_t *keys, const uint32_t *valid, const uint8_t *key_pages, uint32_t key_rows, uint64_t n,
                 int32_t *mm, int32_t *part, hipStream_t s);
 // Zero the records of the key range, scatter one record per build row, then
-// derive the presence bitmap from the records (a map of row ids alone: clear
-// the presence words, scatter row ids and presence bits, count the bits).  Nothing is written when the
+// derive the presence bitmap from the records (for a map of row ids alone,
+// from the slots a row id filled).  Nothing is written when the
 // key range exceeds max_range.  Keys are unique iff counters[0] == counters[1].
 void direct_build(const DirectBuildArgs &a, hipStream_t s);
 

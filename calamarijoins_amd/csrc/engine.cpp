@@ -1004,7 +1004,7 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     a.partials = tmp<uint32_t>(2 * kDirectParts);
     const uint64_t words = a.max_range / 32 + 4;
     a.presence = tmp<uint32_t>(words);
-    a.records = tmp<uint8_t>(a.max_range * a.stride);
+    a.records = tmp<uint8_t>(a.max_range * a.stride + 16);  // + the row-id map's last 16-byte clear
     {
         // zeroed records + one record per row + the bitmap pass over the records
         TimedScope ts(timer, "direct_build",
