@@ -95,6 +95,13 @@ void decode_fixed(const uint8_t *pages, uint32_t num_pages, const uint64_t *row_
 void decode_varchar(const uint8_t *pages, uint32_t num_pages, const uint64_t *row_start,
                     uint64_t *refs, uint32_t *valid_bits, hipStream_t s);
 void gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t *dst, uint64_t n, hipStream_t s);
+constexpr uint32_t kGatherSet = 8;
+struct GatherSet {  // dst[k][i] = src[k][idx[i]] for k < count
+    const uint32_t *src[kGatherSet];
+    uint32_t *dst[kGatherSet];
+    uint32_t count;
+};
+void gather_u32_set(const GatherSet &g, const uint32_t *idx, uint64_t n, hipStream_t s);
 
 // join.hip
 struct KeySource {

@@ -299,4 +299,24 @@ void gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t *dst, uint64_
     CJ_HIP(hipGetLastError());
 }
 
+// Several row-id lists gathered through one index list: the index is read
+// once and one launch serves every list of a join side.
+__global__ void k_gather_u32_set(GatherSet g, const uint32_t *idx, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t ix = idx[i];
+        uint32_t v[kGatherSet];
+#pragma unroll
+        for (uint32_t k = 0; k < kGatherSet; ++k) v[k] = g.src[k < g.count ? k : 0][ix];
+#pragma unroll
+        for (uint32_t k = 0; k < kGatherSet; ++k)
+            if (k < g.count) g.dst[k][i] = v[k];
+    }
+}
+void gather_u32_set(const GatherSet &g, const uint32_t *idx, uint64_t n, hipStream_t s) {
+    if (!n || !g.count) return;
+    if (g.count > kGatherSet) throw InvalidError("too many lists in one gather set");
+    k_gather_u32_set<<<grid_for(n), kWG, 0, s>>>(g, idx, n);
+    CJ_HIP(hipGetLastError());
+}
+
 }  // namespace cj

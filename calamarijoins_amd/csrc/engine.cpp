@@ -722,6 +722,8 @@ class Exec {
     const uint16_t *page_index(DevColumn *c);
     uint32_t flags = 0;
     std::vector<std::pair<uint32_t, uint32_t>> direct_field;  // per output attr: record offset, flag bit
+    void add_gather(GatherSet &g, const uint32_t *old, uint32_t *dst, const uint32_t *idx, uint64_t total);
+    void flush_gather(GatherSet &g, const uint32_t *idx, uint64_t total);
     void materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
                           const std::vector<uint32_t> &remap, uint64_t total, const uint32_t *ob, const uint32_t *op,
                           cj_result *out);
@@ -860,6 +862,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
     res.columnar = false;
     res.rows = total;
     std::map<std::pair<int, uint32_t>, uint32_t> slot_map;
+    GatherSet gs[2] = {};  // the lists each side's index list gathers
     for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
         bool fb = remap[a] < bw;
         const Rel &S = fb ? B : P;
@@ -874,8 +877,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
             const uint32_t *nr = idx;
             if (old && total) {
                 uint32_t *g = tmp<uint32_t>(total);
-                TimedScope ts(timer, "gather_rids", 12 * total);
-                gather_u32(old, idx, g, total, st);
+                add_gather(gs[fb ? 0 : 1], old, g, idx, total);
                 nr = g;
             }
             ns = (uint32_t)res.slots.size();
@@ -889,7 +891,24 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         }
         res.cols.push_back(ColRef{ns, cr.col, cr.type});
     }
+    flush_gather(gs[0], ob, total);
+    flush_gather(gs[1], op, total);
     return res;
+}
+
+// Queue dst = old[idx] into the side's gather set (launched when full).
+void Exec::add_gather(GatherSet &g, const uint32_t *old, uint32_t *dst, const uint32_t *idx, uint64_t total) {
+    if (g.count == kGatherSet) flush_gather(g, idx, total);
+    g.src[g.count] = old;
+    g.dst[g.count] = dst;
+    ++g.count;
+}
+
+void Exec::flush_gather(GatherSet &g, const uint32_t *idx, uint64_t total) {
+    if (!g.count) return;
+    TimedScope ts(timer, "gather_rids", (4 + 8 * (uint64_t)g.count) * total);
+    gather_u32_set(g, idx, total, st);
+    g.count = 0;
 }
 
 // The fused probe + materialization applies to a root join whose probe side
@@ -1197,7 +1216,11 @@ void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, u
                             const std::vector<uint32_t> &remap, uint64_t total, const uint32_t *ob,
                             const uint32_t *op, cj_result *) {
     out_rows = total;
+    out_types.clear();
+    out_cols.clear();
     std::map<std::pair<int, uint32_t>, const uint32_t *> final_rids;
+    std::vector<const uint32_t *> attr_rids(n.num_output_attrs, nullptr);
+    GatherSet gs[2] = {};  // the lists each side's index list gathers
     for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
         const int32_t type = n.output_attrs[a].type;
         out_types.push_back(type);
@@ -1220,17 +1243,26 @@ void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, u
             const uint32_t *old = S.slots[cr.slot].rids;
             if (old) {
                 uint32_t *g = tmp<uint32_t>(total);
-                TimedScope ts(timer, "gather_rids", 12 * total);
-                gather_u32(old, idx, g, total, st);
+                add_gather(gs[fb ? 0 : 1], old, g, idx, total);
                 rids = g;
             } else {
                 rids = idx;
             }
             final_rids[key] = rids;
         }
-        DevColumn *c = column(S.slots[cr.slot].table, cr.col);
-        auto [pages, np] = materialize_column(c, type, rids, total, true);
-        out_cols.push_back({pages, {np, type}});
+        attr_rids[a] = rids;
+        out_cols.push_back({nullptr, {0u, type}});
+    }
+    if (total == 0) return;
+    flush_gather(gs[0], ob, total);
+    flush_gather(gs[1], op, total);
+    for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
+        const bool fb = remap[a] < bw;
+        const Rel &S = fb ? B : P;
+        const ColRef &cr = S.cols[fb ? remap[a] : remap[a] - bw];
+        const int32_t type = n.output_attrs[a].type;
+        auto [pages, np] = materialize_column(column(S.slots[cr.slot].table, cr.col), type, attr_rids[a], total, true);
+        out_cols[a] = {pages, {np, type}};
     }
 }
 
