@@ -804,10 +804,15 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         // (with a direct map every probe row matches at most once: the match
         // bits also let the pair writer skip a second lookup)
         const bool bits = fa.mode == kModeDirect && pk.mode == KEY_COLUMNAR && !(flags & CJ_EXEC_NO_ALIGNED);
+        // a row-id list probing a direct map: the count pass records match
+        // bits so the pair writer reads the keys of the matched rows only
+        const bool inter_bits = fa.mode == kModeDirect && pk.mode == KEY_INTER && !fused &&
+                                !(flags & CJ_EXEC_NO_ALIGNED);
         const bool aligned = fused && bits;
         uint32_t *match_bits = nullptr;
-        const uint64_t probe_rows = pk.mode == KEY_COLUMNAR ? column(P.slots[0].table, P.cols[pattr].col)->num_rows : 0;
-        if (bits) {
+        const uint64_t probe_rows = pk.mode == KEY_COLUMNAR ? column(P.slots[0].table, P.cols[pattr].col)->num_rows
+                                                            : pk.mode == KEY_INTER ? pk.rows : 0;
+        if (bits || inter_bits) {
             const uint64_t words = probe_rows / 32 + 2;
             match_bits = tmp<uint32_t>(words);
             fill_u32(match_bits, 0, words, st);
@@ -815,6 +820,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         {
             TimedScope ts(timer, use_nl ? "nl_probe_count" : "probe_count", key_bytes(pk));
             if (bits) probe_count_match(fa, cnt, match_bits, st);
+            else if (inter_bits) inter_count_match(fa, cnt, match_bits, st);
             else probe_count_columnar(fa, cnt, st);
         }
         scan(cnt, off, units);
@@ -849,6 +855,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
             if (need_p) op = tmp<uint32_t>(total);
             TimedScope ts(timer, use_nl ? "nl_probe_write" : "probe_write", key_bytes(pk) + total * (4ull * (need_b + need_p)));
             if (bits) direct_pairs(fa, match_bits, off, ob, op, st);
+            else if (inter_bits) inter_pairs(fa, match_bits, off, ob, op, st);
             else probe_write_columnar(fa, off, ob, op, st);
         }
     }

@@ -308,6 +308,158 @@ void direct_pairs(const ProbeArgs &a, const uint32_t *match_bits, const uint64_t
     CJ_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------ direct maps, row-id lists
+// One wave per unit of kInterChunk rows of the intermediate; rows are taken
+// 64 a chunk, 8 chunks at a time: row ids, then validity words and keys (a
+// decoded column, or the base column's pages read in place).  Units and
+// chunks start at multiples of 64 rows, so a chunk's match mask is two whole
+// words of the match bitmap.
+constexpr uint32_t kInterBatch = 8;
+
+// Keys of rows r0 + c0 + 64 c + lane (c < kInterBatch) of the intermediate
+// whose unit holds n rows; bit c of the result: the row exists, its key is
+// non-NULL and is not INT32_MIN (a NULL once it went through an
+// intermediate).  Rows outside use[] (bit clear) read the unit's first row.
+__device__ __forceinline__ uint32_t inter_keys(const KeySource &k, uint64_t r0, uint32_t n, uint32_t c0, uint32_t lane,
+                                               uint32_t use, int32_t (&key)[kInterBatch]) {
+    uint64_t rid[kInterBatch];
+    uint32_t vw[kInterBatch];
+#pragma unroll
+    for (uint32_t c = 0; c < kInterBatch; ++c) {
+        const uint32_t r = c0 + 64 * c + lane;
+        const uint64_t row = r0 + ((((use >> c) & 1u) != 0) & (r < n) ? r : 0u);
+        rid[c] = k.rids ? k.rids[row] : row;
+    }
+    if (k.values) {
+#pragma unroll
+        for (uint32_t c = 0; c < kInterBatch; ++c) vw[c] = k.valid_bits[rid[c] >> 5];
+#pragma unroll
+        for (uint32_t c = 0; c < kInterBatch; ++c) key[c] = k.values[rid[c]];
+    } else {
+#pragma unroll
+        for (uint32_t c = 0; c < kInterBatch; ++c) {
+            vw[c] = ~0u;
+            key[c] = dense_page_value(k.dense_pages, k.dense_rows, (uint32_t)rid[c]);
+        }
+    }
+    uint32_t ok = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < kInterBatch; ++c) {
+        const uint32_t r = c0 + 64 * c + lane;
+        ok |= (uint32_t)((r < n) & (((vw[c] >> (rid[c] & 31)) & 1u) != 0) & (key[c] != kI32Null)) << c;
+    }
+    return ok & use;
+}
+
+__global__ __launch_bounds__(kWG) void k_inter_count_match(ProbeArgs a, uint64_t *counts, uint32_t *match_bits) {
+    const uint32_t lane = lane_id();
+    const KeySource &k = a.key;
+    const DirectTable &dt = a.dt;
+    const uint32_t u = blockIdx.x * (kWG / 64) + wave_id();
+    if ((uint64_t)u * kInterChunk >= k.rows) return;
+    const uint64_t r0 = (uint64_t)u * kInterChunk;
+    const uint32_t n = k.rows - r0 < kInterChunk ? (uint32_t)(k.rows - r0) : kInterChunk;
+    uint64_t total = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kInterBatch) {
+        int32_t key[kInterBatch];
+        const uint32_t ok = inter_keys(k, r0, n, c0, lane, ~0u, key);
+        uint32_t slot[kInterBatch], word[kInterBatch], in = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < kInterBatch; ++c) {
+            const int64_t off = (int64_t)key[c] - (int64_t)dt.min_key;
+            const bool m = ((ok >> c) & 1u) & (off >= 0) & ((uint64_t)off < dt.range);
+            slot[c] = m ? (uint32_t)off : 0u;
+            in |= (uint32_t)m << c;
+        }
+        if (dt.full) {
+#pragma unroll
+            for (uint32_t c = 0; c < kInterBatch; ++c) word[c] = ~0u;
+        } else {
+#pragma unroll
+            for (uint32_t c = 0; c < kInterBatch; ++c) word[c] = dt.presence[slot[c] >> 5];
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < kInterBatch; ++c) {
+            const bool m = ((in >> c) & 1u) && ((word[c] >> (slot[c] & 31)) & 1u);
+            const uint64_t mm = __ballot(m);
+            total += (uint32_t)__popcll(mm);
+            const uint32_t s = c0 + 64 * c;
+            if (s < n && lane < 2) match_bits[((r0 + s) >> 5) + lane] = (uint32_t)(mm >> (32 * lane));
+        }
+    }
+    if (lane == 0) counts[u] = total;
+}
+
+void inter_count_match(const ProbeArgs &a, uint64_t *counts, uint32_t *match_bits, hipStream_t s) {
+    const uint64_t units = (a.key.rows + kInterChunk - 1) / kInterChunk;
+    if (!units) return;
+    if (a.mode != kModeDirect || a.key.mode != KEY_INTER) throw InvalidError("inter_count_match needs a direct map and a row-id list");
+    const uint32_t per = kWG / 64;
+    k_inter_count_match<<<(unsigned)((units + per - 1) / per), kWG, 0, s>>>(a, counts, match_bits);
+    CJ_HIP(hipGetLastError());
+}
+
+// Pairs of one unit from its match bits: keys only for the matched rows
+// (the others read the unit's first row), then the build ids.
+__global__ __launch_bounds__(kWG) void k_inter_pairs(ProbeArgs a, const uint32_t *match_bits, const uint64_t *off,
+                                                     uint32_t *ob, uint32_t *op) {
+    const uint32_t lane = lane_id();
+    const KeySource &k = a.key;
+    const DirectTable &dt = a.dt;
+    const uint32_t u = blockIdx.x * (kWG / 64) + wave_id();
+    if ((uint64_t)u * kInterChunk >= k.rows) return;
+    uint64_t o = off[u];
+    if (off[u + 1] == o) return;
+    const uint64_t r0 = (uint64_t)u * kInterChunk;
+    const uint32_t n = k.rows - r0 < kInterChunk ? (uint32_t)(k.rows - r0) : kInterChunk;
+    const uint64_t lt = lanemask_lt();
+    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kInterBatch) {
+        uint32_t use = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < kInterBatch; ++c) {
+            const uint32_t s = c0 + 64 * c;
+            const uint32_t w = match_bits[((r0 + (s < n ? s : 0u)) >> 5) + (lane >> 5)];
+            use |= (uint32_t)((s + lane < n) & (((w >> (lane & 31)) & 1u) != 0)) << c;
+        }
+        int32_t key[kInterBatch];
+        inter_keys(k, r0, n, c0, lane, use, key);
+        uint32_t bid[kInterBatch];
+#pragma unroll
+        for (uint32_t c = 0; c < kInterBatch; ++c) {
+            // a set match bit means the key is in range and present
+            const int64_t so = (int64_t)key[c] - (int64_t)dt.min_key;
+            const bool m = ((use >> c) & 1u) & (so >= 0) & ((uint64_t)so < dt.range);
+            bid[c] = m ? (uint32_t)so : 0u;
+        }
+        if (dt.rid_offset >= 0) {
+#pragma unroll
+            for (uint32_t c = 0; c < kInterBatch; ++c)
+                bid[c] = *reinterpret_cast<const uint32_t *>(dt.records + (uint64_t)bid[c] * dt.stride +
+                                                             (uint32_t)dt.rid_offset);
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < kInterBatch; ++c) {
+            const bool m = (use >> c) & 1u;
+            const uint64_t mm = __ballot(m);
+            if (m) {
+                const uint64_t at = o + (uint64_t)__popcll(mm & lt);
+                if (ob) ob[at] = bid[c];
+                if (op) op[at] = (uint32_t)(r0 + c0 + 64 * c + lane);
+            }
+            o += (uint64_t)__popcll(mm);
+        }
+    }
+}
+
+void inter_pairs(const ProbeArgs &a, const uint32_t *match_bits, const uint64_t *off, uint32_t *ob, uint32_t *op,
+                 hipStream_t s) {
+    const uint64_t units = (a.key.rows + kInterChunk - 1) / kInterChunk;
+    if (!units) return;
+    const uint32_t per = kWG / 64;
+    k_inter_pairs<<<(unsigned)((units + per - 1) / per), kWG, 0, s>>>(a, match_bits, off, ob, op);
+    CJ_HIP(hipGetLastError());
+}
+
 __global__ void k_key_page_out_pages(const uint64_t *counts, uint32_t pages, uint64_t *p4, uint64_t *p8) {
     const uint32_t i = blockIdx.x * kWG + threadIdx.x;
     if (i >= pages) return;

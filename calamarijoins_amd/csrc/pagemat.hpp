@@ -25,6 +25,11 @@ void probe_count_match(const ProbeArgs &a, uint64_t *counts, uint32_t *match_bit
 // may be null.
 void direct_pairs(const ProbeArgs &a, const uint32_t *match_bits, const uint64_t *off, uint32_t *ob, uint32_t *op,
                   hipStream_t s);
+// The same over a row-id list (KEY_INTER probe side, direct map): match bit
+// r = row r of the intermediate; counts per kInterChunk-row unit.
+void inter_count_match(const ProbeArgs &a, uint64_t *counts, uint32_t *match_bits, hipStream_t s);
+void inter_pairs(const ProbeArgs &a, const uint32_t *match_bits, const uint64_t *off, uint32_t *ob, uint32_t *op,
+                 hipStream_t s);
 
 // Output pages each key page produces for the build side:
 // p4[pg] = ceil(m / 1984), p8[pg] = ceil(m / 1007) for its m matches.

@@ -210,3 +210,46 @@ def test_gpu_sparse_pages(engine, key_null, pay_null, flags):
     plan.root = plan.new_join_node(True, 0, 1, 0, 0, [(1, "INT32"), (2, "INT64"), (4, "INT32"), (5, "INT64"),
                                                      (6, "FP64"), (3, "INT32")])
     same_multiset(engine.execute(plan, flags=flags), cjtest.OracleRun(plan, threads=8))
+
+
+@pytest.mark.parametrize("flags", [0, 8, 4, 2], ids=["aligned", "fused", "fused-hash", "general"])
+@pytest.mark.parametrize("seed", range(2))
+def test_gpu_intermediate_probes_direct_maps(engine, seed, flags):
+    """Intermediates probing base tables with unique, sparse keys (direct maps
+    of row ids, probed through the intermediate's match bits): one key column
+    with NULLs and INT32_MIN (decoded, NULL semantics through the
+    intermediate), one without NULLs (read in place from its pages), a
+    non-root and a root join."""
+    rng = random.Random(4100 + seed)
+    I = bruteforce.I32_MIN
+    n = 6000 + 1500 * seed
+    p_rows = []
+    for i in range(n):
+        r = rng.random()
+        k1 = None if r < 0.05 else (I if r < 0.08 else rng.randrange(-5, 9000))
+        p_rows.append([None if rng.random() < 0.03 else rng.randrange(0, 3000), k1, rng.randrange(0, 4000), i])
+    d_keys = list(range(3000))
+    rng.shuffle(d_keys)
+    e_keys = rng.sample(range(0, 9000), 2200)  # sparse: a map of row ids over 9000 values
+    f_keys = rng.sample(range(0, 4000), 1700)
+    tabs = [
+        cjtest.table_from_rows(["INT32"] * 4, p_rows),
+        cjtest.table_from_rows(["INT32", "INT32"], [[k, rng.randrange(-50, 50)] for k in d_keys]),
+        cjtest.table_from_rows(["INT32", "INT32"], [[k, rng.randrange(-50, 50)] for k in e_keys]),
+        cjtest.table_from_rows(["INT32", "INT32"], [[k, None if rng.random() < 0.1 else k] for k in f_keys]),
+    ]
+    plan = cjtest.Plan()
+    for t in tabs:
+        plan.new_input(t)
+    plan.new_scan_node(0, [(0, "INT32"), (1, "INT32"), (2, "INT32"), (3, "INT32")])
+    for i in range(1, 4):
+        plan.new_scan_node(i, [(0, "INT32"), (1, "INT32")])
+    # P.k0 = D.key -> (P.k1, P.k2, P.row, D.pay)
+    j1 = plan.new_join_node(False, 0, 1, 0, 0, [(1, "INT32"), (2, "INT32"), (3, "INT32"), (5, "INT32")])
+    # k1 = E.key -> (k2, row, D.pay, E.pay)
+    j2 = plan.new_join_node(False, j1, 2, 0, 0, [(1, "INT32"), (2, "INT32"), (3, "INT32"), (5, "INT32")])
+    # k2 = F.key -> (row, D.pay, E.pay, F.pay)
+    plan.root = plan.new_join_node(False, j2, 3, 0, 0, [(1, "INT32"), (2, "INT32"), (3, "INT32"), (5, "INT32")])
+    exp = cjtest.OracleRun(plan, threads=8)
+    assert exp.num_rows > 500
+    same_multiset(engine.execute(plan, flags=flags), exp)
