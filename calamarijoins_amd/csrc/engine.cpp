@@ -815,7 +815,8 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         if (bits || inter_bits) {
             const uint64_t words = probe_rows / 32 + 2;
             match_bits = tmp<uint32_t>(words);
-            fill_u32(match_bits, 0, words, st);
+            // the row-id list's count pass stores every word its rows cover
+            if (bits) fill_u32(match_bits, 0, words, st);
         }
         {
             TimedScope ts(timer, use_nl ? "nl_probe_count" : "probe_count", key_bytes(pk));
