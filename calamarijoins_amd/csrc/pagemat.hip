@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kWG) void k_probe_count_match(ProbeArgs a, uint64_t
 // the 32 chunk masks (64 rows each) go to the wave's LDS slice, then lane i
 // assembles global bitmap word i of the segment from them: interior words
 // are stored, the two edge words OR-ed with device atomics.
-constexpr uint32_t kCountBatch = 16;
+constexpr uint32_t kCountBatch = 32;
 
 __global__ __launch_bounds__(kWG) void k_count_match_direct(ProbeArgs a, uint64_t *counts, uint32_t *match_bits) {
     __shared__ uint32_t msk_all[kWG / 64][2 * (kTile / 64) + 2];
