@@ -447,6 +447,9 @@ namespace {
 struct Slot {
     uint32_t table;          // plan input index
     const uint32_t *rids;    // row ids into the base table (nullptr = identity)
+    // non-null: rids holds slots of a direct map of row ids, row id = map[slot]
+    // (resolved only where a row id is read: keys, output columns)
+    const uint32_t *map = nullptr;
 };
 struct ColRef {
     uint32_t slot;
@@ -665,7 +668,7 @@ class Exec {
                 k.values = static_cast<const int32_t *>(v.values);
                 k.valid_bits = v.valid;
             }
-            k.rids = r.slots[cr.slot].rids;
+            k.rids = resolved_rids(r.slots[cr.slot], r.rows);
             k.rows = r.rows;
         }
         return k;
@@ -722,6 +725,14 @@ class Exec {
     const uint16_t *page_index(DevColumn *c);
     uint32_t flags = 0;
     std::vector<std::pair<uint32_t, uint32_t>> direct_field;  // per output attr: record offset, flag bit
+    // Row ids of a slot of `rows` rows (map slots read through the map).
+    const uint32_t *resolved_rids(const Slot &s, uint64_t rows) {
+        if (!s.map || !rows) return s.rids;
+        uint32_t *g = tmp<uint32_t>(rows);
+        TimedScope ts(timer, "gather_rids", 12 * rows);
+        gather_u32(s.map, s.rids, g, rows, st);
+        return g;
+    }
     void add_gather(GatherSet &g, const uint32_t *old, uint32_t *dst, const uint32_t *idx, uint64_t total);
     void flush_gather(GatherSet &g, const uint32_t *idx, uint64_t total);
     void materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
@@ -768,6 +779,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
     KeySource pk = key_source(P, pattr);
     const bool use_nl = B.rows < 8;
     HashTableDev ht{};
+    const uint32_t *lazy_map = nullptr;  // the build side's row ids, read through map slots
     NLKeys *nl = nullptr;
     uint64_t total = 0;
     uint32_t *ob = nullptr, *op = nullptr;
@@ -786,6 +798,13 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
             // a direct map: with the projected build payloads for the fused
             // root writers, or with the build row ids for pair lists
             fa.mode = kModeDirect;
+            if (!fused && !is_root) {
+                // an intermediate keeps the map slots; row ids are read
+                // through the map only for the rows that reach a key or an
+                // output column
+                lazy_map = reinterpret_cast<const uint32_t *>(fa.dt.records);
+                fa.dt.rid_offset = -1;
+            }
         } else {
             // tuple capacity: the rows the key pages actually hold (the
             // declared num_rows only sizes the directory)
@@ -888,8 +907,10 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
                 add_gather(gs[fb ? 0 : 1], old, g, idx, total);
                 nr = g;
             }
+            // the build side of a lazy direct map: ob holds map slots
+            const uint32_t *map = old ? S.slots[cr.slot].map : (fb ? lazy_map : nullptr);
             ns = (uint32_t)res.slots.size();
-            res.slots.push_back(Slot{S.slots[cr.slot].table, total ? nr : nullptr});
+            res.slots.push_back(Slot{S.slots[cr.slot].table, total ? nr : nullptr, total ? map : nullptr});
             if (!total) {  // an empty list still needs a non-null marker
                 res.slots.back().rids = tmp<uint32_t>(1);
             }
@@ -1108,7 +1129,7 @@ void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_
                     ColView v = view(c);
                     mc.values = v.values;
                     mc.valid_bits = v.valid;
-                    mc.rids = S.slots[cr.slot].rids;
+                    mc.rids = resolved_rids(S.slots[cr.slot], S.rows);
                     bytes += total * (uint64_t)width + total / 8;
                 } else {
                     mc.pages = c->pages;
@@ -1264,12 +1285,25 @@ void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, u
     if (total == 0) return;
     flush_gather(gs[0], ob, total);
     flush_gather(gs[1], op, total);
+    std::map<std::pair<int, uint32_t>, const uint32_t *> resolved;  // map slots -> row ids
     for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
         const bool fb = remap[a] < bw;
         const Rel &S = fb ? B : P;
         const ColRef &cr = S.cols[fb ? remap[a] : remap[a] - bw];
         const int32_t type = n.output_attrs[a].type;
-        auto [pages, np] = materialize_column(column(S.slots[cr.slot].table, cr.col), type, attr_rids[a], total, true);
+        const uint32_t *rids = attr_rids[a];
+        if (const uint32_t *map = S.slots[cr.slot].map) {
+            auto key = std::make_pair(fb ? 0 : 1, cr.slot);
+            auto it = resolved.find(key);
+            if (it == resolved.end()) {
+                uint32_t *g = tmp<uint32_t>(total);
+                TimedScope ts(timer, "gather_rids", 12 * total);
+                gather_u32(map, rids, g, total, st);
+                it = resolved.emplace(key, g).first;
+            }
+            rids = it->second;
+        }
+        auto [pages, np] = materialize_column(column(S.slots[cr.slot].table, cr.col), type, rids, total, true);
         out_cols[a] = {pages, {np, type}};
     }
 }

@@ -253,3 +253,36 @@ def test_gpu_intermediate_probes_direct_maps(engine, seed, flags):
     exp = cjtest.OracleRun(plan, threads=8)
     assert exp.num_rows > 500
     same_multiset(engine.execute(plan, flags=flags), exp)
+
+
+@pytest.mark.parametrize("flags", [0, 2], ids=["aligned", "general"])
+def test_gpu_keys_through_direct_map_rows(engine, flags):
+    """A non-root join over a map of row ids keeps map slots for its build
+    side; a later join keyed on a column of that side (probing, and then as
+    the build side of the root) reads its row ids through the map."""
+    rng = random.Random(4242)
+    p_rows = [[rng.randrange(0, 5000), i] for i in range(9000)]
+    d_keys = rng.sample(range(0, 20000), 4000)  # sparse unique keys: a map of row ids
+    d_rows = [[k, None if rng.random() < 0.05 else rng.randrange(0, 700), rng.randrange(-9, 9)] for k in d_keys]
+    e_keys = list(range(700))
+    rng.shuffle(e_keys)
+    e_rows = [[k, k * 3] for k in e_keys]
+    f_rows = [[rng.randrange(0, 9000), rng.randrange(100)] for _ in range(30000)]
+    tabs = [cjtest.table_from_rows(["INT32"] * 2, p_rows), cjtest.table_from_rows(["INT32"] * 3, d_rows),
+            cjtest.table_from_rows(["INT32"] * 2, e_rows), cjtest.table_from_rows(["INT32"] * 2, f_rows)]
+    plan = cjtest.Plan()
+    for t in tabs:
+        plan.new_input(t)
+    plan.new_scan_node(0, [(0, "INT32"), (1, "INT32")])
+    plan.new_scan_node(1, [(0, "INT32"), (1, "INT32"), (2, "INT32")])
+    plan.new_scan_node(2, [(0, "INT32"), (1, "INT32")])
+    plan.new_scan_node(3, [(0, "INT32"), (1, "INT32")])
+    # P.k = D.key -> (P.row, D.fk, D.pay)
+    j1 = plan.new_join_node(False, 0, 1, 0, 0, [(1, "INT32"), (3, "INT32"), (4, "INT32")])
+    # D.fk = E.key -> (P.row, D.pay, E.val)
+    j2 = plan.new_join_node(False, j1, 2, 1, 0, [(0, "INT32"), (2, "INT32"), (4, "INT32")])
+    # F.rowref = P.row, the intermediate as the build side -> (P.row, D.pay, E.val, F.x)
+    plan.root = plan.new_join_node(True, j2, 3, 0, 0, [(0, "INT32"), (1, "INT32"), (2, "INT32"), (4, "INT32")])
+    exp = cjtest.OracleRun(plan, threads=8)
+    assert exp.num_rows > 1000
+    same_multiset(engine.execute(plan, flags=flags), exp)
