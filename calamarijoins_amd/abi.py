@@ -20,6 +20,9 @@ CJ_EXEC_OUTPUT_DEVICE = 1
 CJ_EXEC_NO_FAST_PATH = 2
 CJ_EXEC_NO_DIRECT = 4
 CJ_EXEC_NO_ALIGNED = 8
+# cj_timing.phase_ms indices (the TimingStats phases)
+CJ_PHASE_HASHTABLE_BUILD, CJ_PHASE_HASH_JOIN_PROBE, CJ_PHASE_NESTED_LOOP_JOIN, CJ_PHASE_MATERIALIZE, \
+    CJ_PHASE_SETUP, CJ_PHASE_INTERMEDIATE = range(6)
 
 TYPE_NAMES = {"INT32": CJ_INT32, "INT64": CJ_INT64, "FP64": CJ_FP64, "VARCHAR": CJ_VARCHAR}
 TYPE_IDS = {v: k for k, v in TYPE_NAMES.items()}
@@ -76,6 +79,7 @@ class Timing(C.Structure):
         ("bytes_alg", C.c_uint64),
         ("queue_ms", C.c_double),
         ("service_ms", C.c_double),
+        ("phase_ms", C.c_double * 6),
     ]
 
 

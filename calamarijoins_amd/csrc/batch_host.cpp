@@ -264,7 +264,9 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         job_start[j + 1] = job_start[j] + dec_cols[j]->num_pages;
         valid_words += dec_cols[j]->num_rows / 32 + 2;
         val_bytes += ((dec_cols[j]->num_rows + 2) * width_of(dec_cols[j]->type) + 15) & ~15ull;
-        mixk(dec_cols[j]->num_rows), mixk(dec_cols[j]->type);
+        // the page count is captured as the decode unit count of the graph:
+        // equal row counts with another NULL density give other page counts
+        mixk(dec_cols[j]->num_rows), mixk(dec_cols[j]->type), mixk(dec_cols[j]->num_pages);
     }
     mixk(n), mixk(cfg.grid), mixk(cfg.load_factor), mixk(dec_cols.size());
     // ---- the template workspace (reused while the batch shape repeats)
@@ -363,6 +365,10 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         tpl->njobs = (uint32_t)jobs.size();
         tpl->ndpages = job_start.back();
     }
+    // a template whose captured decode units disagree with this batch's page
+    // counts would decode past (or short of) the pages: never replay it
+    if (tpl->ndpages != job_start.back() || tpl->njobs != (uint32_t)jobs.size())
+        throw DeviceError("batch template does not match the batch's carried columns");
     for (size_t j = 0; j < dec_cols.size(); ++j)
         jobs[j] = BDecode{dec_cols[j]->pages, dec_cols[j]->row_start, (int32_t)width_of(dec_cols[j]->type),
                           tpl->dec_values[j], tpl->dec_valid[j]};

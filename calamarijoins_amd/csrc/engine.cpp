@@ -149,6 +149,57 @@ struct TimedScope {
     ~TimedScope() { t.end(); }
 };
 
+// ========================================================== phase clock
+hipEvent_t Context::take_event() {
+    {
+        std::lock_guard<std::mutex> g(emu);
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+    }
+    hipEvent_t e;
+    CJ_HIP(hipEventCreate(&e));
+    return e;
+}
+void Context::give_event(hipEvent_t e) {
+    std::lock_guard<std::mutex> g(emu);
+    event_pool.push_back(e);
+}
+
+void PhaseClock::mark() {
+    hipEvent_t e = ctx->take_event();
+    marks.emplace_back(e, stack.empty() ? -1 : stack.back());
+    CJ_HIP(hipEventRecord(e, s));
+}
+void PhaseClock::push(int phase) {
+    if (!on) return;
+    stack.push_back(phase);
+    mark();
+}
+void PhaseClock::pop() {
+    if (!on || stack.empty()) return;
+    stack.pop_back();
+    mark();
+}
+void PhaseClock::collect(double *ms) {
+    for (size_t i = 0; i + 1 < marks.size(); ++i) {
+        if (marks[i].second < 0) continue;
+        float t = 0;
+        if (hipEventElapsedTime(&t, marks[i].first, marks[i + 1].first) == hipSuccess) ms[marks[i].second] += t;
+    }
+}
+PhaseClock::~PhaseClock() {
+    for (auto &m : marks) ctx->give_event(m.first);
+}
+
+struct PhaseScope {
+    PhaseClock &c;
+    PhaseScope(PhaseClock &cc, int phase) : c(cc) { c.push(phase); }
+    ~PhaseScope() { c.pop(); }
+};
+
 // Algorithmic (compulsory) bytes of each launch, for the per-kernel
 // roofline: what the kernel must read and write at minimum.
 static uint64_t decode_bytes(const DevColumn &c) {
@@ -171,6 +222,8 @@ Context::Context(const cj_ctx_options &o) : opts(o), pool(0) {
     hipDeviceProp_t prop;
     CJ_HIP(hipGetDeviceProperties(&prop, device));
     dev_name = prop.name;
+    if (dev_name.empty()) dev_name = prop.gcnArchName;
+    else dev_name += std::string(" (") + prop.gcnArchName + ")";
     num_cus = prop.multiProcessorCount;
     max_lds = prop.sharedMemPerBlock;
     size_t free_b = 0, total_b = 0;
@@ -184,6 +237,7 @@ Context::Context(const cj_ctx_options &o) : opts(o), pool(0) {
 Context::~Context() {
     hipSetDevice(device);
     hipDeviceSynchronize();
+    for (hipEvent_t e : event_pool) hipEventDestroy(e);
     release_batch_templates();
     for (auto &kv : tables)
         for (auto &c : kv.second->cols) free_column(*c);
@@ -474,6 +528,8 @@ class Exec {
     Exec(Context &c, const cj_plan_desc &p, hipStream_t s) : ctx(c), plan(p), st(s) {
         timer.on = c.opts.kernel_timing != 0;
         timer.s = s;
+        clock.s = s;
+        clock.ctx = &c;
         cols.resize(p.num_inputs);
         for (uint32_t i = 0; i < p.num_inputs; ++i) cols[i].resize(p.inputs[i].num_columns, nullptr);
     }
@@ -488,6 +544,7 @@ class Exec {
     const cj_plan_desc &plan;
     hipStream_t st;
     KernelTimer timer;
+    PhaseClock clock;  // on when the caller asked for timing
     std::vector<void *> temps;
     std::vector<std::unique_ptr<DevColumn>> own_cols;
     std::vector<std::vector<DevColumn *>> cols;
@@ -531,6 +588,7 @@ class Exec {
     ColView view(DevColumn *c) {
         auto it = views.find(c);
         if (it != views.end()) return it->second;
+        PhaseScope ps(clock, CJ_PHASE_SETUP);
         ColView v;
         v.col = c;
         bool shared = false;
@@ -677,6 +735,7 @@ class Exec {
     // Hash build: partition pass (histogram, scan, scatter) then one
     // workgroup per partition.
     HashTableDev build_hash(const KeySource &k, uint64_t rows, uint64_t cap_rows) {
+        PhaseScope ps(clock, CJ_PHASE_HASHTABLE_BUILD);
         int log_slots = 11;  // directory >= 2048 slots (hashtable.h:255-264)
         while ((1ull << log_slots) < rows) ++log_slots;
         // <= 4096 partitions (LDS histogram of the partition pass) and
@@ -789,6 +848,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         ProbeArgs fa{};
         fa.mode = kModeHash;
         if (use_nl) {
+            PhaseScope ps(clock, CJ_PHASE_NESTED_LOOP_JOIN);
             nl = tmp<NLKeys>(1);
             TimedScope ts(timer, "nl_collect", 0);
             nl_collect(bk, nl, st);
@@ -815,6 +875,9 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         fa.key = pk;
         fa.ht = ht;
         fa.nl = nl;
+        // counting pass .. pair lists: the probe phase (the root writers
+        // below open the materialize phase)
+        PhaseScope probe_phase(clock, use_nl ? CJ_PHASE_NESTED_LOOP_JOIN : CJ_PHASE_HASH_JOIN_PROBE);
         uint32_t units = key_source_units(pk);
         uint64_t *cnt = tmp<uint64_t>(units + 1);
         uint64_t *off = tmp<uint64_t>(units + 1);
@@ -885,6 +948,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         return Rel{};
     }
     // Non-root: late-materialized intermediate (row-id list per slot).
+    PhaseScope inter_phase(clock, CJ_PHASE_INTERMEDIATE);
     Rel res;
     res.columnar = false;
     res.rows = total;
@@ -961,6 +1025,7 @@ bool Exec::fused_ok(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t 
 // when decoded columns are cached, otherwise private to this call.
 const uint16_t *Exec::page_index(DevColumn *c) {
     if (c->byte_prefix) return c->byte_prefix;
+    PhaseScope ps(clock, CJ_PHASE_SETUP);
     bool own = false;
     for (auto &o : own_cols) own |= o.get() == c;
     TimedScope ts(timer, "page_index", c->bitmap_bytes * 3);
@@ -990,6 +1055,7 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     if (flags & CJ_EXEC_NO_DIRECT) return false;
     DevColumn *kc = column(B.slots[0].table, B.cols[battr].col);
     if (kc->num_rows == 0) return false;
+    PhaseScope ps(clock, CJ_PHASE_HASHTABLE_BUILD);
     // record: 8-byte fields, then 4-byte fields, then the u32 of NULL flags
     DirectBuildArgs a{};
     std::vector<uint32_t> build_attrs;
@@ -1086,6 +1152,7 @@ static uint32_t mat_cols_per_wg(bool from_build, uint32_t width, uint32_t ncols)
 
 void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
                       const std::vector<uint32_t> &remap, const ProbeArgs &a, const uint64_t *off, uint64_t total) {
+    PhaseScope ps(clock, CJ_PHASE_MATERIALIZE);
     const bool direct = a.mode == kModeDirect;
     bool need_b = false, need_p = false;
     for (uint32_t i = 0; i < n.num_output_attrs; ++i) (remap[i] < bw ? need_b : need_p) = true;
@@ -1166,6 +1233,7 @@ void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_
 void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
                         const std::vector<uint32_t> &remap, const ProbeArgs &a, const uint64_t *cnt,
                         const uint32_t *match_bits, uint64_t total) {
+    PhaseScope ps(clock, CJ_PHASE_MATERIALIZE);
     out_rows = total;
     out_types.assign(n.num_output_attrs, 0);
     out_cols.assign(n.num_output_attrs, {nullptr, {0u, 0}});
@@ -1244,6 +1312,7 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
 void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,
                             const std::vector<uint32_t> &remap, uint64_t total, const uint32_t *ob,
                             const uint32_t *op, cj_result *) {
+    PhaseScope ps(clock, CJ_PHASE_MATERIALIZE);
     out_rows = total;
     out_types.clear();
     out_cols.clear();
@@ -1384,6 +1453,7 @@ void Exec::filter_root(const std::vector<FNode> &prog, const std::vector<uint32_
         sel_rids(sel, rows, off, rids, st);
     }
     out_rows = total;
+    PhaseScope ps(clock, CJ_PHASE_MATERIALIZE);
     // a filtered base table keeps its values: no INT32_MIN quirk
     // (from_inner_to_column, build_table.cpp:322-360)
     for (uint32_t c = 0; c < t.num_columns; ++c) {
@@ -1441,6 +1511,7 @@ static void run_exec(Context &ctx, const cj_plan_desc &plan, uint32_t flags, cj_
     Exec ex(ctx, plan, s);
     ex.device_output = (flags & CJ_EXEC_OUTPUT_DEVICE) != 0;
     ex.flags = flags;
+    ex.clock.on = timing != nullptr;
     hipEvent_t e0, e1, e2;
     CJ_HIP(hipEventCreate(&e0));
     CJ_HIP(hipEventCreate(&e1));
@@ -1468,6 +1539,7 @@ static void run_exec(Context &ctx, const cj_plan_desc &plan, uint32_t flags, cj_
             impl->host_block = pinned.alloc(total_pages * kPage);
             uint64_t at = 0;
             CJ_HIP(hipEventRecord(e1, s));
+            ex.clock.push(CJ_PHASE_MATERIALIZE);  // the pages' copy to host memory
             for (size_t i = 0; i < impl->cols.size(); ++i) {
                 auto &c = impl->cols[i];
                 uint8_t *h = static_cast<uint8_t *>(impl->host_block) + at * kPage;
@@ -1478,6 +1550,7 @@ static void run_exec(Context &ctx, const cj_plan_desc &plan, uint32_t flags, cj_
                 at += c.num_pages;
             }
             CJ_HIP(hipEventRecord(e2, s));
+            ex.clock.pop();
         }
         CJ_HIP(hipStreamSynchronize(s));
         if (!ex.device_output) {
@@ -1502,6 +1575,12 @@ static void run_exec(Context &ctx, const cj_plan_desc &plan, uint32_t flags, cj_
             timing->device_ms = dev_ms;
             timing->h2d_ms = ex.h2d_ms;
             timing->d2h_ms = d2h;
+            ex.clock.collect(timing->phase_ms);
+            timing->phase_ms[CJ_PHASE_SETUP] += ex.h2d_ms;  // host clock: uploads before the first kernel
+            int64_t *whole[CJ_NUM_PHASES] = {&timing->hashtable_build_ms, &timing->hash_join_probe_ms,
+                                             &timing->nested_loop_join_ms, &timing->materialize_ms,
+                                             &timing->setup_ms, &timing->intermediate_ms};
+            for (int k = 0; k < CJ_NUM_PHASES; ++k) *whole[k] = (int64_t)timing->phase_ms[k];
             uint64_t in_bytes = 0;
             for (DevColumn *c : ex.referenced) in_bytes += (uint64_t)c->num_pages * kPage;
             uint64_t out_bytes = 0;

@@ -100,6 +100,27 @@ struct KernelTimer {
     void collect(std::map<std::string, KernelAgg> &acc);
 };
 
+// Phase clock of one execution (cj_timing's phases): a hipEvent on the
+// execution stream at every phase change; the interval between two marks
+// belongs to the phase that was current after the first.  Phases nest: an
+// inner phase (a page decode inside a build) pauses the outer one, so no
+// interval is counted twice.
+struct PhaseClock {
+    bool on = false;
+    hipStream_t s = nullptr;
+    class Context *ctx = nullptr;
+    std::vector<std::pair<hipEvent_t, int>> marks;  // event, phase current after it (-1: none)
+    std::vector<int> stack;
+    void push(int phase);
+    void pop();
+    // adds each phase's milliseconds to ms[] (the stream must be idle)
+    void collect(double *ms);
+    ~PhaseClock();
+
+  private:
+    void mark();
+};
+
 struct ExecStream {
     hipStream_t s = nullptr;
     bool busy = false;
@@ -171,6 +192,12 @@ class Context {
     std::mutex smu;
     std::condition_variable scv;
     std::vector<ExecStream> streams;
+
+    // timing events recycled across executions (phase clocks)
+    hipEvent_t take_event();
+    void give_event(hipEvent_t e);
+    std::mutex emu;
+    std::vector<hipEvent_t> event_pool;
 };
 
 // Result bookkeeping hidden behind cj_result::opaque.

@@ -92,7 +92,27 @@ typedef struct cj_plan_desc {
     uint64_t root;
 } cj_plan_desc;
 
-/* Reference Contest::TimingStats (plan.h:370-378), plus device detail. */
+/* Reference Contest::TimingStats (plan.h:370-378), plus device detail.
+ * The int64 phase fields are whole milliseconds, truncated as the reference
+ * truncates them (execute.cpp:123-190, 224-266); phase_ms holds the same
+ * phases untruncated.  Phases are measured with hipEvents on the execution
+ * stream (upload of non-resident pages with the host clock) and do not
+ * overlap, so their sum never exceeds total_execution_ms:
+ *   setup            upload of non-resident input pages, page decode, page indexes
+ *   hashtable_build  direct-map / hash-table build of a join's build side
+ *   hash_join_probe  counting pass, offsets, (build, probe) row-id pairs
+ *   nested_loop_join the same for build sides of fewer than 8 rows
+ *   materialize      root result pages, and their copy to host memory
+ *   intermediate     row-id lists of non-root join results */
+enum {
+    CJ_PHASE_HASHTABLE_BUILD = 0,
+    CJ_PHASE_HASH_JOIN_PROBE = 1,
+    CJ_PHASE_NESTED_LOOP_JOIN = 2,
+    CJ_PHASE_MATERIALIZE = 3,
+    CJ_PHASE_SETUP = 4,
+    CJ_PHASE_INTERMEDIATE = 5,
+    CJ_NUM_PHASES = 6
+};
 typedef struct cj_timing {
     int64_t hashtable_build_ms;
     int64_t hash_join_probe_ms;
@@ -107,6 +127,7 @@ typedef struct cj_timing {
     uint64_t bytes_alg; /* algorithmic bytes: input pages read + output written */
     double queue_ms;    /* served plans: submit .. admitted (0 for cj_execute) */
     double service_ms;  /* served plans: admitted .. result ready              */
+    double phase_ms[CJ_NUM_PHASES]; /* CJ_PHASE_* order, untruncated             */
 } cj_timing;
 
 /* A result column. Host output: pages[i] are host pointers valid until

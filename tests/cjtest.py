@@ -145,6 +145,21 @@ def result_rows(r):
     return [tuple(c[i] for c in cols) for i in range(r.num_rows)]
 
 
+def result_fingerprint(r):
+    """Order-independent fingerprint of a result's row multiset: row count,
+    and the wrapping sums of the row hashes and of their squares (for very
+    large results, where sorting every row hash would dominate the test)."""
+    validate_result(r)
+    keep = []
+    desc = result_table_desc(r, keep)
+    out = np.zeros(max(r.num_rows, 1), dtype=np.uint64)
+    rc = oracle().cjo_row_hashes(C.byref(desc), out.ctypes.data)
+    assert rc == 0, oracle().cjo_last_error()
+    h = out[: r.num_rows]
+    with np.errstate(over="ignore"):
+        return (int(r.num_rows), int(h.sum(dtype=np.uint64)), int((h * h).sum(dtype=np.uint64)))
+
+
 def result_table_desc(r, keep: list):
     cols = (abi.ColumnDesc * max(len(r.types), 1))()
     for i, (t, pl) in enumerate(zip(r.types, r.page_lists)):
@@ -155,8 +170,85 @@ def result_table_desc(r, keep: list):
     return abi.TableDesc(r.num_rows, len(r.types), cols)
 
 
+_POP8 = np.array([bin(i).count("1") for i in range(256)], dtype=np.int32)
+
+
+def _column_pages(pl) -> np.ndarray:
+    """A column's pages (list of host addresses) as an (n, 8192) uint8 array
+    (a view when they are contiguous)."""
+    n = len(pl)
+    if n == 0:
+        return np.zeros((0, PAGE), dtype=np.uint8)
+    if all(pl[i + 1] - pl[i] == PAGE for i in range(n - 1)):
+        return np.ctypeslib.as_array((C.c_uint8 * (n * PAGE)).from_address(pl[0])).reshape(n, PAGE)
+    out = np.empty((n, PAGE), dtype=np.uint8)
+    for i, a in enumerate(pl):
+        out[i] = np.ctypeslib.as_array((C.c_uint8 * PAGE).from_address(a))
+    return out
+
+
+def validate_pages(type_id: int, pages: np.ndarray, where: str = "") -> int:
+    """Structural check of a column's pages against the page format
+    (reference README.md:65-104): a value-page holds 1..65535 rows;
+    n_vals equals the number of set bits among the page's n_rows bitmap bits;
+    values (or VARCHAR offsets and chars) plus the bitmap fit in 8192 bytes;
+    VARCHAR end offsets never decrease; 0xffff / 0xfffe long-string pages
+    only in VARCHAR columns, 0xfffe only after 0xffff or 0xfffe, each with
+    1..8188 chars.  Returns the number of rows the pages describe."""
+    n = pages[:, 0].astype(np.int64) | (pages[:, 1].astype(np.int64) << 8)
+    nv = pages[:, 2].astype(np.int64) | (pages[:, 3].astype(np.int64) << 8)
+    long_pg = n >= 0xfffe
+    if type_id != abi.CJ_VARCHAR:
+        assert not long_pg.any(), f"{where}: long-string page in a fixed-width column"
+    if long_pg.any():
+        idx = np.nonzero(n == 0xfffe)[0]
+        assert (idx > 0).all() and long_pg[idx - 1].all(), f"{where}: 0xfffe page without a 0xffff/0xfffe before it"
+        assert ((nv[long_pg] >= 1) & (nv[long_pg] <= PAGE - 4)).all(), f"{where}: long-string page char count"
+    vp = ~long_pg
+    assert (n[vp] >= 1).all(), f"{where}: page without rows"
+    assert (nv[vp] <= n[vp]).all(), f"{where}: more values than rows"
+    nb = (n + 7) // 8
+    w = {abi.CJ_INT32: 4, abi.CJ_INT64: 8, abi.CJ_FP64: 8}.get(type_id, 0)
+    if type_id != abi.CJ_VARCHAR:
+        head = 4 if w == 4 else 8
+        assert (head + w * nv[vp] + nb[vp] <= PAGE).all(), f"{where}: values and bitmap overflow the page"
+    # n_vals == popcount of the first n_rows bitmap bits (the bitmap is the
+    # page's last ceil(n_rows / 8) bytes, row j at byte PAGE - nb + j / 8)
+    for a in range(0, len(n), 2048):
+        sl = slice(a, min(a + 2048, len(n)))
+        pg, nn, bb, ok = pages[sl], n[sl], nb[sl], vp[sl]
+        # only the page tails that hold bitmaps (the widest in this chunk)
+        span = int(bb[ok].max()) if ok.any() else 1
+        tail = pg[:, PAGE - span:]
+        inside = np.arange(PAGE - span, PAGE)[None, :] >= (PAGE - bb)[:, None]
+        cnt = (_POP8[tail] * inside).sum(axis=1)
+        r = nn % 8
+        extra = np.where(r > 0, _POP8[(pg[:, PAGE - 1].astype(np.int64) >> r) & 0xFF], 0)
+        cnt = cnt - extra
+        bad = ok & (cnt != nv[sl])
+        assert not bad.any(), f"{where}: n_vals != bitmap popcount on page {a + int(np.argmax(bad))}"
+    if type_id == abi.CJ_VARCHAR:
+        for i in np.nonzero(vp)[0]:
+            k = int(nv[i])
+            offs = pages[i, 4:4 + 2 * k].view(np.uint16).astype(np.int64) if k else np.zeros(0, np.int64)
+            assert (np.diff(offs) >= 0).all() if k > 1 else True, f"{where}: VARCHAR offsets decrease on page {i}"
+            end = 4 + 2 * k + (int(offs[-1]) if k else 0)
+            assert end + int(nb[i]) <= PAGE, f"{where}: VARCHAR chars and bitmap overflow page {i}"
+    return int(n[vp].sum() + (n == 0xffff).sum())
+
+
+def validate_result(r) -> None:
+    """validate_pages on every column of a result; each column describes
+    exactly num_rows rows."""
+    for i, (t, pl) in enumerate(zip(r.types, r.page_lists)):
+        rows = validate_pages(t, _column_pages(pl), where=f"column {i}")
+        assert rows == r.num_rows, (i, rows, r.num_rows)
+
+
 def result_row_hashes(r) -> np.ndarray:
-    """Sorted per-row hashes: equal arrays <=> equal row multisets (w.h.p.)."""
+    """Sorted per-row hashes: equal arrays <=> equal row multisets (w.h.p.).
+    The pages are checked structurally first (validate_result)."""
+    validate_result(r)
     keep = []
     desc = result_table_desc(r, keep)
     out = np.zeros(max(r.num_rows, 1), dtype=np.uint64)
@@ -206,6 +298,7 @@ def check_kat(case, result):
     """Assert a result against a KAT's expectations."""
     exp = case["expect"]
     assert [abi.TYPE_IDS[t] for t in result.types] == exp["types"], result.types
+    validate_result(result)
     if "num_rows" in exp:
         assert result.num_rows == exp["num_rows"]
     if exp.get("num_rows_positive"):

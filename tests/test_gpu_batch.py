@@ -169,3 +169,43 @@ def test_gpu_batch_wide_and_duplicate_plans(engine, wide_plans):
     for r in res:
         r.to_host()
     check(res, alone, range(len(plans)))
+
+
+def _carried_plan(seed, null_frac):
+    """A one-join plan whose build side carries an INT64 column with
+    `null_frac` NULLs: the same row counts give other page counts."""
+    b = workloads.generate(30_000, seed, [workloads.col(workloads.GEN_UNIFORM, "INT32", 1, 0.0, lo=0, hi=20_000),
+                                          workloads.col(workloads.GEN_I64_FULL, "INT64", 2, null_frac)])
+    p = workloads.generate(90_000, seed + 1, [workloads.col(workloads.GEN_UNIFORM, "INT32", 3, 0.0, lo=0, hi=20_000),
+                                              workloads.col(workloads.GEN_I32_FULL, "INT32", 4, 0.05)])
+    pl = cjtest.Plan()
+    pl.new_input(b)
+    pl.new_input(p)
+    pl.new_scan_node(0, [(0, "INT32"), (1, "INT64")])
+    pl.new_scan_node(1, [(0, "INT32"), (1, "INT32")])
+    pl.root = pl.new_join_node(True, 0, 1, 0, 0, [(1, "INT64"), (3, "INT32"), (0, "INT32")])
+    return b, p, pl
+
+
+def test_gpu_batch_template_carried_page_counts(engine):
+    """Two batches with equal row counts whose carried INT64 column has a
+    different NULL density (so a different page count) must not share a
+    captured template: each result equals its per-plan result."""
+    made = [_carried_plan(501, 0.0), _carried_plan(503, 0.6)]
+    assert made[0][0].num_rows == made[1][0].num_rows and made[0][1].num_rows == made[1][1].num_rows
+    assert made[0][0].columns[1].num_pages != made[1][0].columns[1].num_pages
+    ids = []
+    try:
+        for b, p, _ in made:
+            ids += [engine.register(b), engine.register(p)]
+        for rep in range(2):
+            for b, p, pl in (made if rep == 0 else made[::-1]):
+                r = engine.execute(pl)
+                alone = [(r.num_rows, cjtest.result_row_hashes(r))]
+                r.release()
+                res, st = engine.execute_batch([pl])
+                assert st.plans_batched == 1
+                check(res, alone, [0])
+    finally:
+        for t in ids:
+            engine.unregister(t)

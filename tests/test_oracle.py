@@ -118,6 +118,9 @@ def rand_value(rng, t, key_domain):
         return rng.randrange(-2**63, 2**63)
     if t == "FP64":
         return rng.random() * 1e6
+    if rng.random() < 0.03:
+        # long strings: 0xffff / 0xfffe page runs above 8185 bytes
+        return "".join(chr(rng.randrange(97, 123)) for _ in range(rng.randrange(8185, 20001)))
     return "".join(chr(rng.randrange(97, 123)) for _ in range(rng.randrange(0, 12)))
 
 
@@ -208,3 +211,97 @@ def test_oracle_multithreaded_partitioned_build():
     cols = [cjtest.decode_column_arrays(abi.CJ_INT32, pl, len(pl)) for pl in got.page_lists]
     assert np.array_equal(cols[0][0], cols[1][0])
     assert np.array_equal(np.sort(cols[1][0]), np.sort(pk[pk < n]))
+
+
+# ------------------------------------------------ the checkers themselves
+def _hash_rows(types, rows):
+    t = cjtest.table_from_rows(types, rows)
+
+    class R:
+        pass
+
+    r = R()
+    r.num_rows, r.types = t.num_rows, [c.type for c in t.columns]
+    r.page_lists = [[c.pages.ctypes.data + i * cjtest.PAGE for i in range(c.num_pages)] for c in t.columns]
+    r.keep = t
+    return cjtest.result_row_hashes(r)
+
+
+HASH_TYPES = ["INT32", "INT64", "FP64", "VARCHAR", "INT32"]
+HASH_ROWS = [[i, -7 * i, i / 3.0, "s%d" % (i % 5) if i % 4 else None, None if i % 6 == 0 else 2 * i]
+             for i in range(300)]
+
+
+def test_row_hashes_ignore_row_order():
+    base = _hash_rows(HASH_TYPES, HASH_ROWS)
+    perm = list(HASH_ROWS)
+    random.Random(5).shuffle(perm)
+    assert np.array_equal(base, _hash_rows(HASH_TYPES, perm))
+
+
+@pytest.mark.parametrize("change", ["value", "null", "byte", "swap_cols", "dup_row", "fp_sign", "string_to_null"])
+def test_row_hashes_tell_rows_apart(change):
+    """A changed value, NULL bit, string byte, a swap of two same-typed
+    columns in one row, or a duplicated row changes the sorted hash array."""
+    base = _hash_rows(HASH_TYPES, HASH_ROWS)
+    rows = [list(r) for r in HASH_ROWS]
+    if change == "value":
+        rows[17][1] += 1
+    elif change == "null":
+        rows[19][4] = None
+    elif change == "byte":
+        rows[21][3] = "s2x"[:2] + "y"
+    elif change == "swap_cols":
+        rows[25][0], rows[25][4] = rows[25][4], rows[25][0]
+    elif change == "dup_row":
+        rows[0] = list(rows[1])
+    elif change == "fp_sign":
+        rows[0][2] = -0.0
+    else:
+        rows[22][3] = None
+    assert change == "fp_sign" or rows != [list(r) for r in HASH_ROWS]  # -0.0 == 0.0 in Python
+    assert not np.array_equal(base, _hash_rows(HASH_TYPES, rows))
+
+
+def _pages_of(type_name, values):
+    return cjtest.column_from_values(type_name, values).pages.copy()
+
+
+def test_validator_accepts_codec_pages():
+    rng = random.Random(9)
+    for t in ("INT32", "INT64", "FP64"):
+        vals = [None if rng.random() < 0.3 else rng.randrange(1000) for _ in range(7000)]
+        pg = _pages_of(t, vals)
+        assert cjtest.validate_pages(abi.TYPE_NAMES[t], pg) == len(vals)
+    strs = [None if rng.random() < 0.2 else "x" * rng.randrange(0, 70) for _ in range(3000)] + ["y" * 20000, "z"]
+    pg = _pages_of("VARCHAR", strs)
+    assert cjtest.validate_pages(abi.CJ_VARCHAR, pg) == len(strs)
+
+
+@pytest.mark.parametrize("mistake", ["n_vals", "overflow", "offsets", "lone_fffe", "long_in_fixed", "no_rows",
+                                     "bitmap_bit"])
+def test_validator_rejects_malformed_pages(mistake):
+    """Mistakes planted in oracle-encoded pages (on the CPU) are caught."""
+    if mistake in ("offsets", "lone_fffe"):
+        pg = _pages_of("VARCHAR", ["abc", "de", None, "fghij"] * 50 + ["q" * 9000])
+        t = abi.CJ_VARCHAR
+        if mistake == "offsets":
+            pg[0, 4:6] = np.array([200, 0], dtype=np.uint8)  # first end offset beyond the second
+        else:
+            pg[0, 0:2] = np.array([0xfe, 0xff], dtype=np.uint8)  # page 0 claims to continue a long string
+    else:
+        pg = _pages_of("INT32", [None if i % 3 == 0 else i for i in range(5000)])
+        t = abi.CJ_INT32
+        if mistake == "n_vals":
+            pg[1, 2] ^= 1
+        elif mistake == "overflow":
+            pg[0, 0:2] = np.array([0x00, 0x20], dtype=np.uint8)  # 8192 rows: bitmap alone fills the page
+            pg[0, 2:4] = np.array([0x00, 0x08], dtype=np.uint8)
+        elif mistake == "long_in_fixed":
+            pg[1, 0:2] = np.array([0xff, 0xff], dtype=np.uint8)
+        elif mistake == "no_rows":
+            pg[1, 0:4] = 0
+        else:
+            pg[0, cjtest.PAGE - 3] ^= 0x10  # a bitmap bit flipped: popcount no longer n_vals
+    with pytest.raises(AssertionError):
+        cjtest.validate_pages(t, pg)
