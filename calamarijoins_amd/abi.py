@@ -20,6 +20,7 @@ CJ_EXEC_OUTPUT_DEVICE = 1
 CJ_EXEC_NO_FAST_PATH = 2
 CJ_EXEC_NO_DIRECT = 4
 CJ_EXEC_NO_ALIGNED = 8
+CJ_NOALLOC_SCRATCH_BYTES = 256 << 20
 # cj_timing.phase_ms indices (the TimingStats phases)
 CJ_PHASE_HASHTABLE_BUILD, CJ_PHASE_HASH_JOIN_PROBE, CJ_PHASE_NESTED_LOOP_JOIN, CJ_PHASE_MATERIALIZE, \
     CJ_PHASE_SETUP, CJ_PHASE_INTERMEDIATE = range(6)
@@ -177,6 +178,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "cj_table_unregister": (C.c_int, [C.c_void_p, C.c_uint64]),
         "cj_execute": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, P(Result), P(Timing)]),
         "cj_result_release": (C.c_int, [C.c_void_p, P(Result)]),
+        "cj_execute_noalloc": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, P(C.c_void_p), P(C.c_uint32),
+                                         C.c_uint32, P(C.c_uint64), P(C.c_uint32)]),
         "cj_result_to_host": (C.c_int, [C.c_void_p, P(Result)]),
         "cj_kernel_stats": (C.c_int, [C.c_void_p, P(KernelStat), C.c_uint32, P(C.c_uint32)]),
         "cj_kernel_stats_reset": (C.c_int, [C.c_void_p]),

@@ -134,6 +134,17 @@ int cj_execute(cj_ctx *ctx, const cj_plan_desc *plan, uint32_t flags, cj_result 
     });
 }
 
+int cj_execute_noalloc(cj_ctx *ctx, const cj_plan_desc *plan, uint32_t flags, void *const *out_pages,
+                       const uint32_t *out_cap_pages, uint32_t num_out, uint64_t *out_rows,
+                       uint32_t *out_num_pages) {
+    return guard([&] {
+        if (!ctx || !plan || !out_rows || (num_out && (!out_pages || !out_cap_pages || !out_num_pages)))
+            throw cj::InvalidError("null argument");
+        CJ_HIP(hipSetDevice(C(ctx)->device));
+        C(ctx)->execute_noalloc(*plan, flags, out_pages, out_cap_pages, num_out, out_rows, out_num_pages);
+    });
+}
+
 int cj_result_release(cj_ctx *ctx, cj_result *res) {
     return guard([&] {
         if (!ctx) throw cj::InvalidError("null context");

@@ -19,6 +19,7 @@
 #include "fused.hpp"
 #include "pagemat.hpp"
 #include "filter.hpp"
+#include "noalloc.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -226,6 +227,7 @@ Context::Context(const cj_ctx_options &o) : opts(o), pool(0) {
     else dev_name += std::string(" (") + prop.gcnArchName + ")";
     num_cus = prop.multiProcessorCount;
     max_lds = prop.sharedMemPerBlock;
+    noalloc_init(device);  // this GPU's scratch of cj_execute_noalloc (held until the process ends)
     size_t free_b = 0, total_b = 0;
     CJ_HIP(hipMemGetInfo(&free_b, &total_b));
     pool.set_budget(o.device_budget ? o.device_budget : (uint64_t)(free_b * 0.92));
@@ -534,7 +536,7 @@ class Exec {
         for (uint32_t i = 0; i < p.num_inputs; ++i) cols[i].resize(p.inputs[i].num_columns, nullptr);
     }
     ~Exec() {
-        for (void *p : temps) ctx.pool.free(p);
+        for (void *p : temps) ctx.pool.free(p);  // (none in no-allocation mode)
         for (auto &c : own_cols) ctx.free_column(*c);
         if (rb) ctx.pinned.free(rb);
         if (rb_event) (void)hipEventDestroy(rb_event);
@@ -555,17 +557,48 @@ class Exec {
     uint64_t *scratch = nullptr;
     double h2d_ms = 0;
 
+    // No-allocation mode (cj_execute_noalloc): temporaries are carved from
+    // the GPU's fixed scratch and result pages go to the caller's buffers.
+    uint8_t *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    void *const *user_out = nullptr;
+    const uint32_t *user_cap = nullptr;
+    uint32_t user_n = 0;
+    bool noalloc() const { return arena != nullptr; }
+
     template <class T>
     T *tmp(size_t n) {
+        if (arena) {
+            const size_t bytes = (n * sizeof(T) + 16 + 255) & ~size_t(255);
+            if (arena_used + bytes > arena_cap)
+                throw OomError("no-allocation scratch exhausted: the plan needs more than " +
+                               std::to_string(arena_cap) + " bytes of temporaries (" +
+                               std::to_string(arena_used + bytes) + " so far)");
+            T *p = reinterpret_cast<T *>(arena + arena_used);
+            arena_used += bytes;
+            return p;
+        }
         T *p = ctx.dalloc<T>(n);
         temps.push_back(p);
         return p;
+    }
+    // Room for `need` pages of result column `attr`: a pool block, or the
+    // caller's buffer in no-allocation mode.
+    uint8_t *out_pages(uint32_t attr, uint64_t need) {
+        if (!noalloc()) return static_cast<uint8_t *>(ctx.pool.alloc((size_t)need * kPage + 16));
+        if (attr >= user_n) throw InvalidError("cj_execute_noalloc: fewer output buffers than result columns");
+        if (need > user_cap[attr])
+            throw OomError("cj_execute_noalloc: result column " + std::to_string(attr) + " needs room for " +
+                           std::to_string(need) + " pages, its buffer holds " + std::to_string(user_cap[attr]));
+        return static_cast<uint8_t *>(user_out[attr]);
     }
 
     DevColumn *column(uint32_t table, uint32_t c) {
         if (cols[table][c]) return cols[table][c];
         const cj_column_desc &d = plan.inputs[table].columns[c];
         DevColumn *dc = ctx.find_resident(d);
+        if (!dc && noalloc())
+            throw UnsupportedError("cj_execute_noalloc needs every input registered as a resident table");
         if (!dc) {
             auto t0 = std::chrono::steady_clock::now();
             own_cols.push_back(std::make_unique<DevColumn>());  // owned first: freed if the upload throws
@@ -594,7 +627,12 @@ class Exec {
         bool shared = false;
         for (auto &o : own_cols) shared |= o.get() == c;
         shared = !shared;
-        if (shared && ctx.opts.cache_decoded) {
+        bool cached = false;
+        if (shared && noalloc()) {  // a decode cached earlier serves; none is made here
+            std::lock_guard<std::mutex> g(c->mu);
+            cached = c->values != nullptr;
+        }
+        if (cached || (shared && ctx.opts.cache_decoded && !noalloc())) {
             std::lock_guard<std::mutex> g(c->mu);
             if (!c->values) {
                 TimedScope ts(timer, c->type == CJ_VARCHAR ? "decode_varchar" : "decode_fixed", decode_bytes(*c));
@@ -643,6 +681,10 @@ class Exec {
     uint64_t *rb = nullptr;
     hipEvent_t rb_event = nullptr;  // a read-back's copy is done (the stream may go on)
     uint64_t *readback() {
+        if (noalloc()) {  // pageable words (the copy is staged by the runtime)
+            thread_local uint64_t words[kReadbackWords];
+            return words;
+        }
         if (!rb) rb = static_cast<uint64_t *>(ctx.pinned.alloc(kReadbackWords * 8));
         return rb;
     }
@@ -799,8 +841,9 @@ class Exec {
                           cj_result *out);
     // pages of base column c gathered by row ids (rows of them); int32_quirk:
     // INT32_MIN values are written as NULL (result page builders only)
+    // (attr: the result column the pages belong to)
     std::pair<uint8_t *, uint32_t> materialize_column(DevColumn *c, int32_t type, const uint32_t *rids, uint64_t rows,
-                                                      bool int32_quirk);
+                                                      bool int32_quirk, uint32_t attr);
     // filter of input table 0: the rows passing the program, every column
     void filter_root(const std::vector<FNode> &prog, const std::vector<uint32_t> &used,
                      const std::vector<uint8_t> &lits);
@@ -890,7 +933,9 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         // bits so the pair writer reads the keys of the matched rows only
         const bool inter_bits = fa.mode == kModeDirect && pk.mode == KEY_INTER && !fused &&
                                 !(flags & CJ_EXEC_NO_ALIGNED);
-        const bool aligned = fused && bits;
+        // (not in no-allocation mode: the page-aligned writers' page bound is
+        // only known after the call, so caller buffers could not be sized)
+        const bool aligned = fused && bits && !noalloc();
         uint32_t *match_bits = nullptr;
         const uint64_t probe_rows = pk.mode == KEY_COLUMNAR ? column(P.slots[0].table, P.cols[pattr].col)->num_rows
                                                             : pk.mode == KEY_INTER ? pk.rows : 0;
@@ -913,10 +958,15 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         if (aligned) {
             uint64_t *h = readback();
             CJ_HIP(hipMemcpyAsync(h, off + units, 8, hipMemcpyDeviceToHost, st));
-            if (!rb_event) CJ_HIP(hipEventCreateWithFlags(&rb_event, hipEventDisableTiming));
-            CJ_HIP(hipEventRecord(rb_event, st));
-            page_offsets(cnt, units);
-            CJ_HIP(hipEventSynchronize(rb_event));
+            if (noalloc()) {
+                CJ_HIP(hipStreamSynchronize(st));
+                page_offsets(cnt, units);
+            } else {
+                if (!rb_event) CJ_HIP(hipEventCreateWithFlags(&rb_event, hipEventDisableTiming));
+                CJ_HIP(hipEventRecord(rb_event, st));
+                page_offsets(cnt, units);
+                CJ_HIP(hipEventSynchronize(rb_event));
+            }
             total = h[0];
         } else {
             total = read_u64(off + units);
@@ -1029,7 +1079,7 @@ const uint16_t *Exec::page_index(DevColumn *c) {
     bool own = false;
     for (auto &o : own_cols) own |= o.get() == c;
     TimedScope ts(timer, "page_index", c->bitmap_bytes * 3);
-    if (own || ctx.opts.cache_decoded) {
+    if (!noalloc() && (own || ctx.opts.cache_decoded)) {
         std::lock_guard<std::mutex> g(c->mu);
         if (!c->byte_prefix) {
             uint16_t *bp = static_cast<uint16_t *>(ctx.pool.alloc(c->bitmap_bytes * 2 + 16));
@@ -1214,7 +1264,7 @@ void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_
                 }
                 // INT32: page k per block; 8-byte: pages 2k, 2k + 1 (the last may be unused)
                 const uint64_t np = width == 4 ? blocks : (total + kRowsPage8 - 1) / kRowsPage8;
-                mc.out = static_cast<uint8_t *>(ctx.pool.alloc((size_t)(width == 4 ? blocks : 2 * blocks) * kPage + 16));
+                mc.out = out_pages(i, width == 4 ? blocks : 2 * blocks);
                 bytes += total * (uint64_t)width + total / 8;
                 out_types[i] = type;
                 out_cols[i] = {mc.out, {(uint32_t)np, type}};
@@ -1264,7 +1314,7 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
         c.is_int32 = type == CJ_INT32;
         c.rec_offset = direct_field[i].first;
         c.rec_bit = direct_field[i].second;
-        c.out = static_cast<uint8_t *>(ctx.pool.alloc((size_t)bound * kPage + 16));
+        c.out = out_pages(i, bound);
         bbytes += total * (uint64_t)width + total / 8;
         out_types[i] = type;
         out_cols[i] = {c.out, {0u, type}};
@@ -1296,7 +1346,7 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
         f.width = type == CJ_INT32 ? 4 : 8;
         f.is_int32 = type == CJ_INT32;
         f.max_page_rows = c->max_page_rows;
-        f.out =static_cast<uint8_t *>(ctx.pool.alloc((size_t)c->num_pages * kPage + 16));
+        f.out = out_pages(i, c->num_pages);
         out_types[i] = type;
         out_cols[i] = {f.out, {0u, type}};
         count_at.push_back(pos + c->num_pages);
@@ -1372,13 +1422,13 @@ void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, u
             }
             rids = it->second;
         }
-        auto [pages, np] = materialize_column(column(S.slots[cr.slot].table, cr.col), type, rids, total, true);
+        auto [pages, np] = materialize_column(column(S.slots[cr.slot].table, cr.col), type, rids, total, true, a);
         out_cols[a] = {pages, {np, type}};
     }
 }
 
 std::pair<uint8_t *, uint32_t> Exec::materialize_column(DevColumn *c, int32_t type, const uint32_t *rids,
-                                                        uint64_t total, bool int32_quirk) {
+                                                        uint64_t total, bool int32_quirk, uint32_t attr) {
     const bool in_place = type == CJ_INT32 && c->type == CJ_INT32 && c->dense_rows;
     ColView v;
     if (!in_place) v = view(c);
@@ -1397,13 +1447,13 @@ std::pair<uint8_t *, uint32_t> Exec::materialize_column(DevColumn *c, int32_t ty
         scan(cp, co, chunks);
         uint64_t npg = read_u64(co + chunks);
         np = (uint32_t)npg;
-        pages = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
+        pages = out_pages(attr, np);
         TimedScope ts(timer, "varchar_write", (uint64_t)np * kPage);
         varchar_write(vs, total, chunk, co, pages, st);
     } else {
         uint32_t R = type == CJ_INT32 ? kRowsI32 : kRowsI64;
         np = (uint32_t)((total + R - 1) / R);
-        pages = static_cast<uint8_t *>(ctx.pool.alloc((size_t)np * kPage + 16));
+        pages = out_pages(attr, np);
         FixedSource fs{type == CJ_INT32 ? 4 : 8, type == CJ_INT32 && int32_quirk, v.values, v.valid, rids,
                        in_place ? c->pages : nullptr, in_place ? c->dense_rows : 0u};
         TimedScope ts(timer, type == CJ_INT32 ? "materialize_i32" : "materialize_i64", fixed_mat_bytes(total, type));
@@ -1463,7 +1513,7 @@ void Exec::filter_root(const std::vector<FNode> &prog, const std::vector<uint32_
             out_cols.push_back({nullptr, {0u, type}});
             continue;
         }
-        auto [pages, np] = materialize_column(column(0, c), type, rids, total, false);
+        auto [pages, np] = materialize_column(column(0, c), type, rids, total, false, c);
         out_cols.push_back({pages, {np, type}});
     }
 }
@@ -1649,6 +1699,50 @@ void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, 
             }
         },
         [&](Exec &ex) { ex.join(plan.nodes[plan.root], true, 0, out); });
+}
+
+void Context::execute_noalloc(const cj_plan_desc &plan, uint32_t flags, void *const *out_pages,
+                              const uint32_t *out_cap_pages, uint32_t num_out, uint64_t *out_rows,
+                              uint32_t *out_num_pages) {
+    if (plan.root >= plan.num_nodes) throw InvalidError("plan root out of range");
+    if (plan.nodes[plan.root].kind != CJ_NODE_JOIN) throw UnsupportedError("plan root must be a join node");
+    for (uint32_t i = 0; i < plan.num_inputs; ++i)
+        for (uint32_t c = 0; c < plan.inputs[i].num_columns; ++c)
+            if (plan.inputs[i].columns[c].type < CJ_INT32 || plan.inputs[i].columns[c].type > CJ_VARCHAR)
+                throw InvalidError("bad column type");
+    for (uint32_t i = 0; i < num_out; ++i)
+        if (out_cap_pages[i] && !out_pages[i]) throw InvalidError("null output buffer");
+    std::lock_guard<std::mutex> one_at_a_time(noalloc_mutex(device));
+    hipStream_t s = acquire_stream();
+    struct Rel_ {
+        Context &c;
+        hipStream_t s;
+        ~Rel_() {
+            hipStreamSynchronize(s);  // no kernel may still use the scratch when the next call starts
+            c.release_stream(s);
+        }
+    } rel_guard{*this, s};
+    Exec ex(*this, plan, s);
+    ex.timer.on = false;
+    ex.device_output = true;
+    ex.flags = flags;
+    ex.arena = noalloc_scratch(device);
+    ex.arena_cap = kNoAllocScratchBytes;
+    ex.user_out = out_pages;
+    ex.user_cap = out_cap_pages;
+    ex.user_n = num_out;
+    for (uint32_t i = 0; i < plan.num_nodes; ++i) {  // every scanned column must be resident
+        const cj_node_desc &nd = plan.nodes[i];
+        if (nd.kind != CJ_NODE_SCAN || nd.base_table_id >= plan.num_inputs) continue;
+        for (uint32_t a = 0; a < nd.num_output_attrs; ++a)
+            if (nd.output_attrs[a].index < plan.inputs[nd.base_table_id].num_columns)
+                ex.column((uint32_t)nd.base_table_id, (uint32_t)nd.output_attrs[a].index);
+    }
+    ex.join(plan.nodes[plan.root], true, 0, nullptr);
+    CJ_HIP(hipStreamSynchronize(s));
+    if (ex.out_cols.size() > num_out) throw InvalidError("cj_execute_noalloc: fewer output buffers than result columns");
+    *out_rows = ex.out_rows;
+    for (uint32_t i = 0; i < num_out; ++i) out_num_pages[i] = i < ex.out_cols.size() ? ex.out_cols[i].second.first : 0;
 }
 
 void Context::filter(const cj_table_desc &t, const cj_filter_node *nodes, uint32_t n, uint32_t flags, cj_result *out,

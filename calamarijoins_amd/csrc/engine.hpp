@@ -152,6 +152,11 @@ class Context {
     void unregister_table(uint64_t id);
 
     void execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing);
+    // the same without allocating: temporaries in the GPU's fixed scratch,
+    // result pages in caller buffers (cj_execute_noalloc)
+    void execute_noalloc(const cj_plan_desc &plan, uint32_t flags, void *const *out_pages,
+                         const uint32_t *out_cap_pages, uint32_t num_out, uint64_t *out_rows,
+                         uint32_t *out_num_pages);
     // rows of one table passing a filter program (cj_filter)
     void filter(const cj_table_desc &t, const cj_filter_node *prog, uint32_t n, uint32_t flags, cj_result *out,
                 cj_timing *timing);

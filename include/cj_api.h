@@ -187,6 +187,31 @@ int cj_table_unregister(cj_ctx *ctx, uint64_t id);
 int cj_execute(cj_ctx *ctx, const cj_plan_desc *plan, uint32_t flags,
                cj_result *out, cj_timing *timing);
 int cj_result_release(cj_ctx *ctx, cj_result *res);
+
+/* ---- execute without allocating.  No counterpart in the reference: for
+ *      callers that cannot allocate device memory (code under HIP graph
+ *      capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ *      same plans and results as cj_execute with CJ_EXEC_OUTPUT_DEVICE, but
+ *      the call allocates nothing: every temporary lives in a fixed device
+ *      array of CJ_NOALLOC_SCRATCH_BYTES that the library holds on each GPU
+ *      it runs on until the process ends, and result column i is written
+ *      to the caller's device buffer out_pages[i] (room for
+ *      out_cap_pages[i] pages of 8 KiB).  Limits: every input must be a
+ *      registered (resident) table; a plan whose temporaries exceed the
+ *      scratch, or a column whose pages exceed its buffer, fails with
+ *      CJ_ERR_OOM and a message naming the size it needed (a column of
+ *      8-byte values may need one page of room beyond the pages it fills;
+ *      the root uses the block or the general writers, whose page counts
+ *      follow from the result size).  The
+ *      call is synchronous.  All calls on one GPU share its scratch, which
+ *      keeps the last call's contents (each call clears what it needs):
+ *      the library runs them one at a time, so calls that would overlap on
+ *      one GPU wait for each other.  On success *out_rows and
+ *      out_num_pages[0 .. num_out) give the result. */
+#define CJ_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int cj_execute_noalloc(cj_ctx *ctx, const cj_plan_desc *plan, uint32_t flags, void *const *out_pages,
+                       const uint32_t *out_cap_pages, uint32_t num_out, uint64_t *out_rows,
+                       uint32_t *out_num_pages);
 /* Copy a device-output result into host pages (fills columns[i].pages). */
 int cj_result_to_host(cj_ctx *ctx, cj_result *res);
 
