@@ -74,6 +74,8 @@ void stop_server(cj::Context *c) {
 }
 }  // namespace
 
+void cj::set_last_error(const std::string &msg) { g_err = msg; }
+
 struct cj_tbl {
     std::shared_ptr<cj::TblFile> f;
 };

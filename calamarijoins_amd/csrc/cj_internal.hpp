@@ -44,6 +44,9 @@ struct BusyError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
+// The calling thread's message for cj_last_error() (api.cpp).
+void set_last_error(const std::string &msg);
+
 #define CJ_HIP(expr)                                                                 \
     do {                                                                             \
         hipError_t e_ = (expr);                                                      \
@@ -87,6 +90,20 @@ enum KeyMode : int {
 void scan_exclusive_u64(const uint64_t *in, uint64_t *out, uint64_t n, hipStream_t s,
                         uint64_t *tmp, size_t tmp_elems);
 size_t scan_tmp_elems(uint64_t n);
+// Up to kMultiScanJobs exclusive scans of u64 arrays in two launches for
+// all of them: out[i] = sum of f(in[j]) for j < i, out[n] = the total, with
+// f(x) = x, or ceil(x / div) when div > 0 (pages of div rows for x rows;
+// x below 2^32).  sums: multi_scan_blocks(jobs, njobs) words of scratch.
+constexpr uint32_t kMultiScanJobs = 8;
+constexpr uint64_t kMultiScanMax = uint64_t(1) << 20;  // longest array it takes
+struct ScanJob {
+    const uint64_t *in;
+    uint64_t *out;
+    uint64_t n;
+    uint64_t div;
+};
+void multi_scan_u64(const ScanJob *jobs, uint32_t njobs, uint64_t *sums, hipStream_t s);
+uint32_t multi_scan_blocks(const ScanJob *jobs, uint32_t njobs);
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
 void page_row_counts(const uint8_t *pages, uint32_t num_pages, uint64_t *counts, hipStream_t s);

@@ -93,6 +93,104 @@ __global__ void k_scan_block(const uint64_t *in, uint64_t *out, uint64_t n, cons
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kWG - 1) out[n] = off;
 }
 
+// Two launches for all arrays together: block sums of f(in), then each
+// block adds the sums of the blocks before it (read directly: an array of
+// at most kMultiScanMax elements has at most 512 blocks) and scans its own.
+// Counts are below 2^32 (rows of one unit), so f divides in 32 bits.
+constexpr uint32_t kMsItems = 8;
+constexpr uint64_t kMsBlock = (uint64_t)kWG * kMsItems;  // 2048 elements a workgroup
+struct MultiScanArgs {
+    ScanJob job[kMultiScanJobs];
+    uint32_t first_block[kMultiScanJobs + 1];  // blocks of the jobs before each job
+    uint32_t njobs;
+    uint64_t *sums;  // one per block
+};
+
+__device__ __forceinline__ uint32_t ms_job(const MultiScanArgs &a, uint32_t b) {
+    uint32_t k = 0;
+    while (k + 1 < a.njobs && a.first_block[k + 1] <= b) ++k;
+    return k;
+}
+
+__device__ __forceinline__ uint64_t ms_load(const ScanJob &j, uint64_t i) {
+    const uint64_t x = j.in[i < j.n ? i : j.n - 1];
+    if (i >= j.n) return 0;
+    return j.div ? ((uint32_t)x + (uint32_t)j.div - 1u) / (uint32_t)j.div : x;
+}
+
+__global__ __launch_bounds__(kWG) void k_multi_scan_sums(MultiScanArgs a) {
+    __shared__ uint64_t lds[4];
+    const uint32_t k = ms_job(a, blockIdx.x);
+    const ScanJob &j = a.job[k];
+    const uint64_t base = (uint64_t)(blockIdx.x - a.first_block[k]) * kMsBlock + (uint64_t)threadIdx.x * kMsItems;
+    uint64_t s = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kMsItems; ++i) s += ms_load(j, base + i);
+    uint64_t tot;
+    block_excl_scan<uint64_t>(s, &tot, lds);
+    if (threadIdx.x == 0) a.sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kWG) void k_multi_scan_apply(MultiScanArgs a) {
+    __shared__ uint64_t lds[4];
+    const uint32_t k = ms_job(a, blockIdx.x);
+    const ScanJob &j = a.job[k];
+    const uint32_t b = blockIdx.x - a.first_block[k];
+    uint64_t p = 0;
+    for (uint32_t i = threadIdx.x; i < b; i += kWG) p += a.sums[a.first_block[k] + i];
+    uint64_t boff;
+    block_excl_scan<uint64_t>(p, &boff, lds);
+    const uint64_t base = (uint64_t)b * kMsBlock + (uint64_t)threadIdx.x * kMsItems;
+    uint64_t v[kMsItems], s = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kMsItems; ++i) {
+        v[i] = ms_load(j, base + i);
+        s += v[i];
+    }
+    uint64_t tot;
+    uint64_t off = boff + block_excl_scan<uint64_t>(s, &tot, lds);
+#pragma unroll
+    for (uint32_t i = 0; i < kMsItems; ++i) {
+        if (base + i < j.n) j.out[base + i] = off;
+        off += v[i];
+    }
+    const uint32_t nb = a.first_block[k + 1] - a.first_block[k];
+    if (b == nb - 1 && threadIdx.x == kWG - 1) j.out[j.n] = off;
+}
+
+__global__ void k_multi_scan_empty(MultiScanArgs a) {
+    if (threadIdx.x < a.njobs && a.job[threadIdx.x].n == 0) a.job[threadIdx.x].out[0] = 0;
+}
+
+void multi_scan_u64(const ScanJob *jobs, uint32_t njobs, uint64_t *sums, hipStream_t s) {
+    if (!njobs) return;
+    if (njobs > kMultiScanJobs) throw InvalidError("too many scan jobs in one launch");
+    MultiScanArgs a{};
+    a.njobs = njobs;
+    a.sums = sums;
+    bool empty = false;
+    for (uint32_t i = 0; i < njobs; ++i) {
+        if (jobs[i].n > kMultiScanMax) throw InvalidError("array too long for the multi-array scan");
+        a.job[i] = jobs[i];
+        if (jobs[i].div > 0xffffffffull) throw InvalidError("scan divisor above 32 bits");
+        a.first_block[i + 1] = a.first_block[i] + (uint32_t)((jobs[i].n + kMsBlock - 1) / kMsBlock);
+        empty |= jobs[i].n == 0;
+    }
+    if (empty) k_multi_scan_empty<<<1, 64, 0, s>>>(a);
+    const uint32_t nb = a.first_block[njobs];
+    if (nb) {
+        k_multi_scan_sums<<<nb, kWG, 0, s>>>(a);
+        k_multi_scan_apply<<<nb, kWG, 0, s>>>(a);
+    }
+    CJ_HIP(hipGetLastError());
+}
+
+uint32_t multi_scan_blocks(const ScanJob *jobs, uint32_t njobs) {
+    uint64_t nb = 0;
+    for (uint32_t i = 0; i < njobs; ++i) nb += (jobs[i].n + kMsBlock - 1) / kMsBlock;
+    return (uint32_t)nb;
+}
+
 size_t scan_tmp_elems(uint64_t n) {
     size_t t = 0;
     while (n > kScanBlock) {

@@ -5,6 +5,9 @@
 
 namespace cj {
 
+// Four dwords read with one 16-byte load from a 4-byte-aligned address.
+typedef uint32_t u32x4u __attribute__((ext_vector_type(4), aligned(4)));
+
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 // wave index in a 1-D block, as a scalar (all lanes of a wave share it)
 __device__ __forceinline__ uint32_t wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }

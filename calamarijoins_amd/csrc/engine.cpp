@@ -951,24 +951,28 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
             else if (inter_bits) inter_count_match(fa, cnt, match_bits, st);
             else probe_count_columnar(fa, cnt, st);
         }
-        scan(cnt, off, units);
-        // the page-aligned writers' output page offsets depend only on the
-        // counts: queue them before the read-back so the device works while
-        // the host waits
-        if (aligned) {
+        if (aligned && units <= kMultiScanMax) {
+            // the page-aligned writers' output page offsets depend only on the
+            // counts: one launch scans the counts and the build columns' page
+            // counts (pages of kCap4 / kCap8 rows per key page)
+            po4 = tmp<uint64_t>(units + 1);
+            po8 = tmp<uint64_t>(units + 1);
+            const ScanJob jobs[3] = {{cnt, off, units, 0}, {cnt, po4, units, kCap4}, {cnt, po8, units, kCap8}};
+            {
+                TimedScope ts(timer, "scan3", 32ull * units);
+                multi_scan_u64(jobs, 3, tmp<uint64_t>(multi_scan_blocks(jobs, 3) + 1), st);
+            }
+            po_cnt = cnt;
+            total = read_u64(off + units);
+        } else if (aligned) {
+            scan(cnt, off, units);
             uint64_t *h = readback();
             CJ_HIP(hipMemcpyAsync(h, off + units, 8, hipMemcpyDeviceToHost, st));
-            if (noalloc()) {
-                CJ_HIP(hipStreamSynchronize(st));
-                page_offsets(cnt, units);
-            } else {
-                if (!rb_event) CJ_HIP(hipEventCreateWithFlags(&rb_event, hipEventDisableTiming));
-                CJ_HIP(hipEventRecord(rb_event, st));
-                page_offsets(cnt, units);
-                CJ_HIP(hipEventSynchronize(rb_event));
-            }
+            CJ_HIP(hipStreamSynchronize(st));
+            page_offsets(cnt, units);
             total = h[0];
         } else {
+            scan(cnt, off, units);
             total = read_u64(off + units);
         }
         if (total > 0xffffffffull) throw UnsupportedError("join result exceeds 2^32 rows");
@@ -1325,38 +1329,66 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
         TimedScope ts(timer, "build_pages", bbytes);
         build_pages(ba, st);
     }
-    // ---- probe side: one filtered output page per payload page with a match
+    // ---- probe side: payload page q writes output page q (no scan of the
+    // pages with a match first); pages without a matched row are counted and
+    // closed up afterwards (rare: this strategy needs half the rows to match)
+    const size_t nbuild = count_at.size();
+    std::vector<DevColumn *> probe_src;
+    uint64_t *empties = nullptr;
+    uint32_t nprobe = 0;
+    for (uint32_t i = 0; i < n.num_output_attrs; ++i) nprobe += remap[i] >= bw;
+    if (nprobe) {
+        empties = tmp<uint64_t>(nprobe);
+        fill_u64(empties, 0, nprobe, st);
+    }
     for (uint32_t i = 0; i < n.num_output_attrs; ++i) {
         if (remap[i] < bw) continue;
         const int32_t type = n.output_attrs[i].type;
         const ColRef &cr = P.cols[remap[i] - bw];
         DevColumn *c = column(P.slots[cr.slot].table, cr.col);
-        uint64_t *flag = tmp<uint64_t>(c->num_pages + 1), *pos = tmp<uint64_t>(c->num_pages + 1);
-        {
-            TimedScope ts(timer, "page_match_flags", (uint64_t)c->num_pages * 16 + c->num_rows / 8);
-            page_match_flags(c->pages, c->num_pages, c->row_start, match_bits, flag, st);
-        }
-        scan(flag, pos, c->num_pages);
         FilterArgs f{};
         f.pages = c->pages;
         f.num_pages = c->num_pages;
         f.row_start = c->row_start;
         f.match_bits = match_bits;
-        f.pos = pos;
+        f.pos = nullptr;
+        f.empty_pages = empties + probe_src.size();
         f.width = type == CJ_INT32 ? 4 : 8;
         f.is_int32 = type == CJ_INT32;
         f.max_page_rows = c->max_page_rows;
         f.out = out_pages(i, c->num_pages);
         out_types[i] = type;
         out_cols[i] = {f.out, {0u, type}};
-        count_at.push_back(pos + c->num_pages);
+        count_at.push_back(f.empty_pages);
         count_col.push_back(i);
+        probe_src.push_back(c);
         TimedScope ts(timer, f.width == 4 ? "probe_filter_w4" : "probe_filter_w8",
                       (uint64_t)c->num_pages * kPage + total * (uint64_t)f.width + total / 8);
         probe_filter(f, st);
     }
     const std::vector<uint64_t> np = read_u64s(count_at);
-    for (size_t k = 0; k < np.size(); ++k) out_cols[count_col[k]].second.first = (uint32_t)np[k];
+    for (size_t k = 0; k < nbuild; ++k) out_cols[count_col[k]].second.first = (uint32_t)np[k];
+    for (size_t k = nbuild; k < np.size(); ++k) {
+        DevColumn *c = probe_src[k - nbuild];
+        const uint32_t i = count_col[k];
+        out_cols[i].second.first = c->num_pages - (uint32_t)np[k];
+        if (!np[k]) continue;
+        // close the gaps: the pages with a matched row, in order, into a new block
+        uint64_t *flag = tmp<uint64_t>(c->num_pages + 1), *pos = tmp<uint64_t>(c->num_pages + 1);
+        {
+            TimedScope ts(timer, "page_match_flags", (uint64_t)c->num_pages * 16 + c->num_rows / 8);
+            page_match_flags(c->pages, c->num_pages, c->row_start, match_bits, flag, st);
+        }
+        scan(flag, pos, c->num_pages);
+        uint8_t *dst = static_cast<uint8_t *>(ctx.pool.alloc((size_t)c->num_pages * kPage + 16));
+        {
+            TimedScope ts(timer, "compact_pages", 2ull * c->num_pages * kPage);
+            compact_pages(out_cols[i].first, c->num_pages, pos, dst, st);
+        }
+        CJ_HIP(hipStreamSynchronize(st));
+        ctx.pool.free(out_cols[i].first);
+        out_cols[i].first = dst;
+    }
 }
 
 void Exec::materialize_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_t bw,

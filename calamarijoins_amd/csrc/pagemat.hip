@@ -567,8 +567,8 @@ __device__ __forceinline__ void emit_col(bool w8, CJ_G uint8_t *base, uint32_t l
         const bool next = cross && rank >= room;
         const uint32_t x = o.nn + (uint32_t)__popcll(om & lt) - (next ? nb : o.pb);
         const uint32_t off = o.qoff + (next ? kPage : 0u);
-        if (w8) *reinterpret_cast<CJ_G uint64_t *>(base + (off + 8 + 8 * x)) = ((uint64_t)hi << 32) | lo;
-        else *reinterpret_cast<CJ_G uint32_t *>(base + (off + 4 + 4 * x)) = lo;
+        if (w8) __builtin_nontemporal_store(((uint64_t)hi << 32) | lo, reinterpret_cast<CJ_G uint64_t *>(base + (off + 8 + 8 * x)));
+        else __builtin_nontemporal_store(lo, reinterpret_cast<CJ_G uint32_t *>(base + (off + 4 + 4 * x)));
     }
     uint64_t pk;  // validity bits of the chunk's result rows
     if (om == mm) {
@@ -659,7 +659,9 @@ __global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a, uint32_t 
         int32_t key[kB];
 #pragma unroll
         for (uint32_t c = 0; c < kB; ++c) {
-            key[c] = keys[vi[c] < nv ? vi[c] : vlast];
+            // key pages, match words and output values stream through once:
+            // non-temporal, so the L2 keeps the direct-map records
+            key[c] = __builtin_nontemporal_load(&keys[vi[c] < nv ? vi[c] : vlast]);
             if (vi[c] >= nv) vbits &= ~(1u << c);
         }
         // matched rows come from the count pass's match bitmap (coalesced
@@ -668,7 +670,7 @@ __global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a, uint32_t 
 #pragma unroll
         for (uint32_t c = 0; c < kB; ++c) {
             const uint32_t lr = c0 + 64 * c + lane;
-            mword[c] = mw[(msh + (lr < n ? lr : nlast)) >> 5];
+            mword[c] = __builtin_nontemporal_load(&mw[(msh + (lr < n ? lr : nlast)) >> 5]);
         }
         uint32_t mbits = 0;
         uint4 rec[kB][NQ];
@@ -796,8 +798,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8))) void k
     const uint32_t lane = lane_id();
     const uint32_t q = blockIdx.x * (kWG / 64) + wave_id();
     if (q >= a.num_pages) return;
-    const uint64_t o = a.pos[q];
-    if (a.pos[q + 1] == o) return;  // no matched row
+    const uint64_t o = a.pos ? a.pos[q] : q;
+    if (a.pos && a.pos[q + 1] == o) return;  // no matched row
     const uint8_t *p = a.pages + (size_t)q * kPage;
     const uint32_t n = ld16(p), nv = ld16(p + 2);
     if (n > kWaveMaxRows) return;  // k_probe_filter's page
@@ -876,6 +878,10 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8))) void k
             mbase += tm;
         }
     }
+    if (mbase == 0) {  // (fixed slots only) the slot stays unwritten
+        if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long *>(a.empty_pages), 1ull);
+        return;
+    }
     const uint32_t nbo = (mbase + 7) >> 3;
 #pragma unroll
     for (uint32_t h = 0; h < 2; ++h) {
@@ -905,8 +911,8 @@ __global__ __launch_bounds__(kWG) void k_probe_filter(FilterArgs a) {
     __shared__ uint32_t obm[kPage / 4];  // output bitmap (up to 65535 rows)
     __shared__ uint32_t lds32[32];
     const uint32_t q = blockIdx.x;
-    const uint64_t out_pg = a.pos[q];
-    if (a.pos[q + 1] == out_pg) return;  // no matched row
+    const uint64_t out_pg = a.pos ? a.pos[q] : q;
+    if (a.pos && a.pos[q + 1] == out_pg) return;  // no matched row
     if (a.row_start[q + 1] - a.row_start[q] <= kWaveMaxRows) return;  // k_probe_filter_wave's page
     const uint4 *src = reinterpret_cast<const uint4 *>(a.pages + (size_t)q * kPage);
     for (uint32_t i = threadIdx.x; i < kPage / 16; i += kWG) stage[i] = src[i];
@@ -963,6 +969,10 @@ __global__ __launch_bounds__(kWG) void k_probe_filter(FilterArgs a) {
         run_ok += tok;
     }
     __syncthreads();
+    if (run_m == 0) {  // (fixed slots only) the slot stays unwritten
+        if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long *>(a.empty_pages), 1ull);
+        return;
+    }
     const uint32_t nb = (run_m + 7) >> 3;
     const uint8_t *ob = reinterpret_cast<const uint8_t *>(obm);
     for (uint32_t b = threadIdx.x; b < nb; b += kWG) out[kPage - nb + b] = ob[b];
@@ -982,6 +992,21 @@ void probe_filter(const FilterArgs &a, hipStream_t s) {
         if (a.width == 4) k_probe_filter<4><<<a.num_pages, kWG, 0, s>>>(a);
         else k_probe_filter<8><<<a.num_pages, kWG, 0, s>>>(a);
     }
+    CJ_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kWG) void k_compact_pages(const uint8_t *src, uint32_t num_pages, const uint64_t *pos,
+                                                       uint8_t *dst) {
+    const uint32_t q = blockIdx.x;
+    if (q >= num_pages || pos[q + 1] == pos[q]) return;
+    const uint4 *s = reinterpret_cast<const uint4 *>(src + (size_t)q * kPage);
+    uint4 *d = reinterpret_cast<uint4 *>(dst + pos[q] * kPage);
+    for (uint32_t i = threadIdx.x; i < kPage / 16; i += kWG) d[i] = s[i];
+}
+
+void compact_pages(const uint8_t *src, uint32_t num_pages, const uint64_t *pos, uint8_t *dst, hipStream_t s) {
+    if (!num_pages) return;
+    k_compact_pages<<<num_pages, kWG, 0, s>>>(src, num_pages, pos, dst);
     CJ_HIP(hipGetLastError());
 }
 

@@ -67,7 +67,11 @@ struct FilterArgs {
     uint32_t num_pages;
     const uint64_t *row_start;  // [num_pages + 1]
     const uint32_t *match_bits;
-    const uint64_t *pos;  // output page of each payload page (exclusive scan of nonempty)
+    // output page of each payload page (exclusive scan of nonempty); null:
+    // payload page q writes output page q and a page without a matched row
+    // is left unwritten and counted in *empty_pages (see compact_pages)
+    const uint64_t *pos;
+    uint64_t *empty_pages;
     int32_t width;
     int32_t is_int32;
     uint32_t max_page_rows;  // largest n_rows of a page of the column
@@ -75,6 +79,9 @@ struct FilterArgs {
 };
 // Probe-side output pages, one per payload page with a match.
 void probe_filter(const FilterArgs &a, hipStream_t s);
+// dst page pos[q] = src page q for every page q with pos[q + 1] > pos[q]
+// (closes the gaps the empty pages of a fixed-slot probe_filter left).
+void compact_pages(const uint8_t *src, uint32_t num_pages, const uint64_t *pos, uint8_t *dst, hipStream_t s);
 
 }  // namespace cj
 

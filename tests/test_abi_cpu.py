@@ -15,7 +15,7 @@ import cjtest
 from calamarijoins_amd import abi, workloads
 
 REPO = cjtest.REPO
-HEADERS = ["include/cj_api.h", "include/cj_workload.h"]
+HEADERS = ["include/cj_api.h", "include/cj_workload.h", "include/cj_membench.h"]
 
 
 def declared_functions():

@@ -286,3 +286,36 @@ def test_gpu_keys_through_direct_map_rows(engine, flags):
     exp = cjtest.OracleRun(plan, threads=8)
     assert exp.num_rows > 1000
     same_multiset(engine.execute(plan, flags=flags), exp)
+
+
+def test_gpu_aligned_pages_without_matches(engine):
+    """The page-aligned writers put payload page q at output page q and close
+    up the pages whose rows all miss: a probe table whose first quarter of
+    rows holds keys the build side lacks (whole payload pages without a
+    match) while three quarters match."""
+    nb, npr = 100_000, 400_000
+    rng = np.random.default_rng(11)
+    bkeys = rng.permutation(nb).astype(np.int32)
+    pkeys = np.concatenate([200_000 + np.arange(npr // 4), rng.integers(0, nb, npr - npr // 4)]).astype(np.int32)
+    ones = lambda n: np.ones(n, np.uint8)
+    b = cjtest.Table(nb, [cjtest.Column(abi.CJ_INT32, cjtest.encode_fixed(abi.CJ_INT32, bkeys, ones(nb))),
+                          cjtest.Column(abi.CJ_INT64, cjtest.encode_fixed(abi.CJ_INT64, bkeys.astype(np.int64) * 3,
+                                                                          (rng.random(nb) > 0.05).astype(np.uint8)))])
+    pv = rng.integers(-2**31, 2**31, npr).astype(np.int32)
+    p = cjtest.Table(npr, [cjtest.Column(abi.CJ_INT32, cjtest.encode_fixed(abi.CJ_INT32, pkeys, ones(npr))),
+                           cjtest.Column(abi.CJ_INT32, cjtest.encode_fixed(abi.CJ_INT32, pv,
+                                                                           (rng.random(npr) > 0.05).astype(np.uint8))),
+                           cjtest.Column(abi.CJ_INT64, cjtest.encode_fixed(abi.CJ_INT64, pv.astype(np.int64) << 20,
+                                                                           ones(npr)))])
+    plan = cjtest.Plan()
+    plan.new_input(b)
+    plan.new_input(p)
+    plan.new_scan_node(0, [(0, "INT32"), (1, "INT64")])
+    plan.new_scan_node(1, [(0, "INT32"), (1, "INT32"), (2, "INT64")])
+    plan.root = plan.new_join_node(True, 0, 1, 0, 0, [(1, "INT64"), (3, "INT32"), (4, "INT64"), (2, "INT32")])
+    got = engine.execute(plan, timing=True)
+    exp = cjtest.OracleRun(plan, threads=8)
+    assert got.num_rows == npr - npr // 4
+    same_multiset(got, exp)
+    # fewer probe-side pages than payload pages: the empty ones were closed up
+    assert got.num_pages()[1] < p.columns[1].num_pages
