@@ -67,7 +67,8 @@ __host__ __device__ inline uint64_t hash_key(int32_t key) {
 // Hash table geometry (unchained table with Bloom tags in the directory).
 struct HashGeom {
     int log_slots;   // directory = 2^log_slots entries of u64 (end<<16 | tag)
-    int log_part;    // slots per build partition = 2^log_part
+    int log_part;    // slots per radix partition = 2^log_part
+    int log_sub;     // slots a build workgroup counts in LDS at a time (<= log_part)
     uint32_t num_partitions;
 };
 

@@ -780,12 +780,17 @@ class Exec {
         PhaseScope ps(clock, CJ_PHASE_HASHTABLE_BUILD);
         int log_slots = 11;  // directory >= 2048 slots (hashtable.h:255-264)
         while ((1ull << log_slots) < rows) ++log_slots;
-        // <= 4096 partitions (LDS histogram of the partition pass) and
-        // 2^log_part slots per build workgroup (8 B of LDS per slot)
-        int log_part = std::max(std::min(log_slots, 12), log_slots - 12);
-        if ((8ull << log_part) + 64 > ctx.max_lds)
-            throw UnsupportedError("build side too large for the LDS build");
-        HashGeom g{log_slots, log_part, 1u << (log_slots - log_part)};
+        if (log_slots > 32) throw UnsupportedError("hash build side above 2^32 rows");
+        // partitions of 2^11 .. 2^14 slots, about 256 of them for parallelism
+        // and at most 4096 (the partition pass's LDS histogram): beyond 2^26
+        // slots a partition holds more slots than one LDS pass counts, and
+        // its workgroup counts them 2^14 (128 KiB of LDS) at a time
+        int log_part = std::min(std::max(log_slots - 8, 11), 14);
+        if (log_part > log_slots) log_part = log_slots;
+        if (log_slots - log_part > 12) log_part = log_slots - 12;
+        int log_sub = std::min(log_part, 14);
+        while (log_sub > 10 && (8ull << log_sub) + 256 > ctx.max_lds) --log_sub;
+        HashGeom g{log_slots, log_part, log_sub, 1u << (log_slots - log_part)};
         uint32_t P = g.num_partitions;
         uint64_t *pc = tmp<uint64_t>(P + 1);
         uint64_t *poff = tmp<uint64_t>(P + 1);

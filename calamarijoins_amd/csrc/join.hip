@@ -73,45 +73,109 @@ __device__ __forceinline__ uint64_t slot_of(uint64_t h, int log_slots) { return 
 
 // ---------------------------------------------------- radix partition pass
 // Partition of a key = the top bits of its directory slot, so each build
-// partition owns a contiguous slot range.
+// partition owns a contiguous slot range (hashtable.h:439-478 partitions by
+// hash bits too, into per-thread chunk lists).
+__device__ __forceinline__ uint32_t part_of(int32_t key, const HashGeom &g) {
+    return (uint32_t)(slot_of(hash_key(key), g.log_slots) >> g.log_part);
+}
+
+// Wave-aggregated add on an LDS counter array: the active lanes that carry
+// the same bin find each other with one ballot per bin bit, their lowest
+// lane adds the group's size once, and every lane gets the old value plus
+// its rank in the group (a unique slot).  A skewed key (thousands of
+// duplicates) costs one LDS atomic per wave instead of 64.
+__device__ __forceinline__ uint32_t agg_add(uint32_t *ctr, uint32_t bin, bool active, int bin_bits) {
+    const uint64_t lt = lanemask_lt();
+    uint64_t peers = __ballot(active);
+    for (int b = 0; b < bin_bits; ++b) {
+        const bool bit = (bin >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+    }
+    const uint32_t leader = peers ? (uint32_t)__ffsll((unsigned long long)peers) - 1u : 0u;
+    uint32_t base = 0;
+    if (active && lane_id() == leader) base = atomicAdd(&ctr[bin], (uint32_t)__popcll(peers));
+    base = (uint32_t)__shfl((int)base, (int)leader, 64);
+    return base + (uint32_t)__popcll(peers & lt);
+}
+
+// Histogram of one unit's keys per partition (LDS, wave-aggregated), then
+// one global add per non-empty partition.
 __global__ __launch_bounds__(kWG) void k_part_count(KeySource k, HashGeom g, uint64_t *part_count) {
     extern __shared__ uint32_t hist[];  // num_partitions
     __shared__ uint32_t lds4[4];
+    const int bits = g.log_slots - g.log_part;
     for (uint32_t i = threadIdx.x; i < g.num_partitions; i += kWG) hist[i] = 0;
     __syncthreads();
     const bool want_nulls = k.mode == KEY_INTER;
     for_each_key(k, blockIdx.x, want_nulls, lds4, [&](bool, bool ok, int32_t key, uint32_t) {
-        if (ok) atomicAdd(&hist[slot_of(hash_key(key), g.log_slots) >> g.log_part], 1u);
+        agg_add(hist, ok ? part_of(key, g) : 0u, ok, bits);
     });
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < g.num_partitions; i += kWG)
         if (hist[i]) atomicAdd((unsigned long long *)&part_count[i], (unsigned long long)hist[i]);
 }
 
+// LDS-staged scatter of one unit (at most kUnitTuples tuples): histogram,
+// block scan into per-partition starts in the staging buffer, one global
+// reservation per non-empty partition, the tuples staged grouped by
+// partition, then written out in staged order so consecutive lanes store
+// consecutive addresses of one partition's run.
+constexpr uint32_t kUnitTuples = 2048;  // a key page holds <= 2047 INT32 values; a row-id unit 2048 rows
+
 __global__ __launch_bounds__(kWG) void k_part_scatter(KeySource k, HashGeom g, uint64_t *part_cursor,
                                                       int32_t *tup_keys, uint32_t *tup_rids) {
-    extern __shared__ uint32_t sm[];  // hist[P] then cursor[P] (u32 pairs as u64)
-    uint32_t *hist = sm;
-    uint64_t *cur = reinterpret_cast<uint64_t *>(sm + ((g.num_partitions + 1) & ~1u));
+    extern __shared__ uint32_t sm[];  // start[P] (then the running cursor), delta[P]
+    const uint32_t P = g.num_partitions;
+    uint32_t *cur = sm, *delta = sm + P;
+    __shared__ int32_t skey[kUnitTuples];
+    __shared__ uint32_t srid[kUnitTuples];
     __shared__ uint32_t lds4[4];
-    for (uint32_t i = threadIdx.x; i < g.num_partitions; i += kWG) hist[i] = 0;
+    __shared__ uint32_t wsum[kWG / 64];
+    const int bits = g.log_slots - g.log_part;
+    for (uint32_t i = threadIdx.x; i < P; i += kWG) cur[i] = 0;
     __syncthreads();
     const bool want_nulls = k.mode == KEY_INTER;
     for_each_key(k, blockIdx.x, want_nulls, lds4, [&](bool, bool ok, int32_t key, uint32_t) {
-        if (ok) atomicAdd(&hist[slot_of(hash_key(key), g.log_slots) >> g.log_part], 1u);
+        agg_add(cur, ok ? part_of(key, g) : 0u, ok, bits);
     });
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < g.num_partitions; i += kWG)
-        cur[i] = hist[i] ? atomicAdd((unsigned long long *)&part_cursor[i], (unsigned long long)hist[i]) : 0;
+    // exclusive scan of the counts (each thread owns a run of bins); each
+    // non-empty bin reserves its range of the partition: delta = global
+    // position - staged position
+    const uint32_t per = (P + kWG - 1) / kWG, b0 = threadIdx.x * per;
+    uint32_t local = 0;
+    for (uint32_t i = 0; i < per; ++i)
+        if (b0 + i < P) local += cur[b0 + i];
+    const uint32_t inc = wave_incl_scan<uint32_t>(local);
+    if (lane_id() == 63) wsum[wave_id()] = inc;
+    __syncthreads();
+    uint32_t pre = inc - local;
+    for (uint32_t w = 0; w < wave_id(); ++w) pre += wsum[w];
+    const uint32_t staged = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    for (uint32_t i = 0; i < per; ++i)
+        if (b0 + i < P) {
+            const uint32_t c = cur[b0 + i];
+            const uint64_t gpos = c ? atomicAdd((unsigned long long *)&part_cursor[b0 + i], (unsigned long long)c) : 0;
+            cur[b0 + i] = pre;
+            delta[b0 + i] = (uint32_t)gpos - pre;  // modulo 2^32: tuple positions are below 2^32
+            pre += c;
+        }
     __syncthreads();
     for_each_key(k, blockIdx.x, want_nulls, lds4, [&](bool, bool ok, int32_t key, uint32_t rid) {
+        const uint32_t at = agg_add(cur, ok ? part_of(key, g) : 0u, ok, bits);
         if (ok) {
-            uint32_t p = (uint32_t)(slot_of(hash_key(key), g.log_slots) >> g.log_part);
-            uint64_t pos = atomicAdd((unsigned long long *)&cur[p], 1ull);
-            tup_keys[pos] = key;
-            tup_rids[pos] = rid;
+            skey[at] = key;
+            srid[at] = rid;
         }
     });
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < staged; i += kWG) {
+        const int32_t key = skey[i];
+        const uint32_t pos = delta[part_of(key, g)] + i;
+        tup_keys[pos] = key;
+        tup_rids[pos] = srid[i];
+    }
 }
 
 void build_partition_hist(const KeySource &k, const HashGeom &g, uint64_t *part_count, hipStream_t s) {
@@ -127,8 +191,8 @@ void build_partition_scatter(const KeySource &k, const HashGeom &g, uint64_t *pa
     uint32_t units = key_source_units(k);
     if (!units) return;
     // part_cursor starts as a copy of the partition offsets and is advanced
-    // by one atomic per (workgroup, partition).
-    size_t sh = ((g.num_partitions + 1) & ~1u) * sizeof(uint32_t) + g.num_partitions * sizeof(uint64_t);
+    // by one atomic per (workgroup, non-empty partition).
+    size_t sh = 2 * (size_t)g.num_partitions * sizeof(uint32_t);
     k_part_scatter<<<units, kWG, sh, s>>>(k, g, part_cursor, tup_keys, tup_rids);
     CJ_HIP(hipGetLastError());
 }
@@ -137,7 +201,10 @@ void build_partition_scatter(const KeySource &k, const HashGeom &g, uint64_t *pa
 // build_partition (hashtable.h:197-246): count per slot, exclusive prefix,
 // scatter keys/row ids, OR Bloom tags, write (end << 16) | tag.  1024
 // threads per partition: a hot key of a skewed build side fills one
-// partition, whose workgroup sets the kernel's length.
+// partition, whose workgroup sets the kernel's length.  A partition holds
+// 2^log_part slots; the workgroup counts 2^log_sub of them at a time in LDS
+// (a count and a tag word per slot: up to 16384 slots, 128 KiB), reading
+// the partition's tuples twice per sub-range.
 constexpr uint32_t kBuildWG = 1024;
 
 __global__ __launch_bounds__(kBuildWG) void k_build_partitions(HashGeom g, const uint64_t *part_start,
@@ -145,54 +212,76 @@ __global__ __launch_bounds__(kBuildWG) void k_build_partitions(HashGeom g, const
                                                                const uint32_t *tup_rids, uint64_t *dir,
                                                                int32_t *keys, uint32_t *rids) {
     extern __shared__ uint32_t sm[];
-    const uint32_t S = 1u << g.log_part;
+    const uint32_t S = 1u << g.log_sub;
     uint32_t *cnt = sm, *tag = sm + S;
     __shared__ uint32_t wsum[kBuildWG / 64];
     const uint32_t p = blockIdx.x;
     const uint64_t a = part_start[p], e = part_end[p];
-    for (uint32_t i = threadIdx.x; i < S; i += kBuildWG) { cnt[i] = 0; tag[i] = 0; }
-    __syncthreads();
-    const uint32_t smask = S - 1;
-    for (uint64_t i = a + threadIdx.x; i < e; i += kBuildWG) {
-        uint64_t h = hash_key(tup_keys[i]);
-        uint32_t ls = (uint32_t)slot_of(h, g.log_slots) & smask;
-        atomicAdd(&cnt[ls], 1u);
-        atomicOr(&tag[ls], (uint32_t)c_bloom.t[h & 0x7FF]);
-    }
-    __syncthreads();
-    // exclusive prefix over the S slot counts: each thread owns S/1024 slots
-    const uint32_t per = (S + kBuildWG - 1) / kBuildWG;
-    const uint32_t s0 = threadIdx.x * per;
-    uint32_t local = 0;
-    for (uint32_t i = 0; i < per; ++i)
-        if (s0 + i < S) local += cnt[s0 + i];
-    const uint32_t inc = wave_incl_scan<uint32_t>(local);
-    if (lane_id() == 63) wsum[wave_id()] = inc;
-    __syncthreads();
-    uint32_t pre = inc - local;
-    for (uint32_t w = 0; w < wave_id(); ++w) pre += wsum[w];
-    for (uint32_t i = 0; i < per; ++i)
-        if (s0 + i < S) {
-            uint32_t c = cnt[s0 + i];
-            cnt[s0 + i] = pre;
-            pre += c;
+    const uint32_t pmask = (1u << g.log_part) - 1u;
+    const uint32_t nsub = 1u << (g.log_part - g.log_sub);
+    uint64_t base = a;  // first tuple position of the sub-range
+    for (uint32_t sub = 0; sub < nsub; ++sub) {
+        for (uint32_t i = threadIdx.x; i < S; i += kBuildWG) {
+            cnt[i] = 0;
+            tag[i] = 0;
         }
-    __syncthreads();
-    for (uint64_t i = a + threadIdx.x; i < e; i += kBuildWG) {
-        int32_t key = tup_keys[i];
-        uint32_t ls = (uint32_t)slot_of(hash_key(key), g.log_slots) & smask;
-        uint64_t pos = a + atomicAdd(&cnt[ls], 1u);
-        keys[pos] = key;
-        rids[pos] = tup_rids[i];
+        __syncthreads();
+        for (uint64_t i = a + threadIdx.x; i < e; i += kBuildWG) {
+            const uint64_t h = hash_key(tup_keys[i]);
+            const uint32_t ps = (uint32_t)slot_of(h, g.log_slots) & pmask;
+            if ((ps >> g.log_sub) != sub) continue;
+            const uint32_t ls = ps & (S - 1u);
+            atomicAdd(&cnt[ls], 1u);
+            atomicOr(&tag[ls], (uint32_t)c_bloom.t[h & 0x7FF]);
+        }
+        __syncthreads();
+        // exclusive prefix over the S slot counts: each thread owns S/1024 slots
+        const uint32_t per = (S + kBuildWG - 1) / kBuildWG;
+        const uint32_t s0 = threadIdx.x * per;
+        uint32_t local = 0;
+        for (uint32_t i = 0; i < per; ++i)
+            if (s0 + i < S) local += cnt[s0 + i];
+        const uint32_t inc = wave_incl_scan<uint32_t>(local);
+        if (lane_id() == 63) wsum[wave_id()] = inc;
+        __syncthreads();
+        uint32_t pre = inc - local, subtotal = 0;
+        for (uint32_t w = 0; w < kBuildWG / 64; ++w) {
+            pre += w < wave_id() ? wsum[w] : 0u;
+            subtotal += wsum[w];
+        }
+        for (uint32_t i = 0; i < per; ++i)
+            if (s0 + i < S) {
+                const uint32_t c = cnt[s0 + i];
+                cnt[s0 + i] = pre;
+                pre += c;
+            }
+        __syncthreads();
+        for (uint64_t i = a + threadIdx.x; i < e; i += kBuildWG) {
+            const int32_t key = tup_keys[i];
+            const uint32_t ps = (uint32_t)slot_of(hash_key(key), g.log_slots) & pmask;
+            if ((ps >> g.log_sub) != sub) continue;
+            const uint64_t pos = base + atomicAdd(&cnt[ps & (S - 1u)], 1u);
+            keys[pos] = key;
+            rids[pos] = tup_rids[i];
+        }
+        __syncthreads();
+        uint64_t *d = dir + ((uint64_t)p << g.log_part) + ((uint64_t)sub << g.log_sub);
+        for (uint32_t i = threadIdx.x; i < S; i += kBuildWG)
+            d[i] = ((base + cnt[i]) << 16) | (uint64_t)(tag[i] & 0xFFFF);
+        base += subtotal;
+        __syncthreads();
     }
-    __syncthreads();
-    uint64_t *d = dir + (uint64_t)p * S;
-    for (uint32_t i = threadIdx.x; i < S; i += kBuildWG) d[i] = ((a + cnt[i]) << 16) | (uint64_t)(tag[i] & 0xFFFF);
 }
 
 void build_partitions(const HashGeom &g, const uint64_t *part_off, const int32_t *tup_keys, const uint32_t *tup_rids,
                       uint64_t *dir, int32_t *keys, uint32_t *rids, hipStream_t s) {
-    size_t sh = 2 * (size_t)(1u << g.log_part) * sizeof(uint32_t);
+    const size_t sh = 2 * (size_t)(1u << g.log_sub) * sizeof(uint32_t);
+    static bool lds_set = false;  // room for 128 KiB of dynamic LDS (set once per process)
+    if (!lds_set) {
+        CJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_build_partitions),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 16384 * 4));
+        lds_set = true;
+    }
     k_build_partitions<<<g.num_partitions, kBuildWG, sh, s>>>(g, part_off, part_off + 1, tup_keys, tup_rids, dir,
                                                               keys, rids);
     CJ_HIP(hipGetLastError());

@@ -319,3 +319,30 @@ def test_gpu_aligned_pages_without_matches(engine):
     same_multiset(got, exp)
     # fewer probe-side pages than payload pages: the empty ones were closed up
     assert got.num_pages()[1] < p.columns[1].num_pages
+
+
+@pytest.mark.parametrize("build_rows,log_domain", [(17_000_000, 28), (70_000_000, 30)], ids=["above_2_24", "above_2_26"])
+def test_gpu_large_hash_builds(engine, build_rows, log_domain):
+    """Hash builds of more than 2^24 and more than 2^26 rows (the second
+    counts each radix partition in two LDS passes): uniform keys over a
+    domain too wide for a direct map, probed by a slightly larger table (the
+    smaller side builds), against the oracle."""
+    b = workloads.generate(build_rows, 1701, [workloads.col(workloads.GEN_UNIFORM, "INT32", 1, 0.01, lo=0,
+                                                            hi=1 << log_domain),
+                                              workloads.col(workloads.GEN_I32_FULL, "INT32", 2, 0.02)])
+    p = workloads.generate(build_rows + 1_000_000, 1702, [workloads.col(workloads.GEN_UNIFORM, "INT32", 3, 0.01, lo=0,
+                                                           hi=1 << log_domain),
+                                             workloads.col(workloads.GEN_I32_FULL, "INT32", 4, 0.02)])
+    plan = cjtest.Plan()
+    plan.new_input(b)
+    plan.new_input(p)
+    plan.new_scan_node(0, [(0, "INT32"), (1, "INT32")])
+    plan.new_scan_node(1, [(0, "INT32"), (1, "INT32")])
+    plan.root = plan.new_join_node(True, 0, 1, 0, 0, [(2, "INT32"), (1, "INT32"), (3, "INT32")])
+    exp = cjtest.OracleRun(plan, threads=16)
+    assert exp.num_rows > 500_000
+    print(f"hash build of {b.num_rows} rows: {exp.num_rows} result rows", flush=True)
+    for flags in (0, abi.CJ_EXEC_NO_FAST_PATH):
+        got = engine.execute(plan, flags=flags)
+        same_multiset(got, exp)
+        got.release()
