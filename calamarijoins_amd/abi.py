@@ -81,7 +81,13 @@ class Timing(C.Structure):
         ("queue_ms", C.c_double),
         ("service_ms", C.c_double),
         ("phase_ms", C.c_double * 6),
+        ("served_alone", C.c_int32),
     ]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("admitted", C.c_uint64), ("budget_waits", C.c_uint64), ("retried_alone", C.c_uint64),
+                ("peak_admitted_bytes", C.c_uint64), ("peak_running", C.c_int32)]
 
 
 class ServeOptions(C.Structure):
@@ -174,6 +180,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "cj_ctx_destroy": (C.c_int, [C.c_void_p]),
         "cj_last_error": (C.c_char_p, []),
         "cj_ctx_device_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, P(C.c_int32)]),
+        "cj_ctx_memory": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64),
+                                    P(C.c_uint64)]),
         "cj_table_register": (C.c_int, [C.c_void_p, P(TableDesc), P(C.c_uint64)]),
         "cj_table_unregister": (C.c_int, [C.c_void_p, C.c_uint64]),
         "cj_execute": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, P(Result), P(Timing)]),
@@ -187,7 +195,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "cj_submit": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, P(C.c_uint64)]),
         "cj_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, P(Result), P(Timing)]),
         "cj_serve_stop": (C.c_int, [C.c_void_p]),
-        "cj_plan_footprint": (C.c_int, [P(PlanDesc), P(C.c_uint64)]),
+        "cj_serve_stats_get": (C.c_int, [C.c_void_p, P(ServeStats)]),
+        "cj_plan_footprint": (C.c_int, [C.c_void_p, P(PlanDesc), P(C.c_uint64)]),
         "cj_execute_batch": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, C.c_uint32, P(Result), P(BatchStats)]),
         "cj_tbl_write": (C.c_int, [C.c_char_p, P(TableDesc)]),
         "cj_filter": (C.c_int, [C.c_void_p, P(TableDesc), P(FilterNode), C.c_uint32, C.c_uint32, P(Result),

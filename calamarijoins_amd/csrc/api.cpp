@@ -112,6 +112,19 @@ int cj_ctx_device_info(cj_ctx *ctx, char *name, size_t name_cap, int32_t *num_cu
     });
 }
 
+int cj_ctx_memory(cj_ctx *ctx, int32_t reset_peak, uint64_t *reserved, uint64_t *in_use, uint64_t *peak_in_use,
+                  uint64_t *budget) {
+    return guard([&] {
+        if (!ctx) throw cj::InvalidError("null context");
+        cj::DevicePool &p = C(ctx)->pool;
+        if (reserved) *reserved = p.reserved();
+        if (in_use) *in_use = p.in_use();
+        if (peak_in_use) *peak_in_use = p.peak_in_use();
+        if (budget) *budget = p.budget();
+        if (reset_peak) p.reset_peak();
+    });
+}
+
 int cj_table_register(cj_ctx *ctx, const cj_table_desc *table, uint64_t *out_id) {
     return guard([&] {
         if (!ctx || !table || !out_id) throw cj::InvalidError("null argument");
@@ -213,6 +226,13 @@ int cj_wait(cj_ctx *ctx, uint64_t ticket, int64_t timeout_ms, cj_result *out, cj
     return rc != CJ_OK ? rc : st;
 }
 
+int cj_serve_stats_get(cj_ctx *ctx, cj_serve_stats *out) {
+    return guard([&] {
+        if (!ctx || !out) throw cj::InvalidError("null argument");
+        *out = server_of(ctx)->stats();
+    });
+}
+
 int cj_serve_stop(cj_ctx *ctx) {
     return guard([&] {
         if (!ctx) throw cj::InvalidError("null context");
@@ -294,10 +314,10 @@ int cj_tbl_cache_name(const char *table, const char *filter_text, char *out, siz
     });
 }
 
-int cj_plan_footprint(const cj_plan_desc *plan, uint64_t *bytes) {
+int cj_plan_footprint(cj_ctx *ctx, const cj_plan_desc *plan, uint64_t *bytes) {
     return guard([&] {
         if (!plan || !bytes) throw cj::InvalidError("null argument");
-        *bytes = cj::estimate_plan_bytes(*plan);
+        *bytes = cj::estimate_plan_bytes(*plan, ctx ? C(ctx) : nullptr);
     });
 }
 

@@ -56,6 +56,7 @@ void *DevicePool::alloc(size_t bytes) {
     if (it != free_.end() && it->first <= sz + sz / 2 + (size_t(4) << 20)) {
         void *p = it->second;
         in_use_ += it->first;
+        peak_ = std::max(peak_, in_use_);
         free_.erase(it);
         return p;
     }
@@ -75,6 +76,7 @@ void *DevicePool::alloc(size_t bytes) {
     sizes_[p] = sz;
     reserved_ += sz;
     in_use_ += sz;
+    peak_ = std::max(peak_, in_use_);
     return p;
 }
 

@@ -28,6 +28,12 @@ class DevicePool {
     void free(void *p);
     void trim();  // return cached blocks to the driver
     uint64_t in_use() const { return in_use_; }
+    uint64_t reserved() const { return reserved_; }
+    uint64_t peak_in_use() const { return peak_; }
+    void reset_peak() {
+        std::lock_guard<std::mutex> g(mu_);
+        peak_ = in_use_;
+    }
     uint64_t budget() const { return budget_; }
 
   private:
@@ -37,6 +43,7 @@ class DevicePool {
     uint64_t budget_;
     uint64_t reserved_ = 0;  // bytes obtained from hipMalloc
     uint64_t in_use_ = 0;
+    uint64_t peak_ = 0;
 };
 
 // Cached pinned host buffers (result pages, upload staging).

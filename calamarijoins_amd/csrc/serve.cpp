@@ -37,9 +37,10 @@ PlanCopy::PlanCopy(const cj_plan_desc &p) {
     desc.inputs = inputs.data();
 }
 
-uint64_t estimate_plan_bytes(const cj_plan_desc &p) {
+uint64_t estimate_plan_bytes(const cj_plan_desc &p, Context *ctx) {
     // rows a node produces: scans their table's rows, joins their probe side's
     std::vector<uint64_t> rows(p.num_nodes, 0);
+    std::vector<uint32_t> scans(p.num_nodes, 0);  // base tables under a node: its row-id lists
     uint64_t bytes = 0;
     for (uint32_t i = 0; i < p.num_nodes; ++i) {  // children precede parents in well-formed plans
         const cj_node_desc &n = p.nodes[i];
@@ -47,8 +48,15 @@ uint64_t estimate_plan_bytes(const cj_plan_desc &p) {
             if (n.base_table_id < p.num_inputs) {
                 const cj_table_desc &t = p.inputs[n.base_table_id];
                 rows[i] = t.num_rows;
-                // decoded columns (dense values + validity) when not cached
-                bytes += t.num_rows * (8ull + 1) * n.num_output_attrs;
+                scans[i] = 1;
+                for (uint32_t a = 0; a < n.num_output_attrs; ++a) {
+                    const uint64_t ci = n.output_attrs[a].index;
+                    DevColumn *dc = ctx && ci < t.num_columns ? ctx->find_resident(t.columns[ci]) : nullptr;
+                    if (dc && (dc->dense_rows || ctx->opts.cache_decoded)) continue;  // in place, or cached once
+                    // decoded column (dense values + validity), and its pages when uploaded
+                    bytes += t.num_rows * (8ull + 1);
+                    if (!dc && ci < t.num_columns) bytes += (uint64_t)t.columns[ci].num_pages * 8192;
+                }
             }
             continue;
         }
@@ -56,9 +64,11 @@ uint64_t estimate_plan_bytes(const cj_plan_desc &p) {
         const uint64_t r = n.right < p.num_nodes ? rows[n.right] : 0;
         const uint64_t b = std::min(l, r), pr = std::max(l, r);
         rows[i] = pr;
+        scans[i] = (n.left < p.num_nodes ? scans[n.left] : 0) + (n.right < p.num_nodes ? scans[n.right] : 0);
         // build structure (hash table or direct map), counts, match bitmap,
-        // pair lists, one row-id list per output slot
-        bytes += 40 * b + pr / 4 + 8 * pr + 4ull * pr * n.num_output_attrs;
+        // pair lists, one row-id list per base table that contributes columns
+        const uint64_t lists = std::min<uint64_t>(scans[i], n.num_output_attrs);
+        bytes += 40 * b + pr / 4 + 8 * pr + 4ull * pr * lists;
     }
     // result pages of the root
     if (p.root < p.num_nodes) bytes += rows[p.root] * 9ull * p.nodes[p.root].num_output_attrs;
@@ -85,7 +95,7 @@ uint64_t Server::submit(const cj_plan_desc &plan, uint32_t flags) {
     auto r = std::make_unique<ServeRequest>();
     r->plan = std::make_unique<PlanCopy>(plan);
     r->flags = flags;
-    r->est_bytes = estimate_plan_bytes(r->plan->desc);
+    r->est_bytes = estimate_plan_bytes(r->plan->desc, &ctx_);
     r->t_submit = std::chrono::steady_clock::now();
     uint64_t t;
     {
@@ -100,9 +110,20 @@ uint64_t Server::submit(const cj_plan_desc &plan, uint32_t flags) {
 }
 
 ServeRequest *Server::pick_locked() {
+    if (exclusive_) return nullptr;  // a plan runs alone
     for (size_t i = 0; i < pending_.size(); ++i) {
         ServeRequest *r = pending_[i];
+        if (r->alone) {  // waits at the head until nothing runs, and passes nothing
+            if (running_ > 0) return nullptr;
+            pending_.erase(pending_.begin() + (long)i);
+            exclusive_ = true;
+            return r;
+        }
         const bool fits = admitted_ + r->est_bytes <= budget_ || running_ == 0;
+        if (!fits && !r->budget_waited) {  // counted once per request
+            r->budget_waited = true;
+            ++stats_.budget_waits;
+        }
         if (fits) {
             for (size_t k = 0; k < i; ++k) ++pending_[k]->skipped;
             pending_.erase(pending_.begin() + (long)i);
@@ -121,12 +142,17 @@ void Server::worker() {
         if (!r) return;
         admitted_ += r->est_bytes;
         ++running_;
-        peak_admitted_bytes = std::max(peak_admitted_bytes, admitted_);
-        peak_running = std::max(peak_running, running_);
+        r->overlapped = !running_set_.empty();
+        for (ServeRequest *o : running_set_) o->overlapped = true;
+        running_set_.push_back(r);
+        ++stats_.admitted;
+        stats_.peak_admitted_bytes = std::max(stats_.peak_admitted_bytes, admitted_);
+        stats_.peak_running = std::max(stats_.peak_running, (int32_t)running_);
         r->t_start = std::chrono::steady_clock::now();
         g.unlock();
         int status = CJ_OK;
         std::string err;
+        bool retry = false;
         try {
             ctx_.execute(r->plan->desc, r->flags, &r->result, &r->timing);
         } catch (const InvalidError &e) {
@@ -135,6 +161,7 @@ void Server::worker() {
             status = CJ_ERR_UNSUPPORTED, err = e.what();
         } catch (const OomError &e) {
             status = CJ_ERR_OOM, err = e.what();
+            retry = true;  // decided below: only if others ran beside it
         } catch (const DeviceError &e) {
             status = CJ_ERR_DEVICE, err = e.what();
         } catch (const std::bad_alloc &) {
@@ -146,6 +173,15 @@ void Server::worker() {
         r->t_done = std::chrono::steady_clock::now();
         admitted_ -= r->est_bytes;
         --running_;
+        running_set_.erase(std::find(running_set_.begin(), running_set_.end(), r));
+        if (r->alone) exclusive_ = false;
+        if (retry && !r->alone && r->overlapped) {  // out of memory beside others: once more, alone
+            ++stats_.retried_alone;
+            r->alone = true;
+            pending_.push_front(r);
+            cv_work_.notify_all();
+            continue;
+        }
         r->status = status;
         r->error = err;
         r->done = true;
@@ -169,6 +205,7 @@ int Server::wait(uint64_t ticket, int64_t timeout_ms, cj_result *out, cj_timing 
             *timing = r->timing;
             timing->queue_ms = std::chrono::duration<double, std::milli>(r->t_start - r->t_submit).count();
             timing->service_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_start).count();
+            timing->served_alone = r->alone ? 1 : 0;
         }
     } else if (err) {
         *err = r->error;

@@ -35,6 +35,9 @@ struct ServeRequest {
     uint32_t flags = 0;
     uint64_t est_bytes = 0;
     uint32_t skipped = 0;  // times a later request was admitted first
+    bool alone = false;    // ran out of device memory beside others: runs alone
+    bool overlapped = false;  // another plan ran while this one did
+    bool budget_waited = false;  // did not fit the budget at some admission
     std::chrono::steady_clock::time_point t_submit, t_start, t_done;
     bool done = false;
     int status = 0;
@@ -44,10 +47,15 @@ struct ServeRequest {
 };
 
 // Device bytes a plan may hold at its peak: build structures, match and
-// row-id lists, decoded columns and output pages, from the table sizes
-// (joins are assumed not to grow their probe side, as with key / foreign-key
-// joins).
-uint64_t estimate_plan_bytes(const cj_plan_desc &p);
+// row-id lists, decoded columns and output pages, from the table sizes.
+// A join's result is taken as the rows of its larger side, an upper bound
+// for key / foreign-key joins (each row of the foreign-key side meets at
+// most one key); a join whose result grows beyond that can run out of
+// memory beside other plans, and is then run again alone.
+// With a context, a column read from a resident table costs nothing when its
+// decoded form is cached with the table or its INT32 pages are read in place,
+// and a non-resident column adds its uploaded pages.
+uint64_t estimate_plan_bytes(const cj_plan_desc &p, Context *ctx = nullptr);
 
 class Server {
   public:
@@ -59,8 +67,10 @@ class Server {
     int wait(uint64_t ticket, int64_t timeout_ms, cj_result *out, cj_timing *timing, std::string *err);
     void drain();
 
-    uint64_t peak_admitted_bytes = 0;
-    int peak_running = 0;
+    cj_serve_stats stats() {
+        std::lock_guard<std::mutex> g(mu_);
+        return stats_;
+    }
 
   private:
     void worker();
@@ -74,6 +84,9 @@ class Server {
     std::map<uint64_t, std::unique_ptr<ServeRequest>> all_;
     uint64_t admitted_ = 0;
     int running_ = 0;
+    bool exclusive_ = false;  // a request that must run alone is running
+    cj_serve_stats stats_{};
+    std::vector<ServeRequest *> running_set_;
     bool stop_ = false;
     uint64_t next_ticket_ = 1;
     std::vector<std::thread> threads_;

@@ -2,6 +2,7 @@
 // writer; see include/cj_workload.h.
 #include "../../include/cj_workload.h"
 #include "../../include/cj_api.h"
+#include "cj_internal.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -122,7 +123,12 @@ struct VarcharWriter {
     uint8_t bitmap[kPageSz];
     bool overflow = false;
     VarcharWriter(uint8_t *o, uint64_t mp) : out(o), max_pages(mp) { std::memset(bitmap, 0, sizeof bitmap); }
+    uint8_t scratch[kPageSz];  // the page of a counting pass (out == nullptr)
     uint8_t *new_page() {
+        if (!out) {  // counting pass: pages are counted, not kept
+            ++npages;
+            return scratch;
+        }
         if (npages >= max_pages) {
             overflow = true;
             return nullptr;
@@ -184,6 +190,15 @@ struct VarcharWriter {
     }
 };
 
+// Pages the VARCHAR writer needs for these rows (a counting pass, so the
+// page block is allocated exactly rather than for one page per row).
+uint64_t varchar_page_count(uint64_t n, const uint64_t *offs, const char *bytes, const uint8_t *valid) {
+    auto wr = std::make_unique<VarcharWriter>(nullptr, 0);
+    for (uint64_t i = 0; i < n; ++i) wr->add(bytes + offs[i], offs[i + 1] - offs[i], valid[i] != 0);
+    wr->finish();
+    return wr->npages;
+}
+
 uint8_t *alloc_pages(uint64_t n) {
     void *p = std::aligned_alloc(4096, std::max<uint64_t>(n, 1) * kPageSz);
     if (!p) throw std::bad_alloc();
@@ -199,7 +214,7 @@ struct GenOwned {
     }
 };
 
-thread_local std::string g_werr;
+
 
 }  // namespace
 
@@ -276,7 +291,7 @@ int cj_gen_table_create(uint64_t candidate_rows, double keep_frac, uint64_t seed
                     for (uint64_t r = a; r < b; ++r)
                         if (valid[r]) std::memcpy(bytes.data() + offs[r], g.vocab + words[pick[r]].first, words[pick[r]].second);
                 });
-                uint64_t maxp = n + offs[n] / (kPageSz - 8) + 4;
+                uint64_t maxp = varchar_page_count(n, offs.data(), bytes.data(), valid.data());
                 pages = alloc_pages(maxp);
                 int rc = cj_pages_encode_varchar(n, offs.data(), bytes.data(), valid.data(), pages, maxp, &np);
                 if (rc) { std::free(pages); throw std::runtime_error("varchar encode overflow"); }
@@ -305,7 +320,7 @@ int cj_gen_table_create(uint64_t candidate_rows, double keep_frac, uint64_t seed
                         }
                     }
                 });
-                uint64_t maxp = n + offs[n] / (kPageSz - 8) + 4;
+                uint64_t maxp = varchar_page_count(n, offs.data(), bytes.data(), valid.data());
                 pages = alloc_pages(maxp);
                 int rc = cj_pages_encode_varchar(n, offs.data(), bytes.data(), valid.data(), pages, maxp, &np);
                 if (rc) { std::free(pages); throw std::runtime_error("varchar encode overflow"); }
@@ -388,7 +403,7 @@ int cj_gen_table_create(uint64_t candidate_rows, double keep_frac, uint64_t seed
         out->opaque = own.release();
         return CJ_OK;
     } catch (const std::exception &e) {
-        g_werr = e.what();
+        cj::set_last_error(e.what());
         return CJ_ERR_INVALID;
     }
 }

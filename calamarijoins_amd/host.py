@@ -234,6 +234,16 @@ class Engine:
         abi.check(rc)
         return EngineResult(self, res, tm)
 
+    def serve_stats(self) -> dict:
+        st = abi.ServeStats()
+        abi.check(self.lib.cj_serve_stats_get(self.ctx, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in abi.ServeStats._fields_}
+
+    def memory(self, reset_peak: bool = False) -> dict:
+        v = [C.c_uint64() for _ in range(4)]
+        abi.check(self.lib.cj_ctx_memory(self.ctx, int(reset_peak), *[C.byref(x) for x in v]))
+        return dict(zip(("reserved", "in_use", "peak_in_use", "budget"), (x.value for x in v)))
+
     def serve_stop(self):
         abi.check(self.lib.cj_serve_stop(self.ctx))
         self._inflight = {}
@@ -251,9 +261,12 @@ class Engine:
         return [EngineResult(self, res[i], None) for i in range(n)], st
 
     @staticmethod
-    def footprint(plan: Plan) -> int:
+    def footprint(plan: Plan, engine: "Engine" = None) -> int:
+        """Footprint the serving loop charges for `plan` (with an engine: its
+        resident tables counted as such)."""
         b = C.c_uint64()
-        abi.check(abi.load_library().cj_plan_footprint(C.byref(plan.desc()), C.byref(b)))
+        ctx = engine.ctx if engine is not None else None
+        abi.check(abi.load_library().cj_plan_footprint(ctx, C.byref(plan.desc()), C.byref(b)))
         return b.value
 
     def kernel_stats(self):
@@ -305,3 +318,73 @@ class EngineResult:
             self.release()
         except Exception:
             pass
+
+
+# ------------------------------------------------------ result fingerprints
+_MIX = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(x: np.ndarray) -> np.ndarray:
+    """SplitMix64 finaliser on a uint64 array."""
+    x = x.copy()
+    with np.errstate(over="ignore"):
+        x ^= x >> np.uint64(30)
+        x *= np.uint64(0xBF58476D1CE4E5B9)
+        x ^= x >> np.uint64(27)
+        x *= np.uint64(0x94D049BB133111EB)
+        x ^= x >> np.uint64(31)
+    return x
+
+
+def _column_row_words(type_id: int, page_ptrs) -> np.ndarray:
+    """One uint64 per row of a result column (host pages): the value's bits
+    (8-byte values), the zero-extended INT32, a hash of the VARCHAR bytes;
+    NULL rows get a marker distinct from every value word."""
+    words = []
+    null_word = np.uint64(0xD1B54A32D192ED03)
+    i = 0
+    while i < len(page_ptrs):
+        p = np.ctypeslib.as_array((C.c_uint8 * PAGE).from_address(page_ptrs[i]))
+        n = int(p[0]) | (int(p[1]) << 8)
+        nv = int(p[2]) | (int(p[3]) << 8)
+        if type_id == abi.CJ_VARCHAR and n == 0xFFFF:  # a long string: this page and its 0xfffe continuations
+            chunks = [p[4:4 + nv].tobytes()]
+            i += 1
+            while i < len(page_ptrs):
+                q = np.ctypeslib.as_array((C.c_uint8 * PAGE).from_address(page_ptrs[i]))
+                if (int(q[0]) | (int(q[1]) << 8)) != 0xFFFE:
+                    break
+                chunks.append(q[4:4 + (int(q[2]) | (int(q[3]) << 8))].tobytes())
+                i += 1
+            words.append(np.array([hash(b"".join(chunks)) & 0xFFFFFFFFFFFFFFFF], dtype=np.uint64))
+            continue
+        bits = np.unpackbits(p[PAGE - (n + 7) // 8:], bitorder="little")[:n].astype(bool)
+        w = np.full(n, null_word, dtype=np.uint64)
+        if type_id == abi.CJ_VARCHAR:
+            offs = p[4:4 + 2 * nv].view(np.uint16).astype(np.int64)
+            chars = p[4 + 2 * nv:]
+            start = np.concatenate([[0], offs[:-1]]) if nv else offs
+            vals = np.array([hash(chars[a:b].tobytes()) & 0xFFFFFFFFFFFFFFFF for a, b in zip(start, offs)],
+                            dtype=np.uint64)
+        elif type_id == abi.CJ_INT32:
+            vals = p[4:4 + 4 * nv].view(np.uint32).astype(np.uint64)
+        else:
+            vals = p[8:8 + 8 * nv].view(np.uint64)
+        w[bits] = vals[:int(bits.sum())]
+        words.append(w)
+        i += 1
+    return np.concatenate(words) if words else np.zeros(0, dtype=np.uint64)
+
+
+def result_fingerprint(res: "EngineResult") -> np.ndarray:
+    """The sorted per-row hashes of a result held in host pages: equal
+    arrays mean equal row multisets (with overwhelming probability).  Used
+    to show that a plan gives the same rows alone and in a served mix."""
+    h = np.zeros(res.num_rows, dtype=np.uint64)
+    for k, (t, pl) in enumerate(zip(res.types, res.page_lists)):
+        w = _column_row_words(t, pl)
+        if len(w) != res.num_rows:
+            raise ValueError(f"column {k} holds {len(w)} rows, the result {res.num_rows}")
+        with np.errstate(over="ignore"):
+            h = _mix64(h * np.uint64(0x100000001B3) ^ (w + np.uint64(k + 1) * _MIX))
+    return np.sort(h)

@@ -5,9 +5,13 @@ over HBM-resident tables, run either one at a time (the plain drop-in
 concurrent plans on separate streams).  Latency is measured from a plan's
 arrival to its result being ready.
 
-    python -m calamarijoins_amd.serving [--scale S] [--plans K] [--requests N]
+    python -m calamarijoins_amd.serving [--scale S] [--plans K] [--requests N] [--check]
 
-prints one JSON line with p50 / p99 latency and rows/s of both modes.
+prints one JSON line with p50 / p99 latency and rows/s of both modes, the
+admission budget (by default twice the median plan footprint, so that it
+binds) and the serving loop's counters.  --check then serves the whole stream
+once more and compares every result with its plan's result run alone (row
+multiset fingerprints, calamarijoins_amd.host.result_fingerprint).
 """
 from __future__ import annotations
 
@@ -21,7 +25,7 @@ import threading
 import time
 
 from . import abi, workloads
-from .host import Engine
+from .host import Engine, result_fingerprint
 
 
 def percentile(xs, q):
@@ -103,9 +107,11 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--scale", type=float, default=1.0, help="table sizes x scale (1.0 = 0.1M-36M rows)")
     ap.add_argument("--plans", type=int, default=32, help="distinct plans in the stream")
-    ap.add_argument("--requests", type=int, default=96, help="arrivals per mode")
+    ap.add_argument("--requests", type=int, default=2000, help="arrivals per mode")
     ap.add_argument("--workers", type=int, default=4)
-    ap.add_argument("--admit-gib", type=float, default=64.0)
+    ap.add_argument("--admit-gib", type=float, default=0.0,
+                    help="admission budget; 0 = twice the median plan footprint")
+    ap.add_argument("--check", action="store_true", help="serve once more and compare with each plan alone")
     ap.add_argument("--load", type=float, default=0.75, help="QPS = load / mean plain service time")
     ap.add_argument("--seed", type=int, default=1005)
     args = ap.parse_args(argv)
@@ -128,6 +134,8 @@ def main(argv=None):
         eng.execute(p, flags=flags).release()
         svc.append((time.perf_counter() - s) * 1e3)
     mean_svc = statistics.mean(svc)
+    foot = [Engine.footprint(p, eng) for p in plans]
+    admit = int(args.admit_gib * (1 << 30)) if args.admit_gib > 0 else 2 * int(statistics.median(foot))
     qps = args.load / (mean_svc / 1e3)
     rng = random.Random(args.seed)
     order = [rng.randrange(len(plans)) for _ in range(args.requests)]
@@ -135,11 +143,33 @@ def main(argv=None):
     print(f"mean plain service {mean_svc:.2f} ms -> {qps:.1f} QPS", file=sys.stderr, flush=True)
 
     lat_p, rows_p, span_p = run_plain(eng, plans, order, arrivals, flags)
-    eng.serve_start(args.workers, int(args.admit_gib * (1 << 30)))
+    eng.serve_start(args.workers, admit)
     try:
         lat_s, rows_s, span_s = run_served(eng, plans, order, arrivals, flags)
+        stats = eng.serve_stats()
     finally:
         eng.serve_stop()
+    check = None
+    if args.check:
+        alone = {}
+        for k in sorted(set(order)):
+            r = eng.execute(plans[k], flags=0)
+            alone[k] = (r.num_rows, result_fingerprint(r))
+            r.release()
+        bad = []
+
+        def compare(k, r):
+            r.to_host()
+            if r.num_rows != alone[k][0] or not (result_fingerprint(r) == alone[k][1]).all():
+                bad.append(k)
+
+        eng.serve_start(args.workers, admit)
+        try:
+            run_served(eng, plans, order, arrivals, flags, collect=compare)
+        finally:
+            eng.serve_stop()
+        check = {"requests": len(order), "distinct_plans": len(alone), "mismatches": len(bad)}
+        print(f"check: {check}", file=sys.stderr, flush=True)
     out = {
         "workload": f"C5 served stream: 12 resident tables ({args.scale:g} x 0.1M-36M rows, 1-40% pre-filter), "
                     f"{len(plans)} plans of 3-8 joins, {args.requests} Poisson arrivals at {qps:.1f} QPS "
@@ -148,7 +178,9 @@ def main(argv=None):
                   "rows_per_s": rows_p / span_p, "makespan_s": span_p},
         "served": {"p50_ms": percentile(lat_s, 0.5), "p99_ms": percentile(lat_s, 0.99),
                    "rows_per_s": rows_s / span_s, "makespan_s": span_s, "workers": args.workers,
-                   "admit_bytes": int(args.admit_gib * (1 << 30))},
+                   "admit_bytes": admit, "stats": stats},
+        "plan_footprint_bytes": {"min": min(foot), "median": statistics.median(foot), "max": max(foot)},
+        "check_served_equals_alone": check,
     }
     print(json.dumps(out), flush=True)
     eng.close()

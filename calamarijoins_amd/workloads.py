@@ -91,7 +91,8 @@ def generate(candidate_rows: int, seed: int, cols: list, keep_frac: float = 1.0,
     gt = GenTable()
     rc = _lib().cj_gen_table_create(candidate_rows, keep_frac, seed, len(cols), arr, threads, C.byref(gt))
     if rc != 0:
-        raise RuntimeError("table generation failed")
+        msg = _lib().cj_last_error()
+        raise RuntimeError(f"table generation failed: {msg.decode() if msg else rc}")
     owner = _Owned(gt)
     columns = []
     for i in range(gt.num_columns):

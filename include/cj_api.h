@@ -128,6 +128,8 @@ typedef struct cj_timing {
     double queue_ms;    /* served plans: submit .. admitted (0 for cj_execute) */
     double service_ms;  /* served plans: admitted .. result ready              */
     double phase_ms[CJ_NUM_PHASES]; /* CJ_PHASE_* order, untruncated             */
+    int32_t served_alone; /* served plans: 1 when the plan ran out of device
+                           * memory beside others and was run again alone */
 } cj_timing;
 
 /* A result column. Host output: pages[i] are host pointers valid until
@@ -172,6 +174,11 @@ int cj_ctx_destroy(cj_ctx *ctx);
 const char *cj_last_error(void);
 /* Device name and CU count of the context's GPU (diagnostics). */
 int cj_ctx_device_info(cj_ctx *ctx, char *name, size_t name_cap, int32_t *num_cus);
+/* Device memory of the context's pool (diagnostics): bytes obtained from the
+ * driver, bytes in use now, the most in use at once since the last reset
+ * (reset_peak != 0 resets it to the current use), and the budget. */
+int cj_ctx_memory(cj_ctx *ctx, int32_t reset_peak, uint64_t *reserved, uint64_t *in_use, uint64_t *peak_in_use,
+                  uint64_t *budget);
 
 /* ---- HBM-resident base tables.  A registered table's pages are uploaded
  *      once; later plans whose inputs use the same page pointers read the
@@ -229,8 +236,11 @@ int cj_kernel_stats_reset(cj_ctx *ctx);
  *      at a time: execute.cpp:311-325).  Plans submitted as they arrive are
  *      admitted by estimated device footprint within admit_bytes and run
  *      concurrently, one worker per stream (at most num_streams of the
- *      context).  The plan's descriptor arrays are copied at submit; its
- *      pages must stay valid until cj_wait returns. ------------------------ */
+ *      context).  A plan that runs out of device memory while others run is
+ *      queued again at the head and run alone (nothing else is admitted
+ *      until it finishes; cj_timing.served_alone says so).  The plan's
+ *      descriptor arrays are copied at submit; its pages must stay valid
+ *      until cj_wait returns. --------------------------------------------- */
 typedef struct cj_serve_options {
     int32_t workers;      /* concurrent plans (<= the context's num_streams)  */
     uint64_t admit_bytes; /* footprint budget of the admitted plans           */
@@ -243,8 +253,18 @@ int cj_wait(cj_ctx *ctx, uint64_t ticket, int64_t timeout_ms, cj_result *out, cj
 /* Finishes every submitted plan, then stops the workers.  Results not yet
  * collected are released. */
 int cj_serve_stop(cj_ctx *ctx);
-/* Estimated device footprint the server charges for a plan. */
-int cj_plan_footprint(const cj_plan_desc *plan, uint64_t *bytes);
+/* Counters of the running serving loop since cj_serve_start. */
+typedef struct cj_serve_stats {
+    uint64_t admitted;             /* plans started (a plan run again alone counts twice) */
+    uint64_t budget_waits;         /* plans that waited at least once because the budget was full */
+    uint64_t retried_alone;        /* plans run again alone after running out of memory */
+    uint64_t peak_admitted_bytes;  /* largest sum of admitted footprints */
+    int32_t peak_running;          /* most plans running at once */
+} cj_serve_stats;
+int cj_serve_stats_get(cj_ctx *ctx, cj_serve_stats *out);
+/* Estimated device footprint the server of ctx charges for a plan; ctx may
+ * be NULL (then as if no input were resident). */
+int cj_plan_footprint(cj_ctx *ctx, const cj_plan_desc *plan, uint64_t *bytes);
 
 /* ---- batched execution of many independent plans (no counterpart in the
  *      reference).  Plans that are one join of two resident base-table scans

@@ -84,3 +84,73 @@ def test_gpu_served_error_and_timeout(engine, c5_small):
         r.release()
     finally:
         engine.serve_stop()
+
+
+def test_gpu_served_out_of_memory_runs_alone():
+    """A device budget with room for the tables and one and a half plans:
+    plans admitted together run out of memory, are queued again and run
+    alone (nothing else admitted meanwhile), and every result equals the
+    plan's result run alone.  (Results kept in device memory by the caller
+    count against the budget too; these plans return host pages.)"""
+    b, p = workloads.c2_tables(seed=97, build_rows=1_000_000, probe_rows=12_000_000)
+    plan = workloads.c2_plan(b, p)
+    probe = cjtest.Engine()
+    try:
+        for t in (b, p):
+            probe.register(t)
+        tables = probe.memory(reset_peak=True)["in_use"]
+        r = probe.execute(plan, flags=abi.CJ_EXEC_OUTPUT_DEVICE)
+        peak = probe.memory()["peak_in_use"]
+        r.to_host()
+        want = (r.num_rows, hashes(r))
+        r.release()
+    finally:
+        probe.close()
+    plan_bytes = peak - tables
+    assert plan_bytes > (64 << 20)
+    eng = cjtest.Engine(device_budget=tables + plan_bytes * 3 // 2 + (32 << 20), num_streams=4)
+    try:
+        for t in (b, p):
+            eng.register(t)
+        eng.serve_start(4, 1 << 40)  # admission lets all four in: the device budget is what binds
+        try:
+            # host output: a finished plan's pages leave the device, so results
+            # not yet collected do not hold the budget
+            tickets = [eng.submit(plan, 0) for _ in range(8)]
+            alone = 0
+            for t in tickets:
+                r = eng.wait(t)
+                assert (r.num_rows, None) == (want[0], None) and np.array_equal(hashes(r), want[1])
+                alone += r.timing.served_alone
+                r.release()
+            st = eng.serve_stats()
+        finally:
+            eng.serve_stop()
+        assert alone >= 1 and st["retried_alone"] == alone, (alone, st)
+        assert st["peak_running"] >= 2
+    finally:
+        eng.close()
+
+
+def test_gpu_served_budget_binds(engine, c5_small):
+    """A budget of about two plan footprints: plans wait for admission, at
+    most as many run at once as fit, and results are unchanged."""
+    tables, plans, _ = c5_small
+    foot = sorted(engine.footprint(pl, engine) for pl in plans)
+    budget = 2 * foot[len(foot) // 2]
+    alone = []
+    for pl in plans:
+        r = engine.execute(pl)
+        alone.append((r.num_rows, hashes(r)))
+        r.release()
+    engine.serve_start(4, budget)
+    try:
+        ts = [(k % len(plans), engine.submit(plans[k % len(plans)])) for k in range(3 * len(plans))]
+        for k, t in ts:
+            r = engine.wait(t)
+            assert r.num_rows == alone[k][0] and np.array_equal(hashes(r), alone[k][1]), k
+            r.release()
+        st = engine.serve_stats()
+    finally:
+        engine.serve_stop()
+    assert st["budget_waits"] > 0 and st["peak_admitted_bytes"] <= max(budget, foot[-1]), (st, budget)
