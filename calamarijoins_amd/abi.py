@@ -112,6 +112,9 @@ class BatchStats(C.Structure):
         ("plans_batched", C.c_uint32), ("plans_single", C.c_uint32), ("device_ms", C.c_double),
         ("total_ms", C.c_double), ("output_rows", C.c_uint64), ("load_factor", C.c_uint32), ("grid", C.c_uint32),
         ("tuned", C.c_int32), ("graph_replayed", C.c_int32),
+        ("log_part", C.c_uint32),
+        ("chunk", C.c_uint32),
+        ("join_queue", C.c_uint32),
     ]
 
 

@@ -1,7 +1,7 @@
 """Batched independent joins (C4): 256 single-join plans over HBM-resident
 tables, run either one at a time through the drop-in `execute()` or in one
 `cj_execute_batch` call (shared persistent-kernel launches over a device
-work list, phase graph replayed, autotuned load factor and grid).
+work list, phase graph replayed, autotuned load factor, grid, radix bits and join tile).
 
     python -m calamarijoins_amd.batching [--scale S] [--plans K] [--reps R]
 
@@ -72,7 +72,7 @@ def main(argv=None):
     if rows_batch != rows_seq:
         bad = [k for k in range(len(plans)) if rows_batch[k] != rows_seq[k]]
         raise SystemExit(f"batched row counts differ from per-plan execute for plans {bad[:8]}")
-    tuned = (st.load_factor, st.grid)
+    tuned = (st.load_factor, st.grid, st.log_part, st.chunk, st.join_queue)
     b_ms, b_dev, replayed = [], [], 0
     for _ in range(args.reps):
         s = time.perf_counter()
@@ -97,7 +97,8 @@ def main(argv=None):
                        "gb_per_s": bytes_alg / (seq_w / 1e3) / 1e9},
         "batched": {"wall_ms": b_w, "device_ms": b_d, "rows_per_s": out_rows / (b_w / 1e3),
                     "gb_per_s": bytes_alg / (b_w / 1e3) / 1e9, "device_gb_per_s": bytes_alg / (b_d / 1e3) / 1e9,
-                    "load_factor": tuned[0], "grid": tuned[1], "graph_replays": replayed,
+                    "load_factor": tuned[0], "grid": tuned[1], "log_part": tuned[2], "chunk": tuned[3], "join_queue": tuned[4],
+                    "graph_replays": replayed,
                     "plans_batched": st.plans_batched},
         "speedup_wall": seq_w / b_w,
         "reps": args.reps,

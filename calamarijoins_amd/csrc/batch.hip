@@ -16,8 +16,21 @@ namespace cj {
 
 namespace {
 
-constexpr uint32_t kSlots = 1u << kBatchLogPart;
-constexpr uint32_t kSlotMask = kSlots - 1;
+constexpr uint32_t kMaxSlots = 1u << kBatchMaxLogPart;  // LDS arrays hold the largest partition
+
+// The next join unit: from the phase's work queue (one atomic on one
+// counter per unit claimed, every thread gets the result) or strided by
+// workgroup.  Only the join phases may take the queue: their units are
+// large (thousands of probe tuples) and of uneven cost; page-sized units
+// would serialize on the one counter (about 6 ns a claim on MI355X).
+__device__ __forceinline__ uint32_t next_unit(bool queue, uint32_t *q, uint32_t *slot, uint32_t prev, bool first) {
+    if (!queue) return first ? blockIdx.x : prev + gridDim.x;
+    if (threadIdx.x == 0) *slot = atomicAdd(q, 1u);
+    __syncthreads();
+    const uint32_t g = *slot;
+    __syncthreads();
+    return g;
+}
 
 __device__ __forceinline__ uint64_t bslot_of(int32_t key, uint32_t log_slots) {
     return hash_key(key) >> (64 - log_slots);
@@ -84,12 +97,13 @@ __global__ __launch_bounds__(kWG) void k_bx_part_count(BatchArgs a, uint32_t uni
     extern __shared__ uint32_t hist[];  // max_parts
     __shared__ uint32_t lds32[32];
     uint64_t *count = PROBE ? a.pcount : a.bcount;
+    const uint32_t lp = a.log_part;
     for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
         const uint32_t k = find_owner(pstart, a.nplans, g);
         const BPlan &pl = plans[k];
         const BSide &sd = PROBE ? pl.p : pl.b;
         const uint32_t ls = pl.log_slots;
-        const uint32_t P = 1u << (ls - kBatchLogPart);
+        const uint32_t P = 1u << (ls - lp);
         for (uint32_t i = threadIdx.x; i < P; i += kWG) hist[i] = 0;
         __syncthreads();
         const CJ_G uint8_t *p = glb(sd.k_pages) + (size_t)(g - pstart[k]) * kPage;
@@ -100,7 +114,7 @@ __global__ __launch_bounds__(kWG) void k_bx_part_count(BatchArgs a, uint32_t uni
             const uint32_t valid = load_keys8(p, n, nv, n == nv, t0, &running, key, lds32);
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j)
-                if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(key[j], ls) >> kBatchLogPart], 1u);
+                if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(key[j], ls) >> lp], 1u);
         }
         __syncthreads();
         const uint32_t pb = part_start[k];
@@ -117,18 +131,21 @@ template <bool PROBE>
 __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t units, const BPlan *__restrict__ plans,
                                                          const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ part_start,
                                                          const uint64_t *__restrict__ off) {
-    extern __shared__ uint32_t sm[];  // hist[max_parts] then cursors (u64)
-    uint32_t *hist = sm;
-    uint64_t *cur = reinterpret_cast<uint64_t *>(sm + ((a.max_parts + 1) & ~1u));
+    // cursors (u64) then hist[max_parts]: declared as u64 so the 64-bit LDS
+    // atomics stay 8-byte aligned whatever static LDS precedes the array
+    extern __shared__ uint64_t sm64[];
+    uint64_t *cur = sm64;
+    uint32_t *hist = reinterpret_cast<uint32_t *>(sm64 + a.max_parts);
     __shared__ uint32_t lds32[32];
     uint64_t *gcur = PROBE ? a.pcur : a.bcur;
     int32_t *keys_out = PROBE ? a.pkeys : a.bstage_keys;
+    const uint32_t lp = a.log_part;
     for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
         const uint32_t k = find_owner(pstart, a.nplans, g);
         const BPlan &pl = plans[k];
         const BSide &sd = PROBE ? pl.p : pl.b;
         const uint32_t ls = pl.log_slots;
-        const uint32_t P = 1u << (ls - kBatchLogPart);
+        const uint32_t P = 1u << (ls - lp);
         const uint32_t page = g - pstart[k];
         for (uint32_t i = threadIdx.x; i < P; i += kWG) hist[i] = 0;
         __syncthreads();
@@ -140,7 +157,7 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t u
             const uint32_t valid = load_keys8(p, n, nv, n == nv, t0, &running, key, lds32);
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j)
-                if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(key[j], ls) >> kBatchLogPart], 1u);
+                if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(key[j], ls) >> lp], 1u);
         }
         __syncthreads();
         const uint32_t pb = part_start[k];
@@ -159,7 +176,7 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t u
             for (uint32_t j = 0; j < kRPT; ++j) {
                 pos[j] = 0;
                 if ((valid >> j) & 1u)
-                    pos[j] = atomicAdd((unsigned long long *)&cur[bslot_of(key[j], ls) >> kBatchLogPart], 1ull);
+                    pos[j] = atomicAdd((unsigned long long *)&cur[bslot_of(key[j], ls) >> lp], 1ull);
             }
             // per carried column: all loads, then all stores (a store between
             // two loads would serialize them on memory latency)
@@ -178,27 +195,27 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t u
 __global__ __launch_bounds__(kWG) void k_bx_build(BatchArgs a, const BPlan *__restrict__ plans,
                                                   const uint32_t *__restrict__ part_start, const uint64_t *__restrict__ boff,
                                                   const int32_t *__restrict__ bstage_keys) {
-    __shared__ uint32_t cnt[kSlots];
+    __shared__ uint32_t cnt[kMaxSlots];
     __shared__ uint32_t lds4[4];
+    const uint32_t S = 1u << a.log_part, smask = S - 1;
     for (uint32_t g = blockIdx.x; g < a.nparts; g += gridDim.x) {
         const uint32_t k = find_owner(part_start, a.nplans, g);
         const BPlan &pl = plans[k];
         const uint32_t ls = pl.log_slots;
         const uint64_t t_a = boff[g], t_e = boff[g + 1];
         const uint64_t plan_base = boff[part_start[k]];
-        for (uint32_t i = threadIdx.x; i < kSlots; i += kWG) cnt[i] = 0;
+        for (uint32_t i = threadIdx.x; i < S; i += kWG) cnt[i] = 0;
         __syncthreads();
         for (uint64_t i = t_a + threadIdx.x; i < t_e; i += kWG)
-            atomicAdd(&cnt[(uint32_t)bslot_of(bstage_keys[i], ls) & kSlotMask], 1u);
+            atomicAdd(&cnt[(uint32_t)bslot_of(bstage_keys[i], ls) & smask], 1u);
         __syncthreads();
-        constexpr uint32_t per = kSlots / kWG;
+        const uint32_t per = S / kWG;
         const uint32_t s0 = threadIdx.x * per;
         uint32_t local = 0;
 #pragma unroll
         for (uint32_t i = 0; i < per; ++i) local += cnt[s0 + i];
         uint32_t tot;
         uint32_t pre = block_excl_scan<uint32_t>(local, &tot, lds4);
-#pragma unroll
         for (uint32_t i = 0; i < per; ++i) {
             const uint32_t c = cnt[s0 + i];
             cnt[s0 + i] = pre;
@@ -208,7 +225,7 @@ __global__ __launch_bounds__(kWG) void k_bx_build(BatchArgs a, const BPlan *__re
         const uint32_t ncar = pl.b.ncar;
         for (uint64_t i = t_a + threadIdx.x; i < t_e; i += kWG) {
             const int32_t key = bstage_keys[i];
-            const uint32_t pos = atomicAdd(&cnt[(uint32_t)bslot_of(key, ls) & kSlotMask], 1u);
+            const uint32_t pos = atomicAdd(&cnt[(uint32_t)bslot_of(key, ls) & smask], 1u);
             a.bkeys[t_a + pos] = key;
             const uint64_t src = i - plan_base, dst = t_a + pos - plan_base;
             for (uint32_t c = 0; c < ncar; ++c) {
@@ -223,21 +240,21 @@ __global__ __launch_bounds__(kWG) void k_bx_build(BatchArgs a, const BPlan *__re
             }
         }
         __syncthreads();
-        uint32_t *d = a.dir + pl.dir_base + (uint64_t)(g - part_start[k]) * kSlots;
-        for (uint32_t i = threadIdx.x; i < kSlots; i += kWG) d[i] = cnt[i];
+        uint32_t *d = a.dir + pl.dir_base + ((uint64_t)(g - part_start[k]) << a.log_part);
+        for (uint32_t i = threadIdx.x; i < S; i += kWG) d[i] = cnt[i];
         __syncthreads();
     }
 }
 
 // ------------------------------------------------------------- join units
-// A probe partition is cut into chunks of kBatchChunk tuples; a partition
+// A probe partition is cut into chunks of a.chunk tuples; a partition
 // whose build side is empty has no units (it cannot match).
 __global__ void k_bx_units(BatchArgs a) {
     const uint32_t g = blockIdx.x * kWG + threadIdx.x;
     if (g >= a.nparts) return;
     const uint64_t np = a.poff[g + 1] - a.poff[g];
     const bool build = a.boff[g + 1] > a.boff[g];
-    a.ucount[g] = build ? (np + kBatchChunk - 1) / kBatchChunk : 0;
+    a.ucount[g] = build ? (np + a.chunk - 1) / a.chunk : 0;
 }
 
 // ------------------------------------------------------------------ join
@@ -251,6 +268,7 @@ struct BuildPart {
     const uint32_t *g_ends;
     const int32_t *g_keys;  // bkeys + t_a
     uint64_t g_car0;        // plan-local index of the partition's first tuple
+    uint32_t smask;         // slots per partition - 1
     uint32_t cap;           // LDS layout: keys[cap], values of each carried column, validity bytes
     uint32_t ncar;
     uint32_t w8;            // bit c: carried column c is 8 bytes wide
@@ -371,7 +389,7 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
         }
 #pragma unroll
         for (uint32_t j = 0; j < RPT; ++j) {
-            const uint32_t sl = (uint32_t)bslot_of(key[j], ls) & kSlotMask;
+            const uint32_t sl = (uint32_t)bslot_of(key[j], ls) & bp.smask;
             const uint32_t hi = bp_end<LDS>(bp, sl);
             const uint32_t lo = sl ? bp_end<LDS>(bp, sl - 1) : 0u;
             const bool in = i0 + kWG * j + threadIdx.x < p1;
@@ -474,25 +492,30 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
                                                  const uint64_t *__restrict__ moff, const int32_t *__restrict__ pkeys,
                                                  const int32_t *__restrict__ bkeys, const uint32_t *__restrict__ dir) {
     extern __shared__ __align__(16) uint8_t dyn[];  // keys, carried build values
-    __shared__ uint16_t ends[kSlots];
+    __shared__ uint16_t ends[kMaxSlots];
     __shared__ uint32_t lds32[32];
     __shared__ uint64_t red[4];
     __shared__ uint32_t mbuf[WRITE ? kMatchBuf : 1];
-    const uint64_t total = uoff[a.nparts];
-    for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
+    __shared__ uint32_t unit;
+    const uint64_t total = uoff[a.nparts];  // < 2^32 (max_units)
+    const uint32_t S = 1u << a.log_part;
+    uint32_t *q = a.queue + (WRITE ? kQJoinWrite : kQJoinCount);
+    for (uint32_t u = next_unit(a.join_queue, q, &unit, 0, true); u < total;
+         u = next_unit(a.join_queue, q, &unit, u, false)) {
         const uint32_t g = find_owner<uint64_t>(uoff, a.nparts, u);
         const uint32_t k = find_owner(part_start, a.nplans, g);
         const BPlan &pl = plans[k];
         const uint64_t t_a = boff[g], t_e = boff[g + 1];
         const uint32_t nb = (uint32_t)(t_e - t_a);
         const uint64_t bbase = boff[part_start[k]], pbase = poff[part_start[k]];
-        const uint64_t p0 = poff[g] + (u - uoff[g]) * kBatchChunk;
+        const uint64_t p0 = poff[g] + (u - uoff[g]) * a.chunk;
         const uint64_t pe = poff[g + 1];
-        const uint64_t p1 = p0 + kBatchChunk < pe ? p0 + kBatchChunk : pe;
+        const uint64_t p1 = p0 + a.chunk < pe ? p0 + a.chunk : pe;
         BuildPart bp;
         bp.l_ends = ends;
         bp.l_dyn = dyn;
-        bp.g_ends = dir + pl.dir_base + (uint64_t)(g - part_start[k]) * kSlots;
+        bp.g_ends = dir + pl.dir_base + ((uint64_t)(g - part_start[k]) << a.log_part);
+        bp.smask = S - 1;
         bp.g_keys = bkeys + t_a;
         bp.g_car0 = t_a - bbase;
         const uint32_t ncar = pl.b.ncar;
@@ -506,11 +529,13 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
         if (nb <= a.lds_cap) {
             // stage the partition: each batch of loads is issued before its LDS stores
             {
-                uint32_t ev[kSlots / kWG];
+                uint32_t ev[kMaxSlots / kWG];
 #pragma unroll
-                for (uint32_t q = 0; q < kSlots / kWG; ++q) ev[q] = bp.g_ends[q * kWG + threadIdx.x];
+                for (uint32_t q = 0; q < kMaxSlots / kWG; ++q)
+                    ev[q] = bp.g_ends[q * kWG + threadIdx.x < S ? q * kWG + threadIdx.x : 0];
 #pragma unroll
-                for (uint32_t q = 0; q < kSlots / kWG; ++q) ends[q * kWG + threadIdx.x] = (uint16_t)ev[q];
+                for (uint32_t q = 0; q < kMaxSlots / kWG; ++q)
+                    if (q * kWG + threadIdx.x < S) ends[q * kWG + threadIdx.x] = (uint16_t)ev[q];
             }
             stage_lds<int32_t>(reinterpret_cast<int32_t *>(dyn), bp.g_keys, nb);
             if (WRITE)
@@ -708,7 +733,7 @@ void batch_part_count(const BatchArgs &a, bool probe, uint32_t units, uint32_t g
 
 void batch_part_scatter(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s) {
     if (!units) return;
-    const size_t sh = ((a.max_parts + 1) & ~1u) * sizeof(uint32_t) + a.max_parts * sizeof(uint64_t);
+    const size_t sh = a.max_parts * (sizeof(uint64_t) + sizeof(uint32_t));
     if (probe) k_bx_part_scatter<true><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.ppage_start, a.part_start,
                                                                      a.poff);
     else k_bx_part_scatter<false><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.bpage_start, a.part_start,

@@ -14,10 +14,18 @@
 //   into 8 KiB pages.
 //
 // Every phase is one launch of a persistent kernel that walks a work list
-// covering all plans: global unit g (a key page, a partition, a probe chunk,
-// an output page) maps to (plan, local unit) through a prefix array.  The
+// covering all plans: global unit g (a key page, a partition, a probe
+// chunk, an output page) maps to (plan, local unit) through a prefix array.
+// Workgroups take units strided by the grid; the join phases may instead
+// pull them from a device work queue (one atomic counter per phase).  The
 // phase sequence before the read-back is a hipGraph captured once per batch
-// template and replayed.
+// template and replayed.  The autotuner picks the hash load factor, the
+// radix bits (slots per partition), the join tile (probe tuples per unit),
+// the join scheduling and the grid.
+
+// Work queue counters of the join phases (used when the tuned
+// configuration schedules join units dynamically).
+enum { kQJoinCount = 0, kQJoinWrite, kBatchQueues };
 #pragma once
 #include "cj_internal.hpp"
 
@@ -25,9 +33,9 @@ namespace cj {
 
 constexpr uint32_t kBatchMaxCarry = 4;                // carried columns per side
 constexpr uint32_t kBatchMaxOut = 8;                  // output columns per plan
-constexpr uint32_t kBatchLogPart = 12;                // slots per partition = 4096
+constexpr uint32_t kBatchMaxLogPart = 12;             // slots per partition: 2^11 or 2^12 (tuned)
 constexpr uint64_t kBatchMaxBuildRows = 1ull << 22;   // larger builds run one plan at a time
-constexpr uint32_t kBatchChunk = 8192;                // probe tuples per join unit
+constexpr uint32_t kBatchMaxChunk = 16384;            // probe tuples per join unit: 4096 .. 16384 (tuned)
 constexpr uint32_t kBatchLdsBytes = 64 * 1024;        // LDS of a join workgroup
 constexpr uint32_t kBatchMatchBuf = 2048;             // matches a join workgroup expands at a time
 
@@ -81,6 +89,10 @@ struct BatchArgs {
     const uint64_t *moff;         // exclusive scan
     uint32_t lds_cap;             // build tuples a join workgroup holds in LDS
     uint32_t lds_bytes;
+    uint32_t log_part;            // slots per partition = 2^log_part
+    uint32_t chunk;               // probe tuples per join unit
+    uint32_t join_queue;          // 1: join units from the work queue, 0: strided by workgroup
+    uint32_t *queue;              // [kBatchQueues] join work queue counters (zeroed before each join launch)
 };
 
 // Output columns of one plan (known after the read-back).

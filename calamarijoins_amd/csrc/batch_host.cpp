@@ -170,7 +170,7 @@ void Context::execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flag
     // configuration: the tuned one for this template class, else the default
     const uint64_t cls = ((uint64_t)ceil_log2(nb_plans + 1) << 16) | ((uint64_t)ceil_log2(sum_probe + 1) << 8) |
                          ceil_log2(sum_build + 1);
-    BatchConfig cfg{2, 2048};
+    BatchConfig cfg{2, 2048, 12, 8192, 0};
     bool tuned = false;
     {
         std::lock_guard<std::mutex> g(mu);
@@ -181,12 +181,27 @@ void Context::execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flag
         }
     }
     if (!tuned && (flags & CJ_BATCH_AUTOTUNE)) {
-        const BatchConfig cands[] = {{2, 1024}, {2, 2048}, {2, 4096}, {4, 2048}, {4, 4096}};
+        // load factor x grid x radix bits (slots per partition) x join tile x join scheduling
+        std::vector<BatchConfig> cands;
+        for (uint32_t lf : {2u, 4u})
+            for (uint32_t grid : {2048u, 4096u})
+                for (uint32_t lp : {11u, 12u})
+                    for (uint32_t chunk : {4096u, 8192u, 16384u})
+                        for (uint32_t jq : {0u, 1u}) cands.push_back({lf, grid, lp, chunk, jq});
         double best = 1e30;
+        std::vector<uint64_t> rows0;  // every configuration must give the same per-plan row counts
         for (const BatchConfig &c : cands) {
-            std::vector<cj_result> tmp(n);
-            double ms = run_batch(*this, batched, info, c, flags | CJ_EXEC_OUTPUT_DEVICE, tmp.data(), nullptr);
-            for (uint32_t i : batched) release(&tmp[i]);
+            // the second run replays the captured graph with the pool warm
+            double ms = 0;
+            for (int rep = 0; rep < 2; ++rep) {
+                std::vector<cj_result> tmp(n);
+                ms = run_batch(*this, batched, info, c, flags | CJ_EXEC_OUTPUT_DEVICE, tmp.data(), nullptr);
+                std::vector<uint64_t> rows;
+                for (uint32_t i : batched) rows.push_back(tmp[i].num_rows);
+                for (uint32_t i : batched) release(&tmp[i]);
+                if (rows0.empty()) rows0 = rows;
+                else if (rows != rows0) throw DeviceError("batch configurations disagree on result sizes");
+            }
             if (ms < best) {
                 best = ms;
                 cfg = c;
@@ -199,6 +214,9 @@ void Context::execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flag
     bs.device_ms = run_batch(*this, batched, info, cfg, flags, out, &bs);
     bs.load_factor = cfg.load_factor;
     bs.grid = cfg.grid;
+    bs.log_part = cfg.log_part;
+    bs.chunk = cfg.chunk;
+    bs.join_queue = cfg.join_queue;
     bs.total_ms = ms_since(t_start);
 }
 
@@ -214,6 +232,10 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         ~Rel_() { c.release_stream(s); }
     } rel_guard{ctx, s};
     CJ_HIP(hipSetDevice(ctx.device));
+    // the kernels' LDS arrays hold 2^kBatchMaxLogPart slots, scanned kWG at a time
+    if (cfg.log_part < 8 || cfg.log_part > kBatchMaxLogPart || cfg.chunk == 0 || cfg.chunk > kBatchMaxChunk ||
+        cfg.grid == 0)
+        throw DeviceError("batch configuration out of range");
     // ---- shape of the batch
     std::vector<uint32_t> starts(3 * (n + 1), 0);
     uint32_t *bps = starts.data(), *pps = bps + (n + 1), *pts = pps + (n + 1);
@@ -226,9 +248,9 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
     auto mixk = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
     for (uint32_t k = 0; k < n; ++k) {
         const PlanInfo &pi = info[batched[k]];
-        const uint64_t want = std::max<uint64_t>(1ull << kBatchLogPart, (uint64_t)cfg.load_factor * pi.bk->num_rows);
+        const uint64_t want = std::max<uint64_t>(1ull << cfg.log_part, (uint64_t)cfg.load_factor * pi.bk->num_rows);
         log_slots[k] = std::min<uint32_t>(ceil_log2(want), 30);
-        const uint32_t parts = 1u << (log_slots[k] - kBatchLogPart);
+        const uint32_t parts = 1u << (log_slots[k] - cfg.log_part);
         max_parts = std::max(max_parts, parts);
         dir_total += 1ull << log_slots[k];
         sum_nb += pi.bk->num_rows;
@@ -251,11 +273,11 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         for (DevColumn *c : pi.pcar) mixk(c->type);
     }
     const uint32_t nbp = bps[n], npp = pps[n], nparts = pts[n];
-    const uint64_t max_units64 = nparts + sum_np / kBatchChunk + 1;
+    const uint64_t max_units64 = nparts + sum_np / cfg.chunk + 1;
     if (max_units64 > 0xffffffffull) throw UnsupportedError("batch too large");
     const uint32_t max_units = (uint32_t)max_units64;
     // LDS of a join workgroup: slot ends (u16), the match window (u32), then the partition
-    const uint32_t dyn_budget = kBatchLdsBytes - (2u << kBatchLogPart) - 4 * kBatchMatchBuf - 512;
+    const uint32_t dyn_budget = kBatchLdsBytes - (2u << kBatchMaxLogPart) - 4 * kBatchMatchBuf - 512;
     const uint32_t lds_cap = std::min<uint32_t>((dyn_budget / max_tuple_bytes) & ~15u, 65535u & ~15u);
     std::vector<BDecode> jobs(dec_cols.size());
     std::vector<uint32_t> job_start(dec_cols.size() + 1, 0);
@@ -268,7 +290,7 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         // equal row counts with another NULL density give other page counts
         mixk(dec_cols[j]->num_rows), mixk(dec_cols[j]->type), mixk(dec_cols[j]->num_pages);
     }
-    mixk(n), mixk(cfg.grid), mixk(cfg.load_factor), mixk(dec_cols.size());
+    mixk(n), mixk(cfg.grid), mixk(cfg.load_factor), mixk(cfg.log_part), mixk(cfg.chunk), mixk(cfg.join_queue), mixk(dec_cols.size());
     // ---- the template workspace (reused while the batch shape repeats)
     std::shared_ptr<BatchTemplate> tpl;
     {
@@ -362,6 +384,10 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         a.max_units = max_units;
         a.lds_cap = lds_cap;
         a.lds_bytes = lds_cap * max_tuple_bytes;
+        a.log_part = cfg.log_part;
+        a.chunk = cfg.chunk;
+        a.join_queue = cfg.join_queue;
+        a.queue = reinterpret_cast<uint32_t *>(blk(kBatchQueues * 4));
         tpl->njobs = (uint32_t)jobs.size();
         tpl->ndpages = job_start.back();
     }
@@ -417,6 +443,7 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         fill_u64(A.pcount, 0, nparts + 1, ps);
         fill_u64(A.mcount, 0, (uint64_t)max_units + 1, ps);
         fill_u32(T.valid_all, 0, T.valid_words, ps);
+        fill_u32(A.queue, 0, kBatchQueues, ps);  // the join work queues (a fill kernel, not a memset)
         batch_decode(T.d_jobs, T.d_job_start, T.njobs, T.ndpages, cfg.grid, ps);
         batch_part_count(A, false, nbp, cfg.grid, ps);
         scan_exclusive_u64(A.bcount, const_cast<uint64_t *>(A.boff), nparts, ps, T.scan_tmp, T.scan_elems);
@@ -434,17 +461,16 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         batch_totals(A, T.totals, ps);
         CJ_HIP(hipGetLastError());
     };
-    hipEvent_t e0, e1;
-    CJ_HIP(hipEventCreate(&e0));
-    CJ_HIP(hipEventCreate(&e1));
+    // device time = the phase-1 graph + the phase-2 launches (host work
+    // between them, the capture included, is not counted)
+    hipEvent_t ev[4];
+    for (hipEvent_t &e : ev) CJ_HIP(hipEventCreate(&e));
     struct EvGuard {
-        hipEvent_t a, b;
+        hipEvent_t *e;
         ~EvGuard() {
-            hipEventDestroy(a);
-            hipEventDestroy(b);
+            for (int i = 0; i < 4; ++i) hipEventDestroy(e[i]);
         }
-    } eg{e0, e1};
-    CJ_HIP(hipEventRecord(e0, s));
+    } eg{ev};
     if (fresh || !T.exec) {
         CJ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
@@ -458,7 +484,9 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         CJ_HIP(hipStreamEndCapture(s, &T.graph));
         CJ_HIP(hipGraphInstantiate(&T.exec, T.graph, nullptr, nullptr, 0));
     }
+    CJ_HIP(hipEventRecord(ev[0], s));
     CJ_HIP(hipGraphLaunch(T.exec, s));
+    CJ_HIP(hipEventRecord(ev[1], s));
     // ---- one read-back: the result size of every plan
     std::vector<uint64_t> totals(n);
     {
@@ -536,9 +564,10 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         up2(d_outs, bouts.data(), n * sizeof(BOut));
         up2(d_packs, packs.data(), packs.size() * sizeof(BPack));
         up2(d_ps, pack_start.data(), pack_start.size() * 4);
+        CJ_HIP(hipEventRecord(ev[2], s));
         if (out_rows) batch_join_write(A, d_outs, cfg.grid, s);
         batch_pack(d_packs, d_ps, (uint32_t)packs.size(), pack_start.back(), cfg.grid, s);
-        CJ_HIP(hipEventRecord(e1, s));
+        CJ_HIP(hipEventRecord(ev[3], s));
         CJ_HIP(hipStreamSynchronize(s));
     } catch (...) {
         hipStreamSynchronize(s);
@@ -547,8 +576,10 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
                 if (b.first) pool.free(b.first);
         throw;
     }
-    float dev_ms = 0;
-    hipEventElapsedTime(&dev_ms, e0, e1);
+    float ms1 = 0, ms2 = 0;
+    CJ_HIP(hipEventElapsedTime(&ms1, ev[0], ev[1]));
+    CJ_HIP(hipEventElapsedTime(&ms2, ev[2], ev[3]));
+    const double dev_ms = (double)ms1 + ms2;
     // ---- results (the output blocks now belong to them)
     for (uint32_t k = 0; k < n; ++k) {
         const PlanInfo &pi = info[batched[k]];

@@ -137,6 +137,9 @@ struct BatchTemplate;  // batch.cpp
 struct BatchConfig {
     uint32_t load_factor;  // directory slots per build row (power-of-two rounded)
     uint32_t grid;         // persistent workgroups per batch launch
+    uint32_t log_part;     // radix partitions of 2^log_part directory slots
+    uint32_t chunk;        // probe tuples per join unit (the join tile)
+    uint32_t join_queue;   // 1: join units from a device work queue, 0: strided by workgroup
 };
 
 class Context {

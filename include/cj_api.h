@@ -281,13 +281,16 @@ enum {
 typedef struct cj_batch_stats {
     uint32_t plans_batched;  /* plans run in the shared launches           */
     uint32_t plans_single;   /* plans run one at a time                     */
-    double device_ms;        /* shared launches, first kernel .. last page  */
+    double device_ms;        /* shared launches: phase-1 graph + output phase */
     double total_ms;         /* the whole call                              */
     uint64_t output_rows;    /* rows of the batched plans                   */
     uint32_t load_factor;    /* directory slots per build row (chosen)      */
     uint32_t grid;           /* persistent workgroups per launch (chosen)   */
     int32_t tuned;           /* this call ran the autotuner                 */
     int32_t graph_replayed;  /* the phase-1 graph of an earlier call ran    */
+    uint32_t log_part;       /* slots per radix partition = 2^log_part (chosen) */
+    uint32_t chunk;          /* probe tuples per join unit (chosen)         */
+    uint32_t join_queue;     /* join units from a device work queue (1) or strided (0), chosen */
 } cj_batch_stats;
 int cj_execute_batch(cj_ctx *ctx, const cj_plan_desc *plans, uint32_t n, uint32_t flags, cj_result *out,
                      cj_batch_stats *stats);

@@ -94,11 +94,14 @@ def test_gpu_batch_autotune(engine, c4_small):
     _, plans, alone = c4_small
     sub = plans[4:20]
     res, st = engine.execute_batch(sub, abi.CJ_BATCH_AUTOTUNE)
-    assert st.tuned == 1 and st.load_factor in (1, 2, 4) and st.grid in (1024, 2048, 4096)
+    # the tuner runs every configuration (and fails the call if their result
+    # sizes disagree); the chosen one's results equal the per-plan results
+    assert st.tuned == 1 and st.load_factor in (2, 4) and st.grid in (2048, 4096)
+    assert st.log_part in (11, 12) and st.chunk in (4096, 8192, 16384) and st.join_queue in (0, 1)
     check(res, alone, range(4, 20))
-    lf, grid = st.load_factor, st.grid
+    cfg = (st.load_factor, st.grid, st.log_part, st.chunk, st.join_queue)
     res, st = engine.execute_batch(sub, abi.CJ_BATCH_AUTOTUNE)
-    assert st.tuned == 0 and (st.load_factor, st.grid) == (lf, grid)
+    assert st.tuned == 0 and (st.load_factor, st.grid, st.log_part, st.chunk, st.join_queue) == cfg
     check(res, alone, range(4, 20))
 
 
