@@ -107,6 +107,11 @@ class FilterNode(C.Structure):
                 ("ival", C.c_int64), ("fval", C.c_double), ("sval", C.c_char_p), ("slen", C.c_uint32)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("load_factor", C.c_uint32), ("log_part", C.c_uint32), ("chunk", C.c_uint32),
+                ("join_queue", C.c_uint32), ("grid", C.c_uint32)]
+
+
 class BatchStats(C.Structure):
     _fields_ = [
         ("plans_batched", C.c_uint32), ("plans_single", C.c_uint32), ("device_ms", C.c_double),
@@ -160,7 +165,7 @@ EXPORTS = [
     "cj_table_register", "cj_table_unregister",
     "cj_execute", "cj_result_release", "cj_result_to_host",
     "cj_kernel_stats", "cj_kernel_stats_reset",
-    "cj_serve_start", "cj_submit", "cj_wait", "cj_serve_stop", "cj_plan_footprint", "cj_execute_batch",
+    "cj_serve_start", "cj_submit", "cj_wait", "cj_serve_stop", "cj_plan_footprint", "cj_execute_batch", "cj_batch_config_set",
     "cj_tbl_write", "cj_tbl_open", "cj_tbl_load", "cj_tbl_close", "cj_tbl_cache_name", "cj_filter",
 ]
 
@@ -201,6 +206,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "cj_serve_stats_get": (C.c_int, [C.c_void_p, P(ServeStats)]),
         "cj_plan_footprint": (C.c_int, [C.c_void_p, P(PlanDesc), P(C.c_uint64)]),
         "cj_execute_batch": (C.c_int, [C.c_void_p, P(PlanDesc), C.c_uint32, C.c_uint32, P(Result), P(BatchStats)]),
+        "cj_batch_config_set": (C.c_int, [C.c_void_p, P(BatchConfig)]),
         "cj_tbl_write": (C.c_int, [C.c_char_p, P(TableDesc)]),
         "cj_filter": (C.c_int, [C.c_void_p, P(TableDesc), P(FilterNode), C.c_uint32, C.c_uint32, P(Result),
                                 P(Timing)]),

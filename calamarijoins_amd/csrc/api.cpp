@@ -2,6 +2,7 @@
 // include/cj_api.h).  Exceptions never cross the boundary: they become a
 // negative status plus a per-thread message for cj_last_error().
 #include "engine.hpp"
+#include "batch.hpp"
 #include "serve.hpp"
 #include "tbl.hpp"
 
@@ -237,6 +238,24 @@ int cj_serve_stop(cj_ctx *ctx) {
     return guard([&] {
         if (!ctx) throw cj::InvalidError("null context");
         stop_server(C(ctx));
+    });
+}
+
+int cj_batch_config_set(cj_ctx *ctx, const cj_batch_config *cfg) {
+    return guard([&] {
+        if (!ctx) throw cj::InvalidError("null context");
+        cj::Context *c = C(ctx);
+        std::lock_guard<std::mutex> g(c->mu);
+        if (!cfg) {
+            c->batch_pinned = false;
+            return;
+        }
+        if ((cfg->load_factor != 2 && cfg->load_factor != 4) || cfg->log_part < 8 ||
+            cfg->log_part > cj::kBatchMaxLogPart || cfg->chunk == 0 || cfg->chunk > cj::kBatchMaxChunk ||
+            cfg->join_queue > 1 || cfg->grid > 65536)
+            throw cj::InvalidError("batch configuration out of range");
+        c->batch_pin = cj::BatchConfig{cfg->load_factor, cfg->log_part, cfg->chunk, cfg->join_queue, cfg->grid};
+        c->batch_pinned = true;
     });
 }
 

@@ -53,13 +53,25 @@ __device__ __forceinline__ uint32_t find_owner(const T *start, uint32_t n, T g) 
 __device__ __forceinline__ void carry8(const BSide &sd, uint32_t c, uint32_t valid, uint64_t row0,
                                        const uint64_t pos[kRPT], uint64_t plan_base) {
     if (sd.car_w[c] == 4) {
-        // the decode pass folded NULL into INT32_MIN
-        const CJ_G int32_t *src = glb(reinterpret_cast<const int32_t *>(sd.src[c]));
         CJ_G int32_t *dst = glb(reinterpret_cast<int32_t *>(sd.car[c]));
         int32_t v[kRPT];
+        const uint32_t R = sd.src_rows[c];
+        if (R) {
+            // no NULLs, R rows on every page but the last: read in place
+            const CJ_G uint8_t *pg = glb(sd.src[c]);
 #pragma unroll
-        for (uint32_t j = 0; j < kRPT; ++j)
-            if ((valid >> j) & 1u) v[j] = src[row0 + kWG * j];
+            for (uint32_t j = 0; j < kRPT; ++j)
+                if ((valid >> j) & 1u) {
+                    const uint32_t r = (uint32_t)(row0 + kWG * j), q = r / R;
+                    v[j] = ld32(pg + (size_t)q * kPage + 4 + 4 * (size_t)(r - q * R));
+                }
+        } else {
+            // the decode pass folded NULL into INT32_MIN
+            const CJ_G int32_t *src = glb(reinterpret_cast<const int32_t *>(sd.src[c]));
+#pragma unroll
+            for (uint32_t j = 0; j < kRPT; ++j)
+                if ((valid >> j) & 1u) v[j] = src[row0 + kWG * j];
+        }
 #pragma unroll
         for (uint32_t j = 0; j < kRPT; ++j)
             if ((valid >> j) & 1u) dst[pos[j] - plan_base] = v[j];
@@ -292,80 +304,46 @@ __device__ __forceinline__ int32_t bp_key(const BuildPart &bp, uint32_t t) {
     else return bp.g_keys[t];
 }
 
-constexpr uint32_t kMatchBuf = kBatchMatchBuf;
-// A match in the LDS window: (probe row in the tile) << 22 | build tuple of the partition.
-constexpr uint32_t kMatchTBits = 22;
-constexpr uint32_t kMatchT = (1u << kMatchTBits) - 1;
-static_assert(kBatchMaxBuildRows <= (1ull << kMatchTBits), "build tuple index must fit the match entry");
-
-// Writes output rows base .. base + cnt - 1 from the expanded matches
-// (mbuf: probe row in the tile and build tuple), one column at a time so
-// every store is coalesced.
+// One output row: the join key, build tuple t of the partition (plan-local
+// index g_car0 + t of the slot-ordered carried columns), probe tuple pt
+// (plan-local index of the partition-ordered carried columns).  Lanes of a
+// wave write ascending rows, so the stores of a column stay nearly
+// coalesced.
 template <bool LDS>
-__device__ __forceinline__ void bx_emit(const BPlan &pl, const BOut *bo, const BuildPart &bp, uint64_t base,
-                                        uint32_t cnt, const uint32_t *mbuf, uint64_t pt0) {
-    constexpr uint32_t Q = kMatchBuf / kWG;  // matches per thread (all loads, then all stores)
-    const uint32_t nout = bo->nout;
-    for (uint32_t c = 0; c < nout; ++c) {
-        const uint32_t src = bo->src[c], cj = bo->j[c], w = bo->w[c];
-        uint8_t *dst = bo->dense[c];
-        if (w == 4) {
-            const CJ_G int32_t *gb = glb(reinterpret_cast<const int32_t *>(pl.bslot_car[cj])) + bp.g_car0;
-            const int32_t *lb = reinterpret_cast<const int32_t *>(bp.l_dyn + bp.coff(cj));
-            const CJ_G int32_t *gp = glb(reinterpret_cast<const int32_t *>(pl.p.car[cj])) + pt0;
-            CJ_G int32_t *out = glb(reinterpret_cast<int32_t *>(dst));
-            int32_t v[Q];
+__device__ __forceinline__ void bx_row(const BPlan &pl, const BOut &bo, const BuildPart &bp, uint64_t o, int32_t key,
+                                       uint32_t t, uint64_t pt) {
 #pragma unroll
-            for (uint32_t q = 0; q < Q; ++q) {
-                const uint32_t i = q * kWG + threadIdx.x;
-                if (i >= cnt) continue;
-                if (src == 0) {
-                    v[q] = bp_key<LDS>(bp, mbuf[i] & kMatchT);
-                } else if (src == 1) {
-                    if constexpr (LDS) v[q] = lb[mbuf[i] & kMatchT];
-                    else v[q] = gb[mbuf[i] & kMatchT];
-                } else {
-                    v[q] = gp[mbuf[i] >> kMatchTBits];
-                }
+    for (uint32_t c = 0; c < kBatchMaxOut; ++c) {
+        if (c >= bo.nout) break;
+        const uint32_t src = bo.src[c], cj = bo.j[c];
+        if (bo.w[c] == 4) {
+            int32_t v;
+            if (src == 0) {
+                v = key;  // equal keys: the build key is the probe key
+            } else if (src == 1) {
+                if constexpr (LDS) v = reinterpret_cast<const int32_t *>(bp.l_dyn + bp.coff(cj))[t];
+                else v = glb(reinterpret_cast<const int32_t *>(pl.bslot_car[cj]))[bp.g_car0 + t];
+            } else {
+                v = glb(reinterpret_cast<const int32_t *>(pl.p.car[cj]))[pt];
             }
-#pragma unroll
-            for (uint32_t q = 0; q < Q; ++q) {
-                const uint32_t i = q * kWG + threadIdx.x;
-                if (i < cnt) out[base + i] = v[q];
-            }
+            glb(reinterpret_cast<int32_t *>(bo.dense[c]))[o] = v;
         } else {
-            const CJ_G uint64_t *gb = glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[cj])) + bp.g_car0;
-            const CJ_G uint8_t *gbv = glb(pl.bslot_vld[cj]) + bp.g_car0;
-            const uint64_t *lb = reinterpret_cast<const uint64_t *>(bp.l_dyn + bp.coff(cj));
-            const uint8_t *lbv = bp.l_dyn + bp.voff(cj);
-            const CJ_G uint64_t *gp = glb(reinterpret_cast<const uint64_t *>(pl.p.car[cj])) + pt0;
-            const CJ_G uint8_t *gpv = glb(pl.p.car_vld[cj]) + pt0;
-            CJ_G uint8_t *dv = glb(bo->dvld[c]);
-            CJ_G uint64_t *out = glb(reinterpret_cast<uint64_t *>(dst));
-            uint64_t v[Q];
-            uint32_t vb = 0;
-#pragma unroll
-            for (uint32_t q = 0; q < Q; ++q) {
-                const uint32_t i = q * kWG + threadIdx.x;
-                if (i >= cnt) continue;
-                uint8_t b;
-                if (src == 1) {
-                    const uint32_t t = mbuf[i] & kMatchT;
-                    if constexpr (LDS) v[q] = lb[t], b = lbv[t];
-                    else v[q] = gb[t], b = gbv[t];
+            uint64_t v;
+            uint8_t b;
+            if (src == 1) {
+                if constexpr (LDS) {
+                    v = reinterpret_cast<const uint64_t *>(bp.l_dyn + bp.coff(cj))[t];
+                    b = (bp.l_dyn + bp.voff(cj))[t];
                 } else {
-                    const uint32_t r = mbuf[i] >> kMatchTBits;
-                    v[q] = gp[r], b = gpv[r];
+                    v = glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[cj]))[bp.g_car0 + t];
+                    b = glb(pl.bslot_vld[cj])[bp.g_car0 + t];
                 }
-                vb |= (uint32_t)(b != 0) << q;
+            } else {
+                v = glb(reinterpret_cast<const uint64_t *>(pl.p.car[cj]))[pt];
+                b = glb(pl.p.car_vld[cj])[pt];
             }
-#pragma unroll
-            for (uint32_t q = 0; q < Q; ++q) {
-                const uint32_t i = q * kWG + threadIdx.x;
-                if (i >= cnt) continue;
-                out[base + i] = v[q];
-                dv[base + i] = (uint8_t)((vb >> q) & 1u);
-            }
+            glb(reinterpret_cast<uint64_t *>(bo.dense[c]))[o] = v;
+            glb(bo.dvld[c])[o] = (uint8_t)(b != 0);
         }
     }
 }
@@ -375,8 +353,7 @@ __device__ __forceinline__ void bx_emit(const BPlan &pl, const BOut *bo, const B
 template <bool WRITE, bool LDS, uint32_t RPT>
 __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &pl, const BOut *bo,
                                              const BuildPart &bp, uint64_t p0, uint64_t p1, uint64_t pbase,
-                                             uint64_t obase, uint32_t *lds32, uint32_t *mbuf) {
-    static_assert(!WRITE || kWG * RPT <= (1u << (32 - kMatchTBits)), "tile row must fit the match entry");
+                                             uint64_t obase, uint32_t *lds32) {
     uint64_t mine = 0, base = obase;
     const uint32_t ls = pl.log_slots;
     for (uint64_t i0 = p0; i0 < p1; i0 += kWG * RPT) {
@@ -434,29 +411,22 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
                     run += lds32[4 * j + ww];
                 }
             __syncthreads();
-            // expand the tile's matches into LDS a window at a time, then emit them
-            for (uint32_t w0 = 0; w0 < run; w0 += kMatchBuf) {
-                const uint32_t w1 = run - w0 < kMatchBuf ? run : w0 + kMatchBuf;
+            // every lane writes its own tuples' matches (the first two were
+            // recorded by the probe; more come from a second walk of the slot)
 #pragma unroll
-                for (uint32_t j = 0; j < RPT; ++j) {
-                    if (!m[j] || pre[j] >= w1 || pre[j] + m[j] <= w0) continue;
-                    const uint32_t row = (kWG * j + threadIdx.x) << kMatchTBits;
-                    if (m[j] <= 2 && pre[j] >= w0 && pre[j] + m[j] <= w1) {
-                        // the usual case: the recorded matches, all inside the window
-                        mbuf[pre[j] - w0] = row | f0[j];
-                        if (m[j] == 2) mbuf[pre[j] + 1 - w0] = row | f1[j];
-                        continue;
-                    }
-                    uint32_t q = pre[j];
-                    for (uint32_t t = s[j]; t < e[j] && q < w1; ++t) {
-                        if (bp_key<LDS>(bp, t) != key[j]) continue;
-                        if (q >= w0) mbuf[q - w0] = row | t;
-                        ++q;
+            for (uint32_t j = 0; j < RPT; ++j) {
+                const uint64_t o = base + pre[j];
+                const uint64_t pt = i0 + kWG * j + threadIdx.x - pbase;
+                uint32_t t = f0[j];
+                for (uint32_t r = 0; r < m[j];) {
+                    bx_row<LDS>(pl, *bo, bp, o + r, key[j], t, pt);
+                    if (++r == 1) {
+                        t = f1[j];
+                    } else if (r < m[j]) {
+                        do ++t;
+                        while (bp_key<LDS>(bp, t) != key[j]);
                     }
                 }
-                __syncthreads();
-                bx_emit<LDS>(pl, bo, bp, base + w0, w1 - w0, mbuf, i0 - pbase);
-                __syncthreads();
             }
             base += run;
         }
@@ -491,11 +461,11 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
                                                  const uint64_t *__restrict__ boff, const uint64_t *__restrict__ poff,
                                                  const uint64_t *__restrict__ moff, const int32_t *__restrict__ pkeys,
                                                  const int32_t *__restrict__ bkeys, const uint32_t *__restrict__ dir) {
-    extern __shared__ __align__(16) uint8_t dyn[];  // keys, carried build values
-    __shared__ uint16_t ends[kMaxSlots];
+    // slot ends (u16, 2^log_part of them), then the keys and carried build values
+    extern __shared__ __align__(16) uint8_t dyn[];
+    uint16_t *ends = reinterpret_cast<uint16_t *>(dyn);
     __shared__ uint32_t lds32[32];
     __shared__ uint64_t red[4];
-    __shared__ uint32_t mbuf[WRITE ? kMatchBuf : 1];
     __shared__ uint32_t unit;
     const uint64_t total = uoff[a.nparts];  // < 2^32 (max_units)
     const uint32_t S = 1u << a.log_part;
@@ -513,7 +483,7 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
         const uint64_t p1 = p0 + a.chunk < pe ? p0 + a.chunk : pe;
         BuildPart bp;
         bp.l_ends = ends;
-        bp.l_dyn = dyn;
+        bp.l_dyn = dyn + (2u << a.log_part);
         bp.g_ends = dir + pl.dir_base + ((uint64_t)(g - part_start[k]) << a.log_part);
         bp.smask = S - 1;
         bp.g_keys = bkeys + t_a;
@@ -524,7 +494,11 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
         bp.w8 = 0;
         for (uint32_t c = 0; c < ncar; ++c) bp.w8 |= (pl.b.car_w[c] == 8 ? 1u : 0u) << c;
         uint64_t obase = 0;
-        if (WRITE) obase = moff[u] - moff[uoff[part_start[k]]];
+        BOut ob;  // the plan's output columns (uniform: scalar registers)
+        if (WRITE) {
+            obase = moff[u] - moff[uoff[part_start[k]]];
+            ob = outs[k];
+        }
         uint64_t mine;
         if (nb <= a.lds_cap) {
             // stage the partition: each batch of loads is issued before its LDS stores
@@ -537,22 +511,23 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
                 for (uint32_t q = 0; q < kMaxSlots / kWG; ++q)
                     if (q * kWG + threadIdx.x < S) ends[q * kWG + threadIdx.x] = (uint16_t)ev[q];
             }
-            stage_lds<int32_t>(reinterpret_cast<int32_t *>(dyn), bp.g_keys, nb);
+            uint8_t *tup = dyn + (2u << a.log_part);  // == bp.l_dyn
+            stage_lds<int32_t>(reinterpret_cast<int32_t *>(tup), bp.g_keys, nb);
             if (WRITE)
                 for (uint32_t c = 0; c < ncar; ++c) {
                     if (pl.b.car_w[c] == 4) {
-                        stage_lds<int32_t>(reinterpret_cast<int32_t *>(dyn + bp.coff(c)),
+                        stage_lds<int32_t>(reinterpret_cast<int32_t *>(tup + bp.coff(c)),
                                            glb(reinterpret_cast<const int32_t *>(pl.bslot_car[c])) + bp.g_car0, nb);
                     } else {
-                        stage_lds<uint64_t>(reinterpret_cast<uint64_t *>(dyn + bp.coff(c)),
+                        stage_lds<uint64_t>(reinterpret_cast<uint64_t *>(tup + bp.coff(c)),
                                             glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[c])) + bp.g_car0, nb);
-                        stage_lds<uint8_t>(dyn + bp.voff(c), glb(pl.bslot_vld[c]) + bp.g_car0, nb);
+                        stage_lds<uint8_t>(tup + bp.voff(c), glb(pl.bslot_vld[c]) + bp.g_car0, nb);
                     }
                 }
             __syncthreads();
-            mine = bx_probe<WRITE, true, WRITE ? 4u : kRPT>(pkeys, pl, outs + k, bp, p0, p1, pbase, obase, lds32, mbuf);
+            mine = bx_probe<WRITE, true, WRITE ? 4u : kRPT>(pkeys, pl, &ob, bp, p0, p1, pbase, obase, lds32);
         } else {
-            mine = bx_probe<WRITE, false, WRITE ? 4u : kRPT>(pkeys, pl, outs + k, bp, p0, p1, pbase, obase, lds32, mbuf);
+            mine = bx_probe<WRITE, false, WRITE ? 4u : kRPT>(pkeys, pl, &ob, bp, p0, p1, pbase, obase, lds32);
         }
         if (!WRITE) {
             uint64_t tot;
@@ -723,9 +698,38 @@ __global__ void k_copy_u64(const uint64_t *src, uint64_t *dst, uint64_t n) {
 // -------------------------------------------------------------- launchers
 static unsigned pgrid(uint64_t units, uint32_t grid) { return (unsigned)(units < grid ? units : grid); }
 
+// dynamic LDS of each kernel
+static size_t part_count_lds(const BatchArgs &a) { return a.max_parts * sizeof(uint32_t); }
+static size_t part_scatter_lds(const BatchArgs &a) { return a.max_parts * (sizeof(uint64_t) + sizeof(uint32_t)); }
+static size_t join_count_lds(const BatchArgs &a) { return (2u << a.log_part) + (size_t)a.lds_cap * 4; }  // slot ends + keys
+static size_t join_write_lds(const BatchArgs &a) {
+    return (2u << a.log_part) + (size_t)a.lds_cap * a.tuple_bytes;  // slot ends + tuples
+}
+
+template <class K>
+static uint32_t resident(K kernel, size_t dyn, int cus) {
+    int per = 0;
+    CJ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kWG, dyn));
+    return (uint32_t)(per > 0 ? per : 1) * (uint32_t)cus;
+}
+
+BatchGrids batch_resident_grids(const BatchArgs &a, int device) {
+    int cus = 0;
+    CJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    BatchGrids g;
+    g.decode = resident(k_bx_decode, 0, cus);
+    g.part_count = resident(k_bx_part_count<true>, part_count_lds(a), cus);
+    g.part_scatter = resident(k_bx_part_scatter<true>, part_scatter_lds(a), cus);
+    g.build = resident(k_bx_build, 0, cus);
+    g.join_count = resident(k_bx_join<false>, join_count_lds(a), cus);
+    g.join_write = resident(k_bx_join<true>, join_write_lds(a), cus);
+    g.pack = resident(k_bx_pack, 0, cus);
+    return g;
+}
+
 void batch_part_count(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s) {
     if (!units) return;
-    const size_t sh = a.max_parts * sizeof(uint32_t);
+    const size_t sh = part_count_lds(a);
     if (probe) k_bx_part_count<true><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.ppage_start, a.part_start);
     else k_bx_part_count<false><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.bpage_start, a.part_start);
     CJ_HIP(hipGetLastError());
@@ -733,7 +737,7 @@ void batch_part_count(const BatchArgs &a, bool probe, uint32_t units, uint32_t g
 
 void batch_part_scatter(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s) {
     if (!units) return;
-    const size_t sh = a.max_parts * (sizeof(uint64_t) + sizeof(uint32_t));
+    const size_t sh = part_scatter_lds(a);
     if (probe) k_bx_part_scatter<true><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.ppage_start, a.part_start,
                                                                      a.poff);
     else k_bx_part_scatter<false><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.bpage_start, a.part_start,
@@ -755,14 +759,16 @@ void batch_units(const BatchArgs &a, hipStream_t s) {
 
 void batch_join_count(const BatchArgs &a, uint32_t grid, hipStream_t s) {
     if (!a.max_units) return;
-    k_bx_join<false><<<pgrid(a.max_units, grid), kWG, a.lds_bytes, s>>>(a, nullptr, a.plans, a.part_start, a.uoff, a.boff,
+    const size_t sh = join_count_lds(a);
+    k_bx_join<false><<<pgrid(a.max_units, grid), kWG, sh, s>>>(a, nullptr, a.plans, a.part_start, a.uoff, a.boff,
                                                                         a.poff, a.moff, a.pkeys, a.bkeys, a.dir);
     CJ_HIP(hipGetLastError());
 }
 
 void batch_join_write(const BatchArgs &a, const BOut *outs, uint32_t grid, hipStream_t s) {
     if (!a.max_units) return;
-    k_bx_join<true><<<pgrid(a.max_units, grid), kWG, a.lds_bytes, s>>>(a, outs, a.plans, a.part_start, a.uoff, a.boff,
+    const size_t sh = join_write_lds(a);
+    k_bx_join<true><<<pgrid(a.max_units, grid), kWG, sh, s>>>(a, outs, a.plans, a.part_start, a.uoff, a.boff,
                                                                        a.poff, a.moff, a.pkeys, a.bkeys, a.dir);
     CJ_HIP(hipGetLastError());
 }

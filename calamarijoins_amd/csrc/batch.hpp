@@ -37,7 +37,6 @@ constexpr uint32_t kBatchMaxLogPart = 12;             // slots per partition: 2^
 constexpr uint64_t kBatchMaxBuildRows = 1ull << 22;   // larger builds run one plan at a time
 constexpr uint32_t kBatchMaxChunk = 16384;            // probe tuples per join unit: 4096 .. 16384 (tuned)
 constexpr uint32_t kBatchLdsBytes = 64 * 1024;        // LDS of a join workgroup
-constexpr uint32_t kBatchMatchBuf = 2048;             // matches a join workgroup expands at a time
 
 // One side of one plan.  Carried column j holds, in partition order, the
 // value of the source column for each tuple: INT32 with NULL folded to
@@ -48,8 +47,10 @@ struct BSide {
     const uint64_t *k_rs;    // their row starts
     uint32_t ncar;
     uint32_t car_w[kBatchMaxCarry];             // 4 or 8
-    const uint8_t *src[kBatchMaxCarry];         // dense decoded source column (by row)
+    const uint8_t *src[kBatchMaxCarry];         // dense decoded source column (by row), or its pages
     const uint32_t *src_valid[kBatchMaxCarry];  // its validity bitmap (by row)
+    uint32_t src_rows[kBatchMaxCarry];          // > 0: INT32 without NULLs read in place from its pages,
+                                                // row r = value r % src_rows of page r / src_rows
     uint8_t *car[kBatchMaxCarry];               // carried values, plan-local tuple index
     uint8_t *car_vld[kBatchMaxCarry];           // validity bytes (8-byte columns only)
 };
@@ -88,7 +89,7 @@ struct BatchArgs {
     uint64_t *mcount;             // [max_units + 1] matches per unit
     const uint64_t *moff;         // exclusive scan
     uint32_t lds_cap;             // build tuples a join workgroup holds in LDS
-    uint32_t lds_bytes;
+    uint32_t tuple_bytes;         // LDS bytes of a staged build tuple (key + carried values)
     uint32_t log_part;            // slots per partition = 2^log_part
     uint32_t chunk;               // probe tuples per join unit
     uint32_t join_queue;          // 1: join units from the work queue, 0: strided by workgroup
@@ -122,6 +123,16 @@ struct BPack {
     int32_t width;
     uint8_t *out;
 };
+
+// Workgroups of each persistent kernel that the device holds at once
+// (occupancy at the kernel's registers and LDS x compute units): a launch
+// of that many takes its units in one round.  A larger grid runs in rounds
+// and, with units split evenly by stride, leaves the machine part-idle in
+// the last one.
+struct BatchGrids {
+    uint32_t decode, part_count, part_scatter, build, join_count, join_write, pack;
+};
+BatchGrids batch_resident_grids(const BatchArgs &a, int device);
 
 // Each launcher takes the persistent grid size (workgroups).
 void batch_part_count(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s);

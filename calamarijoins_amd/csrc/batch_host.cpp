@@ -127,6 +127,7 @@ struct BatchTemplate {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t njobs = 0, ndpages = 0;
+    BatchGrids grids{};
     void release(DevicePool &pool) {
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
@@ -170,24 +171,27 @@ void Context::execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flag
     // configuration: the tuned one for this template class, else the default
     const uint64_t cls = ((uint64_t)ceil_log2(nb_plans + 1) << 16) | ((uint64_t)ceil_log2(sum_probe + 1) << 8) |
                          ceil_log2(sum_build + 1);
-    BatchConfig cfg{2, 2048, 12, 8192, 0};
+    BatchConfig cfg{2, 12, 8192, 0, 0};
     bool tuned = false;
     {
         std::lock_guard<std::mutex> g(mu);
         auto it = batch_tuning.find(cls);
-        if (it != batch_tuning.end()) {
+        if (batch_pinned) {
+            cfg = batch_pin;  // cj_batch_config_set
+            tuned = true;
+        } else if (it != batch_tuning.end()) {
             cfg = it->second;
             tuned = true;
         }
     }
     if (!tuned && (flags & CJ_BATCH_AUTOTUNE)) {
-        // load factor x grid x radix bits (slots per partition) x join tile x join scheduling
+        // load factor x radix bits (slots per partition) x join tile x join
+        // scheduling; every launch takes the kernel's resident grid
         std::vector<BatchConfig> cands;
         for (uint32_t lf : {2u, 4u})
-            for (uint32_t grid : {2048u, 4096u})
-                for (uint32_t lp : {11u, 12u})
-                    for (uint32_t chunk : {4096u, 8192u, 16384u})
-                        for (uint32_t jq : {0u, 1u}) cands.push_back({lf, grid, lp, chunk, jq});
+            for (uint32_t lp : {11u, 12u})
+                for (uint32_t chunk : {4096u, 8192u, 16384u})
+                    for (uint32_t jq : {0u, 1u}) cands.push_back({lf, lp, chunk, jq, 0});
         double best = 1e30;
         std::vector<uint64_t> rows0;  // every configuration must give the same per-plan row counts
         for (const BatchConfig &c : cands) {
@@ -213,7 +217,6 @@ void Context::execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flag
     }
     bs.device_ms = run_batch(*this, batched, info, cfg, flags, out, &bs);
     bs.load_factor = cfg.load_factor;
-    bs.grid = cfg.grid;
     bs.log_part = cfg.log_part;
     bs.chunk = cfg.chunk;
     bs.join_queue = cfg.join_queue;
@@ -234,7 +237,7 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
     CJ_HIP(hipSetDevice(ctx.device));
     // the kernels' LDS arrays hold 2^kBatchMaxLogPart slots, scanned kWG at a time
     if (cfg.log_part < 8 || cfg.log_part > kBatchMaxLogPart || cfg.chunk == 0 || cfg.chunk > kBatchMaxChunk ||
-        cfg.grid == 0)
+        cfg.grid > 65536)
         throw DeviceError("batch configuration out of range");
     // ---- shape of the batch
     std::vector<uint32_t> starts(3 * (n + 1), 0);
@@ -242,8 +245,23 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
     std::vector<uint32_t> log_slots(n);
     uint64_t dir_total = 0, sum_nb = 0, sum_np = 0;
     uint32_t max_parts = 1, max_tuple_bytes = 4;
-    std::vector<DevColumn *> dec_cols;
+    std::vector<DevColumn *> dec_cols;  // carried columns decoded into dense arrays first
     std::map<DevColumn *, uint32_t> dec_index;
+    // an INT32 column without NULLs whose pages hold the same row count is
+    // read in place by the partition pass (no decode)
+    auto in_place = [](const DevColumn *c) {
+        return c->type == CJ_INT32 && c->dense_rows > 0 && c->num_rows < (1ull << 32);
+    };
+    // the source of carried column c of a side: its pages, or its decoded arrays
+    auto set_src = [&](BSide &sd, size_t c, DevColumn *col, const BatchTemplate &t) {
+        if (in_place(col)) {
+            sd.src[c] = col->pages, sd.src_valid[c] = nullptr, sd.src_rows[c] = col->dense_rows;
+        } else {
+            const uint32_t j = dec_index.at(col);
+            sd.src[c] = static_cast<const uint8_t *>(t.dec_values[j]), sd.src_valid[c] = t.dec_valid[j];
+            sd.src_rows[c] = 0;
+        }
+    };
     uint64_t key = 1469598103934665603ull;
     auto mixk = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
     for (uint32_t k = 0; k < n; ++k) {
@@ -263,7 +281,9 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         max_tuple_bytes = std::max(max_tuple_bytes, tb);
         for (const auto *side : {&pi.bcar, &pi.pcar})
             for (DevColumn *c : *side)
-                if (!dec_index.count(c)) {
+                if (in_place(c)) {
+                    mixk(c->dense_rows), mixk(c->num_pages);
+                } else if (!dec_index.count(c)) {
                     dec_index[c] = (uint32_t)dec_cols.size();
                     dec_cols.push_back(c);
                 }
@@ -276,9 +296,20 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
     const uint64_t max_units64 = nparts + sum_np / cfg.chunk + 1;
     if (max_units64 > 0xffffffffull) throw UnsupportedError("batch too large");
     const uint32_t max_units = (uint32_t)max_units64;
-    // LDS of a join workgroup: slot ends (u16), the match window (u32), then the partition
-    const uint32_t dyn_budget = kBatchLdsBytes - (2u << kBatchMaxLogPart) - 4 * kBatchMatchBuf - 512;
-    const uint32_t lds_cap = std::min<uint32_t>((dyn_budget / max_tuple_bytes) & ~15u, 65535u & ~15u);
+    // A join workgroup stages a build partition of at most lds_cap tuples in
+    // LDS (slot ends as u16, keys, carried values; a larger partition is
+    // probed in place): twice the largest mean partition of the batch, so a
+    // workgroup's LDS stays small and several workgroups share a CU (the join
+    // passes are latency-bound).
+    uint64_t max_mean = 1;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint64_t parts = 1ull << (log_slots[k] - cfg.log_part);
+        max_mean = std::max<uint64_t>(max_mean, (info[batched[k]].bk->num_rows + parts - 1) / parts);
+    }
+    const uint32_t dyn_budget = kBatchLdsBytes - (2u << cfg.log_part) - 512;
+    const uint32_t cap_budget = std::min<uint32_t>((dyn_budget / max_tuple_bytes) & ~15u, 65535u & ~15u);
+    const uint32_t lds_cap =
+        (uint32_t)std::min<uint64_t>(cap_budget, (std::max<uint64_t>(256, 2 * max_mean) + 15) & ~15ull);
     std::vector<BDecode> jobs(dec_cols.size());
     std::vector<uint32_t> job_start(dec_cols.size() + 1, 0);
     uint64_t valid_words = 0, val_bytes = 0;
@@ -290,7 +321,7 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         // equal row counts with another NULL density give other page counts
         mixk(dec_cols[j]->num_rows), mixk(dec_cols[j]->type), mixk(dec_cols[j]->num_pages);
     }
-    mixk(n), mixk(cfg.grid), mixk(cfg.load_factor), mixk(cfg.log_part), mixk(cfg.chunk), mixk(cfg.join_queue), mixk(dec_cols.size());
+    mixk(n), mixk(cfg.load_factor), mixk(cfg.log_part), mixk(cfg.chunk), mixk(cfg.join_queue), mixk(cfg.grid), mixk(dec_cols.size());
     // ---- the template workspace (reused while the batch shape repeats)
     std::shared_ptr<BatchTemplate> tpl;
     {
@@ -359,10 +390,8 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
                 const uint64_t rows = kc->num_rows + 1;
                 for (size_t c = 0; c < car.size(); ++c) {
                     const uint32_t w = width_of(car[c]->type);
-                    const uint32_t j = dec_index.at(car[c]);
                     sd.car_w[c] = w;
-                    sd.src[c] = static_cast<const uint8_t *>(tpl->dec_values[j]);
-                    sd.src_valid[c] = tpl->dec_valid[j];
+                    set_src(sd, c, car[c], *tpl);
                     sd.car[c] = blk(rows * w);
                     if (w == 8) sd.car_vld[c] = blk(rows);
                     if (build) {
@@ -383,13 +412,15 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         a.nparts = nparts;
         a.max_units = max_units;
         a.lds_cap = lds_cap;
-        a.lds_bytes = lds_cap * max_tuple_bytes;
+        a.tuple_bytes = max_tuple_bytes;
         a.log_part = cfg.log_part;
         a.chunk = cfg.chunk;
         a.join_queue = cfg.join_queue;
         a.queue = reinterpret_cast<uint32_t *>(blk(kBatchQueues * 4));
         tpl->njobs = (uint32_t)jobs.size();
         tpl->ndpages = job_start.back();
+        tpl->grids = batch_resident_grids(a, ctx.device);
+        if (cfg.grid) tpl->grids = BatchGrids{cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid};
     }
     // a template whose captured decode units disagree with this batch's page
     // counts would decode past (or short of) the pages: never replay it
@@ -404,14 +435,8 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         BPlan &b = tpl->hplans[k];
         b.b.k_pages = pi.bk->pages, b.b.k_rs = pi.bk->row_start;
         b.p.k_pages = pi.pk->pages, b.p.k_rs = pi.pk->row_start;
-        for (size_t c = 0; c < pi.bcar.size(); ++c) {
-            const uint32_t j = dec_index.at(pi.bcar[c]);
-            b.b.src[c] = static_cast<const uint8_t *>(tpl->dec_values[j]), b.b.src_valid[c] = tpl->dec_valid[j];
-        }
-        for (size_t c = 0; c < pi.pcar.size(); ++c) {
-            const uint32_t j = dec_index.at(pi.pcar[c]);
-            b.p.src[c] = static_cast<const uint8_t *>(tpl->dec_values[j]), b.p.src_valid[c] = tpl->dec_valid[j];
-        }
+        for (size_t c = 0; c < pi.bcar.size(); ++c) set_src(b.b, c, pi.bcar[c], *tpl);
+        for (size_t c = 0; c < pi.pcar.size(); ++c) set_src(b.p, c, pi.pcar[c], *tpl);
     }
     // ---- upload the descriptors (pinned staging; the copies finish before the graph runs)
     const size_t up_bytes =
@@ -438,25 +463,26 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
     // ---- phase 1: decode, partition both sides, build, units, match counts (captured once)
     BatchTemplate &T = *tpl;
     BatchArgs &A = T.args;
+    const BatchGrids &G = T.grids;
     auto phase1 = [&](hipStream_t ps) {
         fill_u64(A.bcount, 0, nparts + 1, ps);
         fill_u64(A.pcount, 0, nparts + 1, ps);
         fill_u64(A.mcount, 0, (uint64_t)max_units + 1, ps);
         fill_u32(T.valid_all, 0, T.valid_words, ps);
         fill_u32(A.queue, 0, kBatchQueues, ps);  // the join work queues (a fill kernel, not a memset)
-        batch_decode(T.d_jobs, T.d_job_start, T.njobs, T.ndpages, cfg.grid, ps);
-        batch_part_count(A, false, nbp, cfg.grid, ps);
+        batch_decode(T.d_jobs, T.d_job_start, T.njobs, T.ndpages, G.decode, ps);
+        batch_part_count(A, false, nbp, G.part_count, ps);
         scan_exclusive_u64(A.bcount, const_cast<uint64_t *>(A.boff), nparts, ps, T.scan_tmp, T.scan_elems);
         batch_copy_u64(A.boff, A.bcur, nparts + 1, ps);
-        batch_part_scatter(A, false, nbp, cfg.grid, ps);
-        batch_part_count(A, true, npp, cfg.grid, ps);
+        batch_part_scatter(A, false, nbp, G.part_scatter, ps);
+        batch_part_count(A, true, npp, G.part_count, ps);
         scan_exclusive_u64(A.pcount, const_cast<uint64_t *>(A.poff), nparts, ps, T.scan_tmp, T.scan_elems);
         batch_copy_u64(A.poff, A.pcur, nparts + 1, ps);
-        batch_part_scatter(A, true, npp, cfg.grid, ps);
-        batch_build(A, cfg.grid, ps);
+        batch_part_scatter(A, true, npp, G.part_scatter, ps);
+        batch_build(A, G.build, ps);
         batch_units(A, ps);
         scan_exclusive_u64(A.ucount, const_cast<uint64_t *>(A.uoff), nparts, ps, T.scan_tmp, T.scan_elems);
-        batch_join_count(A, cfg.grid, ps);
+        batch_join_count(A, G.join_count, ps);
         scan_exclusive_u64(A.mcount, const_cast<uint64_t *>(A.moff), max_units, ps, T.scan_tmp, T.scan_elems);
         batch_totals(A, T.totals, ps);
         CJ_HIP(hipGetLastError());
@@ -565,8 +591,8 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         up2(d_packs, packs.data(), packs.size() * sizeof(BPack));
         up2(d_ps, pack_start.data(), pack_start.size() * 4);
         CJ_HIP(hipEventRecord(ev[2], s));
-        if (out_rows) batch_join_write(A, d_outs, cfg.grid, s);
-        batch_pack(d_packs, d_ps, (uint32_t)packs.size(), pack_start.back(), cfg.grid, s);
+        if (out_rows) batch_join_write(A, d_outs, G.join_write, s);
+        batch_pack(d_packs, d_ps, (uint32_t)packs.size(), pack_start.back(), G.pack, s);
         CJ_HIP(hipEventRecord(ev[3], s));
         CJ_HIP(hipStreamSynchronize(s));
     } catch (...) {
@@ -607,6 +633,7 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         }
     }
     if (bs) {
+        bs->grid = T.grids.join_write;
         bs->output_rows = out_rows;
         bs->graph_replayed = fresh ? 0 : 1;
     }

@@ -136,10 +136,10 @@ struct ExecStream {
 struct BatchTemplate;  // batch.cpp
 struct BatchConfig {
     uint32_t load_factor;  // directory slots per build row (power-of-two rounded)
-    uint32_t grid;         // persistent workgroups per batch launch
     uint32_t log_part;     // radix partitions of 2^log_part directory slots
     uint32_t chunk;        // probe tuples per join unit (the join tile)
     uint32_t join_queue;   // 1: join units from a device work queue, 0: strided by workgroup
+    uint32_t grid;         // workgroups of every persistent launch; 0 = each kernel's resident grid
 };
 
 class Context {
@@ -174,6 +174,8 @@ class Context {
     void execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flags, cj_result *out, cj_batch_stats *stats);
     void release_batch_templates();
     std::map<uint64_t, BatchConfig> batch_tuning;                       // template class -> tuned config
+    bool batch_pinned = false;                                          // cj_batch_config_set
+    BatchConfig batch_pin{};
     std::map<uint64_t, std::shared_ptr<BatchTemplate>> batch_templates;  // workspaces + phase-1 graphs
     void release(cj_result *res);
     void to_host(cj_result *res);

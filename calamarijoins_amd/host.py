@@ -248,6 +248,15 @@ class Engine:
         abi.check(self.lib.cj_serve_stop(self.ctx))
         self._inflight = {}
 
+    def batch_config(self, load_factor=None, log_part=12, chunk=8192, join_queue=0, grid=0):
+        """Pin the batch configuration (cj_batch_config_set); with no
+        load_factor, unpin it (autotuner / default again)."""
+        if load_factor is None:
+            abi.check(self.lib.cj_batch_config_set(self.ctx, None))
+            return
+        cfg = abi.BatchConfig(load_factor, log_part, chunk, join_queue, grid)
+        abi.check(self.lib.cj_batch_config_set(self.ctx, C.byref(cfg)))
+
     def execute_batch(self, plans, flags: int = 0):
         """Run many plans through cj_execute_batch; returns ([EngineResult],
         abi.BatchStats)."""

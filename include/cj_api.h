@@ -285,7 +285,7 @@ typedef struct cj_batch_stats {
     double total_ms;         /* the whole call                              */
     uint64_t output_rows;    /* rows of the batched plans                   */
     uint32_t load_factor;    /* directory slots per build row (chosen)      */
-    uint32_t grid;           /* persistent workgroups per launch (chosen)   */
+    uint32_t grid;           /* workgroups of the join write launch (resident) */
     int32_t tuned;           /* this call ran the autotuner                 */
     int32_t graph_replayed;  /* the phase-1 graph of an earlier call ran    */
     uint32_t log_part;       /* slots per radix partition = 2^log_part (chosen) */
@@ -294,6 +294,19 @@ typedef struct cj_batch_stats {
 } cj_batch_stats;
 int cj_execute_batch(cj_ctx *ctx, const cj_plan_desc *plans, uint32_t n, uint32_t flags, cj_result *out,
                      cj_batch_stats *stats);
+/* Pin the configuration of later cj_execute_batch calls on ctx (the
+ * autotuner and the default step aside), or unpin it with cfg == NULL.
+ * load_factor 2 or 4; log_part 8..12; chunk 1..16384; join_queue 0 or 1;
+ * grid 0 = each kernel's resident grid, else the workgroups of every
+ * persistent launch (1..65536).  Results do not depend on it. */
+typedef struct cj_batch_config {
+    uint32_t load_factor;  /* directory slots per build row                */
+    uint32_t log_part;     /* slots per radix partition = 2^log_part       */
+    uint32_t chunk;        /* probe tuples per join unit                   */
+    uint32_t join_queue;   /* join units from a device work queue (1)      */
+    uint32_t grid;         /* workgroups per persistent launch, 0 = resident */
+} cj_batch_config;
+int cj_batch_config_set(cj_ctx *ctx, const cj_batch_config *cfg);
 
 /* ---- filters on the GPU: replace the filter step of the harness's scans,
  *      Table::from_csv(attributes, path, filter) (reference

@@ -96,13 +96,37 @@ def test_gpu_batch_autotune(engine, c4_small):
     res, st = engine.execute_batch(sub, abi.CJ_BATCH_AUTOTUNE)
     # the tuner runs every configuration (and fails the call if their result
     # sizes disagree); the chosen one's results equal the per-plan results
-    assert st.tuned == 1 and st.load_factor in (2, 4) and st.grid in (2048, 4096)
+    assert st.tuned == 1 and st.load_factor in (2, 4) and st.grid > 0
     assert st.log_part in (11, 12) and st.chunk in (4096, 8192, 16384) and st.join_queue in (0, 1)
     check(res, alone, range(4, 20))
-    cfg = (st.load_factor, st.grid, st.log_part, st.chunk, st.join_queue)
+    cfg = (st.load_factor, st.log_part, st.chunk, st.join_queue)
     res, st = engine.execute_batch(sub, abi.CJ_BATCH_AUTOTUNE)
-    assert st.tuned == 0 and (st.load_factor, st.grid, st.log_part, st.chunk, st.join_queue) == cfg
+    assert st.tuned == 0 and (st.load_factor, st.log_part, st.chunk, st.join_queue) == cfg
     check(res, alone, range(4, 20))
+
+
+@pytest.mark.parametrize("cfg", [(2, 8, 1000, 0, 0), (4, 11, 4096, 1, 0), (2, 12, 16384, 0, 300), (4, 10, 2047, 1, 7)],
+                         ids=["lp8-tile1000", "lp11-queue", "lp12-grid300", "lp10-grid7-queue"])
+def test_gpu_batch_pinned_configs(engine, c4_small, cfg):
+    """Every configuration gives the per-plan results: radix bits 8..12,
+    tiles that are not a multiple of the probe tile, strided or queued join
+    units, resident or small explicit grids."""
+    _, plans, alone = c4_small
+    engine.batch_config(*cfg)
+    try:
+        res, st = engine.execute_batch(plans, abi.CJ_BATCH_AUTOTUNE)  # a pinned configuration is not tuned
+        assert st.tuned == 0 and (st.load_factor, st.log_part, st.chunk, st.join_queue) == cfg[:4]
+        assert st.grid == (cfg[4] or st.grid) and st.grid > 0
+        check(res, alone, range(len(plans)))
+    finally:
+        engine.batch_config()
+
+
+def test_gpu_batch_config_rejected(engine):
+    for bad in [(3, 12, 8192, 0, 0), (2, 13, 8192, 0, 0), (2, 7, 8192, 0, 0), (2, 12, 0, 0, 0), (2, 12, 16385, 0, 0),
+                (2, 12, 8192, 2, 0), (2, 12, 8192, 0, 65537)]:
+        with pytest.raises(abi.CJError):
+            engine.batch_config(*bad)
 
 
 def test_gpu_batch_empty(engine):
