@@ -304,47 +304,75 @@ __device__ __forceinline__ int32_t bp_key(const BuildPart &bp, uint32_t t) {
     else return bp.g_keys[t];
 }
 
-// One output row: the join key, build tuple t of the partition (plan-local
-// index g_car0 + t of the slot-ordered carried columns), probe tuple pt
-// (plan-local index of the partition-ordered carried columns).  Lanes of a
-// wave write ascending rows, so the stores of a column stay nearly
-// coalesced.
+// The match window of the write pass: (probe row in the tile) << 22 | build
+// tuple of the partition, kMatchWin of them.
+constexpr uint32_t kMatchWin = 2048;
+constexpr uint32_t kMatchTBits = 22;
+constexpr uint32_t kMatchT = (1u << kMatchTBits) - 1;
+static_assert(kBatchMaxBuildRows <= (1ull << kMatchTBits), "build tuple index must fit the match entry");
+static_assert(kWG * 4 <= (1u << (32 - kMatchTBits)), "tile row must fit the match entry");
+
+// A NULL at slot `slot` of page `pgi` of an output column of `rows` rows
+// (R a page): clear its validity bit (page-init set them all) and count it
+// on the page, whose values the page-fix pass then packs.
+__device__ __forceinline__ void bx_null(uint8_t *page, uint32_t pgi, uint32_t slot, uint32_t rows, uint32_t R,
+                                        uint32_t *count) {
+    const uint32_t r0 = pgi * R;
+    const uint32_t n = rows - r0 < R ? rows - r0 : R;
+    const uint32_t byte = kPage - ((n + 7) >> 3) + (slot >> 3);
+    // the word may also hold bytes before the bitmap: AND leaves them alone
+    __hip_atomic_fetch_and(glb(reinterpret_cast<uint32_t *>(page + (byte & ~3u))),
+                           ~(1u << ((byte & 3u) * 8 + (slot & 7u))), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(glb(count), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One output column of a plan while the write pass emits it (uniform).
+struct BCol {
+    uint32_t src, cj, w, rows, pg0;
+    uint8_t *out;
+};
+
+// Output row o of column `col`: the join key, build tuple t of the
+// partition (plan-local index g_car0 + t of the slot-ordered carried
+// columns) or probe tuple pt (plan-local index of the partition-ordered
+// carried columns), stored at slot o % R of page o / R.  Lanes of a wave
+// write ascending rows, so the stores stay nearly coalesced.
 template <bool LDS>
-__device__ __forceinline__ void bx_row(const BPlan &pl, const BOut &bo, const BuildPart &bp, uint64_t o, int32_t key,
-                                       uint32_t t, uint64_t pt) {
-#pragma unroll
-    for (uint32_t c = 0; c < kBatchMaxOut; ++c) {
-        if (c >= bo.nout) break;
-        const uint32_t src = bo.src[c], cj = bo.j[c];
-        if (bo.w[c] == 4) {
-            int32_t v;
-            if (src == 0) {
-                v = key;  // equal keys: the build key is the probe key
-            } else if (src == 1) {
-                if constexpr (LDS) v = reinterpret_cast<const int32_t *>(bp.l_dyn + bp.coff(cj))[t];
-                else v = glb(reinterpret_cast<const int32_t *>(pl.bslot_car[cj]))[bp.g_car0 + t];
-            } else {
-                v = glb(reinterpret_cast<const int32_t *>(pl.p.car[cj]))[pt];
-            }
-            glb(reinterpret_cast<int32_t *>(bo.dense[c]))[o] = v;
+__device__ __forceinline__ void bx_put(const BPlan &pl, const BCol &col, const BuildPart &bp, uint32_t o,
+                                       int32_t key, uint32_t t, uint32_t pt, uint32_t *page_nulls) {
+    if (col.w == 4) {
+        int32_t v;
+        if (col.src == 0) {
+            v = key;  // equal keys: the build key is the probe key
+        } else if (col.src == 1) {
+            if constexpr (LDS) v = reinterpret_cast<const int32_t *>(bp.l_dyn + bp.coff(col.cj))[t];
+            else v = glb(reinterpret_cast<const int32_t *>(pl.bslot_car[col.cj]))[bp.g_car0 + t];
         } else {
-            uint64_t v;
-            uint8_t b;
-            if (src == 1) {
-                if constexpr (LDS) {
-                    v = reinterpret_cast<const uint64_t *>(bp.l_dyn + bp.coff(cj))[t];
-                    b = (bp.l_dyn + bp.voff(cj))[t];
-                } else {
-                    v = glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[cj]))[bp.g_car0 + t];
-                    b = glb(pl.bslot_vld[cj])[bp.g_car0 + t];
-                }
-            } else {
-                v = glb(reinterpret_cast<const uint64_t *>(pl.p.car[cj]))[pt];
-                b = glb(pl.p.car_vld[cj])[pt];
-            }
-            glb(reinterpret_cast<uint64_t *>(bo.dense[c]))[o] = v;
-            glb(bo.dvld[c])[o] = (uint8_t)(b != 0);
+            v = glb(reinterpret_cast<const int32_t *>(pl.p.car[col.cj]))[pt];
         }
+        const uint32_t pgi = o / kRowsI32, slot = o - pgi * kRowsI32;
+        uint8_t *page = col.out + (size_t)pgi * kPage;
+        if (v != kI32Null) glb(reinterpret_cast<int32_t *>(page + 4))[slot] = v;  // page_builders.h:123
+        else bx_null(page, pgi, slot, col.rows, kRowsI32, page_nulls + col.pg0 + pgi);
+    } else {
+        uint64_t v;
+        uint8_t b;
+        if (col.src == 1) {
+            if constexpr (LDS) {
+                v = reinterpret_cast<const uint64_t *>(bp.l_dyn + bp.coff(col.cj))[t];
+                b = (bp.l_dyn + bp.voff(col.cj))[t];
+            } else {
+                v = glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[col.cj]))[bp.g_car0 + t];
+                b = glb(pl.bslot_vld[col.cj])[bp.g_car0 + t];
+            }
+        } else {
+            v = glb(reinterpret_cast<const uint64_t *>(pl.p.car[col.cj]))[pt];
+            b = glb(pl.p.car_vld[col.cj])[pt];
+        }
+        const uint32_t pgi = o / kRowsI64, slot = o - pgi * kRowsI64;
+        uint8_t *page = col.out + (size_t)pgi * kPage;
+        if (b) glb(reinterpret_cast<uint64_t *>(page + 8))[slot] = v;
+        else bx_null(page, pgi, slot, col.rows, kRowsI64, page_nulls + col.pg0 + pgi);
     }
 }
 
@@ -353,7 +381,7 @@ __device__ __forceinline__ void bx_row(const BPlan &pl, const BOut &bo, const Bu
 template <bool WRITE, bool LDS, uint32_t RPT>
 __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &pl, const BOut *bo,
                                              const BuildPart &bp, uint64_t p0, uint64_t p1, uint64_t pbase,
-                                             uint64_t obase, uint32_t *lds32) {
+                                             uint64_t obase, uint32_t *lds32, uint32_t *win, uint32_t *page_nulls) {
     uint64_t mine = 0, base = obase;
     const uint32_t ls = pl.log_slots;
     for (uint64_t i0 = p0; i0 < p1; i0 += kWG * RPT) {
@@ -411,22 +439,41 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
                     run += lds32[4 * j + ww];
                 }
             __syncthreads();
-            // every lane writes its own tuples' matches (the first two were
-            // recorded by the probe; more come from a second walk of the slot)
+            // expand the tile's matches into the LDS window (probe row in
+            // the tile, build tuple), a window at a time; then the threads
+            // emit them column by column, consecutive rows on consecutive
+            // lanes (full lanes, whatever the per-tuple match counts)
+            for (uint32_t w0 = 0; w0 < run; w0 += kMatchWin) {
+                const uint32_t w1 = run - w0 < kMatchWin ? run : w0 + kMatchWin;
 #pragma unroll
-            for (uint32_t j = 0; j < RPT; ++j) {
-                const uint64_t o = base + pre[j];
-                const uint64_t pt = i0 + kWG * j + threadIdx.x - pbase;
-                uint32_t t = f0[j];
-                for (uint32_t r = 0; r < m[j];) {
-                    bx_row<LDS>(pl, *bo, bp, o + r, key[j], t, pt);
-                    if (++r == 1) {
-                        t = f1[j];
-                    } else if (r < m[j]) {
-                        do ++t;
-                        while (bp_key<LDS>(bp, t) != key[j]);
+                for (uint32_t j = 0; j < RPT; ++j) {
+                    if (!m[j] || pre[j] >= w1 || pre[j] + m[j] <= w0) continue;
+                    const uint32_t row = (kWG * j + threadIdx.x) << kMatchTBits;
+                    if (m[j] <= 2 && pre[j] >= w0 && pre[j] + m[j] <= w1) {
+                        win[pre[j] - w0] = row | f0[j];  // the usual case: the recorded matches
+                        if (m[j] == 2) win[pre[j] + 1 - w0] = row | f1[j];
+                        continue;
+                    }
+                    uint32_t q = pre[j];
+                    for (uint32_t t = s[j]; t < e[j] && q < w1; ++t) {
+                        if (bp_key<LDS>(bp, t) != key[j]) continue;
+                        if (q >= w0) win[q - w0] = row | t;
+                        ++q;
                     }
                 }
+                __syncthreads();
+                const uint32_t cnt = w1 - w0;
+#pragma unroll 1
+                for (uint32_t c = 0; c < bo->nout; ++c) {
+                    const BCol col{bo->src[c], bo->j[c], bo->w[c], bo->rows, bo->pg0[c], bo->out[c]};
+#pragma unroll 1
+                    for (uint32_t i = threadIdx.x; i < cnt; i += kWG) {
+                        const uint32_t en = win[i], t = en & kMatchT;
+                        const uint32_t pt = (uint32_t)(i0 - pbase) + (en >> kMatchTBits);
+                        bx_put<LDS>(pl, col, bp, (uint32_t)(base + w0 + i), bp_key<LDS>(bp, t), t, pt, page_nulls);
+                    }
+                }
+                __syncthreads();
             }
             base += run;
         }
@@ -460,13 +507,15 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
                                                  const uint32_t *__restrict__ part_start, const uint64_t *__restrict__ uoff,
                                                  const uint64_t *__restrict__ boff, const uint64_t *__restrict__ poff,
                                                  const uint64_t *__restrict__ moff, const int32_t *__restrict__ pkeys,
-                                                 const int32_t *__restrict__ bkeys, const uint32_t *__restrict__ dir) {
+                                                 const int32_t *__restrict__ bkeys, const uint32_t *__restrict__ dir,
+                                                 uint32_t *page_nulls) {
     // slot ends (u16, 2^log_part of them), then the keys and carried build values
     extern __shared__ __align__(16) uint8_t dyn[];
     uint16_t *ends = reinterpret_cast<uint16_t *>(dyn);
     __shared__ uint32_t lds32[32];
     __shared__ uint64_t red[4];
     __shared__ uint32_t unit;
+    __shared__ uint32_t win[WRITE ? kMatchWin : 1];
     const uint64_t total = uoff[a.nparts];  // < 2^32 (max_units)
     const uint32_t S = 1u << a.log_part;
     uint32_t *q = a.queue + (WRITE ? kQJoinWrite : kQJoinCount);
@@ -494,11 +543,11 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
         bp.w8 = 0;
         for (uint32_t c = 0; c < ncar; ++c) bp.w8 |= (pl.b.car_w[c] == 8 ? 1u : 0u) << c;
         uint64_t obase = 0;
-        BOut ob;  // the plan's output columns (uniform: scalar registers)
-        if (WRITE) {
-            obase = moff[u] - moff[uoff[part_start[k]]];
-            ob = outs[k];
-        }
+        if (WRITE) obase = moff[u] - moff[uoff[part_start[k]]];
+        // the plan's output columns, read through the pointer (uniform
+        // addresses: scalar loads; a local copy indexed by the column loop
+        // would live in scratch memory)
+        const BOut *ob = WRITE ? outs + k : nullptr;
         uint64_t mine;
         if (nb <= a.lds_cap) {
             // stage the partition: each batch of loads is issued before its LDS stores
@@ -525,9 +574,9 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
                     }
                 }
             __syncthreads();
-            mine = bx_probe<WRITE, true, WRITE ? 4u : kRPT>(pkeys, pl, &ob, bp, p0, p1, pbase, obase, lds32);
+            mine = bx_probe<WRITE, true, WRITE ? 4u : kRPT>(pkeys, pl, ob, bp, p0, p1, pbase, obase, lds32, win, page_nulls);
         } else {
-            mine = bx_probe<WRITE, false, WRITE ? 4u : kRPT>(pkeys, pl, &ob, bp, p0, p1, pbase, obase, lds32);
+            mine = bx_probe<WRITE, false, WRITE ? 4u : kRPT>(pkeys, pl, ob, bp, p0, p1, pbase, obase, lds32, win, page_nulls);
         }
         if (!WRITE) {
             uint64_t tot;
@@ -615,73 +664,75 @@ __global__ __launch_bounds__(kWG) void k_bx_decode(const BDecode *__restrict__ j
     }
 }
 
-// -------------------------------------------------------------------- pack
-// Output page k of a column holds result rows [k R, (k + 1) R), R = 1984
-// (INT32) or 1007 (8-byte).  A thread takes 8 consecutive rows (4 for 8-byte
-// values): loads them, ranks the non-NULL ones with one block scan, writes
-// the values and its bitmap byte (nibble).  INT32_MIN is written as NULL
-// (page_builders.h:123).
-__global__ __launch_bounds__(kWG) void k_bx_pack(const BPack *__restrict__ cols, const uint32_t *__restrict__ page_start, uint32_t ncols,
-                                                 uint32_t units) {
-    __shared__ uint32_t lds4[4];
-    for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
+// ---------------------------------------------------------- output pages
+// One wave per output page: header (rows, rows: every value present) and an
+// all-ones validity bitmap at the page end (page_builders.h:92-151 layout),
+// NULL counter zero.  The write pass then stores each value at its slot.
+__global__ __launch_bounds__(kWG) void k_bx_page_init(const BPageCol *__restrict__ cols,
+                                                      const uint32_t *__restrict__ page_start, uint32_t ncols,
+                                                      uint32_t pages, uint32_t *page_nulls) {
+    const uint32_t lane = lane_id();
+    for (uint32_t g = blockIdx.x * (kWG / 64) + wave_id(); g < pages; g += gridDim.x * (kWG / 64)) {
         const uint32_t c = find_owner(page_start, ncols, g);
-        const BPack m = cols[c];
+        const BPageCol m = cols[c];
         const uint32_t R = m.width == 4 ? kRowsI32 : kRowsI64;
-        const uint64_t pg = g - page_start[c];
-        const uint64_t r0 = pg * R;
-        const uint32_t cnt = (uint32_t)(m.rows - r0 < R ? m.rows - r0 : R);
-        CJ_G uint8_t *p = glb(m.out) + pg * kPage;
-        const uint32_t nbytes = (cnt + 7) >> 3;
-        CJ_G uint8_t *bm = p + kPage - nbytes;
+        const uint64_t r0 = (uint64_t)(g - page_start[c]) * R;
+        const uint32_t n = (uint32_t)(m.rows - r0 < R ? m.rows - r0 : R);
+        CJ_G uint8_t *p = glb(m.out) + (uint64_t)(g - page_start[c]) * kPage;
+        const uint32_t nbytes = (n + 7) >> 3;
+        for (uint32_t i = lane; i < nbytes; i += 64) {
+            const uint32_t left = n - 8 * i;  // rows of this byte
+            p[kPage - nbytes + i] = (uint8_t)(left >= 8 ? 0xffu : (1u << left) - 1u);
+        }
+        if (lane == 0) {
+            *reinterpret_cast<CJ_G uint32_t *>(p) = n | (n << 16);
+            page_nulls[g] = 0;
+        }
+    }
+}
+
+// A page whose rows include NULLs: the write pass left every value at its
+// row's slot and cleared the NULL rows' bits; pack the present values in row
+// order (in place: all loads before a barrier, then the stores) and set the
+// header's value count.  One workgroup per page, 8 (4) rows per thread.
+__global__ __launch_bounds__(kWG) void k_bx_page_fix(const BPageCol *__restrict__ cols,
+                                                     const uint32_t *__restrict__ page_start, uint32_t ncols,
+                                                     uint32_t pages, const uint32_t *__restrict__ page_nulls) {
+    __shared__ uint32_t lds4[4];
+    for (uint32_t g = blockIdx.x; g < pages; g += gridDim.x) {
+        if (!page_nulls[g]) continue;
+        const uint32_t c = find_owner(page_start, ncols, g);
+        const BPageCol m = cols[c];
+        const uint32_t R = m.width == 4 ? kRowsI32 : kRowsI64;
+        const uint64_t r0 = (uint64_t)(g - page_start[c]) * R;
+        const uint32_t n = (uint32_t)(m.rows - r0 < R ? m.rows - r0 : R);
+        CJ_G uint8_t *p = glb(m.out) + (uint64_t)(g - page_start[c]) * kPage;
+        const CJ_G uint8_t *bm = p + kPage - ((n + 7) >> 3);
         uint32_t tot;
         if (m.width == 4) {
             const uint32_t i0 = 8 * threadIdx.x;
-            const CJ_G int32_t *src = glb(reinterpret_cast<const int32_t *>(m.dense)) + r0;
+            uint32_t bits = i0 < n ? bm[i0 >> 3] : 0u;  // 8 rows a thread: one bitmap byte
+            CJ_G int32_t *vals = reinterpret_cast<CJ_G int32_t *>(p + 4);
             int32_t v[8];
-            if (i0 + 8 <= cnt) {
-                typedef int32_t v4i __attribute__((ext_vector_type(4)));
-                const v4i a = *reinterpret_cast<const CJ_G v4i *>(src + i0);
-                const v4i b = *reinterpret_cast<const CJ_G v4i *>(src + i0 + 4);
 #pragma unroll
-                for (uint32_t q = 0; q < 4; ++q) v[q] = a[q], v[4 + q] = b[q];
-            } else {
-#pragma unroll
-                for (uint32_t q = 0; q < 8; ++q) v[q] = i0 + q < cnt ? src[i0 + q] : kI32Null;
-            }
-            uint32_t bits = 0;
-#pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) bits |= (v[q] != kI32Null ? 1u : 0u) << q;
-            uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);
-            CJ_G uint32_t *vals = reinterpret_cast<CJ_G uint32_t *>(p + 4);
+            for (uint32_t q = 0; q < 8; ++q) v[q] = vals[i0 + q < n ? i0 + q : 0];
+            uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);  // its barrier orders loads before stores
 #pragma unroll
             for (uint32_t q = 0; q < 8; ++q)
-                if ((bits >> q) & 1u) vals[o++] = (uint32_t)v[q];
-            if (threadIdx.x < nbytes) bm[threadIdx.x] = (uint8_t)bits;
+                if ((bits >> q) & 1u) vals[o++] = v[q];
         } else {
             const uint32_t i0 = 4 * threadIdx.x;
-            const CJ_G uint64_t *src = glb(reinterpret_cast<const uint64_t *>(m.dense)) + r0;
-            const CJ_G uint8_t *sv = glb(m.dvld) + r0;
-            uint64_t v[4];
-            uint32_t bits = 0;
-#pragma unroll
-            for (uint32_t q = 0; q < 4; ++q) {
-                const uint32_t i = i0 + q < cnt ? i0 + q : cnt - 1;
-                v[q] = src[i];
-                bits |= (uint32_t)((i0 + q < cnt) & (sv[i] != 0)) << q;
-            }
-            uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);
+            uint32_t bits = i0 < n ? (bm[i0 >> 3] >> (i0 & 4)) & 15u : 0u;
             CJ_G uint64_t *vals = reinterpret_cast<CJ_G uint64_t *>(p + 8);
+            uint64_t v[4];
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) v[q] = vals[i0 + q < n ? i0 + q : 0];
+            uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);
 #pragma unroll
             for (uint32_t q = 0; q < 4; ++q)
                 if ((bits >> q) & 1u) vals[o++] = v[q];
-            const uint32_t hi = __shfl_down(bits, 1);  // rows i0 + 4 .. i0 + 7 (0 past cnt)
-            if (!(threadIdx.x & 1) && (threadIdx.x >> 1) < nbytes) bm[threadIdx.x >> 1] = (uint8_t)(bits | (hi << 4));
         }
-        if (threadIdx.x == 0) {
-            *reinterpret_cast<CJ_G uint16_t *>(p) = (uint16_t)cnt;
-            *reinterpret_cast<CJ_G uint16_t *>(p + 2) = (uint16_t)tot;
-        }
+        if (threadIdx.x == 0) *reinterpret_cast<CJ_G uint16_t *>(p + 2) = (uint16_t)tot;
         __syncthreads();
     }
 }
@@ -723,7 +774,8 @@ BatchGrids batch_resident_grids(const BatchArgs &a, int device) {
     g.build = resident(k_bx_build, 0, cus);
     g.join_count = resident(k_bx_join<false>, join_count_lds(a), cus);
     g.join_write = resident(k_bx_join<true>, join_write_lds(a), cus);
-    g.pack = resident(k_bx_pack, 0, cus);
+    g.page_init = resident(k_bx_page_init, 0, cus);
+    g.page_fix = resident(k_bx_page_fix, 0, cus);
     return g;
 }
 
@@ -761,15 +813,16 @@ void batch_join_count(const BatchArgs &a, uint32_t grid, hipStream_t s) {
     if (!a.max_units) return;
     const size_t sh = join_count_lds(a);
     k_bx_join<false><<<pgrid(a.max_units, grid), kWG, sh, s>>>(a, nullptr, a.plans, a.part_start, a.uoff, a.boff,
-                                                                        a.poff, a.moff, a.pkeys, a.bkeys, a.dir);
+                                                                        a.poff, a.moff, a.pkeys, a.bkeys, a.dir, nullptr);
     CJ_HIP(hipGetLastError());
 }
 
-void batch_join_write(const BatchArgs &a, const BOut *outs, uint32_t grid, hipStream_t s) {
+void batch_join_write(const BatchArgs &a, const BOut *outs, uint32_t *page_nulls, uint32_t grid, hipStream_t s) {
     if (!a.max_units) return;
     const size_t sh = join_write_lds(a);
     k_bx_join<true><<<pgrid(a.max_units, grid), kWG, sh, s>>>(a, outs, a.plans, a.part_start, a.uoff, a.boff,
-                                                                       a.poff, a.moff, a.pkeys, a.bkeys, a.dir);
+                                                                       a.poff, a.moff, a.pkeys, a.bkeys, a.dir,
+                                                                       page_nulls);
     CJ_HIP(hipGetLastError());
 }
 
@@ -780,10 +833,18 @@ void batch_decode(const BDecode *jobs, const uint32_t *page_start, uint32_t njob
     CJ_HIP(hipGetLastError());
 }
 
-void batch_pack(const BPack *cols, const uint32_t *page_start, uint32_t ncols, uint32_t units, uint32_t grid,
-                hipStream_t s) {
-    if (!units || !ncols) return;
-    k_bx_pack<<<pgrid(units, grid), kWG, 0, s>>>(cols, page_start, ncols, units);
+void batch_page_init(const BPageCol *cols, const uint32_t *page_start, uint32_t ncols, uint32_t pages,
+                     uint32_t *page_nulls, uint32_t grid, hipStream_t s) {
+    if (!pages || !ncols) return;
+    k_bx_page_init<<<pgrid((pages + kWG / 64 - 1) / (kWG / 64), grid), kWG, 0, s>>>(cols, page_start, ncols, pages,
+                                                                                  page_nulls);
+    CJ_HIP(hipGetLastError());
+}
+
+void batch_page_fix(const BPageCol *cols, const uint32_t *page_start, uint32_t ncols, uint32_t pages,
+                    const uint32_t *page_nulls, uint32_t grid, hipStream_t s) {
+    if (!pages || !ncols) return;
+    k_bx_page_fix<<<pgrid(pages, grid), kWG, 0, s>>>(cols, page_start, ncols, pages, page_nulls);
     CJ_HIP(hipGetLastError());
 }
 

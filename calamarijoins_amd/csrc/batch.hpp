@@ -1,17 +1,19 @@
 // batch.hpp -- batched execution of many independent single-join plans.
 //
 // A radix join over all plans at once.  Both sides of every plan are
-// partitioned by the top bits of the key hash; a build partition covers 2^12
-// directory slots, so its tuples (a few thousand) fit in LDS.  The output
-// columns' values travel with the keys through the partition pass
-// ("carried" columns), so no phase gathers by row id: the probe side streams
-// through HBM and every lookup is an LDS access.  Phases:
+// partitioned by the top bits of the key hash; a build partition covers
+// 2^log_part directory slots (2^8 .. 2^12, tuned), so its tuples (a few
+// thousand at most) fit in LDS.  The output columns' values travel with the
+// keys through the partition pass ("carried" columns), so no phase gathers
+// by row id: the probe side streams through HBM and every lookup is an LDS
+// access.  Phases:
 //
-//   decode carried columns -> partition histogram and scatter of both sides
-//   -> per-partition build (slot order, partition-local ends) -> join units
-//   (probe chunks of a partition) -> match count per unit -> [read-back of
-//   per-plan result sizes] -> match write (dense output columns) -> pack
-//   into 8 KiB pages.
+//   decode of the carried columns not read in place -> partition histogram
+//   and scatter of both sides -> per-partition build (slot order,
+//   partition-local ends) -> join units (probe chunks of a partition) ->
+//   match count per unit -> [read-back of per-plan result sizes] -> page
+//   init -> match write (values into their 8 KiB page slots) -> page fix
+//   (pages with NULLs packed).
 //
 // Every phase is one launch of a persistent kernel that walks a work list
 // covering all plans: global unit g (a key page, a partition, a probe
@@ -20,22 +22,23 @@
 // pull them from a device work queue (one atomic counter per phase).  The
 // phase sequence before the read-back is a hipGraph captured once per batch
 // template and replayed.  The autotuner picks the hash load factor, the
-// radix bits (slots per partition), the join tile (probe tuples per unit),
-// the join scheduling and the grid.
-
-// Work queue counters of the join phases (used when the tuned
-// configuration schedules join units dynamically).
-enum { kQJoinCount = 0, kQJoinWrite, kBatchQueues };
+// radix bits (slots per partition), the join tile (probe tuples per unit)
+// and the join scheduling; every launch takes its kernel's resident grid
+// unless cj_batch_config_set pins one.
 #pragma once
 #include "cj_internal.hpp"
 
 namespace cj {
 
+// Work queue counters of the join phases (used when the configuration
+// schedules join units dynamically).
+enum { kQJoinCount = 0, kQJoinWrite, kBatchQueues };
+
 constexpr uint32_t kBatchMaxCarry = 4;                // carried columns per side
 constexpr uint32_t kBatchMaxOut = 8;                  // output columns per plan
-constexpr uint32_t kBatchMaxLogPart = 12;             // slots per partition: 2^11 or 2^12 (tuned)
+constexpr uint32_t kBatchMaxLogPart = 12;             // slots per partition: 2^8 .. 2^12
 constexpr uint64_t kBatchMaxBuildRows = 1ull << 22;   // larger builds run one plan at a time
-constexpr uint32_t kBatchMaxChunk = 16384;            // probe tuples per join unit: 4096 .. 16384 (tuned)
+constexpr uint32_t kBatchMaxChunk = 16384;            // probe tuples per join unit
 constexpr uint32_t kBatchLdsBytes = 64 * 1024;        // LDS of a join workgroup
 
 // One side of one plan.  Carried column j holds, in partition order, the
@@ -57,7 +60,7 @@ struct BSide {
 
 struct BPlan {
     BSide b, p;
-    uint32_t log_slots;  // directory = 2^log_slots slots, partitions of 2^12
+    uint32_t log_slots;  // directory = 2^log_slots slots, partitions of 2^log_part
     uint64_t dir_base;   // first directory entry of the plan
     // build side in slot order (written by the build phase)
     uint8_t *bslot_car[kBatchMaxCarry];
@@ -96,14 +99,20 @@ struct BatchArgs {
     uint32_t *queue;              // [kBatchQueues] join work queue counters (zeroed before each join launch)
 };
 
-// Output columns of one plan (known after the read-back).
+// Output columns of one plan (known after the read-back).  Row o of output
+// column c goes straight to its page: page o / R, slot o % R of the value
+// area (R = 1984 INT32 or 1007 8-byte rows a page; the column's last page is
+// shorter).  The page-init pass wrote every page's header and an all-ones
+// validity bitmap before; a NULL clears its bit and counts on its page, and
+// the page-fix pass packs the values of the pages that hold NULLs.
 struct BOut {
     uint32_t nout;
+    uint32_t rows;              // result rows of the plan (< 2^32)
     uint8_t src[kBatchMaxOut];  // 0 key, 1 build carried, 2 probe carried
     uint8_t j[kBatchMaxOut];    // carried column index
     uint8_t w[kBatchMaxOut];    // 4 or 8
-    uint8_t *dense[kBatchMaxOut];
-    uint8_t *dvld[kBatchMaxOut];  // 8-byte columns
+    uint8_t *out[kBatchMaxOut];  // the column's pages
+    uint32_t pg0[kBatchMaxOut];  // global index of its first page (NULL counters)
 };
 
 // Dense decode of a base column (values + validity bitmap).
@@ -115,13 +124,11 @@ struct BDecode {
     uint32_t *valid;  // zeroed
 };
 
-// One output column of one plan: dense values -> pages.
-struct BPack {
-    const uint8_t *dense;
-    const uint8_t *dvld;
+// One output column of one plan: its pages.
+struct BPageCol {
+    uint8_t *out;
     uint64_t rows;
     int32_t width;
-    uint8_t *out;
 };
 
 // Workgroups of each persistent kernel that the device holds at once
@@ -130,7 +137,7 @@ struct BPack {
 // and, with units split evenly by stride, leaves the machine part-idle in
 // the last one.
 struct BatchGrids {
-    uint32_t decode, part_count, part_scatter, build, join_count, join_write, pack;
+    uint32_t decode, part_count, part_scatter, build, join_count, join_write, page_init, page_fix;
 };
 BatchGrids batch_resident_grids(const BatchArgs &a, int device);
 
@@ -140,11 +147,16 @@ void batch_part_scatter(const BatchArgs &a, bool probe, uint32_t units, uint32_t
 void batch_build(const BatchArgs &a, uint32_t grid, hipStream_t s);
 void batch_units(const BatchArgs &a, hipStream_t s);
 void batch_join_count(const BatchArgs &a, uint32_t grid, hipStream_t s);
-void batch_join_write(const BatchArgs &a, const BOut *outs, uint32_t grid, hipStream_t s);
+void batch_join_write(const BatchArgs &a, const BOut *outs, uint32_t *page_nulls, uint32_t grid, hipStream_t s);
 void batch_decode(const BDecode *jobs, const uint32_t *page_start, uint32_t njobs, uint32_t units, uint32_t grid,
                   hipStream_t s);
-void batch_pack(const BPack *cols, const uint32_t *page_start, uint32_t ncols, uint32_t units, uint32_t grid,
-                hipStream_t s);
+// Output pages: headers, all-ones bitmaps and zero NULL counters of all
+// `pages` pages (page_start: prefix over the columns), then, after the
+// write pass, the values of the pages that counted NULLs packed.
+void batch_page_init(const BPageCol *cols, const uint32_t *page_start, uint32_t ncols, uint32_t pages,
+                     uint32_t *page_nulls, uint32_t grid, hipStream_t s);
+void batch_page_fix(const BPageCol *cols, const uint32_t *page_start, uint32_t ncols, uint32_t pages,
+                    const uint32_t *page_nulls, uint32_t grid, hipStream_t s);
 // totals[k] = matches of plan k's units
 void batch_totals(const BatchArgs &a, uint64_t *totals, hipStream_t s);
 void batch_copy_u64(const uint64_t *src, uint64_t *dst, uint64_t n, hipStream_t s);

@@ -60,6 +60,7 @@ PlanInfo analyse(Context &ctx, const cj_plan_desc &p) {
     const cj_table_desc &bt = build_left ? lt : rt, &ptt = build_left ? rt : lt;
     if (bt.num_rows < 8) return pi;  // nested loop (execute.cpp:236-238)
     if (bt.num_rows > kBatchMaxBuildRows) return pi;
+    if (ptt.num_rows >= (1ull << 31)) return pi;  // plan-local probe tuples and result rows are 32-bit
     const uint64_t ba = build_left ? j.left_attr : j.right_attr, pa = build_left ? j.right_attr : j.left_attr;
     pi.bk = col_of(B, bt, ba);
     pi.pk = col_of(P, ptt, pa);
@@ -306,7 +307,7 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         const uint64_t parts = 1ull << (log_slots[k] - cfg.log_part);
         max_mean = std::max<uint64_t>(max_mean, (info[batched[k]].bk->num_rows + parts - 1) / parts);
     }
-    const uint32_t dyn_budget = kBatchLdsBytes - (2u << cfg.log_part) - 512;
+    const uint32_t dyn_budget = kBatchLdsBytes - (2u << cfg.log_part) - 4 * 2048 - 512;  // slot ends, match window
     const uint32_t cap_budget = std::min<uint32_t>((dyn_budget / max_tuple_bytes) & ~15u, 65535u & ~15u);
     const uint32_t lds_cap =
         (uint32_t)std::min<uint64_t>(cap_budget, (std::max<uint64_t>(256, 2 * max_mean) + 15) & ~15ull);
@@ -420,7 +421,8 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         tpl->njobs = (uint32_t)jobs.size();
         tpl->ndpages = job_start.back();
         tpl->grids = batch_resident_grids(a, ctx.device);
-        if (cfg.grid) tpl->grids = BatchGrids{cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid};
+        if (cfg.grid)
+            tpl->grids = BatchGrids{cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid};
     }
     // a template whose captured decode units disagree with this batch's page
     // counts would decode past (or short of) the pages: never replay it
@@ -524,7 +526,7 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
     }
     uint64_t out_rows = 0;
     for (uint64_t t : totals) out_rows += t;
-    // ---- phase 2: dense output columns, then their pages
+    // ---- phase 2: result pages (headers and bitmaps), values into their slots, pages with NULLs packed
     std::vector<void *> tmp_blocks;
     struct TmpGuard {
         DevicePool &p;
@@ -538,46 +540,40 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         tmp_blocks.push_back(p);
         return static_cast<uint8_t *>(p);
     };
-    uint64_t dense_bytes = 0;
-    for (uint32_t k = 0; k < n; ++k)
-        for (const auto &oc : info[batched[k]].outs)
-            dense_bytes += ((totals[k] * (width_of(oc.type) + (oc.type == CJ_INT32 ? 0 : 1))) + 15) & ~15ull;
-    uint8_t *dense = tblk(std::max<uint64_t>(dense_bytes, 16));
     std::vector<BOut> bouts(n);
-    std::vector<BPack> packs;
-    std::vector<uint32_t> pack_start(1, 0);
+    std::vector<BPageCol> pcols;
+    std::vector<uint32_t> page_start(1, 0);
     std::vector<std::vector<std::pair<uint8_t *, uint32_t>>> out_blocks(n);
-    uint64_t dpos = 0;
     try {
         for (uint32_t k = 0; k < n; ++k) {
             const PlanInfo &pi = info[batched[k]];
             BOut &bo = bouts[k];
             std::memset(&bo, 0, sizeof bo);
             bo.nout = (uint32_t)pi.outs.size();
+            if (totals[k] >= (1ull << 32)) throw UnsupportedError("a batched plan's result exceeds 2^32 rows");
+            bo.rows = (uint32_t)totals[k];
             for (size_t c = 0; c < pi.outs.size(); ++c) {
                 const auto &oc = pi.outs[c];
                 const uint32_t w = width_of(oc.type);
                 bo.src[c] = oc.src, bo.j[c] = oc.j, bo.w[c] = (uint8_t)w;
-                bo.dense[c] = dense + dpos;
-                dpos += totals[k] * w;
-                if (w == 8) {
-                    bo.dvld[c] = dense + dpos;
-                    dpos += totals[k];
-                }
-                dpos = (dpos + 15) & ~15ull;
                 const uint32_t R = w == 4 ? kRowsI32 : kRowsI64;
-                const uint32_t np = (uint32_t)((totals[k] + R - 1) / R);
+                const uint64_t np = (totals[k] + R - 1) / R;
+                if (page_start.back() + np > 0xffffffffull) throw UnsupportedError("batch result too large");
                 uint8_t *blk = np ? static_cast<uint8_t *>(pool.alloc((size_t)np * kPage + 16)) : nullptr;
-                out_blocks[k].push_back({blk, np});
+                out_blocks[k].push_back({blk, (uint32_t)np});
+                bo.out[c] = blk;
+                bo.pg0[c] = page_start.back();
                 if (!np) continue;
-                packs.push_back(BPack{bo.dense[c], bo.dvld[c], totals[k], (int32_t)w, blk});
-                pack_start.push_back(pack_start.back() + np);
+                pcols.push_back(BPageCol{blk, totals[k], (int32_t)w});
+                page_start.push_back(page_start.back() + (uint32_t)np);
             }
         }
+        const uint32_t npages = page_start.back();
         BOut *d_outs = reinterpret_cast<BOut *>(tblk(n * sizeof(BOut)));
-        BPack *d_packs = reinterpret_cast<BPack *>(tblk(std::max<size_t>(packs.size(), 1) * sizeof(BPack)));
-        uint32_t *d_ps = reinterpret_cast<uint32_t *>(tblk(pack_start.size() * 4));
-        const size_t b2 = n * sizeof(BOut) + packs.size() * sizeof(BPack) + pack_start.size() * 4 + 64;
+        BPageCol *d_cols = reinterpret_cast<BPageCol *>(tblk(std::max<size_t>(pcols.size(), 1) * sizeof(BPageCol)));
+        uint32_t *d_ps = reinterpret_cast<uint32_t *>(tblk(page_start.size() * 4));
+        uint32_t *d_nulls = reinterpret_cast<uint32_t *>(tblk(std::max<size_t>(npages, 1) * 4));
+        const size_t b2 = n * sizeof(BOut) + pcols.size() * sizeof(BPageCol) + page_start.size() * 4 + 64;
         uint8_t *stage2 = static_cast<uint8_t *>(pinned.alloc(b2));
         PinGuard pg2{pinned, stage2};
         size_t o2 = 0;
@@ -588,11 +584,12 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
             o2 += bytes;
         };
         up2(d_outs, bouts.data(), n * sizeof(BOut));
-        up2(d_packs, packs.data(), packs.size() * sizeof(BPack));
-        up2(d_ps, pack_start.data(), pack_start.size() * 4);
+        up2(d_cols, pcols.data(), pcols.size() * sizeof(BPageCol));
+        up2(d_ps, page_start.data(), page_start.size() * 4);
         CJ_HIP(hipEventRecord(ev[2], s));
-        if (out_rows) batch_join_write(A, d_outs, G.join_write, s);
-        batch_pack(d_packs, d_ps, (uint32_t)packs.size(), pack_start.back(), G.pack, s);
+        batch_page_init(d_cols, d_ps, (uint32_t)pcols.size(), npages, d_nulls, G.page_init, s);
+        if (out_rows) batch_join_write(A, d_outs, d_nulls, G.join_write, s);
+        batch_page_fix(d_cols, d_ps, (uint32_t)pcols.size(), npages, d_nulls, G.page_fix, s);
         CJ_HIP(hipEventRecord(ev[3], s));
         CJ_HIP(hipStreamSynchronize(s));
     } catch (...) {
