@@ -295,12 +295,12 @@ struct BuildPart {
 
 template <bool LDS>
 __device__ __forceinline__ uint32_t bp_end(const BuildPart &bp, uint32_t s) {
-    if constexpr (LDS) return bp.l_ends[s];
+    if constexpr (LDS) return lds(bp.l_ends)[s];
     else return bp.g_ends[s];
 }
 template <bool LDS>
 __device__ __forceinline__ int32_t bp_key(const BuildPart &bp, uint32_t t) {
-    if constexpr (LDS) return reinterpret_cast<const int32_t *>(bp.l_dyn)[t];
+    if constexpr (LDS) return lds(reinterpret_cast<const int32_t *>(bp.l_dyn))[t];
     else return bp.g_keys[t];
 }
 
@@ -345,7 +345,7 @@ __device__ __forceinline__ void bx_put(const BPlan &pl, const BCol &col, const B
         if (col.src == 0) {
             v = key;  // equal keys: the build key is the probe key
         } else if (col.src == 1) {
-            if constexpr (LDS) v = reinterpret_cast<const int32_t *>(bp.l_dyn + bp.coff(col.cj))[t];
+            if constexpr (LDS) v = lds(reinterpret_cast<const int32_t *>(bp.l_dyn + bp.coff(col.cj)))[t];
             else v = glb(reinterpret_cast<const int32_t *>(pl.bslot_car[col.cj]))[bp.g_car0 + t];
         } else {
             v = glb(reinterpret_cast<const int32_t *>(pl.p.car[col.cj]))[pt];
@@ -359,8 +359,8 @@ __device__ __forceinline__ void bx_put(const BPlan &pl, const BCol &col, const B
         uint8_t b;
         if (col.src == 1) {
             if constexpr (LDS) {
-                v = reinterpret_cast<const uint64_t *>(bp.l_dyn + bp.coff(col.cj))[t];
-                b = (bp.l_dyn + bp.voff(col.cj))[t];
+                v = lds(reinterpret_cast<const uint64_t *>(bp.l_dyn + bp.coff(col.cj)))[t];
+                b = lds(bp.l_dyn + bp.voff(col.cj))[t];
             } else {
                 v = glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[col.cj]))[bp.g_car0 + t];
                 b = glb(pl.bslot_vld[col.cj])[bp.g_car0 + t];
@@ -466,7 +466,7 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
 #pragma unroll 1
                 for (uint32_t c = 0; c < bo->nout; ++c) {
                     const BCol col{bo->src[c], bo->j[c], bo->w[c], bo->rows, bo->pg0[c], bo->out[c]};
-#pragma unroll 1
+#pragma unroll 4
                     for (uint32_t i = threadIdx.x; i < cnt; i += kWG) {
                         const uint32_t en = win[i], t = en & kMatchT;
                         const uint32_t pt = (uint32_t)(i0 - pbase) + (en >> kMatchTBits);

@@ -26,6 +26,14 @@ template <class T>
 __device__ __forceinline__ CJ_G T *glb(T *p) {
     return (CJ_G T *)p;
 }
+// The same for a generic pointer into LDS: ds_* instructions, and the
+// compiler knows its accesses cannot alias global memory (so LDS loads may
+// move above global stores).
+#define CJ_L __attribute__((address_space(3)))
+template <class T>
+__device__ __forceinline__ CJ_L T *lds(T *p) {
+    return (CJ_L T *)p;
+}
 
 __device__ __forceinline__ uint16_t ld16(const uint8_t *p) { return *reinterpret_cast<const uint16_t *>(p); }
 __device__ __forceinline__ int32_t ld32(const uint8_t *p) { return *reinterpret_cast<const int32_t *>(p); }
