@@ -22,9 +22,9 @@
 // pull them from a device work queue (one atomic counter per phase).  The
 // phase sequence before the read-back is a hipGraph captured once per batch
 // template and replayed.  The autotuner picks the hash load factor, the
-// radix bits (slots per partition), the join tile (probe tuples per unit)
-// and the join scheduling; every launch takes its kernel's resident grid
-// unless cj_batch_config_set pins one.
+// radix bits (slots per partition), the join tile (probe tuples per unit),
+// the join scheduling and the grid (each kernel's resident grid, or a fixed
+// workgroup count for every launch); cj_batch_config_set pins all of them.
 #pragma once
 #include "cj_internal.hpp"
 

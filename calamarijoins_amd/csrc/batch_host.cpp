@@ -172,7 +172,7 @@ void Context::execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flag
     // configuration: the tuned one for this template class, else the default
     const uint64_t cls = ((uint64_t)ceil_log2(nb_plans + 1) << 16) | ((uint64_t)ceil_log2(sum_probe + 1) << 8) |
                          ceil_log2(sum_build + 1);
-    BatchConfig cfg{2, 12, 8192, 0, 0};
+    BatchConfig cfg{2, 11, 16384, 1, 4096};  // the best of the 256-join workload's sweep
     bool tuned = false;
     {
         std::lock_guard<std::mutex> g(mu);
@@ -187,12 +187,13 @@ void Context::execute_batch(const cj_plan_desc *plans, uint32_t n, uint32_t flag
     }
     if (!tuned && (flags & CJ_BATCH_AUTOTUNE)) {
         // load factor x radix bits (slots per partition) x join tile x join
-        // scheduling; every launch takes the kernel's resident grid
+        // scheduling x grid (each kernel's resident grid, or 4096 workgroups)
         std::vector<BatchConfig> cands;
         for (uint32_t lf : {2u, 4u})
             for (uint32_t lp : {11u, 12u})
-                for (uint32_t chunk : {4096u, 8192u, 16384u})
-                    for (uint32_t jq : {0u, 1u}) cands.push_back({lf, lp, chunk, jq, 0});
+                for (uint32_t chunk : {4096u, 16384u})
+                    for (uint32_t jq : {0u, 1u})
+                        for (uint32_t grid : {0u, 4096u}) cands.push_back({lf, lp, chunk, jq, grid});
         double best = 1e30;
         std::vector<uint64_t> rows0;  // every configuration must give the same per-plan row counts
         for (const BatchConfig &c : cands) {

@@ -285,7 +285,7 @@ typedef struct cj_batch_stats {
     double total_ms;         /* the whole call                              */
     uint64_t output_rows;    /* rows of the batched plans                   */
     uint32_t load_factor;    /* directory slots per build row (chosen)      */
-    uint32_t grid;           /* workgroups of the join write launch (resident) */
+    uint32_t grid;           /* workgroups of the join write launch (chosen) */
     int32_t tuned;           /* this call ran the autotuner                 */
     int32_t graph_replayed;  /* the phase-1 graph of an earlier call ran    */
     uint32_t log_part;       /* slots per radix partition = 2^log_part (chosen) */
