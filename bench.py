@@ -106,9 +106,10 @@ COPY_BYTES = 1 << 30
 
 
 def copy_calibration():
-    """Three device-to-device copies of COPY_BYTES: rocprofv3's FETCH_SIZE and
-    WRITE_SIZE of this copy kernel should read COPY_BYTES / 1024 KiB each, and
-    the ratio scales the same counters of the join kernels to bytes."""
+    """Three device-to-device copies of COPY_BYTES, a kernel of known traffic
+    for checking how rocprofv3's counters convert to bytes: its read requests
+    (TCC_EA0_RDREQ by size) must add up to COPY_BYTES read, its WRITE_SIZE to
+    COPY_BYTES / 1024 KiB written."""
     Hip().copy_ms(COPY_BYTES, 3)
     print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "kib": COPY_BYTES // 1024}), flush=True)
 
@@ -127,7 +128,8 @@ def page_payload_bytes(ptr, num_pages, type_id):
 
 def pmc_traffic(kernel):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration)."""
+    summary: line-fill reads (TCC_EA0_RDREQ by request size) + WRITE_SIZE,
+    both checked against a copy of known size."""
     try:
         text = open(TRAFFIC_FILE).read()
     except OSError:
@@ -190,6 +192,100 @@ def cpu_baseline(plan, thread_counts, repeats):
     }
 
 
+def thread_counts(spec):
+    counts = []
+    for t in spec.split(","):
+        n = host_cpu_share() if t == "share" else int(t)
+        if n not in counts:
+            counts.append(n)
+    return counts
+
+
+def time_oracle(plans, threads, repeats):
+    """The oracle over `plans` one after another (as execute() runs them),
+    1 cold pass then `repeats` timed passes: (median seconds per pass, output
+    rows of a pass, per-plan seconds of the last pass)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import cj_oracle
+
+    def one_pass():
+        rows, per = 0, []
+        for pl in plans:
+            t0 = time.perf_counter()
+            r = cj_oracle.OracleRun(pl, threads=threads)
+            per.append(time.perf_counter() - t0)
+            rows += r.num_rows
+            del r
+        return rows, per
+
+    one_pass()
+    secs, rows, per = [], 0, []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        rows, per = one_pass()
+        secs.append(time.perf_counter() - t0)
+    return statistics.median(secs), rows, per
+
+
+def other_workload(args):
+    """--workload c3|c4|c5: the workload's GPU harness (run as a child
+    process; its JSON line is kept whole under "gpu"), then the oracle on the
+    same generated plans on this host's cores (a reported baseline, never the
+    measured path), one JSON line."""
+    import subprocess
+    from calamarijoins_amd import workloads
+
+    mod = {"c3": "starbench", "c4": "batching", "c5": "serving"}[args.workload]
+    cmd = [sys.executable, "-m", f"calamarijoins_amd.{mod}"] + (["--check"] if args.workload == "c5" else [])
+    log(f"{args.workload}: running {' '.join(cmd[1:])}")
+    proc = subprocess.run(cmd, cwd=REPO, capture_output=True, text=True)
+    sys.stderr.write(proc.stderr[-4000:])
+    if proc.returncode != 0:
+        print(proc.stdout[-2000:], file=sys.stderr)
+        sys.exit(proc.returncode)
+    gpu = json.loads([ln for ln in proc.stdout.splitlines() if ln.startswith("{")][-1])
+
+    t0 = time.time()
+    if args.workload == "c3":
+        fact, ds, _ = workloads.c3_tables()
+        plans = [workloads.c3_plan(fact, ds)]
+        repeats = args.cpu_repeats
+        gpu_value, unit = gpu["rows_per_s"], "rows/s"
+    elif args.workload == "c4":
+        pairs = [workloads.c4_plan_tables(k) for k in range(256)]
+        plans = [workloads.c4_plan(b, p) for b, p in pairs]
+        repeats = 1  # one pass is ~10 s of 16-thread CPU work
+        gpu_value, unit = gpu["batched"]["rows_per_s"], "rows/s"
+    else:
+        tables, meta = workloads.c5_tables(seed=1005)
+        plans = [workloads.c5_plan(tables, meta, 5000 + k) for k in range(32)]
+        repeats = 3
+        gpu_value, unit = gpu["served"]["rows_per_s"], "rows/s"
+    log(f"{args.workload}: generated {len(plans)} plans for the CPU baseline in {time.time() - t0:.1f}s")
+    cpu = None
+    if not args.no_cpu_baseline:
+        runs = {}
+        for t in thread_counts(args.cpu_threads):
+            sec, rows, per = time_oracle(plans, t, repeats)
+            runs[t] = (sec, rows, per)
+            log(f"{args.workload} cpu baseline: {t} threads, {sec * 1e3:.0f} ms per pass of {len(plans)} plans")
+        best = min(runs, key=lambda t: runs[t][0])
+        sec, rows, per = runs[best]
+        cpu = {"value": rows / sec, "unit": unit, "cores": best, "kind": "port",
+               "sample": f"CPU restatement of execute() on the whole workload ({len(plans)} plan(s) one after "
+                         f"another, {rows} output rows); 1 cold + {repeats} timed pass(es) per thread count, median "
+                         "ms per pass " + ", ".join(f"{t} threads {runs[t][0] * 1e3:.0f}" for t in runs)
+                         + f"; host: {cpu_model()}, {os.cpu_count()} hardware threads, {host_cpu_share()} in this "
+                           "job's share",
+               "median_ms_by_threads": {str(t): runs[t][0] * 1e3 for t in runs}}
+        if args.workload == "c5":
+            ms = sorted(x * 1e3 for x in per)
+            cpu["plan_latency_ms"] = {"p50": ms[len(ms) // 2], "max": ms[-1]}
+    line = {"workload": args.workload, "gpu": gpu, "gpu_value": gpu_value, "unit": unit, "cpu_baseline": cpu,
+            "gpu_over_cpu": (gpu_value / cpu["value"]) if cpu else None}
+    print(json.dumps(line), flush=True)
+
+
 def raw_execute(eng, plan, flags):
     """cj_execute through the bare C ABI (no Python result wrapper):
     (wall ms of the call, cj_result, cj_timing)."""
@@ -215,6 +311,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="only warmup + timed device-output steps (a short program for rocprofv3)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2 (default): the single large join, the headline line; c3/c4/c5: that workload's "
+                         "harness plus its CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -224,6 +323,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload != "c2":
+        other_workload(args)
         return
 
     global abi
@@ -344,11 +446,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        counts = []
-        for t in args.cpu_threads.split(","):
-            n = host_cpu_share() if t == "share" else int(t)
-            if n not in counts:
-                counts.append(n)
+        counts = thread_counts(args.cpu_threads)
         try:
             cpu = cpu_baseline(plan, counts, args.cpu_repeats)
         except Exception as e:  # the oracle is optional on the timing box
