@@ -218,12 +218,15 @@ def time_oracle(plans, threads, repeats):
             del r
         return rows, per
 
+    t0 = time.perf_counter()
     one_pass()
+    log(f"  oracle cold pass, {threads} threads: {time.perf_counter() - t0:.1f} s")
     secs, rows, per = [], 0, []
     for _ in range(repeats):
         t0 = time.perf_counter()
         rows, per = one_pass()
         secs.append(time.perf_counter() - t0)
+        log(f"  oracle timed pass, {threads} threads: {secs[-1]:.1f} s")
     return statistics.median(secs), rows, per
 
 
@@ -259,7 +262,7 @@ def other_workload(args):
     else:
         tables, meta = workloads.c5_tables(seed=1005)
         plans = [workloads.c5_plan(tables, meta, 5000 + k) for k in range(32)]
-        repeats = 3
+        repeats = 1  # a pass is tens of seconds of CPU work
         gpu_value, unit = gpu["served"]["rows_per_s"], "rows/s"
     log(f"{args.workload}: generated {len(plans)} plans for the CPU baseline in {time.time() - t0:.1f}s")
     cpu = None
