@@ -1229,6 +1229,14 @@ void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_
     // page writers per (side, width); each workgroup writes cols_per_wg
     // columns of one block of result rows
     const uint64_t blocks = (total + kBlockRows - 1) / kBlockRows;
+    // row ids of a build-side slot read through its map, resolved once for
+    // all the columns that come from it
+    std::map<uint32_t, const uint32_t *> resolved;
+    auto rids_of = [&](uint32_t slot) {
+        auto it = resolved.find(slot);
+        if (it != resolved.end()) return it->second;
+        return resolved[slot] = resolved_rids(B.slots[slot], B.rows);
+    };
     for (int side = 0; side < 2; ++side)
         for (uint32_t width = 4; width <= 8; width += 4) {
             MatArgs m{};
@@ -1257,7 +1265,7 @@ void Exec::fused_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint32_
                     ColView v = view(c);
                     mc.values = v.values;
                     mc.valid_bits = v.valid;
-                    mc.rids = resolved_rids(S.slots[cr.slot], S.rows);
+                    mc.rids = rids_of(cr.slot);  // fb: S is B
                     bytes += total * (uint64_t)width + total / 8;
                 } else {
                     mc.pages = c->pages;
