@@ -48,55 +48,6 @@ __device__ __forceinline__ uint32_t find_owner(const T *start, uint32_t n, T g) 
     return lo;
 }
 
-// Carried column c of side `sd` for the thread's kRPT rows (row0 + kWG j,
-// those set in `valid`) into plan-local tuples pos[j] - plan_base.
-__device__ __forceinline__ void carry8(const BSide &sd, uint32_t c, uint32_t valid, uint64_t row0,
-                                       const uint64_t pos[kRPT], uint64_t plan_base) {
-    if (sd.car_w[c] == 4) {
-        CJ_G int32_t *dst = glb(reinterpret_cast<int32_t *>(sd.car[c]));
-        int32_t v[kRPT];
-        const uint32_t R = sd.src_rows[c];
-        if (R) {
-            // no NULLs, R rows on every page but the last: read in place
-            const CJ_G uint8_t *pg = glb(sd.src[c]);
-#pragma unroll
-            for (uint32_t j = 0; j < kRPT; ++j)
-                if ((valid >> j) & 1u) {
-                    const uint32_t r = (uint32_t)(row0 + kWG * j), q = r / R;
-                    v[j] = ld32(pg + (size_t)q * kPage + 4 + 4 * (size_t)(r - q * R));
-                }
-        } else {
-            // the decode pass folded NULL into INT32_MIN
-            const CJ_G int32_t *src = glb(reinterpret_cast<const int32_t *>(sd.src[c]));
-#pragma unroll
-            for (uint32_t j = 0; j < kRPT; ++j)
-                if ((valid >> j) & 1u) v[j] = src[row0 + kWG * j];
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < kRPT; ++j)
-            if ((valid >> j) & 1u) dst[pos[j] - plan_base] = v[j];
-    } else {
-        const CJ_G uint64_t *src = glb(reinterpret_cast<const uint64_t *>(sd.src[c]));
-        CJ_G uint64_t *dst = glb(reinterpret_cast<uint64_t *>(sd.car[c]));
-        CJ_G uint8_t *dv = glb(sd.car_vld[c]);
-        const CJ_G uint32_t *sv = glb(sd.src_valid[c]);
-        uint64_t v[kRPT];
-        uint32_t vb = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < kRPT; ++j)
-            if ((valid >> j) & 1u) {
-                v[j] = src[row0 + kWG * j];
-                vb |= (uint32_t)valid_bit(sv, row0 + kWG * j) << j;
-            }
-#pragma unroll
-        for (uint32_t j = 0; j < kRPT; ++j)
-            if ((valid >> j) & 1u) {
-                dst[pos[j] - plan_base] = v[j];
-                dv[pos[j] - plan_base] = (uint8_t)((vb >> j) & 1u);
-            }
-    }
-}
-
 }  // namespace
 
 // ------------------------------------------------------ partition histogram
@@ -138,16 +89,144 @@ __global__ __launch_bounds__(kWG) void k_bx_part_count(BatchArgs a, uint32_t uni
 
 // ------------------------------------------------------ partition scatter
 // Keys go to their partition's range of the side's key array; carried
-// values to the same plan-local position of the carried arrays.
+// values to the same plan-local position of the carried arrays.  Per key
+// page: each non-NULL key takes a rank in its partition (LDS atomic) and is
+// recorded at its value index; one global reservation per non-empty
+// partition; an LDS scan orders the page's tuples by partition; then the
+// keys and each carried column are written out in that order, so
+// consecutive lanes store consecutive slots of a partition's run (a page
+// of a plan with few partitions writes long coalesced runs instead of one
+// scattered store per tuple).
+
+// Carried column c of side sd at table row `row`.
+__device__ __forceinline__ int32_t carried4(const BSide &sd, uint32_t c, uint64_t row) {
+    const uint32_t R = sd.src_rows[c];
+    if (R) {  // no NULLs, R rows on every page but the last: read in place
+        const uint32_t r = (uint32_t)row, q = r / R;
+        return ld32(glb(sd.src[c]) + (size_t)q * kPage + 4 + 4 * (size_t)(r - q * R));
+    }
+    return glb(reinterpret_cast<const int32_t *>(sd.src[c]))[row];  // decoded: NULL folded into INT32_MIN
+}
+
+constexpr uint32_t kRankBits = 11;  // ranks within a page: < 2048 non-NULL keys (a key page holds <= 2047)
+
 template <bool PROBE>
 __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t units, const BPlan *__restrict__ plans,
                                                          const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ part_start,
-                                                         const uint64_t *__restrict__ off) {
-    // cursors (u64) then hist[max_parts]: declared as u64 so the 64-bit LDS
-    // atomics stay 8-byte aligned whatever static LDS precedes the array
+                                                         const uint64_t *__restrict__ off, uint32_t lds_parts) {
+    // global run start of each partition (u64) then the counts / local
+    // starts (u32): declared as u64 so the u64 half stays 8-byte aligned
+    // whatever static LDS precedes the array; lds_parts >= the plans' P
     extern __shared__ uint64_t sm64[];
+    uint64_t *gb = sm64;
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(sm64 + lds_parts);
+    __shared__ int32_t akey[kTile];    // by value index: the key
+    __shared__ uint32_t ainfo[kTile];  // partition << kRankBits | rank
+    __shared__ uint16_t arow[kTile];   // row in the page
+    __shared__ uint16_t order[kTile];  // partition order -> value index
+    __shared__ uint32_t lds32[32];
+    __shared__ uint32_t lds4[4];
+    uint64_t *gcur = PROBE ? a.pcur : a.bcur;
+    int32_t *keys_out = PROBE ? a.pkeys : a.bstage_keys;
+    const uint32_t lp = a.log_part;
+    for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
+        const uint32_t k = find_owner(pstart, a.nplans, g);
+        const BPlan &pl = plans[k];
+        const BSide &sd = PROBE ? pl.p : pl.b;
+        const uint32_t ls = pl.log_slots;
+        const uint32_t P = 1u << (ls - lp);
+        const uint32_t page = g - pstart[k];
+        for (uint32_t i = threadIdx.x; i < P; i += kWG) cnt[i] = 0;
+        __syncthreads();
+        const CJ_G uint8_t *p = glb(sd.k_pages) + (size_t)page * kPage;
+        const uint32_t n = ld16(p), nv = ld16(p + 2);
+        uint32_t running = 0;
+        for (uint32_t t0 = 0; t0 < n; t0 += kTile) {
+            int32_t key[kRPT];
+            uint32_t vi[kRPT];
+            const uint32_t valid = load_keys8(p, n, nv, n == nv, t0, &running, key, lds32, vi);
+#pragma unroll
+            for (uint32_t j = 0; j < kRPT; ++j)
+                if ((valid >> j) & 1u) {
+                    const uint32_t q = (uint32_t)(bslot_of(key[j], ls) >> lp);
+                    const uint32_t rank = atomicAdd(&cnt[q], 1u);
+                    akey[vi[j]] = key[j];
+                    ainfo[vi[j]] = q << kRankBits | rank;
+                    arow[vi[j]] = (uint16_t)(t0 + kWG * j + threadIdx.x);
+                }
+            __syncthreads();  // lds32 of the next tile's ranking
+        }
+        // one reservation per non-empty partition, then the counts become
+        // the partitions' starts in the page's partition order
+        const uint32_t pb = part_start[k];
+        for (uint32_t i = threadIdx.x; i < P; i += kWG)
+            gb[i] = cnt[i] ? atomicAdd((unsigned long long *)&gcur[pb + i], (unsigned long long)cnt[i]) : 0;
+        const uint32_t per = (P + kWG - 1) / kWG;
+        const uint32_t c0 = threadIdx.x * per;
+        uint32_t local = 0;
+        for (uint32_t i = 0; i < per && c0 + i < P; ++i) local += cnt[c0 + i];
+        uint32_t nval;
+        uint32_t pre = block_excl_scan<uint32_t>(local, &nval, lds4);  // barrier: the reservations read cnt first
+        for (uint32_t i = 0; i < per && c0 + i < P; ++i) {
+            const uint32_t c = cnt[c0 + i];
+            cnt[c0 + i] = pre;
+            pre += c;
+        }
+        __syncthreads();
+        for (uint32_t v = threadIdx.x; v < nval; v += kWG) {
+            const uint32_t in = ainfo[v];
+            order[cnt[in >> kRankBits] + (in & ((1u << kRankBits) - 1))] = (uint16_t)v;
+        }
+        __syncthreads();
+        // write-out in partition order: keys, then each carried column
+        const uint64_t plan_base = off[pb];
+        const uint64_t r0 = glb(sd.k_rs)[page];
+        for (uint32_t s = threadIdx.x; s < nval; s += kWG) {
+            const uint32_t v = order[s], in = ainfo[v];
+            keys_out[gb[in >> kRankBits] + (in & ((1u << kRankBits) - 1))] = akey[v];
+        }
+        const uint32_t ncar = sd.ncar;
+        for (uint32_t c = 0; c < ncar; ++c) {
+            if (sd.car_w[c] == 4) {
+                CJ_G int32_t *dst = glb(reinterpret_cast<int32_t *>(sd.car[c]));
+                for (uint32_t s = threadIdx.x; s < nval; s += kWG) {
+                    const uint32_t v = order[s], in = ainfo[v];
+                    dst[gb[in >> kRankBits] + (in & ((1u << kRankBits) - 1)) - plan_base] = carried4(sd, c, r0 + arow[v]);
+                }
+            } else {
+                const CJ_G uint64_t *src = glb(reinterpret_cast<const uint64_t *>(sd.src[c]));
+                const CJ_G uint32_t *sv = glb(sd.src_valid[c]);
+                CJ_G uint64_t *dst = glb(reinterpret_cast<uint64_t *>(sd.car[c]));
+                CJ_G uint8_t *dv = glb(sd.car_vld[c]);
+                for (uint32_t s = threadIdx.x; s < nval; s += kWG) {
+                    const uint32_t v = order[s], in = ainfo[v];
+                    const uint64_t row = r0 + arow[v];
+                    const uint64_t d = gb[in >> kRankBits] + (in & ((1u << kRankBits) - 1)) - plan_base;
+                    dst[d] = src[row];
+                    dv[d] = (uint8_t)valid_bit(sv, row);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The same scatter for plans with many partitions per page (a page's runs
+// would be a tuple or two: staging buys nothing there): every tuple takes
+// its position from an LDS cursor and is stored directly.  The global
+// reservations are issued kResPer at a time per thread before any result is
+// awaited.
+constexpr uint32_t kResPer = 16;
+
+template <bool PROBE>
+__global__ __launch_bounds__(kWG) void k_bx_part_scatter_direct(BatchArgs a, uint32_t units,
+                                                                const BPlan *__restrict__ plans,
+                                                                const uint32_t *__restrict__ pstart,
+                                                                const uint32_t *__restrict__ part_start,
+                                                                const uint64_t *__restrict__ off, uint32_t lds_parts) {
+    extern __shared__ uint64_t sm64[];  // cursors (u64) then hist[lds_parts] (u32)
     uint64_t *cur = sm64;
-    uint32_t *hist = reinterpret_cast<uint32_t *>(sm64 + a.max_parts);
+    uint32_t *hist = reinterpret_cast<uint32_t *>(sm64 + lds_parts);
     __shared__ uint32_t lds32[32];
     uint64_t *gcur = PROBE ? a.pcur : a.bcur;
     int32_t *keys_out = PROBE ? a.pkeys : a.bstage_keys;
@@ -170,13 +249,25 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t u
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j)
                 if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(key[j], ls) >> lp], 1u);
+            __syncthreads();
+        }
+        const uint32_t pb = part_start[k];
+        for (uint32_t i0 = 0; i0 < P; i0 += kWG * kResPer) {
+            uint64_t r[kResPer];
+#pragma unroll
+            for (uint32_t q = 0; q < kResPer; ++q) {
+                const uint32_t i = i0 + q * kWG + threadIdx.x;
+                const uint32_t h = i < P ? hist[i] : 0u;
+                r[q] = h ? atomicAdd((unsigned long long *)&gcur[pb + i], (unsigned long long)h) : 0;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < kResPer; ++q) {
+                const uint32_t i = i0 + q * kWG + threadIdx.x;
+                if (i < P) cur[i] = r[q];
+            }
         }
         __syncthreads();
-        const uint32_t pb = part_start[k];
         const uint64_t plan_base = off[pb];
-        for (uint32_t i = threadIdx.x; i < P; i += kWG)
-            cur[i] = hist[i] ? atomicAdd((unsigned long long *)&gcur[pb + i], (unsigned long long)hist[i]) : 0;
-        __syncthreads();
         const uint64_t r0 = glb(sd.k_rs)[page];
         const uint32_t ncar = sd.ncar;
         running = 0;
@@ -190,14 +281,44 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t u
                 if ((valid >> j) & 1u)
                     pos[j] = atomicAdd((unsigned long long *)&cur[bslot_of(key[j], ls) >> lp], 1ull);
             }
-            // per carried column: all loads, then all stores (a store between
-            // two loads would serialize them on memory latency)
-            for (uint32_t c = 0; c < ncar; ++c) carry8(sd, c, valid, r0 + t0 + threadIdx.x, pos, plan_base);
+            // per carried column: all loads, then all stores
+            for (uint32_t c = 0; c < ncar; ++c) {
+                if (sd.car_w[c] == 4) {
+                    int32_t v[kRPT];
+#pragma unroll
+                    for (uint32_t j = 0; j < kRPT; ++j)
+                        if ((valid >> j) & 1u) v[j] = carried4(sd, c, r0 + t0 + kWG * j + threadIdx.x);
+                    CJ_G int32_t *dst = glb(reinterpret_cast<int32_t *>(sd.car[c]));
+#pragma unroll
+                    for (uint32_t j = 0; j < kRPT; ++j)
+                        if ((valid >> j) & 1u) dst[pos[j] - plan_base] = v[j];
+                } else {
+                    const CJ_G uint64_t *src = glb(reinterpret_cast<const uint64_t *>(sd.src[c]));
+                    const CJ_G uint32_t *sv = glb(sd.src_valid[c]);
+                    uint64_t v[kRPT];
+                    uint32_t vb = 0;
+#pragma unroll
+                    for (uint32_t j = 0; j < kRPT; ++j)
+                        if ((valid >> j) & 1u) {
+                            const uint64_t row = r0 + t0 + kWG * j + threadIdx.x;
+                            v[j] = src[row];
+                            vb |= (uint32_t)valid_bit(sv, row) << j;
+                        }
+                    CJ_G uint64_t *dst = glb(reinterpret_cast<uint64_t *>(sd.car[c]));
+                    CJ_G uint8_t *dv = glb(sd.car_vld[c]);
+#pragma unroll
+                    for (uint32_t j = 0; j < kRPT; ++j)
+                        if ((valid >> j) & 1u) {
+                            dst[pos[j] - plan_base] = v[j];
+                            dv[pos[j] - plan_base] = (uint8_t)((vb >> j) & 1u);
+                        }
+                }
+            }
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j)
                 if ((valid >> j) & 1u) keys_out[pos[j]] = key[j];
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -751,7 +872,7 @@ static unsigned pgrid(uint64_t units, uint32_t grid) { return (unsigned)(units <
 
 // dynamic LDS of each kernel
 static size_t part_count_lds(const BatchArgs &a) { return a.max_parts * sizeof(uint32_t); }
-static size_t part_scatter_lds(const BatchArgs &a) { return a.max_parts * (sizeof(uint64_t) + sizeof(uint32_t)); }
+static size_t scatter_lds(uint32_t parts) { return parts * (sizeof(uint64_t) + sizeof(uint32_t)); }
 static size_t join_count_lds(const BatchArgs &a) { return (2u << a.log_part) + (size_t)a.lds_cap * 4; }  // slot ends + keys
 static size_t join_write_lds(const BatchArgs &a) {
     return (2u << a.log_part) + (size_t)a.lds_cap * a.tuple_bytes;  // slot ends + tuples
@@ -764,13 +885,38 @@ static uint32_t resident(K kernel, size_t dyn, int cus) {
     return (uint32_t)(per > 0 ? per : 1) * (uint32_t)cus;
 }
 
+// A kernel whose static + dynamic LDS must fit a workgroup's share of the
+// CU; dynamic LDS above 64 KiB needs the kernel attribute raised.
+template <class K>
+static void allow_lds(K kernel, size_t dyn, int device, const char *what) {
+    hipFuncAttributes fa;
+    CJ_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kernel)));
+    int max_lds = 0;
+    CJ_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    if (fa.sharedSizeBytes + dyn > (size_t)max_lds)
+        throw UnsupportedError(std::string("batch configuration: ") + what +
+                               " needs more LDS than a workgroup has (too many partitions for 2^log_part slots)");
+    if (dyn > 64 * 1024)
+        CJ_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+}
+
 BatchGrids batch_resident_grids(const BatchArgs &a, int device) {
+    allow_lds(k_bx_part_count<true>, part_count_lds(a), device, "the partition histogram");
+    allow_lds(k_bx_part_count<false>, part_count_lds(a), device, "the partition histogram");
+    allow_lds(k_bx_part_scatter<true>, scatter_lds(a.staged_parts), device, "the partition scatter");
+    allow_lds(k_bx_part_scatter<false>, scatter_lds(a.staged_parts), device, "the partition scatter");
+    allow_lds(k_bx_part_scatter_direct<true>, scatter_lds(a.direct_parts), device, "the partition scatter");
+    allow_lds(k_bx_part_scatter_direct<false>, scatter_lds(a.direct_parts), device, "the partition scatter");
+    allow_lds(k_bx_join<true>, join_write_lds(a), device, "the join");
+    allow_lds(k_bx_join<false>, join_count_lds(a), device, "the join");
     int cus = 0;
     CJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     BatchGrids g;
     g.decode = resident(k_bx_decode, 0, cus);
     g.part_count = resident(k_bx_part_count<true>, part_count_lds(a), cus);
-    g.part_scatter = resident(k_bx_part_scatter<true>, part_scatter_lds(a), cus);
+    g.part_scatter = resident(k_bx_part_scatter<true>, scatter_lds(a.staged_parts), cus);
+    g.part_scatter_direct = resident(k_bx_part_scatter_direct<true>, scatter_lds(a.direct_parts), cus);
     g.build = resident(k_bx_build, 0, cus);
     g.join_count = resident(k_bx_join<false>, join_count_lds(a), cus);
     g.join_write = resident(k_bx_join<true>, join_write_lds(a), cus);
@@ -787,13 +933,22 @@ void batch_part_count(const BatchArgs &a, bool probe, uint32_t units, uint32_t g
     CJ_HIP(hipGetLastError());
 }
 
-void batch_part_scatter(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s) {
+void batch_part_scatter(const BatchArgs &a, bool probe, const uint32_t *pstart, uint32_t units, bool staged,
+                        uint32_t grid, hipStream_t s) {
     if (!units) return;
-    const size_t sh = part_scatter_lds(a);
-    if (probe) k_bx_part_scatter<true><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.ppage_start, a.part_start,
-                                                                     a.poff);
-    else k_bx_part_scatter<false><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, a.bpage_start, a.part_start,
-                                                                      a.boff);
+    const uint32_t parts = staged ? a.staged_parts : a.direct_parts;
+    const size_t sh = scatter_lds(parts);
+    const uint64_t *off = probe ? a.poff : a.boff;
+    if (staged) {
+        if (probe) k_bx_part_scatter<true><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off, parts);
+        else k_bx_part_scatter<false><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off, parts);
+    } else {
+        if (probe)
+            k_bx_part_scatter_direct<true><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off, parts);
+        else
+            k_bx_part_scatter_direct<false><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off,
+                                                                                 parts);
+    }
     CJ_HIP(hipGetLastError());
 }
 

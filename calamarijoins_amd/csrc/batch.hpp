@@ -72,8 +72,14 @@ struct BatchArgs {
     uint32_t nplans;
     const uint32_t *bpage_start;  // [nplans + 1] prefix of build key pages
     const uint32_t *ppage_start;  // [nplans + 1] prefix of probe key pages
+    // the same prefixes counting only the pages of plans with at most / more
+    // than kStagedMaxParts partitions (the two scatter kernels' work lists)
+    const uint32_t *bpage_start_s, *bpage_start_d;
+    const uint32_t *ppage_start_s, *ppage_start_d;
     const uint32_t *part_start;   // [nplans + 1] prefix of partitions
     uint32_t max_parts;           // largest partition count of a plan (LDS histogram)
+    uint32_t staged_parts;        // largest partition count of a plan scattered through LDS staging
+    uint32_t direct_parts;        // largest partition count of a plan scattered directly
     uint32_t nparts;
     // build side
     uint64_t *bcount, *bcur;      // [nparts + 1]
@@ -137,13 +143,21 @@ struct BPageCol {
 // and, with units split evenly by stride, leaves the machine part-idle in
 // the last one.
 struct BatchGrids {
-    uint32_t decode, part_count, part_scatter, build, join_count, join_write, page_init, page_fix;
+    uint32_t decode, part_count, part_scatter, part_scatter_direct, build, join_count, join_write, page_init, page_fix;
 };
+// Also checks that every kernel's LDS fits a workgroup (UnsupportedError if
+// not) and raises the dynamic-LDS limit where a kernel needs more than 64 KiB;
+// called when a batch template is made, outside graph capture.
 BatchGrids batch_resident_grids(const BatchArgs &a, int device);
 
 // Each launcher takes the persistent grid size (workgroups).
 void batch_part_count(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s);
-void batch_part_scatter(const BatchArgs &a, bool probe, uint32_t units, uint32_t grid, hipStream_t s);
+// Plans with at most kStagedMaxParts partitions scatter through LDS staging
+// (a page's tuples ordered by partition, written out in runs); plans with
+// more scatter each tuple directly.  pstart / units: the kernel's work list.
+constexpr uint32_t kStagedMaxParts = 256;
+void batch_part_scatter(const BatchArgs &a, bool probe, const uint32_t *pstart, uint32_t units, bool staged,
+                        uint32_t grid, hipStream_t s);
 void batch_build(const BatchArgs &a, uint32_t grid, hipStream_t s);
 void batch_units(const BatchArgs &a, hipStream_t s);
 void batch_join_count(const BatchArgs &a, uint32_t grid, hipStream_t s);

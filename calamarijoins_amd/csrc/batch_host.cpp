@@ -242,8 +242,12 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         cfg.grid > 65536)
         throw DeviceError("batch configuration out of range");
     // ---- shape of the batch
-    std::vector<uint32_t> starts(3 * (n + 1), 0);
+    // prefixes: build pages, probe pages, partitions; then build and probe
+    // pages of the plans scattered through LDS staging / directly
+    std::vector<uint32_t> starts(7 * (n + 1), 0);
     uint32_t *bps = starts.data(), *pps = bps + (n + 1), *pts = pps + (n + 1);
+    uint32_t *bss = pts + (n + 1), *bsd = bss + (n + 1), *pss = bsd + (n + 1), *psd = pss + (n + 1);
+    uint32_t staged_parts = 1, direct_parts = 1;
     std::vector<uint32_t> log_slots(n);
     uint64_t dir_total = 0, sum_nb = 0, sum_np = 0;
     uint32_t max_parts = 1, max_tuple_bytes = 4;
@@ -277,6 +281,12 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         sum_np += pi.pk->num_rows;
         bps[k + 1] = bps[k] + pi.bk->num_pages;
         pps[k + 1] = pps[k] + pi.pk->num_pages;
+        const bool staged = parts <= kStagedMaxParts;
+        (staged ? staged_parts : direct_parts) = std::max(staged ? staged_parts : direct_parts, parts);
+        bss[k + 1] = bss[k] + (staged ? pi.bk->num_pages : 0);
+        bsd[k + 1] = bsd[k] + (staged ? 0 : pi.bk->num_pages);
+        pss[k + 1] = pss[k] + (staged ? pi.pk->num_pages : 0);
+        psd[k + 1] = psd[k] + (staged ? 0 : pi.pk->num_pages);
         pts[k + 1] = pts[k] + parts;
         uint32_t tb = 4;
         for (DevColumn *c : pi.bcar) tb += width_of(c->type) + (c->type == CJ_INT32 ? 0 : 1);
@@ -410,7 +420,13 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         a.bpage_start = tpl->d_starts;
         a.ppage_start = tpl->d_starts + (n + 1);
         a.part_start = tpl->d_starts + 2 * (n + 1);
+        a.bpage_start_s = tpl->d_starts + 3 * (n + 1);
+        a.bpage_start_d = tpl->d_starts + 4 * (n + 1);
+        a.ppage_start_s = tpl->d_starts + 5 * (n + 1);
+        a.ppage_start_d = tpl->d_starts + 6 * (n + 1);
         a.max_parts = max_parts;
+        a.staged_parts = staged_parts;
+        a.direct_parts = direct_parts;
         a.nparts = nparts;
         a.max_units = max_units;
         a.lds_cap = lds_cap;
@@ -423,7 +439,8 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         tpl->ndpages = job_start.back();
         tpl->grids = batch_resident_grids(a, ctx.device);
         if (cfg.grid)
-            tpl->grids = BatchGrids{cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid};
+            tpl->grids =
+                BatchGrids{cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid};
     }
     // a template whose captured decode units disagree with this batch's page
     // counts would decode past (or short of) the pages: never replay it
@@ -477,11 +494,13 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         batch_part_count(A, false, nbp, G.part_count, ps);
         scan_exclusive_u64(A.bcount, const_cast<uint64_t *>(A.boff), nparts, ps, T.scan_tmp, T.scan_elems);
         batch_copy_u64(A.boff, A.bcur, nparts + 1, ps);
-        batch_part_scatter(A, false, nbp, G.part_scatter, ps);
+        batch_part_scatter(A, false, A.bpage_start_s, bss[n], true, G.part_scatter, ps);
+        batch_part_scatter(A, false, A.bpage_start_d, bsd[n], false, G.part_scatter_direct, ps);
         batch_part_count(A, true, npp, G.part_count, ps);
         scan_exclusive_u64(A.pcount, const_cast<uint64_t *>(A.poff), nparts, ps, T.scan_tmp, T.scan_elems);
         batch_copy_u64(A.poff, A.pcur, nparts + 1, ps);
-        batch_part_scatter(A, true, npp, G.part_scatter, ps);
+        batch_part_scatter(A, true, A.ppage_start_s, pss[n], true, G.part_scatter, ps);
+        batch_part_scatter(A, true, A.ppage_start_d, psd[n], false, G.part_scatter_direct, ps);
         batch_build(A, G.build, ps);
         batch_units(A, ps);
         scan_exclusive_u64(A.ucount, const_cast<uint64_t *>(A.uoff), nparts, ps, T.scan_tmp, T.scan_elems);

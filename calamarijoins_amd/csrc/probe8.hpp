@@ -49,7 +49,8 @@ __device__ __forceinline__ uint32_t rank_slots(uint32_t pred, uint32_t rank[S], 
 // all 8 are in flight together.  P: a generic or a global (glb) page pointer.
 template <class P>
 __device__ __forceinline__ uint32_t load_keys8(P p, uint32_t n, uint32_t nv, bool dense, uint32_t t0,
-                                               uint32_t *running, int32_t key[kRPT], uint32_t *lds32) {
+                                               uint32_t *running, int32_t key[kRPT], uint32_t *lds32,
+                                               uint32_t *vi_out = nullptr) {
     const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
     uint32_t valid = 0;
     uint32_t vi[kRPT];
@@ -82,6 +83,10 @@ __device__ __forceinline__ uint32_t load_keys8(P p, uint32_t n, uint32_t nv, boo
     for (uint32_t j = 0; j < kRPT; ++j) {
         key[j] = ld32(p + 4 + 4 * (size_t)(vi[j] < nv ? vi[j] : vlast));
         if (vi[j] >= nv) valid &= ~(1u << j);  // malformed bitmap: stay inside the value area
+    }
+    if (vi_out) {  // the value index of each key (its position among the page's non-NULL values)
+#pragma unroll
+        for (uint32_t j = 0; j < kRPT; ++j) vi_out[j] = vi[j];
     }
     return valid;
 }
