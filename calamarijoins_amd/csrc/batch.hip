@@ -453,19 +453,19 @@ struct BCol {
     uint8_t *out;
 };
 
-// Output row o of column `col`: the join key, build tuple t of the
-// partition (plan-local index g_car0 + t of the slot-ordered carried
-// columns) or probe tuple pt (plan-local index of the partition-ordered
-// carried columns), stored at slot o % R of page o / R.  Lanes of a wave
-// write ascending rows, so the stores stay nearly coalesced.
-template <bool LDS>
+// Output row o of column `col` (width W, source SRC: 0 the join key, 1 the
+// build side, 2 the probe side): the key, build tuple t of the partition
+// (plan-local index g_car0 + t of the slot-ordered carried columns) or
+// probe tuple pt (plan-local index of the partition-ordered carried
+// columns), stored at slot o % R of page o / R.
+template <int W, int SRC, bool LDS>
 __device__ __forceinline__ void bx_put(const BPlan &pl, const BCol &col, const BuildPart &bp, uint32_t o,
                                        int32_t key, uint32_t t, uint32_t pt, uint32_t *page_nulls) {
-    if (col.w == 4) {
+    if constexpr (W == 4) {
         int32_t v;
-        if (col.src == 0) {
+        if constexpr (SRC == 0) {
             v = key;  // equal keys: the build key is the probe key
-        } else if (col.src == 1) {
+        } else if constexpr (SRC == 1) {
             if constexpr (LDS) v = lds(reinterpret_cast<const int32_t *>(bp.l_dyn + bp.coff(col.cj)))[t];
             else v = glb(reinterpret_cast<const int32_t *>(pl.bslot_car[col.cj]))[bp.g_car0 + t];
         } else {
@@ -478,7 +478,7 @@ __device__ __forceinline__ void bx_put(const BPlan &pl, const BCol &col, const B
     } else {
         uint64_t v;
         uint8_t b;
-        if (col.src == 1) {
+        if constexpr (SRC == 1) {
             if constexpr (LDS) {
                 v = lds(reinterpret_cast<const uint64_t *>(bp.l_dyn + bp.coff(col.cj)))[t];
                 b = lds(bp.l_dyn + bp.voff(col.cj))[t];
@@ -494,6 +494,21 @@ __device__ __forceinline__ void bx_put(const BPlan &pl, const BCol &col, const B
         uint8_t *page = col.out + (size_t)pgi * kPage;
         if (b) glb(reinterpret_cast<uint64_t *>(page + 8))[slot] = v;
         else bx_null(page, pgi, slot, col.rows, kRowsI64, page_nulls + col.pg0 + pgi);
+    }
+}
+
+// One column of a match window: rows ob + i for entries i of win (probe row
+// in the tile << kMatchTBits | build tuple), consecutive rows on
+// consecutive lanes; the column's width and source are template
+// parameters, so the loop holds no per-entry branch on them.
+template <int W, int SRC, bool LDS>
+__device__ __forceinline__ void bx_emit_col(const BPlan &pl, const BCol &col, const BuildPart &bp, const uint32_t *win,
+                                            uint32_t cnt, uint32_t ob, uint32_t pt0, uint32_t *page_nulls) {
+#pragma unroll 4
+    for (uint32_t i = threadIdx.x; i < cnt; i += kWG) {
+        const uint32_t en = win[i], t = en & kMatchT;
+        bx_put<W, SRC, LDS>(pl, col, bp, ob + i, SRC == 0 ? bp_key<LDS>(bp, t) : 0, t, pt0 + (en >> kMatchTBits),
+                            page_nulls);
     }
 }
 
@@ -584,14 +599,17 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
                 }
                 __syncthreads();
                 const uint32_t cnt = w1 - w0;
+                const uint32_t ob = (uint32_t)(base + w0), pt0 = (uint32_t)(i0 - pbase);
 #pragma unroll 1
                 for (uint32_t c = 0; c < bo->nout; ++c) {
                     const BCol col{bo->src[c], bo->j[c], bo->w[c], bo->rows, bo->pg0[c], bo->out[c]};
-#pragma unroll 4
-                    for (uint32_t i = threadIdx.x; i < cnt; i += kWG) {
-                        const uint32_t en = win[i], t = en & kMatchT;
-                        const uint32_t pt = (uint32_t)(i0 - pbase) + (en >> kMatchTBits);
-                        bx_put<LDS>(pl, col, bp, (uint32_t)(base + w0 + i), bp_key<LDS>(bp, t), t, pt, page_nulls);
+                    if (col.w == 4) {
+                        if (col.src == 0) bx_emit_col<4, 0, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                        else if (col.src == 1) bx_emit_col<4, 1, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                        else bx_emit_col<4, 2, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                    } else {  // 8-byte columns are never keys
+                        if (col.src == 1) bx_emit_col<8, 1, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                        else bx_emit_col<8, 2, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
                     }
                 }
                 __syncthreads();
