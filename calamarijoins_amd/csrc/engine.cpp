@@ -1186,9 +1186,23 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
                       rids ? kc->num_rows * 8ull : kc->num_rows * (4ull + a.stride + a.stride) + kc->num_rows / 8);
         direct_build(a, st);
     }
-    uint32_t *h = reinterpret_cast<uint32_t *>(readback());
-    CJ_HIP(hipMemcpyAsync(h, st4, 16, hipMemcpyDeviceToHost, st));
-    CJ_HIP(hipStreamSynchronize(st));
+    uint32_t hk[4];
+    bool known;
+    {
+        std::lock_guard<std::mutex> g(kc->mu);
+        known = kc->dkey_known;
+        if (known) hk[0] = (uint32_t)kc->dkey_lo, hk[1] = (uint32_t)kc->dkey_hi, hk[2] = kc->dkey_keyed, hk[3] = kc->dkey_present;
+    }
+    if (!known) {
+        uint32_t *rb = reinterpret_cast<uint32_t *>(readback());
+        CJ_HIP(hipMemcpyAsync(rb, st4, 16, hipMemcpyDeviceToHost, st));
+        CJ_HIP(hipStreamSynchronize(st));
+        std::copy(rb, rb + 4, hk);
+        std::lock_guard<std::mutex> g(kc->mu);
+        kc->dkey_lo = (int32_t)hk[0], kc->dkey_hi = (int32_t)hk[1], kc->dkey_keyed = hk[2], kc->dkey_present = hk[3];
+        kc->dkey_known = true;
+    }
+    const uint32_t *h = hk;
     const int32_t lo = (int32_t)h[0], hi = (int32_t)h[1];
     if (lo > hi) return false;  // no non-NULL key
     const uint64_t range = (uint64_t)((int64_t)hi - (int64_t)lo) + 1;

@@ -75,7 +75,14 @@ struct DevColumn {
     uint32_t dense_rows = 0;          // INT32 without NULLs, this many rows on every page but
                                       // a shorter last one: rows are read in place (0: no)
     bool loaded = false;        // pages + row_start present
-    std::mutex mu;              // guards lazy decode
+    // key statistics of a direct-map build over this column, recorded by the
+    // first such build and reused by later ones (a registered column's
+    // values do not change), so they skip the build's read-back: the
+    // non-NULL keys' min and max, the keyed rows and the slots they filled
+    bool dkey_known = false;
+    int32_t dkey_lo = 0, dkey_hi = 0;
+    uint32_t dkey_keyed = 0, dkey_present = 0;
+    std::mutex mu;              // guards lazy decode and the key statistics
 };
 
 struct DevTable {
