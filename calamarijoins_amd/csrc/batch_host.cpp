@@ -344,6 +344,17 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
             ctx.batch_templates.erase(it);  // in use: not shared with a concurrent batch
         }
     }
+    // a template taken from (or not yet in) the context's cache goes back to
+    // it at the end; if anything throws before that, its blocks and graph
+    // are released here
+    struct TplGuard {
+        std::shared_ptr<BatchTemplate> &t;
+        DevicePool &pool;
+        bool armed = true;
+        ~TplGuard() {
+            if (armed && t) t->release(pool);
+        }
+    } tpl_guard{tpl, pool};
     bool fresh = false;
     if (!tpl) {
         fresh = true;
@@ -659,6 +670,7 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         auto &slot = ctx.batch_templates[key];
         if (slot) slot->release(pool);
         slot = tpl;
+        tpl_guard.armed = false;
         // keep the workspace of a few recent shapes only
         while (ctx.batch_templates.size() > 4) {
             auto it = ctx.batch_templates.begin();

@@ -236,3 +236,35 @@ def test_gpu_batch_template_carried_page_counts(engine):
     finally:
         for t in ids:
             engine.unregister(t)
+
+
+def test_gpu_batch_config_beyond_lds_fails_cleanly(engine):
+    """2^8 slots a partition over a 4M-row build side (load factor 4) is 65536
+    partitions: the scatter's per-partition LDS arrays cannot fit a
+    workgroup, so the call fails before any launch; the tuned configurations
+    then run the same batch."""
+    b = workloads.generate(4_000_000, 61, [workloads.col(workloads.GEN_UNIFORM, "INT32", 1, 0.0, lo=0, hi=4_000_000),
+                                           workloads.col(workloads.GEN_I32_FULL, "INT32", 2)])
+    # the smaller side builds: the probe side must be the larger one
+    p = workloads.generate(5_000_000, 62, [workloads.col(workloads.GEN_UNIFORM, "INT32", 3, 0.0, lo=0, hi=4_000_000),
+                                        workloads.col(workloads.GEN_I32_FULL, "INT32", 4)])
+    plan = workloads.c4_plan(b, p)
+    ids = [engine.register(b), engine.register(p)]
+    try:
+        exp = engine.execute(plan)
+        want = cjtest.result_fingerprint(exp)
+        exp.release()
+        engine.batch_config(4, 8, 8192, 0, 0)
+        try:
+            in_use = engine.memory()["in_use"]
+            with pytest.raises(abi.CJError, match="LDS"):
+                engine.execute_batch([plan])
+            assert engine.memory()["in_use"] == in_use  # the failed call's workspace was released
+        finally:
+            engine.batch_config()
+        res, st = engine.execute_batch([plan])
+        assert st.plans_batched == 1 and cjtest.result_fingerprint(res[0]) == want
+        res[0].release()
+    finally:
+        for t in ids:
+            engine.unregister(t)
