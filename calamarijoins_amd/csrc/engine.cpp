@@ -16,6 +16,8 @@
 // an intermediate inserts them as INT32_MIN (hashtable.h:357), and INT32
 // output values equal to INT32_MIN come out NULL (page_builders.h:123).
 #include "engine.hpp"
+
+#include <atomic>
 #include "fused.hpp"
 #include "pagemat.hpp"
 #include "filter.hpp"
@@ -153,6 +155,11 @@ struct TimedScope {
 };
 
 // ========================================================== phase clock
+uint64_t next_column_uid() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
+}
+
 hipEvent_t Context::take_event() {
     {
         std::lock_guard<std::mutex> g(emu);
@@ -970,7 +977,28 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
                 multi_scan_u64(jobs, 3, tmp<uint64_t>(multi_scan_blocks(jobs, 3) + 1), st);
             }
             po_cnt = cnt;
-            total = read_u64(off + units);
+            // both key columns are base-table columns and every probe row
+            // matches at most once: the count is a property of the two
+            // columns, read back once and kept on the probe key column
+            DevColumn *pkc = column(P.slots[0].table, P.cols[pattr].col);
+            const uint64_t buid = column(B.slots[0].table, B.cols[battr].col)->uid;
+            bool known = false;
+            {
+                std::lock_guard<std::mutex> g(pkc->mu);
+                for (auto &mt : pkc->match_totals)
+                    if (mt.first == buid) total = mt.second, known = true;
+            }
+            // (trusted only where it leads to the page-aligned writers, whose
+            // writes are bounded by their blocks; other strategies read the
+            // count back)
+            if (known && !(total && 2 * total >= probe_rows)) known = false;
+            if (!known) {
+                total = read_u64(off + units);
+                std::lock_guard<std::mutex> g(pkc->mu);
+                bool have = false;
+                for (auto &mt : pkc->match_totals) have |= mt.first == buid;
+                if (!have) pkc->match_totals.emplace_back(buid, total);
+            }
         } else if (aligned) {
             scan(cnt, off, units);
             uint64_t *h = readback();
@@ -1323,6 +1351,7 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
     const uint32_t kp = a.key.num_pages;
     std::vector<const uint64_t *> count_at;  // device page count per output column
     std::vector<uint32_t> count_col;
+    std::vector<uint64_t> bounds;  // pages of each build-side column's block
     // ---- build side
     BuildPagesArgs ba{};
     ba.key = a.key;
@@ -1348,6 +1377,8 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
         c.rec_offset = direct_field[i].first;
         c.rec_bit = direct_field[i].second;
         c.out = out_pages(i, bound);
+        c.cap_pages = bound;
+        bounds.push_back(bound);
         bbytes += total * (uint64_t)width + total / 8;
         out_types[i] = type;
         out_cols[i] = {c.out, {0u, type}};
@@ -1396,7 +1427,11 @@ void Exec::aligned_root(const cj_node_desc &n, const Rel &B, const Rel &P, uint3
         probe_filter(f, st);
     }
     const std::vector<uint64_t> np = read_u64s(count_at);
-    for (size_t k = 0; k < nbuild; ++k) out_cols[count_col[k]].second.first = (uint32_t)np[k];
+    for (size_t k = 0; k < nbuild; ++k) {
+        if (np[k] > bounds[k])
+            throw DeviceError("a join result outgrew its page block: a registered table's pages changed");
+        out_cols[count_col[k]].second.first = (uint32_t)np[k];
+    }
     for (size_t k = nbuild; k < np.size(); ++k) {
         DevColumn *c = probe_src[k - nbuild];
         const uint32_t i = count_col[k];

@@ -60,6 +60,8 @@ class PinnedPool {
 };
 
 // A base-table column in HBM.
+uint64_t next_column_uid();
+
 struct DevColumn {
     int32_t type = 0;
     uint32_t num_pages = 0;
@@ -82,6 +84,11 @@ struct DevColumn {
     bool dkey_known = false;
     int32_t dkey_lo = 0, dkey_hi = 0;
     uint32_t dkey_keyed = 0, dkey_present = 0;
+    // as a probe key column: the match count of a page-aligned join against a
+    // direct map of each build key column (by that column's uid), for the
+    // same reason
+    std::vector<std::pair<uint64_t, uint64_t>> match_totals;
+    const uint64_t uid = next_column_uid();  // unique over the process (a freed column's address is reused)
     std::mutex mu;              // guards lazy decode and the key statistics
 };
 

@@ -621,6 +621,13 @@ __global__ __launch_bounds__(kWG) void k_build_pages(BuildPagesArgs a, uint32_t 
     CJ_G uint8_t *base[NC];
     ColPick pick[NC];
     OutCol oc[NC];
+    // a block sized from a match count that no longer holds must not be
+    // overrun: a key page whose pages would pass a block's end writes
+    // nothing (the host compares the page counts with the blocks afterwards)
+    for (uint32_t k = 0; k < nc; ++k) {
+        const BuildPageCol &col = a.cols[c_lo + k];
+        if ((col.width == 8 ? a.p8_off : a.p4_off)[pg + 1] > col.cap_pages) return;
+    }
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
         const BuildPageCol &col = a.cols[c_lo + (k < (int)nc ? k : 0)];

@@ -43,6 +43,7 @@ struct BuildPageCol {
     int32_t is_int32;  // INT32_MIN -> NULL
     uint32_t rec_offset, rec_bit;
     uint8_t *out;      // page block of this column
+    uint64_t cap_pages;  // pages the block holds: a key page whose pages would pass it writes nothing
 };
 
 struct BuildPagesArgs {

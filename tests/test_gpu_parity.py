@@ -84,6 +84,30 @@ def test_gpu_c2_scaled_bit_exact(engine, flags):
     assert got.num_rows == int(np.sum(pk < 200_000))
 
 
+def test_gpu_repeated_resident_plans_reuse_key_statistics(engine):
+    """A plan over registered tables run again: the direct maps' key
+    statistics and the page-aligned join's match count are taken from the
+    first run (no read-back); every run equals the oracle, on the aligned
+    path (C2 shape) and through a multi-join plan (C3 shape, direct maps of
+    row ids over the dimension keys)."""
+    b, p = workloads.c2_tables(seed=77, build_rows=150_000, probe_rows=2_000_000)
+    c2 = workloads.c2_plan(b, p)
+    f, ds, _ = workloads.c3_tables(fact_rows=workloads.C3_FACT // 100, dims=tuple(d // 100 for d in workloads.C3_DIMS))
+    c3 = workloads.c3_plan(f, ds)
+    ids = [engine.register(t) for t in [b, p, f] + list(ds)]
+    try:
+        for plan in (c2, c3):
+            exp = cjtest.OracleRun(plan, threads=8)
+            for _ in range(3):
+                same_multiset(engine.execute(plan), exp)
+                r = engine.execute(plan, flags=abi.CJ_EXEC_OUTPUT_DEVICE)
+                r.to_host()
+                same_multiset(r, exp)
+    finally:
+        for t in ids:
+            engine.unregister(t)
+
+
 @pytest.mark.parametrize("scale", [100, 10])
 def test_gpu_c3_star_pipeline_bit_exact(engine, scale):
     """Left-deep 4-join star with VARCHAR (incl. long strings) through
