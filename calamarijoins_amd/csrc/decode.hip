@@ -242,20 +242,28 @@ void page_row_counts(const uint8_t *pages, uint32_t num_pages, uint64_t *counts,
 }
 
 // ------------------------------------------------------------ dense decode
-// One wave per page, no barriers.  For each batch of 8 chunks of 64 rows,
-// the bitmap bytes (sparse pages) and then the values are loaded with
-// clamped in-page addresses, all in flight together; value indices of a
-// sparse page come from ballots of the row bits.  Validity words wholly
-// inside a page whose first row is 32-aligned are stored; the others
-// (shared with a neighbouring page) are OR-ed with atomics.
+// Segments of 512 rows (8 chunks of 64), no barriers: a page's segments go
+// to as many waves as a dense page has segments (4 for INT32, 2 for 8-byte
+// values), each wave taking every kDecodeWaves-th segment, so a column of
+// full pages runs one segment per wave (the loads of a segment are two
+// dependent round trips; one wave walking a whole page would chain eight).
+// A wave starts a segment of a sparse page from the set bits of the bitmap
+// bytes before it.  The bitmap bytes of the segment, then its values, are
+// loaded with clamped in-page addresses, all in flight together; value
+// indices come from ballots of the row bits.  Validity words wholly inside a
+// page whose first row is 32-aligned are stored; the others (shared with a
+// neighbouring page) are OR-ed with atomics.
 constexpr uint32_t kDecodeBatch = 8;
 
 template <int W>
 __global__ __launch_bounds__(kWG) void k_decode_fixed(const uint8_t *pages, uint32_t num_pages,
                                                       const uint64_t *row_start, void *values, uint32_t *valid_bits) {
     using T = typename std::conditional<W == 4, uint32_t, uint64_t>::type;
+    constexpr uint32_t kSeg = 64 * kDecodeBatch;
+    constexpr uint32_t kDecodeWaves = W == 4 ? 4 : 2;  // segments of a dense page
     const uint32_t lane = lane_id();
-    const uint32_t pg = blockIdx.x * (kWG / 64) + wave_id();
+    const uint32_t gw = blockIdx.x * (kWG / 64) + wave_id();
+    const uint32_t pg = gw / kDecodeWaves;
     if (pg >= num_pages) return;
     const uint8_t *p = pages + (size_t)pg * kPage;
     const uint32_t n = ld16(p), nv = ld16(p + 2);
@@ -267,8 +275,14 @@ __global__ __launch_bounds__(kWG) void k_decode_fixed(const uint8_t *pages, uint
     const uint64_t lt = lanemask_lt();
     const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
     const bool aligned = (base & 31) == 0;
-    uint32_t vbase = 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kDecodeBatch) {
+    for (uint32_t c0 = (gw % kDecodeWaves) * kSeg; c0 < n; c0 += kDecodeWaves * kSeg) {
+        uint32_t vbase = 0;  // values before the segment (sparse pages)
+        if (!dense) {
+            uint32_t pc = 0;
+            for (uint32_t b = lane; b < c0 / 8; b += 64) pc += (uint32_t)__popc(bm[b]);
+            for (int o = 32; o > 0; o >>= 1) pc += __shfl_xor(pc, o, 64);
+            vbase = pc;
+        }
         uint32_t okbits = 0;
         uint32_t vi[kDecodeBatch];
         if (dense) {
@@ -325,7 +339,8 @@ __global__ __launch_bounds__(kWG) void k_decode_fixed(const uint8_t *pages, uint
 void decode_fixed(const uint8_t *pages, uint32_t num_pages, const uint64_t *row_start, int width,
                   void *values, uint32_t *valid_bits, hipStream_t s) {
     if (!num_pages) return;
-    const uint32_t grid = (num_pages + kWG / 64 - 1) / (kWG / 64);
+    const uint64_t waves = (uint64_t)num_pages * (width == 4 ? 4 : 2);
+    const uint32_t grid = (uint32_t)((waves + kWG / 64 - 1) / (kWG / 64));
     if (width == 4) k_decode_fixed<4><<<grid, kWG, 0, s>>>(pages, num_pages, row_start, values, valid_bits);
     else k_decode_fixed<8><<<grid, kWG, 0, s>>>(pages, num_pages, row_start, values, valid_bits);
     CJ_HIP(hipGetLastError());

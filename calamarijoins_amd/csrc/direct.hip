@@ -124,9 +124,11 @@ void key_minmax(const int32_t *keys, const uint32_t *valid, const uint8_t *key_p
     CJ_HIP(hipGetLastError());
 }
 
+__device__ __forceinline__ int32_t direct_lo(const DirectBuildArgs &a) { return a.known ? a.lo : a.minmax[0]; }
+
 // Key range of the build side, or 0 when it has no key or exceeds max_range.
 __device__ __forceinline__ uint64_t direct_range(const DirectBuildArgs &a) {
-    const int32_t lo = a.minmax[0], hi = a.minmax[1];
+    const int32_t lo = direct_lo(a), hi = a.known ? a.hi : a.minmax[1];
     if (lo > hi) return 0;
     const uint64_t r = (uint64_t)((int64_t)hi - (int64_t)lo) + 1;
     return r <= a.max_range ? r : 0;
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(kWG) void k_direct_scatter(DirectBuildArgs a) {
         if (a.keys && !valid_bit(a.key_valid, i)) continue;
         ++cnt;
         const int32_t key = a.keys ? a.keys[i] : dense_page_value(a.key_pages, a.key_rows, (uint32_t)i);
-        const uint64_t slot = (uint64_t)((int64_t)key - (int64_t)a.minmax[0]);
+        const uint64_t slot = (uint64_t)((int64_t)key - (int64_t)direct_lo(a));
         uint4 q[NQ];
 #pragma unroll
         for (int c = 0; c < NQ; ++c) q[c] = make_uint4(0, 0, 0, 0);
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(kWG) void k_rid_scatter(DirectBuildArgs a) {
         if (a.keys && !valid_bit(a.key_valid, i)) continue;
         ++cnt;
         const int32_t key = a.keys ? a.keys[i] : dense_page_value(a.key_pages, a.key_rows, (uint32_t)i);
-        rids[(uint64_t)((int64_t)key - (int64_t)a.minmax[0])] = (uint32_t)i;
+        rids[(uint64_t)((int64_t)key - (int64_t)direct_lo(a))] = (uint32_t)i;
     }
     for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
     if (lane_id() == 0) red[wave_id()] = cnt;
@@ -314,10 +316,20 @@ __global__ __launch_bounds__(kWG) void k_rid_present(DirectBuildArgs a) {
 static void rid_build(const DirectBuildArgs &a, hipStream_t s) {
     if (a.stride != 4 || a.rid_offset != 0) throw InvalidError("bad row-id map layout");
     const uint64_t zg = ((a.max_range + 3) / 4 + kWG - 1) / kWG;
-    k_rid_zero<<<(unsigned)(zg < 8192 ? zg : 8192), kWG, 0, s>>>(a);
+    if (!(a.known && a.full)) k_rid_zero<<<(unsigned)(zg < 8192 ? zg : 8192), kWG, 0, s>>>(a);
     uint64_t sg = (a.rows + kWG - 1) / kWG;
     sg = sg < kDirectParts ? sg : kDirectParts;
     k_rid_scatter<<<(unsigned)sg, kWG, 0, s>>>(a);
+    if (a.known) {
+        CJ_HIP(hipGetLastError());
+        if (!a.full) {
+            const uint64_t steps = (a.max_range + 63) / 64;
+            uint64_t pg = (steps + kWG / 64 - 1) / (kWG / 64);
+            k_rid_present<<<(unsigned)(pg < kDirectParts ? pg : kDirectParts), kWG, 0, s>>>(a);
+            CJ_HIP(hipGetLastError());
+        }
+        return;
+    }
     const uint64_t steps = (a.max_range + 63) / 64;
     uint64_t pg = (steps + kWG / 64 - 1) / (kWG / 64);
     pg = pg < kDirectParts ? pg : kDirectParts;
@@ -332,7 +344,7 @@ void direct_build(const DirectBuildArgs &a, hipStream_t s) {
     if (a.stride % 16 || a.stride == 0 || a.stride > 80) throw InvalidError("bad direct-map record stride");
     const uint64_t chunks = a.max_range * (a.stride / 16);
     const uint64_t zg = (chunks + kWG - 1) / kWG;
-    k_direct_zero<<<(unsigned)(zg < 8192 ? zg : 8192), kWG, 0, s>>>(a);
+    if (!(a.known && a.full)) k_direct_zero<<<(unsigned)(zg < 8192 ? zg : 8192), kWG, 0, s>>>(a);
     uint64_t sg = (a.rows + kWG - 1) / kWG;
     sg = sg < kDirectParts ? sg : kDirectParts;
     switch (a.stride / 16) {
@@ -345,8 +357,8 @@ void direct_build(const DirectBuildArgs &a, hipStream_t s) {
     const uint64_t steps = (a.max_range + 63) / 64;
     uint64_t pg = (steps + kWG / 64 - 1) / (kWG / 64);
     pg = pg < kDirectParts ? pg : kDirectParts;
-    k_direct_presence<<<(unsigned)pg, kWG, 0, s>>>(a);
-    k_direct_finish<<<1, kWG, 0, s>>>(a, (uint32_t)sg, (uint32_t)pg);
+    if (!(a.known && a.full)) k_direct_presence<<<(unsigned)pg, kWG, 0, s>>>(a);
+    if (!a.known) k_direct_finish<<<1, kWG, 0, s>>>(a, (uint32_t)sg, (uint32_t)pg);
     CJ_HIP(hipGetLastError());
 }
 

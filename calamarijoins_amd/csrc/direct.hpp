@@ -6,7 +6,7 @@ namespace cj {
 
 constexpr uint32_t kDirectMaxFields = 8;
 constexpr uint32_t kDirectPresentBit = 31;  // flags bit: the slot holds a build row
-constexpr uint32_t kDirectParts = 1024;     // workgroup partials of each build pass
+constexpr uint32_t kDirectParts = 4096;     // workgroup partials of each build pass
 constexpr uint32_t kMinmaxParts = 1024;     // workgroup partials of key_minmax
 
 struct DirectField {
@@ -36,6 +36,13 @@ struct DirectBuildArgs {
     DirectField fields[kDirectMaxFields];
     int32_t rid_offset;  // >= 0: the build row id is stored at this record offset; with no
                          // fields the records are the row ids alone (stride 4, offset 0)
+    // Statistics of an earlier build over the same key column.  known != 0:
+    // the key range is [lo, hi] (minmax is not read, key_minmax is not run)
+    // and the keys are unique.  full != 0 as well (every key of the range
+    // present): the build is the scatter alone, since it writes every slot
+    // of the range and the probes skip the bitmap.
+    uint32_t known, full;
+    int32_t lo, hi;
 };
 
 // The table as the probe sees it.
@@ -58,6 +65,8 @@ void key_minmax(const int32_t *keys, const uint32_t *valid, const uint8_t *key_p
 // derive the presence bitmap from the records (for a map of row ids alone,
 // from the slots a row id filled).  Nothing is written when the
 // key range exceeds max_range.  Keys are unique iff counters[0] == counters[1].
+// With known statistics the counters are not written, and a full range is
+// neither zeroed nor given a bitmap.
 void direct_build(const DirectBuildArgs &a, hipStream_t s);
 
 }  // namespace cj

@@ -1196,9 +1196,35 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
     // a map of row ids is 16 bytes a slot with nothing to fill but the keyed
     // rows: sparser keys (filtered primary keys) still take it
     if (rids) a.max_range = std::min<uint64_t>(16 * kc->num_rows + 1024, uint64_t(1) << 25);
+    // statistics recorded by an earlier build over this key column (a
+    // registered column's keys do not change): the key range, and whether
+    // the keys are unique and fill it, are known before anything runs
+    uint32_t hk[4];
+    bool known, counted = false;
+    {
+        std::lock_guard<std::mutex> g(kc->mu);
+        known = kc->dkey_known;
+        counted = kc->dkey_counted;
+        if (known) hk[0] = (uint32_t)kc->dkey_lo, hk[1] = (uint32_t)kc->dkey_hi, hk[2] = kc->dkey_keyed, hk[3] = kc->dkey_present;
+    }
+    if (known) {
+        const int32_t lo = (int32_t)hk[0], hi = (int32_t)hk[1];
+        if (lo > hi) return false;
+        const uint64_t range = (uint64_t)((int64_t)hi - (int64_t)lo) + 1;
+        if (range > a.max_range) return false;
+        if (counted) {
+            if (hk[2] != hk[3]) return false;
+            a.known = 1;
+            a.full = hk[3] == range;
+            a.lo = lo;
+            a.hi = hi;
+        } else {
+            known = false;  // the earlier build's map was too small to count: this one counts
+        }
+    }
     uint32_t *st4 = tmp<uint32_t>(4);  // min, max, keyed rows, present slots (written by kernels)
     int32_t *mm = reinterpret_cast<int32_t *>(st4);
-    {
+    if (!known) {
         TimedScope ts(timer, "key_minmax", kc->num_rows * 4 + kc->num_rows / 8);
         key_minmax(a.keys, a.key_valid, a.key_pages, a.key_rows, kc->num_rows, mm, tmp<int32_t>(2 * kMinmaxParts), st);
     }
@@ -1214,21 +1240,17 @@ bool Exec::try_direct(const cj_node_desc &n, const Rel &B, uint32_t battr, uint3
                       rids ? kc->num_rows * 8ull : kc->num_rows * (4ull + a.stride + a.stride) + kc->num_rows / 8);
         direct_build(a, st);
     }
-    uint32_t hk[4];
-    bool known;
-    {
-        std::lock_guard<std::mutex> g(kc->mu);
-        known = kc->dkey_known;
-        if (known) hk[0] = (uint32_t)kc->dkey_lo, hk[1] = (uint32_t)kc->dkey_hi, hk[2] = kc->dkey_keyed, hk[3] = kc->dkey_present;
-    }
     if (!known) {
         uint32_t *rb = reinterpret_cast<uint32_t *>(readback());
         CJ_HIP(hipMemcpyAsync(rb, st4, 16, hipMemcpyDeviceToHost, st));
         CJ_HIP(hipStreamSynchronize(st));
         std::copy(rb, rb + 4, hk);
+        const int32_t lo = (int32_t)hk[0], hi = (int32_t)hk[1];
         std::lock_guard<std::mutex> g(kc->mu);
-        kc->dkey_lo = (int32_t)hk[0], kc->dkey_hi = (int32_t)hk[1], kc->dkey_keyed = hk[2], kc->dkey_present = hk[3];
+        kc->dkey_lo = lo, kc->dkey_hi = hi, kc->dkey_keyed = hk[2], kc->dkey_present = hk[3];
         kc->dkey_known = true;
+        // the counters count only when the map held the key range
+        kc->dkey_counted = lo <= hi && (uint64_t)((int64_t)hi - (int64_t)lo) + 1 <= a.max_range;
     }
     const uint32_t *h = hk;
     const int32_t lo = (int32_t)h[0], hi = (int32_t)h[1];

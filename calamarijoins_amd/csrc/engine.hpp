@@ -79,9 +79,11 @@ struct DevColumn {
     bool loaded = false;        // pages + row_start present
     // key statistics of a direct-map build over this column, recorded by the
     // first such build and reused by later ones (a registered column's
-    // values do not change), so they skip the build's read-back: the
-    // non-NULL keys' min and max, the keyed rows and the slots they filled
-    bool dkey_known = false;
+    // values do not change), so they skip the key range pass and the
+    // build's read-back: the non-NULL keys' min and max, and, when that
+    // build's map held the range (dkey_counted), the keyed rows and the
+    // slots they filled
+    bool dkey_known = false, dkey_counted = false;
     int32_t dkey_lo = 0, dkey_hi = 0;
     uint32_t dkey_keyed = 0, dkey_present = 0;
     // as a probe key column: the match count of a page-aligned join against a

@@ -108,6 +108,47 @@ def test_gpu_repeated_resident_plans_reuse_key_statistics(engine):
             engine.unregister(t)
 
 
+def test_gpu_registered_key_statistics_recount_for_a_wider_map(engine):
+    """Key statistics recorded by a build whose direct map could not hold the
+    key range (a root join: a map of records, at most ~2x the build rows)
+    carry no duplicate count; a later build with a wider map (a non-root
+    join: a map of row ids) over the same registered key column counts the
+    keys itself, finds the duplicates and takes the hash table.  Both plans,
+    run twice each, equal the oracle."""
+    rng = random.Random(777)
+    keys = rng.sample(range(0, 20000), 3800)
+    keys += [rng.choice(keys) for _ in range(200)]  # duplicates
+    keys += [0, 19999]
+    rng.shuffle(keys)
+    d_rows = [[k, rng.randrange(-500, 500)] for k in keys]
+    p_rows = [[rng.randrange(0, 20000), i] for i in range(9000)]
+    e_rows = [[i, rng.randrange(100)] for i in range(9000)]
+    d, p, e = (cjtest.table_from_rows(["INT32"] * 2, r) for r in (d_rows, p_rows, e_rows))
+    root = cjtest.Plan()
+    for t in (p, d):
+        root.new_input(t)
+    root.new_scan_node(0, [(0, "INT32"), (1, "INT32")])
+    root.new_scan_node(1, [(0, "INT32"), (1, "INT32")])
+    root.root = root.new_join_node(False, 0, 1, 0, 0, [(1, "INT32"), (3, "INT32")])
+    two = cjtest.Plan()
+    for t in (p, d, e):
+        two.new_input(t)
+    two.new_scan_node(0, [(0, "INT32"), (1, "INT32")])
+    two.new_scan_node(1, [(0, "INT32"), (1, "INT32")])
+    two.new_scan_node(2, [(0, "INT32"), (1, "INT32")])
+    j1 = two.new_join_node(False, 0, 1, 0, 0, [(1, "INT32"), (3, "INT32")])
+    two.root = two.new_join_node(True, j1, 2, 0, 0, [(0, "INT32"), (1, "INT32"), (3, "INT32")])
+    ids = [engine.register(t) for t in (d, p, e)]
+    try:
+        for plan in (root, two, root, two):
+            exp = cjtest.OracleRun(plan, threads=8)
+            assert exp.num_rows > 1000
+            same_multiset(engine.execute(plan), exp)
+    finally:
+        for t in ids:
+            engine.unregister(t)
+
+
 @pytest.mark.parametrize("scale", [100, 10])
 def test_gpu_c3_star_pipeline_bit_exact(engine, scale):
     """Left-deep 4-join star with VARCHAR (incl. long strings) through
