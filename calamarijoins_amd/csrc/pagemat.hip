@@ -91,6 +91,93 @@ __global__ __launch_bounds__(kWG) void k_probe_count_match(ProbeArgs a, uint64_t
 // are stored, the two edge words OR-ed with device atomics.
 constexpr uint32_t kCountBatch = 32;
 
+// Segment-local row mask msk (32-bit word k: rows 32k .. 32k + 31 of the
+// nseg rows from global row g0) into the global bitmap: interior words are
+// stored, the two edge words (shared with a neighbouring page) OR-ed.
+__device__ __forceinline__ void store_segment_bits(const uint32_t *msk, uint32_t *match_bits, uint64_t g0,
+                                                   uint32_t nseg, uint32_t lane) {
+    const uint32_t sh = (uint32_t)(g0 & 31);
+    const uint32_t nwords = (sh + nseg + 31) >> 5;
+    for (uint32_t i = lane; i < nwords; i += 64) {
+        // bits [32 i - sh, 32 i - sh + 32) of the segment's row mask
+        uint32_t w;
+        if (i == 0) {
+            w = msk[0] << sh;
+        } else {
+            const uint32_t st = 32 * i - sh, d = st >> 5, o = st & 31;
+            w = o ? (msk[d] >> o) | (msk[d + 1] << (32 - o)) : msk[d];
+        }
+        const bool edge = (i == 0 && sh) || (i == nwords - 1 && ((sh + nseg) & 31));
+        uint32_t *dst = match_bits + (g0 >> 5) + i;
+        if (edge) {
+            if (w) atomicOr(dst, w);
+        } else {
+            *dst = w;
+        }
+    }
+}
+
+// A key page without NULL keys and of at most kTile rows, whole: lane l
+// takes rows 4l .. 4l + 3 of each 256-row block with one 16-byte load (all
+// eight blocks in flight together); the four match bits of a lane are
+// merged into 32-bit row-mask words across groups of eight lanes.  Returns
+// the page's matches; msk receives the row mask.
+constexpr uint32_t kDenseBlk = kTile / 256;  // 256-row blocks of a page count_dense_page takes
+
+// The 16-byte key loads of count_dense_page, issued before the page header
+// is known: rows past the page's last are read from inside the page (the 16
+// bytes at +4 + 4 r stay below 8192 for r <= 2043) and dropped.
+__device__ __forceinline__ void load_dense_keys(const uint8_t *p, uint32_t lane, u32x4u (&kv)[kDenseBlk]) {
+#pragma unroll
+    for (uint32_t b = 0; b < kDenseBlk; ++b) {
+        const uint32_t r = 256 * b + 4 * lane;
+        kv[b] = *reinterpret_cast<const u32x4u *>(p + 4 + 4 * (r < 2040 ? r : 2040));
+    }
+}
+
+__device__ __forceinline__ uint32_t count_dense_page(const DirectTable &dt, const u32x4u (&kv)[kDenseBlk], uint32_t n,
+                                                     uint32_t *msk, uint32_t lane) {
+    constexpr uint32_t kBlk = kDenseBlk;
+    uint32_t slot[kBlk][4], in[kBlk];
+#pragma unroll
+    for (uint32_t b = 0; b < kBlk; ++b) {
+        in[b] = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t r = 256 * b + 4 * lane + j;
+            const int64_t off = (int64_t)(int32_t)kv[b][j] - (int64_t)dt.min_key;
+            const bool ok = (r < n) & (off >= 0) & ((uint64_t)off < dt.range);
+            slot[b][j] = ok ? (uint32_t)off : 0u;
+            in[b] |= (uint32_t)ok << j;
+        }
+    }
+    if (!dt.full) {
+        uint32_t word[kBlk][4];
+#pragma unroll
+        for (uint32_t b = 0; b < kBlk; ++b)
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) word[b][j] = dt.presence[slot[b][j] >> 5];
+#pragma unroll
+        for (uint32_t b = 0; b < kBlk; ++b)
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j)
+                if (!((word[b][j] >> (slot[b][j] & 31)) & 1u)) in[b] &= ~(1u << j);
+    }
+    uint32_t total = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < kBlk; ++b) {
+        total += (uint32_t)__popc(in[b]);
+        uint32_t v = in[b] << (4 * (lane & 7));
+        v |= __shfl_xor(v, 1, 64);
+        v |= __shfl_xor(v, 2, 64);
+        v |= __shfl_xor(v, 4, 64);
+        if ((lane & 7) == 0) msk[8 * b + (lane >> 3)] = v;
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) total += __shfl_xor(total, d, 64);
+    return total;
+}
+
 __global__ __launch_bounds__(kWG) void k_count_match_direct(ProbeArgs a, uint64_t *counts, uint32_t *match_bits) {
     __shared__ uint32_t msk_all[kWG / 64][2 * (kTile / 64) + 2];
     const uint32_t lane = lane_id();
@@ -99,14 +186,28 @@ __global__ __launch_bounds__(kWG) void k_count_match_direct(ProbeArgs a, uint64_
     const uint32_t pg = blockIdx.x * (kWG / 64) + wave_id();
     if (pg >= k.num_pages) return;
     const uint8_t *p = k.pages + (size_t)pg * kPage;
+    // the keys of a dense page are loaded together with its header (one
+    // round trip instead of two); the empty asm keeps the loads here
+    u32x4u kv[kDenseBlk];
+    load_dense_keys(p, lane, kv);
     const uint32_t n = ld16(p), nv = ld16(p + 2);
+    const uint64_t r0 = k.row_start[pg];
+#pragma unroll
+    for (uint32_t b = 0; b < kDenseBlk; ++b) asm volatile("" : "+v"(kv[b]));
     const bool dense = n == nv;
     const uint8_t *bm = p + kPage - ((n + 7) >> 3);
     const int32_t *keys = reinterpret_cast<const int32_t *>(p + 4);
-    const uint64_t r0 = k.row_start[pg];
     uint32_t *msk = msk_all[wave_id()];
     const uint64_t lt = lanemask_lt();
     const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
+    if (dense && n <= kTile) {
+        const uint32_t total = count_dense_page(dt, kv, n, msk, lane);
+        if (lane == 0) msk[2 * (kTile / 64)] = msk[2 * (kTile / 64) + 1] = 0;
+        __builtin_amdgcn_wave_barrier();
+        store_segment_bits(msk, match_bits, r0, n, lane);
+        if (lane == 0) counts[pg] = total;
+        return;
+    }
     uint32_t vbase = 0;
     uint64_t total = 0;
     for (uint32_t s0 = 0; s0 < n; s0 += kTile) {
@@ -170,26 +271,7 @@ __global__ __launch_bounds__(kWG) void k_count_match_direct(ProbeArgs a, uint64_
         }
         if (lane == 0) msk[2 * (kTile / 64)] = msk[2 * (kTile / 64) + 1] = 0;
         __builtin_amdgcn_wave_barrier();
-        const uint64_t g0 = r0 + s0;
-        const uint32_t sh = (uint32_t)(g0 & 31);
-        const uint32_t nwords = (sh + nseg + 31) >> 5;
-        for (uint32_t i = lane; i < nwords; i += 64) {
-            // bits [32 i - sh, 32 i - sh + 32) of the segment's row mask
-            uint32_t w;
-            if (i == 0) {
-                w = msk[0] << sh;
-            } else {
-                const uint32_t st = 32 * i - sh, d = st >> 5, o = st & 31;
-                w = o ? (msk[d] >> o) | (msk[d + 1] << (32 - o)) : msk[d];
-            }
-            const bool edge = (i == 0 && sh) || (i == nwords - 1 && ((sh + nseg) & 31));
-            uint32_t *dst = match_bits + (g0 >> 5) + i;
-            if (edge) {
-                if (w) atomicOr(dst, w);
-            } else {
-                *dst = w;
-            }
-        }
+        store_segment_bits(msk, match_bits, r0 + s0, nseg, lane);
         __builtin_amdgcn_wave_barrier();
     }
     if (lane == 0) counts[pg] = total;
