@@ -453,14 +453,15 @@ struct BCol {
     uint8_t *out;
 };
 
-// Output row o of column `col` (width W, source SRC: 0 the join key, 1 the
-// build side, 2 the probe side): the key, build tuple t of the partition
-// (plan-local index g_car0 + t of the slot-ordered carried columns) or
-// probe tuple pt (plan-local index of the partition-ordered carried
-// columns), stored at slot o % R of page o / R.
+// The value of output column `col` (width W, source SRC: 0 the join key, 1
+// the build side, 2 the probe side) for match entry (key, build tuple t of
+// the partition, probe tuple pt): the key, carried build value g_car0 + t
+// (plan-local index of the slot-ordered carried columns) or carried probe
+// value pt (plan-local index of the partition-ordered carried columns).
+// `ok` is false for a NULL.
 template <int W, int SRC, bool LDS>
-__device__ __forceinline__ void bx_put(const BPlan &pl, const BCol &col, const BuildPart &bp, uint32_t o,
-                                       int32_t key, uint32_t t, uint32_t pt, uint32_t *page_nulls) {
+__device__ __forceinline__ uint64_t bx_get(const BPlan &pl, const BCol &col, const BuildPart &bp, int32_t key,
+                                           uint32_t t, uint32_t pt, bool &ok) {
     if constexpr (W == 4) {
         int32_t v;
         if constexpr (SRC == 0) {
@@ -471,10 +472,8 @@ __device__ __forceinline__ void bx_put(const BPlan &pl, const BCol &col, const B
         } else {
             v = glb(reinterpret_cast<const int32_t *>(pl.p.car[col.cj]))[pt];
         }
-        const uint32_t pgi = o / kRowsI32, slot = o - pgi * kRowsI32;
-        uint8_t *page = col.out + (size_t)pgi * kPage;
-        if (v != kI32Null) glb(reinterpret_cast<int32_t *>(page + 4))[slot] = v;  // page_builders.h:123
-        else bx_null(page, pgi, slot, col.rows, kRowsI32, page_nulls + col.pg0 + pgi);
+        ok = v != kI32Null;  // page_builders.h:123
+        return (uint32_t)v;
     } else {
         uint64_t v;
         uint8_t b;
@@ -490,25 +489,58 @@ __device__ __forceinline__ void bx_put(const BPlan &pl, const BCol &col, const B
             v = glb(reinterpret_cast<const uint64_t *>(pl.p.car[col.cj]))[pt];
             b = glb(pl.p.car_vld[col.cj])[pt];
         }
-        const uint32_t pgi = o / kRowsI64, slot = o - pgi * kRowsI64;
-        uint8_t *page = col.out + (size_t)pgi * kPage;
-        if (b) glb(reinterpret_cast<uint64_t *>(page + 8))[slot] = v;
-        else bx_null(page, pgi, slot, col.rows, kRowsI64, page_nulls + col.pg0 + pgi);
+        ok = b != 0;
+        return v;
     }
 }
+
+// Store output row o of column `col`: slot o % R of page o / R.
+template <int W>
+__device__ __forceinline__ void bx_store(const BCol &col, uint32_t o, uint64_t v, bool ok, uint32_t *page_nulls) {
+    constexpr uint32_t R = W == 4 ? kRowsI32 : kRowsI64;
+    const uint32_t pgi = o / R, slot = o - pgi * R;
+    uint8_t *page = col.out + (size_t)pgi * kPage;
+    if (ok) {
+        if constexpr (W == 4) glb(reinterpret_cast<uint32_t *>(page + 4))[slot] = (uint32_t)v;
+        else glb(reinterpret_cast<uint64_t *>(page + 8))[slot] = v;
+    } else {
+        bx_null(page, pgi, slot, col.rows, R, page_nulls + col.pg0 + pgi);
+    }
+}
+
+constexpr uint32_t kEmitBatch = 4;  // window entries per thread whose loads are issued together
 
 // One column of a match window: rows ob + i for entries i of win (probe row
 // in the tile << kMatchTBits | build tuple), consecutive rows on
 // consecutive lanes; the column's width and source are template
-// parameters, so the loop holds no per-entry branch on them.
+// parameters, so the loop holds no per-entry branch on them.  A batch of
+// entries per thread reads the window, the keys and the values with
+// clamped indices (every load of the batch in flight together), then
+// stores the rows that exist.
 template <int W, int SRC, bool LDS>
 __device__ __forceinline__ void bx_emit_col(const BPlan &pl, const BCol &col, const BuildPart &bp, const uint32_t *win,
                                             uint32_t cnt, uint32_t ob, uint32_t pt0, uint32_t *page_nulls) {
-#pragma unroll 4
-    for (uint32_t i = threadIdx.x; i < cnt; i += kWG) {
-        const uint32_t en = win[i], t = en & kMatchT;
-        bx_put<W, SRC, LDS>(pl, col, bp, ob + i, SRC == 0 ? bp_key<LDS>(bp, t) : 0, t, pt0 + (en >> kMatchTBits),
-                            page_nulls);
+    const CJ_L uint32_t *lw = lds(win);
+    for (uint32_t i0 = threadIdx.x; i0 < cnt; i0 += kEmitBatch * kWG) {
+        uint32_t en[kEmitBatch];
+#pragma unroll
+        for (uint32_t b = 0; b < kEmitBatch; ++b) {
+            const uint32_t i = i0 + b * kWG;
+            en[b] = lw[i < cnt ? i : cnt - 1];
+        }
+        uint64_t v[kEmitBatch];
+        bool ok[kEmitBatch];
+#pragma unroll
+        for (uint32_t b = 0; b < kEmitBatch; ++b) {
+            const uint32_t t = en[b] & kMatchT;
+            v[b] = bx_get<W, SRC, LDS>(pl, col, bp, SRC == 0 ? bp_key<LDS>(bp, t) : 0, t,
+                                       pt0 + (en[b] >> kMatchTBits), ok[b]);
+        }
+#pragma unroll
+        for (uint32_t b = 0; b < kEmitBatch; ++b) {
+            const uint32_t i = i0 + b * kWG;
+            if (i < cnt) bx_store<W>(col, ob + i, v[b], ok[b], page_nulls);
+        }
     }
 }
 
@@ -520,13 +552,23 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
                                              uint64_t obase, uint32_t *lds32, uint32_t *win, uint32_t *page_nulls) {
     uint64_t mine = 0, base = obase;
     const uint32_t ls = pl.log_slots;
+    // the keys of the next tile are loaded while this one is probed
+    // (clamped: past the unit's last tuple the loads read its last key; a
+    // unit always holds a probe tuple, k_bx_units)
+    int32_t nkey[RPT];
+#pragma unroll
+    for (uint32_t j = 0; j < RPT; ++j) {
+        const uint64_t idx = p0 + kWG * j + threadIdx.x;
+        nkey[j] = pkeys[idx < p1 ? idx : p1 - 1];
+    }
     for (uint64_t i0 = p0; i0 < p1; i0 += kWG * RPT) {
         int32_t key[RPT];
         uint32_t s[RPT], e[RPT], m[RPT];
 #pragma unroll
         for (uint32_t j = 0; j < RPT; ++j) {
-            const uint64_t idx = i0 + kWG * j + threadIdx.x;
-            key[j] = pkeys[idx < p1 ? idx : p1 - 1];
+            key[j] = nkey[j];
+            const uint64_t idx = i0 + kWG * (RPT + j) + threadIdx.x;
+            nkey[j] = pkeys[idx < p1 ? idx : p1 - 1];
         }
 #pragma unroll
         for (uint32_t j = 0; j < RPT; ++j) {
