@@ -203,6 +203,102 @@ __global__ __launch_bounds__(kWG) void k_direct_scatter(DirectBuildArgs a) {
     }
 }
 
+constexpr uint32_t kScatterRows = 4;  // rows per thread whose loads are issued together
+
+// The validity word of a field or key read in place (no NULLs).
+__device__ const uint32_t kAllValid = ~0u;
+
+// k_direct_scatter for NF (<= 3) fields: a thread takes kScatterRows rows
+// (kWG apart) per step and issues every load of them (key, validity words,
+// value words) before it builds and stores their records.  The field loop
+// is unrolled and the loads are unconditional (addresses are selected, not
+// branched on: a field read in place or decoded, an 8-byte value's high
+// word at +4 of the low one, a 4-byte value reading its own word twice), so
+// no load waits for another.  KIN: the keys are read in place.
+template <int NF, int NQ, bool KIN>
+__global__ __launch_bounds__(kWG) void k_direct_scatter_nf(DirectBuildArgs a) {
+    __shared__ uint32_t red[kWG / 64];
+    const uint64_t range = direct_range(a);
+    const int64_t lo = direct_lo(a);
+    const uint64_t n = a.rows;
+    uint32_t cnt = 0;
+    const uint64_t step = (uint64_t)gridDim.x * kWG * kScatterRows;
+    for (uint64_t i0 = blockIdx.x * (uint64_t)kWG * kScatterRows + threadIdx.x; range && i0 < n; i0 += step) {
+        uint64_t ic[kScatterRows];
+        int32_t key[kScatterRows];
+        uint32_t kw[kScatterRows];
+#pragma unroll
+        for (uint32_t r = 0; r < kScatterRows; ++r) {
+            const uint64_t i = i0 + r * kWG;
+            ic[r] = i < n ? i : n - 1;
+            if constexpr (KIN) {
+                key[r] = dense_page_value(a.key_pages, a.key_rows, (uint32_t)ic[r]);
+                kw[r] = ~0u;
+            } else {
+                key[r] = a.keys[ic[r]];
+                kw[r] = a.key_valid[ic[r] >> 5];
+            }
+        }
+        uint32_t vw[NF][kScatterRows], lo32[NF][kScatterRows], hi32[NF][kScatterRows];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const DirectField &fd = a.fields[f];
+            const uint32_t hoff = fd.width == 8 ? 4u : 0u;
+            const uint32_t dr = fd.dense_rows ? fd.dense_rows : 1u;  // (in place only)
+#pragma unroll
+            for (uint32_t r = 0; r < kScatterRows; ++r) {
+                const uint32_t pg = (uint32_t)ic[r] / dr;
+                const uint8_t *inplace = fd.pages + (size_t)pg * kPage + 4 + 4 * (size_t)((uint32_t)ic[r] - pg * dr);
+                const uint8_t *dense = static_cast<const uint8_t *>(fd.values) + ic[r] * (uint64_t)fd.width;
+                const uint8_t *p = fd.values ? dense : inplace;
+                const uint32_t *vp = fd.valid_bits ? fd.valid_bits + (ic[r] >> 5) : &kAllValid;
+                lo32[f][r] = *reinterpret_cast<const uint32_t *>(p);
+                hi32[f][r] = *reinterpret_cast<const uint32_t *>(p + hoff);
+                vw[f][r] = *vp;
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kScatterRows; ++r) {
+            const uint64_t i = i0 + r * kWG;
+            if (i >= n || !((kw[r] >> (ic[r] & 31)) & 1u)) continue;
+            ++cnt;
+            const uint64_t slot = (uint64_t)((int64_t)key[r] - lo);
+            uint4 q[NQ];
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) q[c] = make_uint4(0, 0, 0, 0);
+            uint32_t flags = 1u << kDirectPresentBit;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const DirectField &fd = a.fields[f];
+                const bool ok = (vw[f][r] >> (ic[r] & 31)) & 1u;
+                flags |= (uint32_t)ok << f;
+                rec_put<NQ>(q, fd.offset, ok ? lo32[f][r] : 0u);
+                if (fd.width == 8) rec_put<NQ>(q, fd.offset + 4, ok ? hi32[f][r] : 0u);
+            }
+            if (a.rid_offset >= 0) rec_put<NQ>(q, (uint32_t)a.rid_offset, (uint32_t)i);
+            rec_put<NQ>(q, a.flags_offset, flags);
+            uint4 *dst = reinterpret_cast<uint4 *>(a.records + slot * a.stride);
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) dst[c] = q[c];
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if (lane_id() == 0) red[wave_id()] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kWG / 64; ++w) t += red[w];
+        a.partials[blockIdx.x] = t;
+    }
+}
+
+template <int NQ, bool KIN>
+static void launch_scatter_nf(const DirectBuildArgs &a, unsigned g, hipStream_t s) {
+    if (a.nfields == 1) k_direct_scatter_nf<1, NQ, KIN><<<g, kWG, 0, s>>>(a);
+    else if (a.nfields == 2) k_direct_scatter_nf<2, NQ, KIN><<<g, kWG, 0, s>>>(a);
+    else k_direct_scatter_nf<3, NQ, KIN><<<g, kWG, 0, s>>>(a);
+}
+
 // 64 slots per wave step: ballot of their present bits -> two presence words.
 __global__ __launch_bounds__(kWG) void k_direct_presence(DirectBuildArgs a) {
     __shared__ uint32_t red[kWG / 64];
@@ -347,12 +443,24 @@ void direct_build(const DirectBuildArgs &a, hipStream_t s) {
     if (!(a.known && a.full)) k_direct_zero<<<(unsigned)(zg < 8192 ? zg : 8192), kWG, 0, s>>>(a);
     uint64_t sg = (a.rows + kWG - 1) / kWG;
     sg = sg < kDirectParts ? sg : kDirectParts;
-    switch (a.stride / 16) {
-        case 1: k_direct_scatter<1><<<(unsigned)sg, kWG, 0, s>>>(a); break;
-        case 2: k_direct_scatter<2><<<(unsigned)sg, kWG, 0, s>>>(a); break;
-        case 3: k_direct_scatter<3><<<(unsigned)sg, kWG, 0, s>>>(a); break;
-        case 4: k_direct_scatter<4><<<(unsigned)sg, kWG, 0, s>>>(a); break;
-        default: k_direct_scatter<5><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+    const uint32_t nq = a.stride / 16;
+    if (a.nfields >= 1 && a.nfields <= 3 && nq <= 2) {
+        // batched loads; a thread takes kScatterRows rows a step
+        uint64_t bg = (a.rows + (uint64_t)kWG * kScatterRows - 1) / ((uint64_t)kWG * kScatterRows);
+        bg = bg < kDirectParts ? bg : kDirectParts;
+        sg = bg;
+        const unsigned g = (unsigned)bg;
+        const bool kin = a.keys == nullptr;
+        if (nq == 1) kin ? launch_scatter_nf<1, true>(a, g, s) : launch_scatter_nf<1, false>(a, g, s);
+        else kin ? launch_scatter_nf<2, true>(a, g, s) : launch_scatter_nf<2, false>(a, g, s);
+    } else {
+        switch (nq) {
+            case 1: k_direct_scatter<1><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+            case 2: k_direct_scatter<2><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+            case 3: k_direct_scatter<3><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+            case 4: k_direct_scatter<4><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+            default: k_direct_scatter<5><<<(unsigned)sg, kWG, 0, s>>>(a); break;
+        }
     }
     const uint64_t steps = (a.max_range + 63) / 64;
     uint64_t pg = (steps + kWG / 64 - 1) / (kWG / 64);
