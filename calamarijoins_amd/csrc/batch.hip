@@ -325,37 +325,173 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter_direct(BatchArgs a, uin
 // --------------------------------------------- per-partition build (slot order)
 // build_partition (hashtable.h:197-246): count per slot, exclusive prefix,
 // place the tuples; the directory keeps each slot's partition-local end.
-__global__ __launch_bounds__(kWG) void k_bx_build(BatchArgs a, const BPlan *__restrict__ plans,
-                                                  const uint32_t *__restrict__ part_start, const uint64_t *__restrict__ boff,
-                                                  const int32_t *__restrict__ bstage_keys) {
-    __shared__ uint32_t cnt[kMaxSlots];
-    __shared__ uint32_t lds4[4];
-    const uint32_t S = 1u << a.log_part, smask = S - 1;
-    for (uint32_t g = blockIdx.x; g < a.nparts; g += gridDim.x) {
-        const uint32_t k = find_owner(part_start, a.nplans, g);
-        const BPlan &pl = plans[k];
-        const uint32_t ls = pl.log_slots;
-        const uint64_t t_a = boff[g], t_e = boff[g + 1];
-        const uint64_t plan_base = boff[part_start[k]];
-        for (uint32_t i = threadIdx.x; i < S; i += kWG) cnt[i] = 0;
-        __syncthreads();
-        for (uint64_t i = t_a + threadIdx.x; i < t_e; i += kWG)
-            atomicAdd(&cnt[(uint32_t)bslot_of(bstage_keys[i], ls) & smask], 1u);
-        __syncthreads();
-        const uint32_t per = S / kWG;
-        const uint32_t s0 = threadIdx.x * per;
-        uint32_t local = 0;
+// A partition of at most kBatchSortCap tuples is ordered by (slot, key), so
+// the matches of a probe key are consecutive and a slot can be searched by
+// bisection: keys and source tuples in LDS, each slot sorted by one thread
+// (insertion by bisection; a key already in place, such as the next of a
+// run of equal keys, costs one comparison), then written out in order.  A
+// larger one is left in slot order.
+// A long slot (many copies of a few keys) is ordered by the whole
+// workgroup, one distinct key per round, ascending: the smallest key left and
+// its count c (reductions), then the copies outside the first c places and
+// the other keys inside them are listed (in LDS, any order) and swapped pair
+// by pair.  O(slot length / kWG) a round; an insertion sort by one thread
+// would move each copy of one key past the copies of another.
+constexpr uint32_t kLongSlot = 32;  // longer slots are ordered by the workgroup
+
+__device__ __forceinline__ int32_t block_min_i32(int32_t v, int32_t *red) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    if (lane_id() == 0) red[wave_id()] = v;
+    __syncthreads();
+    int32_t r = red[0];
 #pragma unroll
-        for (uint32_t i = 0; i < per; ++i) local += cnt[s0 + i];
-        uint32_t tot;
-        uint32_t pre = block_excl_scan<uint32_t>(local, &tot, lds4);
-        for (uint32_t i = 0; i < per; ++i) {
-            const uint32_t c = cnt[s0 + i];
-            cnt[s0 + i] = pre;
-            pre += c;
+    for (uint32_t w = 1; w < kWG / 64; ++w) r = min(r, red[w]);
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane_id() == 0) red[wave_id()] = v;
+    __syncthreads();
+    uint32_t r = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kWG / 64; ++w) r += red[w];
+    __syncthreads();
+    return r;
+}
+
+// lists: 2 x half u16 entries (positions < 2^16); cnt2: two LDS counters.
+__device__ void group_slot_wg(int32_t *skey, uint16_t *sidx, uint32_t lo, uint32_t hi, uint16_t *lists, uint32_t half,
+                              uint32_t *cnt2, int32_t *redi, uint32_t *redu) {
+    uint32_t w = lo;
+    while (w < hi) {
+        int32_t mn = INT32_MAX;
+        for (uint32_t i = w + threadIdx.x; i < hi; i += kWG) mn = min(mn, skey[i]);
+        mn = block_min_i32(mn, redi);
+        uint32_t c = 0;
+        for (uint32_t i = w + threadIdx.x; i < hi; i += kWG) c += skey[i] == mn;
+        c = block_sum_u32(c, redu);
+        if (w + c >= hi) break;  // the largest key is left: in place
+        if (threadIdx.x == 0) cnt2[0] = cnt2[1] = 0;
+        __syncthreads();
+        // others in [w, w + c) and copies in [w + c, hi): as many of each
+        // (at most half of the slot: c of one kind and hi - w - c of the
+        // other bound them both)
+        for (uint32_t i = w + threadIdx.x; i < w + c; i += kWG)
+            if (skey[i] != mn) lists[atomicAdd(&cnt2[0], 1u)] = (uint16_t)i;
+        for (uint32_t i = w + c + threadIdx.x; i < hi; i += kWG)
+            if (skey[i] == mn) lists[half + atomicAdd(&cnt2[1], 1u)] = (uint16_t)i;
+        __syncthreads();
+        const uint32_t np = cnt2[0];
+        for (uint32_t q = threadIdx.x; q < np; q += kWG) {
+            const uint32_t x = lists[q], y = lists[half + q];
+            const int32_t tk = skey[x];
+            const uint16_t ti = sidx[x];
+            skey[x] = skey[y];
+            sidx[x] = sidx[y];
+            skey[y] = tk;
+            sidx[y] = ti;
         }
         __syncthreads();
-        const uint32_t ncar = pl.b.ncar;
+        w += c;
+    }
+}
+
+__device__ __forceinline__ void bx_build_part(const BatchArgs &a, const BPlan *__restrict__ plans,
+                                              const uint32_t *__restrict__ part_start, const uint64_t *__restrict__ boff,
+                                              const int32_t *__restrict__ bstage_keys, uint32_t g, uint32_t sort_cap,
+                                              uint32_t *cnt, uint32_t *lds4, int32_t *skey, uint16_t *sidx,
+                                              uint16_t *lists, uint32_t *longs, uint32_t *misc, int32_t *redi) {
+    const uint32_t S = 1u << a.log_part, smask = S - 1;
+    const uint32_t k = find_owner(part_start, a.nplans, g);
+    const BPlan &pl = plans[k];
+    const uint32_t ls = pl.log_slots;
+    const uint64_t t_a = boff[g], t_e = boff[g + 1];
+    const uint64_t plan_base = boff[part_start[k]];
+    const bool sorted = t_e - t_a <= sort_cap;
+    for (uint32_t i = threadIdx.x; i < S; i += kWG) cnt[i] = 0;
+    __syncthreads();
+    for (uint64_t i = t_a + threadIdx.x; i < t_e; i += kWG)
+        atomicAdd(&cnt[(uint32_t)bslot_of(bstage_keys[i], ls) & smask], 1u);
+    __syncthreads();
+    const uint32_t per = S / kWG;
+    const uint32_t s0 = threadIdx.x * per;
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < per; ++i) local += cnt[s0 + i];
+    uint32_t tot;
+    uint32_t pre = block_excl_scan<uint32_t>(local, &tot, lds4);
+    for (uint32_t i = 0; i < per; ++i) {
+        const uint32_t c = cnt[s0 + i];
+        cnt[s0 + i] = pre;
+        pre += c;
+    }
+    __syncthreads();
+    const uint32_t ncar = pl.b.ncar;
+    if (sorted) {
+        // slot order in LDS: cnt[s] ends as the end of slot s
+        if (threadIdx.x == 0) misc[2] = 0;  // long slots found
+        for (uint64_t i = t_a + threadIdx.x; i < t_e; i += kWG) {
+            const int32_t key = bstage_keys[i];
+            const uint32_t pos = atomicAdd(&cnt[(uint32_t)bslot_of(key, ls) & smask], 1u);
+            skey[pos] = key;
+            sidx[pos] = (uint16_t)(i - t_a);
+        }
+        __syncthreads();
+        for (uint32_t sl = threadIdx.x; sl < S; sl += kWG) {
+            const uint32_t lo = sl ? cnt[sl - 1] : 0u, hi = cnt[sl];
+            if (hi - lo > kLongSlot) {
+                longs[atomicAdd(&misc[2], 1u)] = sl;
+                continue;
+            }
+            for (uint32_t i = lo + 1; i < hi; ++i) {
+                const int32_t kx = skey[i];
+                if (skey[i - 1] <= kx) continue;
+                const uint16_t ix = sidx[i];
+                // first of [lo, i) above kx, then [that, i) one place up
+                uint32_t x = lo, y = i - 1;
+                while (x < y) {
+                    const uint32_t mid = (x + y) >> 1;
+                    if (skey[mid] > kx) y = mid;
+                    else x = mid + 1;
+                }
+                uint32_t j = i;
+                for (; j >= x + 4; j -= 4) {
+                    const int32_t k0 = skey[j - 1], k1 = skey[j - 2], k2 = skey[j - 3], k3 = skey[j - 4];
+                    const uint16_t i0 = sidx[j - 1], i1 = sidx[j - 2], i2 = sidx[j - 3], i3 = sidx[j - 4];
+                    skey[j] = k0, skey[j - 1] = k1, skey[j - 2] = k2, skey[j - 3] = k3;
+                    sidx[j] = i0, sidx[j - 1] = i1, sidx[j - 2] = i2, sidx[j - 3] = i3;
+                }
+                for (; j > x; --j) {
+                    skey[j] = skey[j - 1];
+                    sidx[j] = sidx[j - 1];
+                }
+                skey[x] = kx;
+                sidx[x] = ix;
+            }
+        }
+        __syncthreads();
+        const uint32_t nlong = misc[2];
+        for (uint32_t q = 0; q < nlong; ++q) {
+            const uint32_t sl = longs[q];
+            group_slot_wg(skey, sidx, sl ? cnt[sl - 1] : 0u, cnt[sl], lists, sort_cap / 2, misc, redi, lds4);
+        }
+        const uint32_t nb = (uint32_t)(t_e - t_a);
+        for (uint32_t pos = threadIdx.x; pos < nb; pos += kWG) {
+            a.bkeys[t_a + pos] = skey[pos];
+            const uint64_t src = t_a + sidx[pos] - plan_base, dst = t_a + pos - plan_base;
+            for (uint32_t c = 0; c < ncar; ++c) {
+                if (pl.b.car_w[c] == 4) {
+                    glb(reinterpret_cast<int32_t *>(pl.bslot_car[c]))[dst] =
+                        glb(reinterpret_cast<const int32_t *>(pl.b.car[c]))[src];
+                } else {
+                    glb(reinterpret_cast<uint64_t *>(pl.bslot_car[c]))[dst] =
+                        glb(reinterpret_cast<const uint64_t *>(pl.b.car[c]))[src];
+                    glb(pl.bslot_vld[c])[dst] = glb(pl.b.car_vld[c])[src];
+                }
+            }
+        }
+    } else {
         for (uint64_t i = t_a + threadIdx.x; i < t_e; i += kWG) {
             const int32_t key = bstage_keys[i];
             const uint32_t pos = atomicAdd(&cnt[(uint32_t)bslot_of(key, ls) & smask], 1u);
@@ -372,9 +508,56 @@ __global__ __launch_bounds__(kWG) void k_bx_build(BatchArgs a, const BPlan *__re
                 }
             }
         }
+    }
+    __syncthreads();
+    uint32_t *d = a.dir + pl.dir_base + ((uint64_t)(g - part_start[k]) << a.log_part);
+    for (uint32_t i = threadIdx.x; i < S; i += kWG) d[i] = cnt[i];
+    __syncthreads();
+}
+
+// The first launch (big = false) takes the partitions of at most sort_cap =
+// lds_cap tuples (sorted) and those above kBatchSortCap (slot order only),
+// strided by workgroup; the second (big = true, sort_cap = kBatchSortCap,
+// more LDS) the ones in between: a workgroup checks kWG partitions at once
+// and builds the ones it finds.
+__global__ __launch_bounds__(kWG) void k_bx_build(BatchArgs a, const BPlan *__restrict__ plans,
+                                                  const uint32_t *__restrict__ part_start, const uint64_t *__restrict__ boff,
+                                                  const int32_t *__restrict__ bstage_keys, uint32_t sort_cap, bool big) {
+    __shared__ uint32_t cnt[kMaxSlots];
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t found[kWG];
+    __shared__ uint32_t nfound;
+    __shared__ uint32_t misc[4];
+    __shared__ int32_t redi[kWG / 64];
+    // long slots of a partition: at most its tuples / (kLongSlot + 1)
+    __shared__ uint32_t longs[kBatchSortCap / (kLongSlot + 1) + 1];
+    // keys, source tuples, and the swap lists of the long slots' ordering
+    extern __shared__ __align__(16) uint8_t bdyn[];
+    int32_t *skey = reinterpret_cast<int32_t *>(bdyn);
+    uint16_t *sidx = reinterpret_cast<uint16_t *>(bdyn + 4 * (size_t)sort_cap);
+    uint16_t *lists = reinterpret_cast<uint16_t *>(bdyn + 6 * (size_t)sort_cap);
+    if (!big) {
+        for (uint32_t g = blockIdx.x; g < a.nparts; g += gridDim.x) {
+            const uint64_t nb = boff[g + 1] - boff[g];
+            if (nb > a.lds_cap && nb <= kBatchSortCap) continue;  // the second launch's
+            bx_build_part(a, plans, part_start, boff, bstage_keys, g, sort_cap, cnt, lds4, skey, sidx, lists, longs, misc,
+                          redi);
+        }
+        return;
+    }
+    for (uint32_t g0 = blockIdx.x * kWG; g0 < a.nparts; g0 += gridDim.x * kWG) {
+        if (threadIdx.x == 0) nfound = 0;
         __syncthreads();
-        uint32_t *d = a.dir + pl.dir_base + ((uint64_t)(g - part_start[k]) << a.log_part);
-        for (uint32_t i = threadIdx.x; i < S; i += kWG) d[i] = cnt[i];
+        const uint32_t g = g0 + threadIdx.x;
+        if (g < a.nparts) {
+            const uint64_t nb = boff[g + 1] - boff[g];
+            if (nb > a.lds_cap && nb <= kBatchSortCap) found[atomicAdd(&nfound, 1u)] = g;
+        }
+        __syncthreads();
+        const uint32_t nf = nfound;
+        for (uint32_t f = 0; f < nf; ++f)
+            bx_build_part(a, plans, part_start, boff, bstage_keys, found[f], sort_cap, cnt, lds4, skey, sidx, lists,
+                          longs, misc, redi);
         __syncthreads();
     }
 }
@@ -428,6 +611,14 @@ __device__ __forceinline__ int32_t bp_key(const BuildPart &bp, uint32_t t) {
 // The match window of the write pass: (probe row in the tile) << 22 | build
 // tuple of the partition, kMatchWin of them.
 constexpr uint32_t kMatchWin = 2048;
+// Match descriptor of a probe tuple whose build partition is staged in LDS
+// (written over its key by the count pass, read by the write pass): the
+// partition-local index of its first matching build tuple (the partition is
+// ordered by slot, then key, so its matches are consecutive) and their
+// count, 1023 standing for "1023 or more".
+constexpr uint32_t kDescFirstBits = 22;
+constexpr uint32_t kDescFirst = (1u << kDescFirstBits) - 1;
+constexpr uint32_t kDescMany = 1023;
 constexpr uint32_t kMatchTBits = 22;
 constexpr uint32_t kMatchT = (1u << kMatchTBits) - 1;
 static_assert(kBatchMaxBuildRows <= (1ull << kMatchTBits), "build tuple index must fit the match entry");
@@ -549,7 +740,8 @@ __device__ __forceinline__ void bx_emit_col(const BPlan &pl, const BCol &col, co
 template <bool WRITE, bool LDS, uint32_t RPT>
 __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &pl, const BOut *bo,
                                              const BuildPart &bp, uint64_t p0, uint64_t p1, uint64_t pbase,
-                                             uint64_t obase, uint32_t *lds32, uint32_t *win, uint32_t *page_nulls) {
+                                             uint64_t obase, uint32_t *lds32, uint32_t *win, uint32_t *page_nulls,
+                                             uint32_t *desc, bool sorted) {
     uint64_t mine = 0, base = obase;
     const uint32_t ls = pl.log_slots;
     // the keys of the next tile are loaded while this one is probed
@@ -579,25 +771,57 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
             s[j] = in ? lo : 0u;
             e[j] = in ? hi : 0u;
         }
-        uint32_t f0[RPT], f1[RPT];  // first two matching build tuples (write pass)
+        // first two matching build tuples (write pass over a partition not
+        // ordered by key; count pass over an ordered one: the first)
+        uint32_t f0[RPT], f1[RPT];
+        if (!WRITE && sorted) {
+            // the slot's keys are ascending: its run of the probe key by bisection
 #pragma unroll
-        for (uint32_t j = 0; j < RPT; ++j) {
-            uint32_t c = 0, a0 = 0, a1 = 0;
-            for (uint32_t t = s[j]; t < e[j]; ++t) {
-                const bool hit = bp_key<LDS>(bp, t) == key[j];
-                if (WRITE && hit) {
-                    a1 = c == 1 ? t : a1;
-                    a0 = c == 0 ? t : a0;
+            for (uint32_t j = 0; j < RPT; ++j) {
+                uint32_t lo = s[j], hi = e[j];
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (bp_key<LDS>(bp, mid) < key[j]) lo = mid + 1;
+                    else hi = mid;
                 }
-                c += hit;
+                uint32_t ul = lo, uh = e[j];
+                while (ul < uh) {
+                    const uint32_t mid = (ul + uh) >> 1;
+                    if (bp_key<LDS>(bp, mid) <= key[j]) ul = mid + 1;
+                    else uh = mid;
+                }
+                m[j] = ul - lo;
+                f0[j] = lo;
+                f1[j] = 0;
             }
-            m[j] = c;
-            f0[j] = a0;
-            f1[j] = a1;
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < RPT; ++j) {
+                uint32_t c = 0, a0 = 0, a1 = 0;
+                for (uint32_t t = s[j]; t < e[j]; ++t) {
+                    const bool hit = bp_key<LDS>(bp, t) == key[j];
+                    if (WRITE && hit) {
+                        a1 = c == 1 ? t : a1;
+                        a0 = c == 0 ? t : a0;
+                    }
+                    c += hit;
+                }
+                m[j] = c;
+                f0[j] = a0;
+                f1[j] = a1;
+            }
         }
         if constexpr (!WRITE) {
 #pragma unroll
-            for (uint32_t j = 0; j < RPT; ++j) mine += m[j];
+            for (uint32_t j = 0; j < RPT; ++j) {
+                mine += m[j];
+                if (sorted) {
+                    // the tuple's key is not read again: its descriptor
+                    // takes its place
+                    const uint64_t idx = i0 + kWG * j + threadIdx.x;
+                    if (idx < p1) desc[idx] = (min(m[j], kDescMany) << kDescFirstBits) | (m[j] ? f0[j] : 0u);
+                }
+            }
         } else {
             // exclusive prefix in tuple order i0 + 256 j + t
             const uint32_t w = wave_id(), lane = lane_id();
@@ -662,6 +886,89 @@ __device__ __forceinline__ uint64_t bx_probe(const int32_t *pkeys, const BPlan &
     return mine;
 }
 
+// The write pass over a partition ordered by (slot, key) (staged in LDS or
+// read in place), from the count pass's match descriptors: no key lookups,
+// no candidate walks.  Per tile of kWG *
+// RPT probe tuples: the descriptors (the next tile's loaded while this one
+// is written), a prefix of the threads' match counts (a thread's matches
+// take consecutive rows; a plan's result is a row multiset, so any order of
+// the tile's matches serves), then the window: entry (tile row, first + x)
+// for match x of a tuple, emitted column by column.
+template <uint32_t RPT, bool LDS>
+__device__ __forceinline__ void bx_write_desc(const uint32_t *desc, const BPlan &pl, const BOut *bo,
+                                              const BuildPart &bp, uint32_t nb, uint64_t p0, uint64_t p1,
+                                              uint64_t pbase, uint64_t obase, uint32_t *lds32, uint32_t *win,
+                                              uint32_t *page_nulls) {
+    uint64_t base = obase;
+    uint32_t nd[RPT];
+#pragma unroll
+    for (uint32_t j = 0; j < RPT; ++j) {
+        const uint64_t idx = p0 + kWG * j + threadIdx.x;
+        nd[j] = desc[idx < p1 ? idx : p1 - 1];
+    }
+    const uint32_t w = wave_id(), lane = lane_id();
+    for (uint64_t i0 = p0; i0 < p1; i0 += kWG * RPT) {
+        uint32_t m[RPT], f[RPT], mt = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < RPT; ++j) {
+            const uint32_t d = nd[j];
+            const uint64_t idx = i0 + kWG * (RPT + j) + threadIdx.x;
+            nd[j] = desc[idx < p1 ? idx : p1 - 1];
+            const bool in = i0 + kWG * j + threadIdx.x < p1;
+            f[j] = d & kDescFirst;
+            m[j] = in ? d >> kDescFirstBits : 0u;
+            if (m[j] == kDescMany) {
+                // a longer run: its equal keys are consecutive in the partition
+                const int32_t kk = bp_key<LDS>(bp, f[j]);
+                uint32_t t = f[j] + kDescMany;
+                while (t < nb && bp_key<LDS>(bp, t) == kk) ++t;
+                m[j] = t - f[j];
+            }
+            mt += m[j];
+        }
+        const uint32_t inc = wave_incl_scan<uint32_t>(mt);
+        if (lane == 63) lds32[w] = inc;
+        __syncthreads();
+        uint32_t run = 0, pre = inc - mt;
+#pragma unroll
+        for (uint32_t ww = 0; ww < kWG / 64; ++ww) {
+            const uint32_t x = lds32[ww];
+            pre += ww < w ? x : 0u;
+            run += x;
+        }
+        __syncthreads();  // every wave has read lds32 before the next tile writes it
+        for (uint32_t w0 = 0; w0 < run; w0 += kMatchWin) {
+            const uint32_t w1 = run - w0 < kMatchWin ? run : w0 + kMatchWin;
+            // the thread's entries [pre, pre + mt) that fall in [w0, w1)
+            uint32_t q = pre;
+#pragma unroll
+            for (uint32_t j = 0; j < RPT; ++j) {
+                const uint32_t row = (kWG * j + threadIdx.x) << kMatchTBits;
+                const uint32_t a = q > w0 ? q : w0, b = q + m[j] < w1 ? q + m[j] : w1;
+                for (uint32_t x = a; x < b; ++x) lds(win)[x - w0] = row | (f[j] + (x - q));
+                q += m[j];
+            }
+            __syncthreads();
+            const uint32_t cnt = w1 - w0;
+            const uint32_t ob = (uint32_t)(base + w0), pt0 = (uint32_t)(i0 - pbase);
+#pragma unroll 1
+            for (uint32_t c = 0; c < bo->nout; ++c) {
+                const BCol col{bo->src[c], bo->j[c], bo->w[c], bo->rows, bo->pg0[c], bo->out[c]};
+                if (col.w == 4) {
+                    if (col.src == 0) bx_emit_col<4, 0, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                    else if (col.src == 1) bx_emit_col<4, 1, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                    else bx_emit_col<4, 2, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                } else {  // 8-byte columns are never keys
+                    if (col.src == 1) bx_emit_col<8, 1, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                    else bx_emit_col<8, 2, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                }
+            }
+            __syncthreads();
+        }
+        base += run;
+    }
+}
+
 }  // namespace
 
 // dst[0 .. n) = src[0 .. n), eight loads per thread in flight before the
@@ -687,7 +994,7 @@ template <bool WRITE>
 __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__restrict__ outs, const BPlan *__restrict__ plans,
                                                  const uint32_t *__restrict__ part_start, const uint64_t *__restrict__ uoff,
                                                  const uint64_t *__restrict__ boff, const uint64_t *__restrict__ poff,
-                                                 const uint64_t *__restrict__ moff, const int32_t *__restrict__ pkeys,
+                                                 const uint64_t *__restrict__ moff, int32_t *pkeys,
                                                  const int32_t *__restrict__ bkeys, const uint32_t *__restrict__ dir,
                                                  uint32_t *page_nulls) {
     // slot ends (u16, 2^log_part of them), then the keys and carried build values
@@ -732,7 +1039,8 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
         uint64_t mine;
         if (nb <= a.lds_cap) {
             // stage the partition: each batch of loads is issued before its LDS stores
-            {
+            // (the write pass reads descriptors, not slots: no slot ends)
+            if (!WRITE) {
                 uint32_t ev[kMaxSlots / kWG];
 #pragma unroll
                 for (uint32_t q = 0; q < kMaxSlots / kWG; ++q)
@@ -755,9 +1063,30 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
                     }
                 }
             __syncthreads();
-            mine = bx_probe<WRITE, true, WRITE ? 4u : kRPT>(pkeys, pl, ob, bp, p0, p1, pbase, obase, lds32, win, page_nulls);
+            if constexpr (WRITE) {
+                bx_write_desc<4u, true>(reinterpret_cast<const uint32_t *>(pkeys), pl, ob, bp, nb, p0, p1, pbase,
+                                        obase, lds32, win, page_nulls);
+                mine = 0;
+            } else {
+                mine = bx_probe<false, true, kRPT>(pkeys, pl, ob, bp, p0, p1, pbase, obase, lds32, win, page_nulls,
+                                                   reinterpret_cast<uint32_t *>(pkeys), true);
+            }
         } else {
-            mine = bx_probe<WRITE, false, WRITE ? 4u : kRPT>(pkeys, pl, ob, bp, p0, p1, pbase, obase, lds32, win, page_nulls);
+            // a partition too large for LDS is read in place; ordered by
+            // key when it holds at most kBatchSortCap tuples (k_bx_build)
+            const bool sorted = nb <= kBatchSortCap;
+            if constexpr (WRITE) {
+                if (sorted)
+                    bx_write_desc<4u, false>(reinterpret_cast<const uint32_t *>(pkeys), pl, ob, bp, nb, p0, p1, pbase,
+                                             obase, lds32, win, page_nulls);
+                else
+                    bx_probe<true, false, 4u>(pkeys, pl, ob, bp, p0, p1, pbase, obase, lds32, win, page_nulls, nullptr,
+                                              false);
+                mine = 0;
+            } else {
+                mine = bx_probe<false, false, kRPT>(pkeys, pl, ob, bp, p0, p1, pbase, obase, lds32, win, page_nulls,
+                                                    reinterpret_cast<uint32_t *>(pkeys), sorted);
+            }
         }
         if (!WRITE) {
             uint64_t tot;
@@ -933,6 +1262,9 @@ static unsigned pgrid(uint64_t units, uint32_t grid) { return (unsigned)(units <
 // dynamic LDS of each kernel
 static size_t part_count_lds(const BatchArgs &a) { return a.max_parts * sizeof(uint32_t); }
 static size_t scatter_lds(uint32_t parts) { return parts * (sizeof(uint64_t) + sizeof(uint32_t)); }
+// keys, source tuples and swap lists of the partitions a build launch orders
+static size_t build_lds(const BatchArgs &a) { return (size_t)a.lds_cap * (sizeof(int32_t) + 2 * sizeof(uint16_t)); }
+static size_t build_big_lds() { return (size_t)kBatchSortCap * (sizeof(int32_t) + 2 * sizeof(uint16_t)); }
 static size_t join_count_lds(const BatchArgs &a) { return (2u << a.log_part) + (size_t)a.lds_cap * 4; }  // slot ends + keys
 static size_t join_write_lds(const BatchArgs &a) {
     return (2u << a.log_part) + (size_t)a.lds_cap * a.tuple_bytes;  // slot ends + tuples
@@ -968,6 +1300,7 @@ BatchGrids batch_resident_grids(const BatchArgs &a, int device) {
     allow_lds(k_bx_part_scatter<false>, scatter_lds(a.staged_parts), device, "the partition scatter");
     allow_lds(k_bx_part_scatter_direct<true>, scatter_lds(a.direct_parts), device, "the partition scatter");
     allow_lds(k_bx_part_scatter_direct<false>, scatter_lds(a.direct_parts), device, "the partition scatter");
+    allow_lds(k_bx_build, std::max(build_lds(a), build_big_lds()), device, "the partition build");
     allow_lds(k_bx_join<true>, join_write_lds(a), device, "the join");
     allow_lds(k_bx_join<false>, join_count_lds(a), device, "the join");
     int cus = 0;
@@ -977,7 +1310,8 @@ BatchGrids batch_resident_grids(const BatchArgs &a, int device) {
     g.part_count = resident(k_bx_part_count<true>, part_count_lds(a), cus);
     g.part_scatter = resident(k_bx_part_scatter<true>, scatter_lds(a.staged_parts), cus);
     g.part_scatter_direct = resident(k_bx_part_scatter_direct<true>, scatter_lds(a.direct_parts), cus);
-    g.build = resident(k_bx_build, 0, cus);
+    g.build = resident(k_bx_build, build_lds(a), cus);
+    g.build_big = resident(k_bx_build, build_big_lds(), cus);
     g.join_count = resident(k_bx_join<false>, join_count_lds(a), cus);
     g.join_write = resident(k_bx_join<true>, join_write_lds(a), cus);
     g.page_init = resident(k_bx_page_init, 0, cus);
@@ -1012,9 +1346,12 @@ void batch_part_scatter(const BatchArgs &a, bool probe, const uint32_t *pstart, 
     CJ_HIP(hipGetLastError());
 }
 
-void batch_build(const BatchArgs &a, uint32_t grid, hipStream_t s) {
+void batch_build(const BatchArgs &a, uint32_t grid, uint32_t grid_big, hipStream_t s) {
     if (!a.nparts) return;
-    k_bx_build<<<pgrid(a.nparts, grid), kWG, 0, s>>>(a, a.plans, a.part_start, a.boff, a.bstage_keys);
+    k_bx_build<<<pgrid(a.nparts, grid), kWG, build_lds(a), s>>>(a, a.plans, a.part_start, a.boff, a.bstage_keys,
+                                                                 a.lds_cap, false);
+    k_bx_build<<<pgrid(a.nparts, grid_big), kWG, build_big_lds(), s>>>(a, a.plans, a.part_start, a.boff,
+                                                                        a.bstage_keys, kBatchSortCap, true);
     CJ_HIP(hipGetLastError());
 }
 

@@ -40,6 +40,8 @@ constexpr uint32_t kBatchMaxLogPart = 12;             // slots per partition: 2^
 constexpr uint64_t kBatchMaxBuildRows = 1ull << 22;   // larger builds run one plan at a time
 constexpr uint32_t kBatchMaxChunk = 16384;            // probe tuples per join unit
 constexpr uint32_t kBatchLdsBytes = 64 * 1024;        // LDS of a join workgroup
+constexpr uint32_t kBatchSortCap = 16384;             // partitions of at most this many build tuples are
+                                                      // ordered by (slot, key) (partition-local indices fit 16 bits)
 
 // One side of one plan.  Carried column j holds, in partition order, the
 // value of the source column for each tuple: INT32 with NULL folded to
@@ -143,7 +145,8 @@ struct BPageCol {
 // and, with units split evenly by stride, leaves the machine part-idle in
 // the last one.
 struct BatchGrids {
-    uint32_t decode, part_count, part_scatter, part_scatter_direct, build, join_count, join_write, page_init, page_fix;
+    uint32_t decode, part_count, part_scatter, part_scatter_direct, build, build_big, join_count, join_write, page_init,
+        page_fix;
 };
 // Also checks that every kernel's LDS fits a workgroup (UnsupportedError if
 // not) and raises the dynamic-LDS limit where a kernel needs more than 64 KiB;
@@ -158,7 +161,7 @@ void batch_part_count(const BatchArgs &a, bool probe, uint32_t units, uint32_t g
 constexpr uint32_t kStagedMaxParts = 256;
 void batch_part_scatter(const BatchArgs &a, bool probe, const uint32_t *pstart, uint32_t units, bool staged,
                         uint32_t grid, hipStream_t s);
-void batch_build(const BatchArgs &a, uint32_t grid, hipStream_t s);
+void batch_build(const BatchArgs &a, uint32_t grid, uint32_t grid_big, hipStream_t s);
 void batch_units(const BatchArgs &a, hipStream_t s);
 void batch_join_count(const BatchArgs &a, uint32_t grid, hipStream_t s);
 void batch_join_write(const BatchArgs &a, const BOut *outs, uint32_t *page_nulls, uint32_t grid, hipStream_t s);

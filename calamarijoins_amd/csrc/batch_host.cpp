@@ -322,6 +322,8 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
     const uint32_t cap_budget = std::min<uint32_t>((dyn_budget / max_tuple_bytes) & ~15u, 65535u & ~15u);
     const uint32_t lds_cap =
         (uint32_t)std::min<uint64_t>(cap_budget, (std::max<uint64_t>(256, 2 * max_mean) + 15) & ~15ull);
+    // a partition staged in LDS is also one the build orders by key
+    if (lds_cap > kBatchSortCap) throw InvalidError("batch: join LDS capacity above the build's sort capacity");
     std::vector<BDecode> jobs(dec_cols.size());
     std::vector<uint32_t> job_start(dec_cols.size() + 1, 0);
     uint64_t valid_words = 0, val_bytes = 0;
@@ -451,7 +453,7 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         tpl->grids = batch_resident_grids(a, ctx.device);
         if (cfg.grid)
             tpl->grids =
-                BatchGrids{cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid};
+                BatchGrids{cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid, cfg.grid};
     }
     // a template whose captured decode units disagree with this batch's page
     // counts would decode past (or short of) the pages: never replay it
@@ -512,7 +514,7 @@ static double run_batch(Context &ctx, const std::vector<uint32_t> &batched, cons
         batch_copy_u64(A.poff, A.pcur, nparts + 1, ps);
         batch_part_scatter(A, true, A.ppage_start_s, pss[n], true, G.part_scatter, ps);
         batch_part_scatter(A, true, A.ppage_start_d, psd[n], false, G.part_scatter_direct, ps);
-        batch_build(A, G.build, ps);
+        batch_build(A, G.build, G.build_big, ps);
         batch_units(A, ps);
         scan_exclusive_u64(A.ucount, const_cast<uint64_t *>(A.uoff), nparts, ps, T.scan_tmp, T.scan_elems);
         batch_join_count(A, G.join_count, ps);
