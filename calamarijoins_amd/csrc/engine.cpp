@@ -51,6 +51,18 @@ void DevicePool::trim() {
     free_.clear();
 }
 
+PoolMeter &pool_meter() {
+    static thread_local PoolMeter m;
+    return m;
+}
+
+static void meter_take(int64_t sz) {
+    PoolMeter &m = pool_meter();
+    if (!m.on) return;
+    m.cur += sz;
+    m.peak = std::max(m.peak, m.cur);
+}
+
 void *DevicePool::alloc(size_t bytes) {
     size_t sz = pool_round(bytes ? bytes : 1);
     std::lock_guard<std::mutex> g(mu_);
@@ -59,6 +71,7 @@ void *DevicePool::alloc(size_t bytes) {
         void *p = it->second;
         in_use_ += it->first;
         peak_ = std::max(peak_, in_use_);
+        meter_take((int64_t)it->first);
         free_.erase(it);
         return p;
     }
@@ -79,6 +92,7 @@ void *DevicePool::alloc(size_t bytes) {
     reserved_ += sz;
     in_use_ += sz;
     peak_ = std::max(peak_, in_use_);
+    meter_take((int64_t)sz);
     return p;
 }
 
@@ -88,6 +102,7 @@ void DevicePool::free(void *p) {
     auto it = sizes_.find(p);
     if (it == sizes_.end()) return;
     in_use_ -= it->second;
+    meter_take(-(int64_t)it->second);
     free_.emplace(it->second, p);
 }
 
@@ -1796,6 +1811,43 @@ static void run_exec(Context &ctx, const cj_plan_desc &plan, uint32_t flags, cj_
     }
 }
 
+// FNV-1a over a plan's structure and its inputs' pages: equal for repeats of
+// one plan over the same (resident) tables.
+static uint64_t plan_signature(const cj_plan_desc &p) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&](uint64_t v) {
+        for (int i = 0; i < 8; ++i) {
+            h ^= (v >> (8 * i)) & 0xff;
+            h *= 0x100000001b3ull;
+        }
+    };
+    mix(p.num_nodes), mix(p.root), mix(p.num_inputs);
+    for (uint32_t i = 0; i < p.num_nodes; ++i) {
+        const cj_node_desc &n = p.nodes[i];
+        mix((uint64_t)n.kind), mix(n.left), mix(n.right), mix(n.left_attr), mix(n.right_attr);
+        mix(n.build_left), mix(n.base_table_id), mix(n.num_output_attrs);
+        for (uint32_t a = 0; a < n.num_output_attrs; ++a) mix(n.output_attrs[a].index), mix((uint64_t)n.output_attrs[a].type);
+    }
+    for (uint32_t i = 0; i < p.num_inputs; ++i) {
+        const cj_table_desc &t = p.inputs[i];
+        mix(t.num_rows), mix(t.num_columns);
+        for (uint32_t c = 0; c < t.num_columns; ++c) {
+            const cj_column_desc &col = t.columns[c];
+            mix((uint64_t)col.type), mix(col.num_pages);
+            mix(col.num_pages ? (uint64_t)(uintptr_t)col.pages[0] : 0);
+            mix(col.num_pages ? (uint64_t)(uintptr_t)col.pages[col.num_pages - 1] : 0);
+        }
+    }
+    return h;
+}
+
+uint64_t Context::learned_footprint(const cj_plan_desc &plan) {
+    const uint64_t sig = plan_signature(plan);
+    std::lock_guard<std::mutex> g(learned_mu);
+    auto it = learned.find(sig);
+    return it == learned.end() ? 0 : it->second;
+}
+
 void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing) {
     std::memset(out, 0, sizeof *out);
     if (plan.root >= plan.num_nodes) throw InvalidError("plan root out of range");
@@ -1804,6 +1856,14 @@ void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, 
         for (uint32_t c = 0; c < plan.inputs[i].num_columns; ++c)
             if (plan.inputs[i].columns[c].type < CJ_INT32 || plan.inputs[i].columns[c].type > CJ_VARCHAR)
                 throw InvalidError("bad column type");
+    // meter this thread's pool blocks while the plan runs (its
+    // temporaries and result pages) and record the peak by plan signature
+    PoolMeter &meter = pool_meter();
+    struct MeterScope {
+        PoolMeter &m;
+        explicit MeterScope(PoolMeter &mm) : m(mm) { m = PoolMeter{true, 0, 0}; }
+        ~MeterScope() { m.on = false; }
+    } ms{meter};
     run_exec(
         *this, plan, flags, out, timing,
         [&](Exec &ex) {
@@ -1817,6 +1877,11 @@ void Context::execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, 
             }
         },
         [&](Exec &ex) { ex.join(plan.nodes[plan.root], true, 0, out); });
+    const uint64_t peak = (uint64_t)std::max<int64_t>(meter.peak, 0);
+    const uint64_t sig = plan_signature(plan);
+    std::lock_guard<std::mutex> g(learned_mu);
+    uint64_t &l = learned[sig];
+    l = std::max(l, peak);
 }
 
 void Context::execute_noalloc(const cj_plan_desc &plan, uint32_t flags, void *const *out_pages,

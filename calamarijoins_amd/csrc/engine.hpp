@@ -17,6 +17,15 @@
 
 namespace cj {
 
+// Device bytes one thread's allocations hold while it meters them (the
+// blocks it takes from the pool, less the ones it returns): the footprint of
+// one execution, which runs on one thread.
+struct PoolMeter {
+    bool on = false;
+    int64_t cur = 0, peak = 0;
+};
+PoolMeter &pool_meter();  // this thread's
+
 // Caching device allocator: blocks are rounded to 2 MiB (small ones to
 // 4 KiB) and recycled best-fit, so steady-state execution calls no hipMalloc.
 class DevicePool {
@@ -178,6 +187,13 @@ class Context {
     void unregister_table(uint64_t id);
 
     void execute(const cj_plan_desc &plan, uint32_t flags, cj_result *out, cj_timing *timing);
+    // Peak device bytes an execution of a plan took (its temporaries and
+    // result pages, the blocks as the pool hands them out), recorded by plan
+    // signature (structure and input pages: with resident tables, the same
+    // plan); 0 if the plan has not run on this context.
+    uint64_t learned_footprint(const cj_plan_desc &plan);
+    std::mutex learned_mu;
+    std::unordered_map<uint64_t, uint64_t> learned;
     // the same without allocating: temporaries in the GPU's fixed scratch,
     // result pages in caller buffers (cj_execute_noalloc)
     void execute_noalloc(const cj_plan_desc &plan, uint32_t flags, void *const *out_pages,

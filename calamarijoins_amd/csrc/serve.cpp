@@ -72,7 +72,16 @@ uint64_t estimate_plan_bytes(const cj_plan_desc &p, Context *ctx) {
     }
     // result pages of the root
     if (p.root < p.num_nodes) bytes += rows[p.root] * 9ull * p.nodes[p.root].num_output_attrs;
-    return bytes + (64ull << 20);
+    bytes += 64ull << 20;
+    // a plan that has run on this context: its recorded peak, with a margin
+    // (the bound above assumes every join's output as large as its larger
+    // input; for key / foreign-key joins over filtered tables the real
+    // footprint is tens to thousands of times smaller)
+    if (ctx) {
+        const uint64_t seen = ctx->learned_footprint(p);
+        if (seen) bytes = std::min<uint64_t>(bytes, seen + seen / 4 + (8ull << 20));
+    }
+    return bytes;
 }
 
 Server::Server(Context &ctx, int workers, uint64_t admit_bytes) : ctx_(ctx), budget_(admit_bytes) {

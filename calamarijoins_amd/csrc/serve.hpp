@@ -54,7 +54,9 @@ struct ServeRequest {
 // memory beside other plans, and is then run again alone.
 // With a context, a column read from a resident table costs nothing when its
 // decoded form is cached with the table or its INT32 pages are read in place,
-// and a non-resident column adds its uploaded pages.
+// and a non-resident column adds its uploaded pages; and a plan that has run
+// on the context is charged its recorded peak plus a quarter (and 8 MiB), if
+// that is less.
 uint64_t estimate_plan_bytes(const cj_plan_desc &p, Context *ctx = nullptr);
 
 class Server {

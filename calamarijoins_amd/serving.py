@@ -8,8 +8,12 @@ arrival to its result being ready.
     python -m calamarijoins_amd.serving [--scale S] [--plans K] [--requests N] [--check]
 
 prints one JSON line with p50 / p99 latency and rows/s of both modes, the
-admission budget (by default twice the median plan footprint, so that it
-binds) and the serving loop's counters.  --check then serves the whole stream
+admission budget (by default twice the median plan footprint bounded from
+the plans' shapes; --tight: 1.5 times the largest footprint the server
+charges once the plans have run, their recorded peaks, so that it binds) and
+the serving loop's counters, and the capacity of both modes (the first
+--capacity requests issued at once: requests and rows per second).  --check
+then serves the whole stream
 once more and compares every result with its plan's result run alone (row
 multiset fingerprints, calamarijoins_amd.host.result_fingerprint).
 """
@@ -110,10 +114,14 @@ def main(argv=None):
     ap.add_argument("--requests", type=int, default=2000, help="arrivals per mode")
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--admit-gib", type=float, default=0.0,
-                    help="admission budget; 0 = twice the median plan footprint")
+                    help="admission budget; 0 = twice the median plan footprint from the plans' shapes")
+    ap.add_argument("--tight", action="store_true",
+                    help="admission budget 1.5 x the largest footprint the server charges (learned after a run)")
     ap.add_argument("--check", action="store_true", help="serve once more and compare with each plan alone")
     ap.add_argument("--load", type=float, default=0.75, help="QPS = load / mean plain service time")
     ap.add_argument("--seed", type=int, default=1005)
+    ap.add_argument("--capacity", type=int, default=1000,
+                    help="requests served back to back in each mode for the capacity figure (0: skip)")
     args = ap.parse_args(argv)
 
     t0 = time.time()
@@ -125,6 +133,9 @@ def main(argv=None):
     for tb in tables:
         eng.register(tb)
     flags = abi.CJ_EXEC_OUTPUT_DEVICE
+    # footprints from the plans' shapes (before any has run), then the ones
+    # the server charges once each plan has run: its recorded peak + 25 %
+    foot_shape = [Engine.footprint(p, eng) for p in plans]
     # plain service time of each plan (warm: decoded columns cached)
     for p in plans:
         eng.execute(p, flags=flags).release()
@@ -135,7 +146,12 @@ def main(argv=None):
         svc.append((time.perf_counter() - s) * 1e3)
     mean_svc = statistics.mean(svc)
     foot = [Engine.footprint(p, eng) for p in plans]
-    admit = int(args.admit_gib * (1 << 30)) if args.admit_gib > 0 else 2 * int(statistics.median(foot))
+    if args.admit_gib > 0:
+        admit = int(args.admit_gib * (1 << 30))
+    elif args.tight:
+        admit = int(1.5 * max(foot))  # binds: two of the largest plans never run together
+    else:
+        admit = 2 * int(statistics.median(foot_shape))
     qps = args.load / (mean_svc / 1e3)
     rng = random.Random(args.seed)
     order = [rng.randrange(len(plans)) for _ in range(args.requests)]
@@ -149,6 +165,21 @@ def main(argv=None):
         stats = eng.serve_stats()
     finally:
         eng.serve_stop()
+    # capacity: the same requests all at once (no arrival waits), one at a
+    # time against the serving loop, as requests per second
+    cap = None
+    if args.capacity:
+        ncap = min(args.capacity, len(order))
+        zero = [0.0] * ncap
+        _, rows_pc, span_pc = run_plain(eng, plans, order[:ncap], zero, flags)
+        eng.serve_start(args.workers, admit)
+        try:
+            _, rows_sc, span_sc = run_served(eng, plans, order[:ncap], zero, flags)
+        finally:
+            eng.serve_stop()
+        cap = {"requests": ncap, "plain_qps": ncap / span_pc, "served_qps": ncap / span_sc,
+               "plain_rows_per_s": rows_pc / span_pc, "served_rows_per_s": rows_sc / span_sc}
+        print(f"capacity: {cap}", file=sys.stderr, flush=True)
     check = None
     if args.check:
         alone = {}
@@ -180,6 +211,9 @@ def main(argv=None):
                    "rows_per_s": rows_s / span_s, "makespan_s": span_s, "workers": args.workers,
                    "admit_bytes": admit, "stats": stats},
         "plan_footprint_bytes": {"min": min(foot), "median": statistics.median(foot), "max": max(foot)},
+        "plan_footprint_bytes_from_shape": {"min": min(foot_shape), "median": statistics.median(foot_shape),
+                                            "max": max(foot_shape)},
+        "capacity": cap,
         "check_served_equals_alone": check,
     }
     print(json.dumps(out), flush=True)

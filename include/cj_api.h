@@ -263,7 +263,10 @@ typedef struct cj_serve_stats {
 } cj_serve_stats;
 int cj_serve_stats_get(cj_ctx *ctx, cj_serve_stats *out);
 /* Estimated device footprint the server of ctx charges for a plan; ctx may
- * be NULL (then as if no input were resident). */
+ * be NULL (then as if no input were resident).  An upper bound from the plan's
+ * shape (every join's output as large as its larger input), or, once the plan
+ * has run on ctx, its recorded peak plus a quarter when that is less: the
+ * peak device bytes (temporaries and result pages) of its executions. */
 int cj_plan_footprint(cj_ctx *ctx, const cj_plan_desc *plan, uint64_t *bytes);
 
 /* ---- batched execution of many independent plans (no counterpart in the

@@ -154,3 +154,29 @@ def test_gpu_served_budget_binds(engine, c5_small):
     finally:
         engine.serve_stop()
     assert st["budget_waits"] > 0 and st["peak_admitted_bytes"] <= max(budget, foot[-1]), (st, budget)
+
+
+def test_gpu_learned_footprint():
+    """A plan's footprint charge starts from the bound on its shape and, once
+    the plan has run on the context, falls to its recorded peak plus a
+    quarter: never below the device memory the run actually held."""
+    tables, meta = workloads.c5_tables(scale=0.01, keep_lo=0.5, keep_hi=0.95)
+    plans = [workloads.c5_plan(tables, meta, 5000 + k) for k in range(4)]
+    eng = cjtest.Engine(cache_decoded=0)
+    try:
+        for tb in tables:
+            eng.register(tb)
+        for pl in plans:
+            shape = eng.footprint(pl, eng)
+            base = eng.memory(reset_peak=True)["in_use"]
+            r = eng.execute(pl, flags=abi.CJ_EXEC_OUTPUT_DEVICE)
+            held = eng.memory()["peak_in_use"] - base
+            r.release()
+            charged = eng.footprint(pl, eng)
+            assert charged <= shape, (charged, shape)
+            assert charged >= held, (charged, held)
+            # a repeat keeps the charge (the recorded peak is the largest seen)
+            eng.execute(pl, flags=abi.CJ_EXEC_OUTPUT_DEVICE).release()
+            assert eng.footprint(pl, eng) >= charged
+    finally:
+        eng.close()
