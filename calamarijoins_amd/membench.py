@@ -50,6 +50,19 @@ def gather_ms(idx: np.ndarray, table_bytes: int, record_bytes: int, reps: int = 
     return ms.value
 
 
+def gather16_mode_ms(idx: np.ndarray, table_bytes: int, mode: int, reps: int = 5) -> float:
+    """16-byte records: mode 0 plain loads, 1 non-temporal loads, 2 lookups
+    routed by table eighth to the XCDs (cj_membench_gather16_mode)."""
+    lib = _lib()
+    lib.cj_membench_gather16_mode.restype = C.c_int
+    lib.cj_membench_gather16_mode.argtypes = [C.c_int32, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32,
+                                              C.POINTER(C.c_double)]
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    ms = C.c_double()
+    abi.check(lib.cj_membench_gather16_mode(0, idx.ctypes.data, len(idx), table_bytes, mode, reps, C.byref(ms)))
+    return ms.value
+
+
 def c2_lookup_slots(build_rows: int, probe_rows: int) -> np.ndarray:
     """Direct-map slots of the matched probe keys of the single large join,
     in probe order (its INT32 key pages hold no NULLs: values at +4)."""
@@ -87,6 +100,13 @@ def main():
         out[name] = {"lookups": len(idx), "table_bytes": tb, "record_bytes": w, "ms": ms,
                      "G_lookups_per_s": len(idx) / (ms / 1e3) / 1e9,
                      "GBps_requested": len(idx) * w / (ms / 1e3) / 1e9}
+        print(f"{name} done", file=sys.stderr, flush=True)
+    # the C2 lookup stream with another cache policy, or routed by table
+    # slice to the XCDs (an upper bound for XCD-aware routing of the gather)
+    for name, mode in (("gather16_c2_zipf_64MB_nontemporal", 1), ("gather16_c2_zipf_64MB_xcd_sliced", 2)):
+        ms = gather16_mode_ms(slots, table16, mode, args.reps)
+        out[name] = {"lookups": n, "table_bytes": table16, "record_bytes": 16, "ms": ms,
+                     "G_lookups_per_s": n / (ms / 1e3) / 1e9, "GBps_requested": n * 16 / (ms / 1e3) / 1e9}
         print(f"{name} done", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
