@@ -588,6 +588,10 @@ struct BuildPart {
     uint32_t cap;           // LDS layout: keys[cap], values of each carried column, validity bytes
     uint32_t ncar;
     uint32_t w8;            // bit c: carried column c is 8 bytes wide
+    // write pass: the tile's values of the probe side's carried column 0,
+    // staged in LDS by tile row (low words, high words, validity bytes)
+    uint32_t *pt_lo, *pt_hi;
+    uint8_t *pt_vb;
     // byte offset of carried column c's values / validity bytes in LDS
     __device__ uint32_t coff(uint32_t c) const {
         return cap * (4 + 4 * c + 4 * __popc(w8 & ((1u << c) - 1)));
@@ -645,10 +649,12 @@ struct BCol {
 };
 
 // The value of output column `col` (width W, source SRC: 0 the join key, 1
-// the build side, 2 the probe side) for match entry (key, build tuple t of
-// the partition, probe tuple pt): the key, carried build value g_car0 + t
-// (plan-local index of the slot-ordered carried columns) or carried probe
-// value pt (plan-local index of the partition-ordered carried columns).
+// the build side, 2 the probe side, 3 the probe side's carried column 0
+// staged in LDS for the tile) for match entry (key, build tuple t of the
+// partition, probe tuple pt): the key, carried build value g_car0 + t
+// (plan-local index of the slot-ordered carried columns), carried probe
+// value pt (plan-local index of the partition-ordered carried columns) or,
+// staged, the value of tile row pt.
 // `ok` is false for a NULL.
 template <int W, int SRC, bool LDS>
 __device__ __forceinline__ uint64_t bx_get(const BPlan &pl, const BCol &col, const BuildPart &bp, int32_t key,
@@ -660,6 +666,8 @@ __device__ __forceinline__ uint64_t bx_get(const BPlan &pl, const BCol &col, con
         } else if constexpr (SRC == 1) {
             if constexpr (LDS) v = lds(reinterpret_cast<const int32_t *>(bp.l_dyn + bp.coff(col.cj)))[t];
             else v = glb(reinterpret_cast<const int32_t *>(pl.bslot_car[col.cj]))[bp.g_car0 + t];
+        } else if constexpr (SRC == 3) {
+            v = (int32_t)lds(bp.pt_lo)[pt];
         } else {
             v = glb(reinterpret_cast<const int32_t *>(pl.p.car[col.cj]))[pt];
         }
@@ -676,6 +684,9 @@ __device__ __forceinline__ uint64_t bx_get(const BPlan &pl, const BCol &col, con
                 v = glb(reinterpret_cast<const uint64_t *>(pl.bslot_car[col.cj]))[bp.g_car0 + t];
                 b = glb(pl.bslot_vld[col.cj])[bp.g_car0 + t];
             }
+        } else if constexpr (SRC == 3) {
+            v = ((uint64_t)lds(bp.pt_hi)[pt] << 32) | lds(bp.pt_lo)[pt];
+            b = lds(bp.pt_vb)[pt];
         } else {
             v = glb(reinterpret_cast<const uint64_t *>(pl.p.car[col.cj]))[pt];
             b = glb(pl.p.car_vld[col.cj])[pt];
@@ -899,21 +910,50 @@ __device__ __forceinline__ void bx_write_desc(const uint32_t *desc, const BPlan 
                                               const BuildPart &bp, uint32_t nb, uint64_t p0, uint64_t p1,
                                               uint64_t pbase, uint64_t obase, uint32_t *lds32, uint32_t *win,
                                               uint32_t *page_nulls) {
+    static_assert(RPT <= 4, "the staged probe values hold four rows a thread");
     uint64_t base = obase;
-    uint32_t nd[RPT];
+    // the probe side's carried column 0 (if any) is loaded with the
+    // descriptors, a tile ahead, and staged in LDS, so emitting it reads no
+    // global memory
+    const bool stage = pl.p.ncar > 0;
+    const bool stage8 = stage && pl.p.car_w[0] == 8;
+    const CJ_G uint32_t *c32 = glb(reinterpret_cast<const uint32_t *>(pl.p.car[0]));
+    const CJ_G uint8_t *cvb = glb(pl.p.car_vld[0]);
+    uint32_t nd[RPT], nlo[RPT], nhi[RPT], nvb[RPT];
+    auto load_tile = [&](uint64_t t0) {
 #pragma unroll
-    for (uint32_t j = 0; j < RPT; ++j) {
-        const uint64_t idx = p0 + kWG * j + threadIdx.x;
-        nd[j] = desc[idx < p1 ? idx : p1 - 1];
-    }
+        for (uint32_t j = 0; j < RPT; ++j) {
+            const uint64_t idx = t0 + kWG * j + threadIdx.x;
+            const uint64_t ic = idx < p1 ? idx : p1 - 1;
+            nd[j] = desc[ic];
+            const uint64_t pi = ic - pbase;  // plan-local index of the carried values
+            nlo[j] = stage ? (stage8 ? c32[2 * pi] : c32[pi]) : 0u;
+            nhi[j] = stage8 ? c32[2 * pi + 1] : 0u;
+            nvb[j] = stage8 ? cvb[pi] : 0u;
+        }
+    };
+    load_tile(p0);
     const uint32_t w = wave_id(), lane = lane_id();
     for (uint64_t i0 = p0; i0 < p1; i0 += kWG * RPT) {
         uint32_t m[RPT], f[RPT], mt = 0;
+        uint32_t d_[RPT];
 #pragma unroll
         for (uint32_t j = 0; j < RPT; ++j) {
-            const uint32_t d = nd[j];
-            const uint64_t idx = i0 + kWG * (RPT + j) + threadIdx.x;
-            nd[j] = desc[idx < p1 ? idx : p1 - 1];
+            d_[j] = nd[j];
+            // staged for this tile (read after the window barrier; the
+            // previous tile's emission ended with a barrier)
+            if (stage) {
+                lds(bp.pt_lo)[kWG * j + threadIdx.x] = nlo[j];
+                if (stage8) {
+                    lds(bp.pt_hi)[kWG * j + threadIdx.x] = nhi[j];
+                    lds(bp.pt_vb)[kWG * j + threadIdx.x] = (uint8_t)nvb[j];
+                }
+            }
+        }
+        load_tile(i0 + kWG * RPT);
+#pragma unroll
+        for (uint32_t j = 0; j < RPT; ++j) {
+            const uint32_t d = d_[j];
             const bool in = i0 + kWG * j + threadIdx.x < p1;
             f[j] = d & kDescFirst;
             m[j] = in ? d >> kDescFirstBits : 0u;
@@ -954,18 +994,25 @@ __device__ __forceinline__ void bx_write_desc(const uint32_t *desc, const BPlan 
 #pragma unroll 1
             for (uint32_t c = 0; c < bo->nout; ++c) {
                 const BCol col{bo->src[c], bo->j[c], bo->w[c], bo->rows, bo->pg0[c], bo->out[c]};
+                const bool staged = col.src == 2 && col.cj == 0;  // (stage holds: column 0 exists)
                 if (col.w == 4) {
                     if (col.src == 0) bx_emit_col<4, 0, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
                     else if (col.src == 1) bx_emit_col<4, 1, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                    else if (staged) bx_emit_col<4, 3, LDS>(pl, col, bp, win, cnt, ob, 0u, page_nulls);
                     else bx_emit_col<4, 2, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
                 } else {  // 8-byte columns are never keys
                     if (col.src == 1) bx_emit_col<8, 1, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
+                    else if (staged) bx_emit_col<8, 3, LDS>(pl, col, bp, win, cnt, ob, 0u, page_nulls);
                     else bx_emit_col<8, 2, LDS>(pl, col, bp, win, cnt, ob, pt0, page_nulls);
                 }
             }
             __syncthreads();
         }
         base += run;
+        // the next tile's staging overwrites the probe values: every
+        // thread is past this tile's emission (the window loop ends with a
+        // barrier; a tile without matches has none, so wait here)
+        if (!run) __syncthreads();
     }
 }
 
@@ -1004,6 +1051,9 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
     __shared__ uint64_t red[4];
     __shared__ uint32_t unit;
     __shared__ uint32_t win[WRITE ? kMatchWin : 1];
+    // write pass: the tile's values of the probe side's carried column 0
+    __shared__ uint32_t ptile_lo[WRITE ? kWG * 4 : 1], ptile_hi[WRITE ? kWG * 4 : 1];
+    __shared__ uint8_t ptile_vb[WRITE ? kWG * 4 : 1];
     const uint64_t total = uoff[a.nparts];  // < 2^32 (max_units)
     const uint32_t S = 1u << a.log_part;
     uint32_t *q = a.queue + (WRITE ? kQJoinWrite : kQJoinCount);
@@ -1030,6 +1080,9 @@ __global__ __launch_bounds__(kWG) void k_bx_join(BatchArgs a, const BOut *__rest
         bp.ncar = ncar;
         bp.w8 = 0;
         for (uint32_t c = 0; c < ncar; ++c) bp.w8 |= (pl.b.car_w[c] == 8 ? 1u : 0u) << c;
+        bp.pt_lo = ptile_lo;
+        bp.pt_hi = ptile_hi;
+        bp.pt_vb = ptile_vb;
         uint64_t obase = 0;
         if (WRITE) obase = moff[u] - moff[uoff[part_start[k]]];
         // the plan's output columns, read through the pointer (uniform
