@@ -976,7 +976,7 @@ Rel Exec::join(const cj_node_desc &n, bool is_root, int depth, cj_result *out) {
         }
         {
             TimedScope ts(timer, use_nl ? "nl_probe_count" : "probe_count", key_bytes(pk));
-            if (bits) probe_count_match(fa, cnt, match_bits, st);
+            if (bits) probe_count_match(fa, cnt, match_bits, column(P.slots[0].table, P.cols[pattr].col)->dense_rows, st);
             else if (inter_bits) inter_count_match(fa, cnt, match_bits, st);
             else probe_count_columnar(fa, cnt, st);
         }

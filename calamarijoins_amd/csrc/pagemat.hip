@@ -178,6 +178,10 @@ __device__ __forceinline__ uint32_t count_dense_page(const DirectTable &dt, cons
     return total;
 }
 
+// DENSE: every key page holds n == nv <= kTile rows (known on the host from
+// the column), so only count_dense_page is compiled in: the general path's
+// batched registers would cap the occupancy at 3 waves a SIMD.
+template <bool DENSE>
 __global__ __launch_bounds__(kWG) void k_count_match_direct(ProbeArgs a, uint64_t *counts, uint32_t *match_bits) {
     __shared__ uint32_t msk_all[kWG / 64][2 * (kTile / 64) + 2];
     const uint32_t lane = lane_id();
@@ -200,7 +204,7 @@ __global__ __launch_bounds__(kWG) void k_count_match_direct(ProbeArgs a, uint64_
     uint32_t *msk = msk_all[wave_id()];
     const uint64_t lt = lanemask_lt();
     const uint32_t nlast = n ? n - 1 : 0, vlast = nv ? nv - 1 : 0;
-    if (dense && n <= kTile) {
+    if (DENSE || (dense && n <= kTile)) {
         const uint32_t total = count_dense_page(dt, kv, n, msk, lane);
         if (lane == 0) msk[2 * (kTile / 64)] = msk[2 * (kTile / 64) + 1] = 0;
         __builtin_amdgcn_wave_barrier();
@@ -208,82 +212,86 @@ __global__ __launch_bounds__(kWG) void k_count_match_direct(ProbeArgs a, uint64_
         if (lane == 0) counts[pg] = total;
         return;
     }
-    uint32_t vbase = 0;
-    uint64_t total = 0;
-    for (uint32_t s0 = 0; s0 < n; s0 += kTile) {
-        const uint32_t nseg = n - s0 < kTile ? n - s0 : kTile;
-        for (uint32_t b0 = 0; b0 < kTile / 64; b0 += kCountBatch) {
-            uint32_t vbits = 0, vi[kCountBatch];
-            if (dense) {
+    if constexpr (!DENSE) {  // sparse pages, or pages of more than kTile rows
+        uint32_t vbase = 0;
+        uint64_t total = 0;
+        for (uint32_t s0 = 0; s0 < n; s0 += kTile) {
+            const uint32_t nseg = n - s0 < kTile ? n - s0 : kTile;
+            for (uint32_t b0 = 0; b0 < kTile / 64; b0 += kCountBatch) {
+                uint32_t vbits = 0, vi[kCountBatch];
+                if (dense) {
+#pragma unroll
+                    for (uint32_t c = 0; c < kCountBatch; ++c) {
+                        const uint32_t lr = s0 + 64 * (b0 + c) + lane;
+                        vi[c] = lr;
+                        if (lr < n) vbits |= 1u << c;
+                    }
+                } else {
+                    uint32_t byte[kCountBatch];
+#pragma unroll
+                    for (uint32_t c = 0; c < kCountBatch; ++c) {
+                        const uint32_t lr = s0 + 64 * (b0 + c) + lane;
+                        byte[c] = bm[(lr < n ? lr : nlast) >> 3];
+                    }
+#pragma unroll
+                    for (uint32_t c = 0; c < kCountBatch; ++c) {
+                        const uint32_t lr = s0 + 64 * (b0 + c) + lane;
+                        const bool v = lr < n && ((byte[c] >> (lr & 7)) & 1u);
+                        const uint64_t vm = __ballot(v);
+                        vi[c] = vbase + (uint32_t)__popcll(vm & lt);
+                        vbase += (uint32_t)__popcll(vm);
+                        if (v) vbits |= 1u << c;
+                    }
+                }
+                // all key loads first, unconditionally (a load under a
+                // short-circuit condition is issued and awaited one at a time)
+                int32_t key[kCountBatch];
+#pragma unroll
+                for (uint32_t c = 0; c < kCountBatch; ++c) key[c] = keys[vi[c] < nv ? vi[c] : vlast];
+                uint32_t slot[kCountBatch], word[kCountBatch], in = 0;
 #pragma unroll
                 for (uint32_t c = 0; c < kCountBatch; ++c) {
-                    const uint32_t lr = s0 + 64 * (b0 + c) + lane;
-                    vi[c] = lr;
-                    if (lr < n) vbits |= 1u << c;
+                    const int64_t off = (int64_t)key[c] - (int64_t)dt.min_key;
+                    const bool ok = ((vbits >> c) & 1u) & (vi[c] < nv) & (off >= 0) & ((uint64_t)off < dt.range);
+                    slot[c] = ok ? (uint32_t)off : 0u;
+                    in |= (uint32_t)ok << c;
                 }
-            } else {
-                uint32_t byte[kCountBatch];
+                if (dt.full) {
+#pragma unroll
+                    for (uint32_t c = 0; c < kCountBatch; ++c) word[c] = ~0u;
+                } else {
+#pragma unroll
+                    for (uint32_t c = 0; c < kCountBatch; ++c) word[c] = dt.presence[slot[c] >> 5];
+                }
 #pragma unroll
                 for (uint32_t c = 0; c < kCountBatch; ++c) {
-                    const uint32_t lr = s0 + 64 * (b0 + c) + lane;
-                    byte[c] = bm[(lr < n ? lr : nlast) >> 3];
-                }
-#pragma unroll
-                for (uint32_t c = 0; c < kCountBatch; ++c) {
-                    const uint32_t lr = s0 + 64 * (b0 + c) + lane;
-                    const bool v = lr < n && ((byte[c] >> (lr & 7)) & 1u);
-                    const uint64_t vm = __ballot(v);
-                    vi[c] = vbase + (uint32_t)__popcll(vm & lt);
-                    vbase += (uint32_t)__popcll(vm);
-                    if (v) vbits |= 1u << c;
+                    const bool m = ((in >> c) & 1u) && ((word[c] >> (slot[c] & 31)) & 1u);
+                    const uint64_t mm = __ballot(m);
+                    total += (uint32_t)__popcll(mm);
+                    if (lane == 0) {
+                        msk[2 * (b0 + c)] = (uint32_t)mm;
+                        msk[2 * (b0 + c) + 1] = (uint32_t)(mm >> 32);
+                    }
                 }
             }
-            // all key loads first, unconditionally (a load under a
-            // short-circuit condition is issued and awaited one at a time)
-            int32_t key[kCountBatch];
-#pragma unroll
-            for (uint32_t c = 0; c < kCountBatch; ++c) key[c] = keys[vi[c] < nv ? vi[c] : vlast];
-            uint32_t slot[kCountBatch], word[kCountBatch], in = 0;
-#pragma unroll
-            for (uint32_t c = 0; c < kCountBatch; ++c) {
-                const int64_t off = (int64_t)key[c] - (int64_t)dt.min_key;
-                const bool ok = ((vbits >> c) & 1u) & (vi[c] < nv) & (off >= 0) & ((uint64_t)off < dt.range);
-                slot[c] = ok ? (uint32_t)off : 0u;
-                in |= (uint32_t)ok << c;
-            }
-            if (dt.full) {
-#pragma unroll
-                for (uint32_t c = 0; c < kCountBatch; ++c) word[c] = ~0u;
-            } else {
-#pragma unroll
-                for (uint32_t c = 0; c < kCountBatch; ++c) word[c] = dt.presence[slot[c] >> 5];
-            }
-#pragma unroll
-            for (uint32_t c = 0; c < kCountBatch; ++c) {
-                const bool m = ((in >> c) & 1u) && ((word[c] >> (slot[c] & 31)) & 1u);
-                const uint64_t mm = __ballot(m);
-                total += (uint32_t)__popcll(mm);
-                if (lane == 0) {
-                    msk[2 * (b0 + c)] = (uint32_t)mm;
-                    msk[2 * (b0 + c) + 1] = (uint32_t)(mm >> 32);
-                }
-            }
+            if (lane == 0) msk[2 * (kTile / 64)] = msk[2 * (kTile / 64) + 1] = 0;
+            __builtin_amdgcn_wave_barrier();
+            store_segment_bits(msk, match_bits, r0 + s0, nseg, lane);
+            __builtin_amdgcn_wave_barrier();
         }
-        if (lane == 0) msk[2 * (kTile / 64)] = msk[2 * (kTile / 64) + 1] = 0;
-        __builtin_amdgcn_wave_barrier();
-        store_segment_bits(msk, match_bits, r0 + s0, nseg, lane);
-        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) counts[pg] = total;
     }
-    if (lane == 0) counts[pg] = total;
 }
 
-void probe_count_match(const ProbeArgs &a, uint64_t *counts, uint32_t *match_bits, hipStream_t s) {
+void probe_count_match(const ProbeArgs &a, uint64_t *counts, uint32_t *match_bits, uint32_t dense_rows,
+                       hipStream_t s) {
     const uint32_t units = a.key.num_pages;
     if (!units) return;
     const uint32_t grid = units < kPagesGrid ? units : kPagesGrid;
     if (a.mode == kModeDirect) {
-        const uint32_t per = kWG / 64;
-        k_count_match_direct<<<(units + per - 1) / per, kWG, 0, s>>>(a, counts, match_bits);
+        const uint32_t per = kWG / 64, g = (units + per - 1) / per;
+        if (dense_rows && dense_rows <= kTile) k_count_match_direct<true><<<g, kWG, 0, s>>>(a, counts, match_bits);
+        else k_count_match_direct<false><<<g, kWG, 0, s>>>(a, counts, match_bits);
     } else if (a.mode == kModeNL) {
         k_probe_count_match<kModeNL><<<grid, kWG, 0, s>>>(a, counts, match_bits);
     } else {
@@ -874,16 +882,31 @@ void page_match_flags(const uint8_t *pages, uint32_t num_pages, const uint64_t *
 constexpr uint32_t kWaveMaxRows = 4096;  // pages k_probe_filter_wave handles
 constexpr uint32_t kWaveBatch = 16;      // 64-row chunks whose loads are issued together
 
-// One wave per payload page of at most kWaveMaxRows rows, no barriers: for
-// each batch of 16 (8-byte values: 8) chunks of 64 rows, the page-bitmap bytes and match words
-// are loaded, value indices come from ballots of the validity bits, all
-// value loads are issued, then matched values are stored compacted and the
-// output bitmap is assembled in registers (lane j holds words j and 64 + j).
+// Set bits of m below this lane.
+__device__ __forceinline__ uint32_t rank_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// One wave per payload page of at most kWaveMaxRows rows, no barriers.  The
+// page's validity words and match words are loaded once, lane j holding
+// words j and 64 + j (a validity word from two aligned loads of the
+// byte-aligned page bitmap, a match word from two words of the global
+// bitmap: funnel shifts).  A 64-row chunk then takes its two words of each
+// from those lanes into scalar registers (readlane): the value index of a
+// row is the chunk's value base plus the valid rows below it (mbcnt), the
+// chunk's value loads are issued for a batch of 16 (8-byte values: 8)
+// chunks, and the matched values are stored compacted.  The output validity
+// bits are the matched rows' bits in order: all ones up to the match count
+// but for matched rows without a value (NULLs), whose ranks a scalar loop
+// clears; they are assembled in registers (lane j holds output words j and
+// 64 + j).  Vector work per chunk is a few lane ops; the bitmaps' bit
+// arithmetic runs on the scalar unit.
 template <int W>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8))) void k_probe_filter_wave(FilterArgs a) {
     using T = typename std::conditional<W == 4, uint32_t, uint64_t>::type;
     // 8-byte values: half the batch, so that 8 waves fit a SIMD
     constexpr uint32_t kB = W == 4 ? kWaveBatch : kWaveBatch / 2;
+    static_assert(32 % kB == 0, "a batch's chunks take their words from one register of the staged pair");
     const uint32_t lane = lane_id();
     const uint32_t q = blockIdx.x * (kWG / 64) + wave_id();
     if (q >= a.num_pages) return;
@@ -893,79 +916,90 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8))) void k
     const uint32_t n = ld16(p), nv = ld16(p + 2);
     if (n > kWaveMaxRows) return;  // k_probe_filter's page
     const uint64_t rs = a.row_start[q];
-    const bool dense = n == nv;
-    const uint8_t *bm = p + kPage - ((n + 7) >> 3);
     const T *vals = reinterpret_cast<const T *>(p + (W == 4 ? 4 : 8));
     uint8_t *out = a.out + o * kPage;
     T *ovals = reinterpret_cast<T *>(out + (W == 4 ? 4 : 8));
+    // staged words: rows 32 w .. 32 w + 31 of the page, w = lane + 64 h (bits
+    // of rows past n cleared; loads clamped into the page and the bitmap)
+    const uint32_t bm0 = kPage - ((n + 7) >> 3);  // page offset of the bitmap
     const uint32_t *mw = a.match_bits + (rs >> 5);
     const uint32_t msh = (uint32_t)(rs & 31);
+    uint32_t vword[2], mword[2];
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t w = lane + 64 * h, row0 = 32 * w;
+        const uint32_t keep = row0 >= n ? 0u : (n - row0 >= 32 ? ~0u : (1u << (n - row0)) - 1u);
+        const uint32_t wc = row0 < n ? w : 0u;
+        const uint32_t off = bm0 + 4 * wc;
+        const uint32_t a0 = (off & ~3u) < kPage - 4 ? (off & ~3u) : kPage - 4;
+        const uint32_t a1 = a0 + 4 < kPage ? a0 + 4 : a0;  // bytes past the page end are rows past n
+        const uint32_t d0 = *reinterpret_cast<const uint32_t *>(p + a0);
+        const uint32_t d1 = *reinterpret_cast<const uint32_t *>(p + a1);
+        const uint32_t m0 = mw[wc], m1 = mw[wc + 1];
+        vword[h] = (n == nv ? ~0u : __builtin_amdgcn_alignbyte(d1, d0, off & 3u)) & keep;
+        mword[h] = __builtin_amdgcn_alignbit(m1, m0, msh) & keep;
+    }
     // lane j: words j and 64 + j of the output validity bitmap (a page of
-    // this kernel has at most 4096 rows)
-    uint32_t bw = 0, bw1 = 0;
-    const uint64_t lt = lanemask_lt();
+    // this kernel has at most 4096 rows): all ones up to the result's row
+    // count (cut at the end) but for the result rows of matched rows
+    // without a value, cleared as they are found
+    uint32_t bw = ~0u, bw1 = ~0u;
     uint32_t vbase = 0, mbase = 0, obase = 0;
-    // every load below is unconditional with a clamped in-page address, so
-    // the compiler issues a whole batch before the first wait
-    const uint32_t nlast = n - 1, vlast = nv ? nv - 1 : 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += 64 * kB) {
-        uint32_t vbits = 0, mbits = 0;
-        uint32_t byte[kB], word[kB];
-#pragma unroll
-        for (uint32_t c = 0; c < kB; ++c) {
-            const uint32_t lr = c0 + 64 * c + lane;
-            const uint32_t lc = lr < n ? lr : nlast;
-            byte[c] = bm[lc >> 3];
-            word[c] = mw[(msh + lc) >> 5];
-        }
-#pragma unroll
-        for (uint32_t c = 0; c < kB; ++c) {
-            const uint32_t lr = c0 + 64 * c + lane;
-            const uint32_t lc = lr < n ? lr : nlast;
-            const bool in = lr < n;
-            if (in && (dense || ((byte[c] >> (lc & 7)) & 1u))) vbits |= 1u << c;
-            if (in && ((word[c] >> ((msh + lc) & 31)) & 1u)) mbits |= 1u << c;
-        }
+    const uint32_t vlast = nv ? nv - 1 : 0, nch = (n + 63) >> 6;
+    for (uint32_t c0 = 0; c0 < nch; c0 += kB) {
+        // chunk c0 + c: words 2 (c0 + c) and 2 (c0 + c) + 1, all in register
+        // c0 / 32 of the staged pair (chunks past the page's end read
+        // cleared words)
+        const uint32_t vsrc = c0 >= 32 ? vword[1] : vword[0], msrc = c0 >= 32 ? mword[1] : mword[0];
         T v[kB];
-        uint32_t okbits = 0;
+        uint32_t vi[kB];
 #pragma unroll
         for (uint32_t c = 0; c < kB; ++c) {
-            const uint64_t vm = __ballot((vbits >> c) & 1u);
-            const uint32_t vi = vbase + (uint32_t)__popcll(vm & lt);
-            vbase += (uint32_t)__popcll(vm);
-            v[c] = vals[vi < nv ? vi : vlast];
-            if ((((vbits & mbits) >> c) & 1u) && vi < nv) okbits |= 1u << c;
+            const uint32_t l = (2 * (c0 + c)) & 63;
+            const uint32_t vlo = (uint32_t)__builtin_amdgcn_readlane(vsrc, l);
+            const uint32_t vhi = (uint32_t)__builtin_amdgcn_readlane(vsrc, l + 1);
+            vi[c] = vbase + __builtin_amdgcn_mbcnt_hi(vhi, __builtin_amdgcn_mbcnt_lo(vlo, 0u));
+            v[c] = vals[vi[c] < nv ? vi[c] : vlast];
+            vbase += (uint32_t)__popc(vlo) + (uint32_t)__popc(vhi);
         }
+        uint32_t lb = (2 * c0) & 63;
 #pragma unroll
         for (uint32_t c = 0; c < kB; ++c) {
-            bool ok = (okbits >> c) & 1u;
+            // (an opaque lane base: the chunk's word reads stay here instead
+            // of all being hoisted into scalar registers)
+            asm volatile("" : "+s"(lb));
+            const uint32_t l = lb + 2 * c;
+            const uint64_t vm = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vsrc, l + 1) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane(vsrc, l);
+            const uint64_t mc = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(msrc, l + 1) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane(msrc, l);
+            // matched and valid (the lane masks are the scalar words), with
+            // the value index inside the page's values
+            bool ok = __builtin_amdgcn_inverse_ballot_w64(vm & mc) && vi[c] < nv;
             if (W == 4 && a.is_int32 && (int32_t)(uint32_t)v[c] == kI32Null) ok = false;  // page_builders.h:123
             const uint64_t om = __ballot(ok);
-            const bool matched = (mbits >> c) & 1u;
-            const uint64_t mm = __ballot(matched);
-            if (ok) ovals[obase + (uint32_t)__popcll(om & lt)] = v[c];
-            // validity bits of the chunk's result rows (matched lanes in
-            // order), OR-ed into the lanes' bitmap words
-            const uint32_t tm = (uint32_t)__popcll(mm);
-            uint64_t pk;
-            if (om == mm) {
-                pk = tm == 64 ? ~0ull : (1ull << tm) - 1ull;
-            } else {
-                const uint32_t rank = (uint32_t)__popcll(mm & lt);
-                const uint32_t dst = matched ? rank : tm + (lane - rank);
-                const int got = __builtin_amdgcn_ds_permute((int)(dst << 2), ok ? 1 : 0);
-                pk = __ballot(lane < tm && got != 0);
-            }
-            if (tm) {
-                or_bits(bw, pk, mbase, lane);
-                if (mbase + tm > 2048) {
-                    if (mbase >= 2048) or_bits(bw1, pk, mbase - 2048, lane);
-                    else or_bits(bw1, pk >> (2048 - mbase), 0, lane);
+            if (ok) ovals[obase + rank_below(om)] = v[c];
+            // the chunk's result rows are its matched rows in row order; one
+            // without a value (a hole) leaves its bit clear
+            for (uint64_t holes = mc & ~om; holes; holes &= holes - 1) {
+                const uint32_t pos = mbase + (uint32_t)__popcll(mc & ((1ull << __builtin_ctzll(holes)) - 1ull));
+                const uint32_t wd = pos >> 5, bit = 1u << (pos & 31);
+                if (lane == (wd & 63)) {
+                    if (wd < 64) bw &= ~bit;
+                    else bw1 &= ~bit;
                 }
             }
             obase += (uint32_t)__popcll(om);
-            mbase += tm;
+            mbase += (uint32_t)__popcll(mc);
         }
+    }
+    // bits past the result's rows cleared
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t r0w = 32 * (lane + 64 * h);
+        const uint32_t keep = r0w >= mbase ? 0u : (mbase - r0w >= 32 ? ~0u : (1u << (mbase - r0w)) - 1u);
+        if (h) bw1 &= keep;
+        else bw &= keep;
     }
     if (mbase == 0) {  // (fixed slots only) the slot stays unwritten
         if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long *>(a.empty_pages), 1ull);

@@ -15,8 +15,10 @@
 namespace cj {
 
 // Counting pass that also sets bit r of `match_bits` (zeroed by the caller)
-// for every probe row r with a match.
-void probe_count_match(const ProbeArgs &a, uint64_t *counts, uint32_t *match_bits, hipStream_t s);
+// for every probe row r with a match.  dense_rows: the key column's pages
+// hold no NULLs and at most this many rows each (0: unknown).
+void probe_count_match(const ProbeArgs &a, uint64_t *counts, uint32_t *match_bits, uint32_t dense_rows,
+                       hipStream_t s);
 
 // Pair lists from the match bits of probe_count_match (direct mode, key
 // pages): the matched rows of key page pg, in row order, as (build id, probe
