@@ -32,8 +32,11 @@ __device__ __forceinline__ uint32_t next_unit(bool queue, uint32_t *q, uint32_t 
     return g;
 }
 
-__device__ __forceinline__ uint64_t bslot_of(int32_t key, uint32_t log_slots) {
-    return hash_key(key) >> (64 - log_slots);
+// Directory slot of a key in a plan of 2^log_slots slots (log_slots <= 30):
+// the top bits of a 32-bit multiplicative (Fibonacci) hash, one multiply.
+// Keys of a dense domain spread evenly over the slots.
+__device__ __forceinline__ uint32_t bslot_of(int32_t key, uint32_t log_slots) {
+    return ((uint32_t)key * 0x9E3779B1u) >> (32 - log_slots);
 }
 
 // Largest k < n with start[k] <= g (entries with no units share a start).
@@ -54,37 +57,56 @@ __device__ __forceinline__ uint32_t find_owner(const T *start, uint32_t n, T g) 
 // Partition of a key = top bits of its directory slot (a plan's partitions
 // own contiguous slot ranges).  NULL keys take no part in a join
 // (hashtable.h:439-478, hash_join.h:156-244).
+//
+// A workgroup takes a contiguous range of key pages (so consecutive pages of
+// one plan) and keeps one LDS histogram while the plan stays the same; it
+// adds the histogram to the global counts when the plan changes and at the
+// end -- one global atomic per partition per range and plan instead of per
+// page.  Counting needs no row positions: a page's non-NULL keys are its
+// value array, read directly (the bitmap only places them in rows).
 template <bool PROBE>
 __global__ __launch_bounds__(kWG) void k_bx_part_count(BatchArgs a, uint32_t units, const BPlan *__restrict__ plans,
                                                        const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ part_start) {
     extern __shared__ uint32_t hist[];  // max_parts
-    __shared__ uint32_t lds32[32];
     uint64_t *count = PROBE ? a.pcount : a.bcount;
     const uint32_t lp = a.log_part;
-    for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
-        const uint32_t k = find_owner(pstart, a.nplans, g);
-        const BPlan &pl = plans[k];
-        const BSide &sd = PROBE ? pl.p : pl.b;
-        const uint32_t ls = pl.log_slots;
-        const uint32_t P = 1u << (ls - lp);
-        for (uint32_t i = threadIdx.x; i < P; i += kWG) hist[i] = 0;
-        __syncthreads();
-        const CJ_G uint8_t *p = glb(sd.k_pages) + (size_t)(g - pstart[k]) * kPage;
-        const uint32_t n = ld16(p), nv = ld16(p + 2);
-        uint32_t running = 0;
-        for (uint32_t t0 = 0; t0 < n; t0 += kTile) {
-            int32_t key[kRPT];
-            const uint32_t valid = load_keys8(p, n, nv, n == nv, t0, &running, key, lds32);
-#pragma unroll
-            for (uint32_t j = 0; j < kRPT; ++j)
-                if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(key[j], ls) >> lp], 1u);
-        }
+    const uint32_t per = (units + gridDim.x - 1) / gridDim.x;
+    const uint32_t g0 = blockIdx.x * per, g1 = g0 + per < units ? g0 + per : units;
+    uint32_t k = ~0u, P = 0, ls = 0;
+    auto flush = [&]() {
         __syncthreads();
         const uint32_t pb = part_start[k];
         for (uint32_t i = threadIdx.x; i < P; i += kWG)
             if (hist[i]) atomicAdd((unsigned long long *)&count[pb + i], (unsigned long long)hist[i]);
-        __syncthreads();
+    };
+    for (uint32_t g = g0; g < g1; ++g) {
+        const uint32_t kg = find_owner(pstart, a.nplans, g);
+        if (kg != k) {
+            if (k != ~0u) flush();
+            __syncthreads();  // the histogram is read before it is cleared
+            k = kg;
+            ls = plans[k].log_slots;
+            P = 1u << (ls - lp);
+            for (uint32_t i = threadIdx.x; i < P; i += kWG) hist[i] = 0;
+            __syncthreads();
+        }
+        const BSide &sd = PROBE ? plans[k].p : plans[k].b;
+        const CJ_G uint8_t *p = glb(sd.k_pages) + (size_t)(g - pstart[k]) * kPage;
+        const uint32_t nv = ld16(p + 2);
+        const CJ_G int32_t *vals = reinterpret_cast<const CJ_G int32_t *>(p + 4);
+        for (uint32_t v0 = 0; v0 < nv; v0 += kTile) {
+            int32_t key[kRPT];
+#pragma unroll
+            for (uint32_t j = 0; j < kRPT; ++j) {
+                const uint32_t v = v0 + kWG * j + threadIdx.x;
+                key[j] = vals[v < nv ? v : nv - 1];
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kRPT; ++j)
+                if (v0 + kWG * j + threadIdx.x < nv) atomicAdd(&hist[bslot_of(key[j], ls) >> lp], 1u);
+        }
     }
+    if (k != ~0u) flush();
 }
 
 // ------------------------------------------------------ partition scatter

@@ -331,6 +331,14 @@ static void validate_page(const uint8_t *p, int32_t type, uint32_t idx, uint16_t
     size_t bm = (n + 7u) / 8u;
     size_t need = type == CJ_VARCHAR ? 4 + 2 * (size_t)nv : (type == CJ_INT32 ? 4 + 4 * (size_t)nv : 8 + 8 * (size_t)nv);
     if (need + bm > kPage) throw InvalidError("page values and bitmap overflow 8 KiB");
+    if (nv < n) {  // a page with NULLs: its values are the rows whose bit is set, so the two counts agree
+        const uint8_t *b = p + kPage - bm;
+        uint32_t ones = 0;
+        for (size_t i = 0; i + 1 < bm; ++i) ones += (uint32_t)__builtin_popcount(b[i]);
+        const uint32_t tail = n & 7u;
+        ones += (uint32_t)__builtin_popcount(tail ? b[bm - 1] & ((1u << tail) - 1u) : b[bm - 1]);
+        if (ones != nv) throw InvalidError("page bitmap marks a different number of values than its header");
+    }
 }
 
 // Upload a column's pages (gathered through pinned staging, 32 MiB chunks,
