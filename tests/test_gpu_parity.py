@@ -246,6 +246,37 @@ def test_gpu_rejects_bad_plans(engine):
         engine.execute(plan)
 
 
+def test_gpu_rejects_page_bitmap_value_count_mismatch(engine):
+    """A page with NULLs lists exactly its valid rows' values: a bitmap that
+    marks one valid row fewer than the header's value count is rejected at
+    upload (kernels that count a page's keys from its value array and ones
+    that place them by the bitmap must see the same keys)."""
+    rows = [[i if i % 3 else None, i] for i in range(300)]
+    good = cjtest.table_from_rows(["INT32", "INT32"], rows)
+    bad = cjtest.table_from_rows(["INT32", "INT32"], rows)
+    page = bad.columns[0].pages[0]
+    n = int(page[0]) | int(page[1]) << 8
+    nv = int(page[2]) | int(page[3]) << 8
+    assert nv < n
+    bm0 = 8192 - (n + 7) // 8
+    for r in range(n):  # clear the first valid row's bit
+        if page[bm0 + r // 8] >> (r % 8) & 1:
+            page[bm0 + r // 8] &= ~(1 << (r % 8)) & 0xFF
+            break
+    for t, ok in ((good, True), (bad, False)):
+        plan = cjtest.Plan()
+        plan.new_input(t)
+        plan.new_input(t)
+        plan.new_scan_node(0, [(0, "INT32"), (1, "INT32")])
+        plan.new_scan_node(1, [(0, "INT32"), (1, "INT32")])
+        plan.root = plan.new_join_node(True, 0, 1, 0, 0, [(1, "INT32"), (3, "INT32")])
+        if ok:
+            same_multiset(engine.execute(plan), cjtest.OracleRun(plan, threads=1))
+        else:
+            with pytest.raises(abi.CJError):
+                engine.execute(plan)
+
+
 @pytest.mark.parametrize("flags", PATHS, ids=PATH_IDS)
 @pytest.mark.parametrize("key_null,pay_null", [(0.3, 0.95), (0.6, 0.97), (0.0, 0.5)],
                          ids=["sparse-payload", "sparse-keys", "half-null"])
