@@ -120,6 +120,23 @@ __global__ __launch_bounds__(kWG) void k_bx_part_count(BatchArgs a, uint32_t uni
 // of a plan with few partitions writes long coalesced runs instead of one
 // scattered store per tuple).
 
+// The key pages of a scatter launch dealt to the eight XCDs in contiguous
+// eighths: workgroup b runs on XCD b % 8 (workgroups are dispatched to the
+// XCDs in turn), so the pages one XCD takes at a time are consecutive pages
+// of a few plans, and the partial cache lines that its scattered stores
+// leave in those plans' partition runs fill up in that XCD's L2 before they
+// are written back (instead of eight L2s each writing back a piece of every
+// line).  A grid that is not a multiple of 8 strides plainly.
+struct XcdUnits {
+    uint32_t first, end, step;
+};
+__device__ __forceinline__ XcdUnits xcd_units(uint32_t units) {
+    if (gridDim.x < 8 || gridDim.x % 8) return {blockIdx.x, units, gridDim.x};
+    const uint32_t x = blockIdx.x & 7;
+    const uint32_t lo = (uint32_t)((uint64_t)units * x / 8), hi = (uint32_t)((uint64_t)units * (x + 1) / 8);
+    return {lo + (blockIdx.x >> 3), hi, gridDim.x >> 3};
+}
+
 // Carried column c of side sd at table row `row`.
 __device__ __forceinline__ int32_t carried4(const BSide &sd, uint32_t c, uint64_t row) {
     const uint32_t R = sd.src_rows[c];
@@ -142,8 +159,9 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t u
     extern __shared__ uint64_t sm64[];
     uint64_t *gb = sm64;
     uint32_t *cnt = reinterpret_cast<uint32_t *>(sm64 + lds_parts);
-    __shared__ int32_t akey[kTile];    // by value index: the key
-    __shared__ uint32_t ainfo[kTile];  // partition << kRankBits | rank
+    // (a tuple's key is read back from the page's value array at the
+    // write-out: no LDS copy, so eight workgroups fit a CU)
+    __shared__ uint32_t ainfo[kTile];  // by value index: partition << kRankBits | rank
     __shared__ uint16_t arow[kTile];   // row in the page
     __shared__ uint16_t order[kTile];  // partition order -> value index
     __shared__ uint32_t lds32[32];
@@ -151,7 +169,8 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t u
     uint64_t *gcur = PROBE ? a.pcur : a.bcur;
     int32_t *keys_out = PROBE ? a.pkeys : a.bstage_keys;
     const uint32_t lp = a.log_part;
-    for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
+    const XcdUnits xu = xcd_units(units);
+    for (uint32_t g = xu.first; g < xu.end; g += xu.step) {
         const uint32_t k = find_owner(pstart, a.nplans, g);
         const BPlan &pl = plans[k];
         const BSide &sd = PROBE ? pl.p : pl.b;
@@ -172,7 +191,6 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t u
                 if ((valid >> j) & 1u) {
                     const uint32_t q = (uint32_t)(bslot_of(key[j], ls) >> lp);
                     const uint32_t rank = atomicAdd(&cnt[q], 1u);
-                    akey[vi[j]] = key[j];
                     ainfo[vi[j]] = q << kRankBits | rank;
                     arow[vi[j]] = (uint16_t)(t0 + kWG * j + threadIdx.x);
                 }
@@ -203,9 +221,10 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t u
         // write-out in partition order: keys, then each carried column
         const uint64_t plan_base = off[pb];
         const uint64_t r0 = glb(sd.k_rs)[page];
+        const CJ_G int32_t *vals = reinterpret_cast<const CJ_G int32_t *>(p + 4);
         for (uint32_t s = threadIdx.x; s < nval; s += kWG) {
             const uint32_t v = order[s], in = ainfo[v];
-            keys_out[gb[in >> kRankBits] + (in & ((1u << kRankBits) - 1))] = akey[v];
+            keys_out[gb[in >> kRankBits] + (in & ((1u << kRankBits) - 1))] = vals[v];
         }
         const uint32_t ncar = sd.ncar;
         for (uint32_t c = 0; c < ncar; ++c) {
@@ -253,7 +272,8 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter_direct(BatchArgs a, uin
     uint64_t *gcur = PROBE ? a.pcur : a.bcur;
     int32_t *keys_out = PROBE ? a.pkeys : a.bstage_keys;
     const uint32_t lp = a.log_part;
-    for (uint32_t g = blockIdx.x; g < units; g += gridDim.x) {
+    const XcdUnits xu = xcd_units(units);
+    for (uint32_t g = xu.first; g < xu.end; g += xu.step) {
         const uint32_t k = find_owner(pstart, a.nplans, g);
         const BPlan &pl = plans[k];
         const BSide &sd = PROBE ? pl.p : pl.b;
@@ -1333,6 +1353,11 @@ __global__ void k_copy_u64(const uint64_t *src, uint64_t *dst, uint64_t n) {
 
 // -------------------------------------------------------------- launchers
 static unsigned pgrid(uint64_t units, uint32_t grid) { return (unsigned)(units < grid ? units : grid); }
+// a scatter grid: a multiple of 8 where it has 8 workgroups or more (xcd_units)
+static unsigned sgrid(uint64_t units, uint32_t grid) {
+    const unsigned g = pgrid(units, grid);
+    return g >= 8 ? g & ~7u : g;
+}
 
 // dynamic LDS of each kernel
 static size_t part_count_lds(const BatchArgs &a) { return a.max_parts * sizeof(uint32_t); }
@@ -1409,13 +1434,13 @@ void batch_part_scatter(const BatchArgs &a, bool probe, const uint32_t *pstart, 
     const size_t sh = scatter_lds(parts);
     const uint64_t *off = probe ? a.poff : a.boff;
     if (staged) {
-        if (probe) k_bx_part_scatter<true><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off, parts);
-        else k_bx_part_scatter<false><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off, parts);
+        if (probe) k_bx_part_scatter<true><<<sgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off, parts);
+        else k_bx_part_scatter<false><<<sgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off, parts);
     } else {
         if (probe)
-            k_bx_part_scatter_direct<true><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off, parts);
+            k_bx_part_scatter_direct<true><<<sgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off, parts);
         else
-            k_bx_part_scatter_direct<false><<<pgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off,
+            k_bx_part_scatter_direct<false><<<sgrid(units, grid), kWG, sh, s>>>(a, units, a.plans, pstart, a.part_start, off,
                                                                                  parts);
     }
     CJ_HIP(hipGetLastError());
