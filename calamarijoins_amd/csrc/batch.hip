@@ -222,12 +222,19 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter(BatchArgs a, uint32_t u
         const uint64_t plan_base = off[pb];
         const uint64_t r0 = glb(sd.k_rs)[page];
         const CJ_G int32_t *vals = reinterpret_cast<const CJ_G int32_t *>(p + 4);
+        const uint32_t ncar = sd.ncar;
+        // keys and a first 4-byte carried column in one pass (their loads
+        // in flight together), other columns after
+        const bool first4 = ncar > 0 && sd.car_w[0] == 4;
+        CJ_G int32_t *dst0 = first4 ? glb(reinterpret_cast<int32_t *>(sd.car[0])) : nullptr;
         for (uint32_t s = threadIdx.x; s < nval; s += kWG) {
             const uint32_t v = order[s], in = ainfo[v];
-            keys_out[gb[in >> kRankBits] + (in & ((1u << kRankBits) - 1))] = vals[v];
+            const uint64_t d = gb[in >> kRankBits] + (in & ((1u << kRankBits) - 1));
+            const int32_t key = vals[v];
+            if (first4) dst0[d - plan_base] = carried4(sd, 0, r0 + arow[v]);
+            keys_out[d] = key;
         }
-        const uint32_t ncar = sd.ncar;
-        for (uint32_t c = 0; c < ncar; ++c) {
+        for (uint32_t c = first4 ? 1 : 0; c < ncar; ++c) {
             if (sd.car_w[c] == 4) {
                 CJ_G int32_t *dst = glb(reinterpret_cast<int32_t *>(sd.car[c]));
                 for (uint32_t s = threadIdx.x; s < nval; s += kWG) {
@@ -1018,7 +1025,8 @@ __device__ __forceinline__ void bx_write_desc(const uint32_t *desc, const BPlan 
             pre += ww < w ? x : 0u;
             run += x;
         }
-        __syncthreads();  // every wave has read lds32 before the next tile writes it
+        // (the next tile writes lds32 after this tile's window barriers, or
+        // the barrier of a tile without matches: every wave has read it)
         for (uint32_t w0 = 0; w0 < run; w0 += kMatchWin) {
             const uint32_t w1 = run - w0 < kMatchWin ? run : w0 + kMatchWin;
             // the thread's entries [pre, pre + mt) that fall in [w0, w1)
