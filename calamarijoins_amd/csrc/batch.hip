@@ -287,19 +287,27 @@ __global__ __launch_bounds__(kWG) void k_bx_part_scatter_direct(BatchArgs a, uin
         const uint32_t ls = pl.log_slots;
         const uint32_t P = 1u << (ls - lp);
         const uint32_t page = g - pstart[k];
-        for (uint32_t i = threadIdx.x; i < P; i += kWG) hist[i] = 0;
-        __syncthreads();
         const CJ_G uint8_t *p = glb(sd.k_pages) + (size_t)page * kPage;
-        const uint32_t n = ld16(p), nv = ld16(p + 2);
-        uint32_t running = 0;
-        for (uint32_t t0 = 0; t0 < n; t0 += kTile) {
+        // the histogram counts the page's non-NULL keys straight from its
+        // value array (at most 2047 values: one slot a thread; addresses stay
+        // inside the page), loaded with the header
+        {
             int32_t key[kRPT];
-            const uint32_t valid = load_keys8(p, n, nv, n == nv, t0, &running, key, lds32);
+#pragma unroll
+            for (uint32_t j = 0; j < kRPT; ++j) {
+                const uint32_t v = kWG * j + threadIdx.x;
+                key[j] = ld32(p + 4 + 4 * (v < kTile - 2 ? v : kTile - 2));
+            }
+            const uint32_t nv0 = ld16(p + 2);
+            for (uint32_t i = threadIdx.x; i < P; i += kWG) hist[i] = 0;
+            __syncthreads();
 #pragma unroll
             for (uint32_t j = 0; j < kRPT; ++j)
-                if ((valid >> j) & 1u) atomicAdd(&hist[bslot_of(key[j], ls) >> lp], 1u);
+                if (kWG * j + threadIdx.x < nv0) atomicAdd(&hist[bslot_of(key[j], ls) >> lp], 1u);
             __syncthreads();
         }
+        const uint32_t n = ld16(p), nv = ld16(p + 2);
+        uint32_t running = 0;
         const uint32_t pb = part_start[k];
         for (uint32_t i0 = 0; i0 < P; i0 += kWG * kResPer) {
             uint64_t r[kResPer];
