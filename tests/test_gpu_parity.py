@@ -149,6 +149,36 @@ def test_gpu_registered_key_statistics_recount_for_a_wider_map(engine):
             engine.unregister(t)
 
 
+@pytest.mark.parametrize("dim_rows", [1_000_000, 2_400_000], ids=["one-window", "two-windows"])
+def test_gpu_count_pass_presence_bitmap_in_lds(engine, dim_rows):
+    """A probe side of more than 4096 dense key pages against a direct map
+    whose keys leave gaps in their range (30 % of a permuted domain kept):
+    the counting pass takes the presence bitmap into LDS, in one window
+    (1M-key range) or two (2.4M); a two-join plan whose first join builds
+    that map equals the oracle."""
+    fact = workloads.generate(8_600_000, 4242, [
+        workloads.col(workloads.GEN_UNIFORM, "INT32", 20, 0.0, lo=0, hi=dim_rows),
+        workloads.col(workloads.GEN_UNIFORM, "INT32", 21, 0.0, lo=0, hi=1000),
+        workloads.col(workloads.GEN_UNIFORM, "INT32", 22, 0.0, lo=-50, hi=50)])
+    dim = workloads.generate(dim_rows, 4243, [
+        workloads.col(workloads.GEN_PERM, "INT32", 1, 0.0, domain=dim_rows, perm_stream=50),
+        workloads.col(workloads.GEN_UNIFORM, "INT32", 2, 0.0, lo=0, hi=100)], keep_frac=0.3)
+    small = workloads.generate(1000, 4244, [
+        workloads.col(workloads.GEN_PERM, "INT32", 1, 0.0, domain=1000, perm_stream=51),
+        workloads.col(workloads.GEN_UNIFORM, "INT32", 2, 0.0, lo=0, hi=7)], keep_frac=0.5)
+    p = cjtest.Plan()
+    for t in (fact, dim, small):
+        p.new_input(t)
+    f = p.new_scan_node(0, [(0, "INT32"), (1, "INT32"), (2, "INT32")])
+    d = p.new_scan_node(1, [(0, "INT32"), (1, "INT32")])
+    sm = p.new_scan_node(2, [(0, "INT32"), (1, "INT32")])
+    j1 = p.new_join_node(False, f, d, 0, 0, [(1, "INT32"), (2, "INT32"), (4, "INT32")])
+    p.root = p.new_join_node(False, j1, sm, 0, 0, [(1, "INT32"), (2, "INT32"), (4, "INT32")])
+    exp = cjtest.OracleRun(p, threads=8)
+    assert exp.num_rows > 10_000
+    same_multiset(engine.execute(p), exp)
+
+
 @pytest.mark.parametrize("scale", [100, 10])
 def test_gpu_c3_star_pipeline_bit_exact(engine, scale):
     """Left-deep 4-join star with VARCHAR (incl. long strings) through
