@@ -51,6 +51,21 @@ __device__ __forceinline__ uint32_t find_owner(const T *start, uint32_t n, T g) 
     return lo;
 }
 
+// find_owner that keeps its last answer's unit range: the units a wave
+// takes one after another mostly lie in the range of the one before, so most
+// calls skip the search (a chain of dependent scalar loads).
+struct OwnerCache {
+    uint32_t k = 0, lo = 1, hi = 0;
+};
+__device__ __forceinline__ uint32_t owner_of(OwnerCache &c, const uint32_t *start, uint32_t n, uint32_t g) {
+    if (g < c.lo || g >= c.hi) {
+        c.k = find_owner(start, n, g);
+        c.lo = start[c.k];
+        c.hi = start[c.k + 1];
+    }
+    return c.k;
+}
+
 }  // namespace
 
 // ------------------------------------------------------ partition histogram
@@ -1293,13 +1308,19 @@ __global__ __launch_bounds__(kWG) void k_bx_page_init(const BPageCol *__restrict
                                                       const uint32_t *__restrict__ page_start, uint32_t ncols,
                                                       uint32_t pages, uint32_t *page_nulls) {
     const uint32_t lane = lane_id();
+    OwnerCache oc;
+    uint32_t mc = ~0u;  // the column whose descriptor m holds
+    BPageCol m{};
     for (uint32_t g = blockIdx.x * (kWG / 64) + wave_id(); g < pages; g += gridDim.x * (kWG / 64)) {
-        const uint32_t c = find_owner(page_start, ncols, g);
-        const BPageCol m = cols[c];
+        const uint32_t c = owner_of(oc, page_start, ncols, g);
+        if (c != mc) {
+            m = cols[c];
+            mc = c;
+        }
         const uint32_t R = m.width == 4 ? kRowsI32 : kRowsI64;
-        const uint64_t r0 = (uint64_t)(g - page_start[c]) * R;
+        const uint64_t r0 = (uint64_t)(g - oc.lo) * R;  // (oc.lo = page_start[c])
         const uint32_t n = (uint32_t)(m.rows - r0 < R ? m.rows - r0 : R);
-        CJ_G uint8_t *p = glb(m.out) + (uint64_t)(g - page_start[c]) * kPage;
+        CJ_G uint8_t *p = glb(m.out) + (uint64_t)(g - oc.lo) * kPage;
         const uint32_t nbytes = (n + 7) >> 3;
         for (uint32_t i = lane; i < nbytes; i += 64) {
             const uint32_t left = n - 8 * i;  // rows of this byte
@@ -1320,41 +1341,53 @@ __global__ __launch_bounds__(kWG) void k_bx_page_fix(const BPageCol *__restrict_
                                                      const uint32_t *__restrict__ page_start, uint32_t ncols,
                                                      uint32_t pages, const uint32_t *__restrict__ page_nulls) {
     __shared__ uint32_t lds4[4];
-    for (uint32_t g = blockIdx.x; g < pages; g += gridDim.x) {
-        if (!page_nulls[g]) continue;
-        const uint32_t c = find_owner(page_start, ncols, g);
-        const BPageCol m = cols[c];
-        const uint32_t R = m.width == 4 ? kRowsI32 : kRowsI64;
-        const uint64_t r0 = (uint64_t)(g - page_start[c]) * R;
-        const uint32_t n = (uint32_t)(m.rows - r0 < R ? m.rows - r0 : R);
-        CJ_G uint8_t *p = glb(m.out) + (uint64_t)(g - page_start[c]) * kPage;
-        const CJ_G uint8_t *bm = p + kPage - ((n + 7) >> 3);
-        uint32_t tot;
-        if (m.width == 4) {
-            const uint32_t i0 = 8 * threadIdx.x;
-            uint32_t bits = i0 < n ? bm[i0 >> 3] : 0u;  // 8 rows a thread: one bitmap byte
-            CJ_G int32_t *vals = reinterpret_cast<CJ_G int32_t *>(p + 4);
-            int32_t v[8];
-#pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) v[q] = vals[i0 + q < n ? i0 + q : 0];
-            uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);  // its barrier orders loads before stores
-#pragma unroll
-            for (uint32_t q = 0; q < 8; ++q)
-                if ((bits >> q) & 1u) vals[o++] = v[q];
-        } else {
-            const uint32_t i0 = 4 * threadIdx.x;
-            uint32_t bits = i0 < n ? (bm[i0 >> 3] >> (i0 & 4)) & 15u : 0u;
-            CJ_G uint64_t *vals = reinterpret_cast<CJ_G uint64_t *>(p + 8);
-            uint64_t v[4];
-#pragma unroll
-            for (uint32_t q = 0; q < 4; ++q) v[q] = vals[i0 + q < n ? i0 + q : 0];
-            uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);
-#pragma unroll
-            for (uint32_t q = 0; q < 4; ++q)
-                if ((bits >> q) & 1u) vals[o++] = v[q];
-        }
-        if (threadIdx.x == 0) *reinterpret_cast<CJ_G uint16_t *>(p + 2) = (uint16_t)tot;
+    __shared__ uint32_t flagged[kWG], nflag;
+    // the NULL counts of kWG pages are read at once (one per thread) and the
+    // pages that have any are listed; the workgroup then packs them one by one
+    for (uint32_t base = blockIdx.x * kWG; base < pages; base += gridDim.x * kWG) {
+        if (threadIdx.x == 0) nflag = 0;
         __syncthreads();
+        const uint32_t gt = base + threadIdx.x;
+        if (gt < pages && page_nulls[gt]) flagged[atomicAdd(&nflag, 1u)] = gt;
+        __syncthreads();
+        const uint32_t nf = nflag;
+        for (uint32_t fi = 0; fi < nf; ++fi) {
+            const uint32_t g = flagged[fi];
+            const uint32_t c = find_owner(page_start, ncols, g);
+            const BPageCol m = cols[c];
+            const uint32_t R = m.width == 4 ? kRowsI32 : kRowsI64;
+            const uint64_t r0 = (uint64_t)(g - page_start[c]) * R;
+            const uint32_t n = (uint32_t)(m.rows - r0 < R ? m.rows - r0 : R);
+            CJ_G uint8_t *p = glb(m.out) + (uint64_t)(g - page_start[c]) * kPage;
+            const CJ_G uint8_t *bm = p + kPage - ((n + 7) >> 3);
+            uint32_t tot;
+            if (m.width == 4) {
+                const uint32_t i0 = 8 * threadIdx.x;
+                uint32_t bits = i0 < n ? bm[i0 >> 3] : 0u;  // 8 rows a thread: one bitmap byte
+                CJ_G int32_t *vals = reinterpret_cast<CJ_G int32_t *>(p + 4);
+                int32_t v[8];
+#pragma unroll
+                for (uint32_t q = 0; q < 8; ++q) v[q] = vals[i0 + q < n ? i0 + q : 0];
+                uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);  // its barrier orders loads before stores
+#pragma unroll
+                for (uint32_t q = 0; q < 8; ++q)
+                    if ((bits >> q) & 1u) vals[o++] = v[q];
+            } else {
+                const uint32_t i0 = 4 * threadIdx.x;
+                uint32_t bits = i0 < n ? (bm[i0 >> 3] >> (i0 & 4)) & 15u : 0u;
+                CJ_G uint64_t *vals = reinterpret_cast<CJ_G uint64_t *>(p + 8);
+                uint64_t v[4];
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) v[q] = vals[i0 + q < n ? i0 + q : 0];
+                uint32_t o = block_excl_scan<uint32_t>(__popc(bits), &tot, lds4);
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q)
+                    if ((bits >> q) & 1u) vals[o++] = v[q];
+            }
+            if (threadIdx.x == 0) *reinterpret_cast<CJ_G uint16_t *>(p + 2) = (uint16_t)tot;
+            __syncthreads();
+        }
+        __syncthreads();  // flagged and nflag are rewritten for the next pages
     }
 }
 
@@ -1512,7 +1545,7 @@ void batch_page_init(const BPageCol *cols, const uint32_t *page_start, uint32_t 
 void batch_page_fix(const BPageCol *cols, const uint32_t *page_start, uint32_t ncols, uint32_t pages,
                     const uint32_t *page_nulls, uint32_t grid, hipStream_t s) {
     if (!pages || !ncols) return;
-    k_bx_page_fix<<<pgrid(pages, grid), kWG, 0, s>>>(cols, page_start, ncols, pages, page_nulls);
+    k_bx_page_fix<<<pgrid((pages + kWG - 1) / kWG, grid), kWG, 0, s>>>(cols, page_start, ncols, pages, page_nulls);
     CJ_HIP(hipGetLastError());
 }
 
