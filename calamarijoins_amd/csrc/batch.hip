@@ -1321,11 +1321,17 @@ __global__ __launch_bounds__(kWG) void k_bx_page_init(const BPageCol *__restrict
         const uint64_t r0 = (uint64_t)(g - oc.lo) * R;  // (oc.lo = page_start[c])
         const uint32_t n = (uint32_t)(m.rows - r0 < R ? m.rows - r0 : R);
         CJ_G uint8_t *p = glb(m.out) + (uint64_t)(g - oc.lo) * kPage;
-        const uint32_t nbytes = (n + 7) >> 3;
-        for (uint32_t i = lane; i < nbytes; i += 64) {
-            const uint32_t left = n - 8 * i;  // rows of this byte
-            p[kPage - nbytes + i] = (uint8_t)(left >= 8 ? 0xffu : (1u << left) - 1u);
+        // the bitmap ends at the page end: its last byte holds the last rows;
+        // whole 4-byte words from the first aligned one, bytes before it
+        const uint32_t nbytes = (n + 7) >> 3, b0 = kPage - nbytes, a0 = (b0 + 3) & ~3u;
+        const uint32_t tail = n & 7;  // rows of the last byte (0: a full byte)
+        for (uint32_t w = a0 / 4 + lane; w < kPage / 4; w += 64) {
+            uint32_t v = ~0u;
+            if (w == kPage / 4 - 1 && tail) v = 0x00ffffffu | (((1u << tail) - 1u) << 24);
+            reinterpret_cast<CJ_G uint32_t *>(p)[w] = v;
         }
+        if (lane < a0 - b0)  // (<= 3 bytes; the page's last byte only when no whole word is left)
+            p[b0 + lane] = (uint8_t)(b0 + lane == kPage - 1 && tail ? (1u << tail) - 1u : 0xffu);
         if (lane == 0) {
             *reinterpret_cast<CJ_G uint32_t *>(p) = n | (n << 16);
             page_nulls[g] = 0;
